@@ -1,0 +1,139 @@
+"""Benchmark: simulated ms per wall-second of the 1M-GR eyelid network on one GPU.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c1]
+
+A step is one 1 ms network step (all GR, GO, SC, BC, PC, IO, NC) over one
+row of a synthetic MF spike train that is resident in HBM before the timed
+region (CS on a seeded 3% of MFs at 100 Hz in 500-1000 ms of every 2,000 ms
+trial, the US as error drive at 999 ms).  Prints one JSON line.
+
+roofline: the granule-cell kernel's algorithmic bytes per launch (numGR x
+104 B at fan-ins (4,3,2), 120 B at (4,3,4); the compulsory per-GR bytes with
+the reference's data types) divided by its mean duration, timed with HIP events
+on the stream it runs on, against MI355X's 8 TB/s.
+
+cpu_baseline: the CPU oracle (a C restatement of the same step, one thread),
+timed on this host over a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "simulated ms per wall-second, 1M-GR net; batched network-ms/s vs plain"
+HBM_PEAK_GBS = 8000.0
+WORKLOADS = {
+    "c1": "1M-GR eyelid network: 1,048,576 GR; 1,024 GO; 1,024 MF; 32 PC (32,768 PF each); 128 BC; "
+          "512 SC; 4 IO; 8 NC; 4 MF + 3 GO inputs and 2 GO targets per GR, seed 0; 2,000 ms trials",
+    "c2": "4M-GR dense network: 4,194,304 GR; 4,096 GO; 4,096 MF; 32 PC; 128 BC; 512 SC; 4 IO; 8 NC; "
+          "4 GO targets per GR in a GO-grid window",
+    "c0": "16K-GR network: 16,384 GR; 256 GO; 256 MF; 8 PC; 32 BC; 128 SC; 1 IO; 2 NC",
+}
+
+
+def bytes_per_gr(n_out: int) -> int:
+    # 5 x FP32 state r+w (40) + apBuf r+w (8) + 4*(nMF+nGOin) indices (28)
+    # + 8*nGOout target+mask (16 or 32) + PF delay mask (4) + weight read (4)
+    # + (history r+w + weight write)/5 (4)
+    return 40 + 8 + 28 + 8 * n_out + 4 + 4 + 4
+
+
+def cpu_baseline(cfg: str, sample_steps: int):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import orc
+    from cerebellum_simulation_amd import configs
+
+    st = configs.build_state(cfg, seed=0)
+    ref = orc.Oracle.from_state(st, precision=32)
+    tr = configs.mf_train(st.sizes.numMF, sample_steps, seed=11)
+    t0 = time.perf_counter()
+    ref.run_mf_train(tr)
+    dt = time.perf_counter() - t0
+    return {"value": round(sample_steps / dt, 3), "unit": "sim-ms/s", "cores": 1, "kind": "port",
+            "sample": f"{sample_steps} steps of the {cfg} network from its initial state, oracle binary32 "
+                      f"build (oracle/build/liborc.so), one thread, {dt:.2f} s; host {platform.processor() or platform.machine()}"
+                      f" with {os.cpu_count()} logical CPUs"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--config", default="c1")
+    ap.add_argument("--profile-steps", type=int, default=400)
+    ap.add_argument("--cpu-steps", type=int, default=0, help="0 = pick a ~15 s sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the network step runs "
+              "on a single MI355X; see DESIGN.md for how it would shard)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from cerebellum_simulation_amd import Network, configs
+
+    t_build = time.perf_counter()
+    st = configs.build_state(args.config, seed=0)
+    net = Network.from_state(st)
+    s = st.sizes
+    n = args.warmup + args.steps
+    tr = configs.mf_train(s.numMF, n + args.profile_steps, seed=1)
+    err = configs.us_err_drive(n + args.profile_steps, st.num_zones)
+    dtr = torch.from_numpy(tr).to("cuda:0")
+    torch.cuda.synchronize()
+    print(f"[bench] {args.config}: built in {time.perf_counter() - t_build:.1f} s, "
+          f"{net.device_bytes() / 2**20:.0f} MiB on device", file=sys.stderr, flush=True)
+
+    net.run_mf_train(dtr[: args.warmup], err_drive=err[: args.warmup])
+    net.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    net.run_mf_train(dtr[args.warmup:n], err_drive=err[args.warmup:n])
+    net.sync()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    sim_ms_per_s = args.steps / elapsed
+    print(f"[bench] timed {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
+
+    # GR-kernel duration with HIP events on the handle's stream
+    gr_us, step_us = net.profile_steps(dtr[n:n + args.profile_steps])
+    n_out = s.maxnumpGRfromGRtoGO
+    alg_bytes = s.numGR * bytes_per_gr(n_out)
+    achieved = alg_bytes / (gr_us * 1e-6) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "gr_step_kernel", "kernel_us": round(gr_us, 3),
+                "algorithmic_bytes_per_launch": alg_bytes, "step_us_events": round(step_us, 3)}
+    traffic_file = os.path.join(ROOT, "profiles", f"traffic_{args.config}.json")
+    if os.path.exists(traffic_file):
+        with open(traffic_file) as f:
+            roofline["traffic"] = json.load(f).get("hbm_bytes_per_launch")
+
+    cpu = None
+    if not args.no_cpu:
+        cpu_steps = args.cpu_steps or {"c0": 5000, "c1": 120, "c2": 30}.get(args.config, 100)
+        cpu = cpu_baseline(args.config, cpu_steps)
+
+    out = {
+        "metric": METRIC, "value": round(sim_ms_per_s, 2), "unit": "sim-ms/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1000.0 * elapsed / args.steps, 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic: seeded Bernoulli MF trains, seeded synthetic connectivity",
+        "config": {"workload": WORKLOADS.get(args.config, args.config), "name": args.config,
+                   "numGR": s.numGR, "numGO": s.numGO, "numMF": s.numMF, "numPC": s.numPC,
+                   "parallelism": "single GPU"},
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,171 @@
+"""ctypes view of the two C-ABIs (include/cbm_hip.h, include/cbm_state.h).
+
+The struct layouts and the field enum are read from the C header itself, so
+the Python mirror cannot drift from the library it binds.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+HEADER = os.path.join(ROOT, "include", "cbm_hip.h")
+LIBDIR = os.path.join(PKG, "lib")
+
+_CTYPES = {"float": C.c_float, "uint32_t": C.c_uint32, "int32_t": C.c_int32}
+
+
+def _strip_comments(text: str) -> str:
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return re.sub(r"//[^\n]*", "", text)
+
+
+def _parse_struct(text: str, name: str) -> list[tuple[str, type]]:
+    m = re.search(r"typedef struct %s \{(.*?)\} %s;" % (name, name), text, flags=re.S)
+    if not m:
+        raise RuntimeError(f"struct {name} not found in {HEADER}")
+    fields = []
+    for decl in m.group(1).split(";"):
+        decl = " ".join(decl.split())
+        if not decl:
+            continue
+        typ, rest = decl.split(" ", 1)
+        for member in rest.split(","):
+            fields.append((member.strip(), _CTYPES[typ]))
+    return fields
+
+
+def _parse_enum(text: str, name: str) -> dict[str, int]:
+    m = re.search(r"enum %s \{(.*?)\};" % name, text, flags=re.S)
+    if not m:
+        raise RuntimeError(f"enum {name} not found in {HEADER}")
+    out, nxt = {}, 0
+    for item in m.group(1).split(","):
+        item = item.strip()
+        if not item:
+            continue
+        if "=" in item:
+            k, v = (x.strip() for x in item.split("="))
+            nxt = int(v, 0)
+        else:
+            k = item
+        out[k] = nxt
+        nxt += 1
+    return out
+
+
+_TEXT = _strip_comments(open(HEADER).read())
+
+
+class Sizes(C.Structure):
+    _fields_ = _parse_struct(_TEXT, "cbm_sizes")
+
+
+class ActParams(C.Structure):
+    _fields_ = _parse_struct(_TEXT, "cbm_act_params")
+
+
+class Options(C.Structure):
+    _fields_ = _parse_struct(_TEXT, "cbm_options")
+
+
+class GenSpec(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("numZones", C.c_uint32), ("innetSeed", C.c_int32),
+                ("nMFperGR", C.c_uint32), ("nGOperGR", C.c_uint32), ("nGOoutPerGR", C.c_uint32),
+                ("nMFperGO", C.c_uint32), ("grgoWindowX", C.c_uint32), ("grgoWindowY", C.c_uint32),
+                ("mzoneConSeed", C.c_int32 * 8), ("mzoneRngSeed", C.c_int32 * 8)]
+
+
+FIELDS: dict[str, int] = {k[len("CBM_F_"):]: v for k, v in _parse_enum(_TEXT, "cbm_field").items()
+                          if k != "CBM_F_COUNT"}
+FIELD_COUNT = len(FIELDS)
+FIELD_NAMES = {v: k for k, v in FIELDS.items()}
+KIND_CON, KIND_STATE, KIND_RUNTIME = 0, 1, 2
+
+_hip = None
+_state = None
+
+
+def _load(name: str) -> C.CDLL:
+    path = os.path.join(LIBDIR, name)
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with `python -m cerebellum_simulation_amd.build`")
+    return C.CDLL(path)
+
+
+def hip_lib() -> C.CDLL:
+    """libcbm_hip.so with argument types declared."""
+    global _hip
+    if _hip is None:
+        L = _load("libcbm_hip.so")
+        P, I, S, V = C.c_void_p, C.c_int, C.c_size_t, C.c_void_p
+        L.cbm_abi_version.restype = I
+        L.cbm_last_error.restype = C.c_char_p
+        L.cbm_device_count.restype = I
+        L.cbm_create.argtypes = [C.POINTER(Sizes), C.POINTER(ActParams), C.POINTER(Options), C.POINTER(P)]
+        L.cbm_destroy.argtypes = [P]
+        L.cbm_write.argtypes = [P, I, I, V, S]
+        L.cbm_read.argtypes = [P, I, I, V, S]
+        L.cbm_commit.argtypes = [P]
+        L.cbm_set_mf.argtypes = [P, V]
+        L.cbm_set_err_drive.argtypes = [P, I, C.c_float]
+        L.cbm_set_grpc_steps.argtypes = [P, I, C.c_float, C.c_float]
+        L.cbm_reset_grpc_steps.argtypes = [P, I]
+        L.cbm_set_ginc_grgo.argtypes = [P, C.c_float]
+        L.cbm_reset_ginc_grgo.argtypes = [P]
+        L.cbm_step.argtypes = [P, I]
+        L.cbm_run_mf_train.argtypes = [P, V, S, I, V]
+        L.cbm_sync.argtypes = [P]
+        L.cbm_stream.argtypes = [P]
+        L.cbm_stream.restype = P
+        L.cbm_profile_steps.argtypes = [P, V, S, I, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.cbm_device_bytes.argtypes = [P, C.POINTER(C.c_size_t)]
+        L.cbm_field_info.argtypes = [I, C.POINTER(Sizes), C.POINTER(S), C.POINTER(S), C.POINTER(S),
+                                     C.POINTER(I), C.POINTER(I), C.POINTER(I)]
+        if L.cbm_abi_version() != 1:
+            raise RuntimeError("libcbm_hip.so ABI version mismatch")
+        _hip = L
+    return _hip
+
+
+def state_lib() -> C.CDLL:
+    """libcbm_state.so with argument types declared."""
+    global _state
+    if _state is None:
+        L = _load("libcbm_state.so")
+        P, I, V = C.c_void_p, C.c_int, C.c_void_p
+        L.cbm_state_last_error.restype = C.c_char_p
+        L.cbm_act_params_from_text.argtypes = [C.c_char_p, C.POINTER(ActParams)]
+        L.cbm_state_create.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(GenSpec), C.POINTER(P)]
+        L.cbm_state_destroy.argtypes = [P]
+        L.cbm_state_sizes.argtypes = [P, C.POINTER(Sizes)]
+        L.cbm_state_params.argtypes = [P, C.POINTER(ActParams)]
+        L.cbm_state_field.argtypes = [P, I, I, C.POINTER(V), C.POINTER(C.c_size_t)]
+        L.cbm_state_load_into.argtypes = [P, P]
+        L.cbm_sfmt_sample.argtypes = [I, C.c_uint64, I, I, I, I, V]
+        _state = L
+    return _state
+
+
+class CbmError(RuntimeError):
+    """A cbm_* call returned an error code."""
+
+
+def check(rc: int, where: str, lib: C.CDLL | None = None) -> None:
+    if rc != 0:
+        L = lib or hip_lib()
+        getter = L.cbm_state_last_error if hasattr(L, "cbm_state_last_error") and L is _state else L.cbm_last_error
+        msg = getter().decode(errors="replace")
+        raise CbmError(f"{where} failed ({rc}): {msg}")
+
+
+def field_info(field: int, sizes: Sizes) -> dict:
+    """Shape of a field: elem bytes, rows, cols, zoned, kind, is_float."""
+    e, r, c = C.c_size_t(), C.c_size_t(), C.c_size_t()
+    z, k, f = C.c_int(), C.c_int(), C.c_int()
+    check(hip_lib().cbm_field_info(field, C.byref(sizes), C.byref(e), C.byref(r), C.byref(c),
+                                   C.byref(z), C.byref(k), C.byref(f)), "cbm_field_info")
+    return dict(elem=e.value, rows=r.value, cols=c.value, zoned=bool(z.value), kind=k.value,
+                is_float=bool(f.value))

@@ -1,0 +1,230 @@
+"""Python host mirror of the network step: ``NetState`` (CBMState side) and
+``Network`` (the CBMSimCore-like handle over the HIP C-ABI).
+
+The method names follow the reference's CBMSimCore / InNetInterface /
+MZoneInterface (CBM_CORE_LIB_MGPU_DEV/CBMCoreInclude/interface/*.h):
+``calc_activity`` is ``calcActivity``, ``update_mf_input`` is
+``updateMFInput``, ``export_*`` are the export accessors.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _abi
+from ._abi import FIELDS, ActParams, GenSpec, Options, Sizes, check, field_info
+
+_SIGNED = {"NUMP_GR_FROM_MF_TO_GR", "NUMP_GR_FROM_GO_TO_GR", "NUMP_GR_FROM_GR_TO_GO",
+           "NUMP_MF_FROM_MF_TO_GO", "NUMP_GO_GABA_OUT_GOGO", "NUMP_GO_COUP_IN_GOGO",
+           "PFPC_PLAST_TIMER_IO", "ZONE_RNG_SEED"}
+
+
+def field_dtype(name: str, info: dict) -> np.dtype:
+    if info["elem"] == 1:
+        return np.dtype(np.uint8)
+    if info["elem"] == 8:
+        return np.dtype(np.uint64)
+    if info["is_float"]:
+        return np.dtype(np.float32)
+    return np.dtype(np.int32 if name in _SIGNED else np.uint32)
+
+
+def field_shape(info: dict) -> tuple:
+    return (info["rows"], info["cols"]) if info["cols"] != 1 else (info["rows"],)
+
+
+class NetState:
+    """A complete network description built by libcbm_state from actParams /
+    conParams texts and a generator spec (CBMState's role)."""
+
+    def __init__(self, act_text: str, con_text: str, *, num_zones: int = 1, innet_seed: int = 0,
+                 n_mf_per_gr: int = 4, n_go_per_gr: int = 3, n_go_out_per_gr: int = 2,
+                 n_mf_per_go: int = 16, grgo_window: tuple[int, int] = (0, 0),
+                 mzone_con_seeds=None, mzone_rng_seeds=None):
+        L = _abi.state_lib()
+        g = GenSpec()
+        g.kind = 0
+        g.numZones = num_zones
+        g.innetSeed = innet_seed
+        g.nMFperGR, g.nGOperGR, g.nGOoutPerGR, g.nMFperGO = n_mf_per_gr, n_go_per_gr, n_go_out_per_gr, n_mf_per_go
+        g.grgoWindowX, g.grgoWindowY = grgo_window
+        for z in range(8):
+            g.mzoneConSeed[z] = mzone_con_seeds[z] if mzone_con_seeds else innet_seed + 101 + z
+            g.mzoneRngSeed[z] = mzone_rng_seeds[z] if mzone_rng_seeds else innet_seed + 1001 + z
+        h = C.c_void_p()
+        rc = L.cbm_state_create(act_text.encode(), con_text.encode(), C.byref(g), C.byref(h))
+        if rc:
+            raise _abi.CbmError(f"cbm_state_create failed ({rc}): {L.cbm_state_last_error().decode()}")
+        self._h = h
+        self.sizes = Sizes()
+        self.params = ActParams()
+        L.cbm_state_sizes(h, C.byref(self.sizes))
+        L.cbm_state_params(h, C.byref(self.params))
+        self.num_zones = num_zones
+
+    def field(self, name: str, zone: int = 0) -> np.ndarray:
+        """numpy view (no copy) of a field in the CBMState layout."""
+        info = field_info(FIELDS[name], self.sizes)
+        ptr, nb = C.c_void_p(), C.c_size_t()
+        L = _abi.state_lib()
+        rc = L.cbm_state_field(self._h, FIELDS[name], zone, C.byref(ptr), C.byref(nb))
+        if rc:
+            raise _abi.CbmError(L.cbm_state_last_error().decode())
+        dt = field_dtype(name, info)
+        n = nb.value // dt.itemsize
+        if n == 0 or not ptr.value:
+            return np.zeros(field_shape(info), dt)
+        buf = (C.c_uint8 * nb.value).from_address(ptr.value)
+        buf._owner = self  # the view keeps the state alive
+        return np.frombuffer(buf, dtype=dt, count=n).reshape(field_shape(info))
+
+    def load_into(self, net: "Network") -> None:
+        L = _abi.state_lib()
+        rc = L.cbm_state_load_into(self._h, net._h)
+        if rc:
+            raise _abi.CbmError(f"cbm_state_load_into failed: {L.cbm_state_last_error().decode()}")
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            _abi.state_lib().cbm_state_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def act_params_from_text(text: str) -> ActParams:
+    p = ActParams()
+    L = _abi.state_lib()
+    rc = L.cbm_act_params_from_text(text.encode(), C.byref(p))
+    if rc:
+        raise _abi.CbmError(L.cbm_state_last_error().decode())
+    return p
+
+
+class Network:
+    """One network on one GPU behind the cbm_* C-ABI (CBMSimCore's role)."""
+
+    def __init__(self, sizes: Sizes, params: ActParams, *, grgo_lag: int = 2, device: int = 0,
+                 cells_per_thread: int = 0):
+        L = _abi.hip_lib()
+        o = Options()
+        o.grgoLag, o.device, o.grCellsPerThread = grgo_lag, device, cells_per_thread
+        h = C.c_void_p()
+        check(L.cbm_create(C.byref(sizes), C.byref(params), C.byref(o), C.byref(h)), "cbm_create")
+        self._h = h
+        self.sizes = sizes
+        self.params = params
+        self._info = {}
+
+    @classmethod
+    def from_state(cls, state: NetState, **kw) -> "Network":
+        net = cls(state.sizes, state.params, **kw)
+        state.load_into(net)
+        return net
+
+    # ------------------------------------------------------------ fields
+    def info(self, name: str) -> dict:
+        if name not in self._info:
+            self._info[name] = field_info(FIELDS[name], self.sizes)
+        return self._info[name]
+
+    def read(self, name: str, zone: int = 0) -> np.ndarray:
+        info = self.info(name)
+        out = np.empty(field_shape(info), field_dtype(name, info))
+        check(_abi.hip_lib().cbm_read(self._h, FIELDS[name], zone, out.ctypes.data, out.nbytes), f"cbm_read {name}")
+        return out
+
+    def write(self, name: str, value, zone: int = 0) -> None:
+        info = self.info(name)
+        arr = np.ascontiguousarray(np.asarray(value, dtype=field_dtype(name, info)).reshape(field_shape(info)))
+        check(_abi.hip_lib().cbm_write(self._h, FIELDS[name], zone, arr.ctypes.data, arr.nbytes), f"cbm_write {name}")
+
+    def commit(self) -> None:
+        check(_abi.hip_lib().cbm_commit(self._h), "cbm_commit")
+
+    # ------------------------------------------------------------ step API
+    def update_mf_input(self, mf) -> None:
+        arr = np.ascontiguousarray(mf, dtype=np.uint8)
+        assert arr.size == self.sizes.numMF
+        check(_abi.hip_lib().cbm_set_mf(self._h, arr.ctypes.data), "cbm_set_mf")
+
+    def update_err_drive(self, zone: int, rel: float) -> None:
+        check(_abi.hip_lib().cbm_set_err_drive(self._h, zone, rel), "cbm_set_err_drive")
+
+    def set_grpc_plast_steps(self, zone: int, ltd: float, ltp: float) -> None:
+        check(_abi.hip_lib().cbm_set_grpc_steps(self._h, zone, ltd, ltp), "cbm_set_grpc_steps")
+
+    def reset_grpc_plast_steps(self, zone: int) -> None:
+        check(_abi.hip_lib().cbm_reset_grpc_steps(self._h, zone), "cbm_reset_grpc_steps")
+
+    def set_ginc_gr_to_go(self, inc: float) -> None:
+        check(_abi.hip_lib().cbm_set_ginc_grgo(self._h, inc), "cbm_set_ginc_grgo")
+
+    def reset_ginc_gr_to_go(self) -> None:
+        check(_abi.hip_lib().cbm_reset_ginc_grgo(self._h), "cbm_reset_ginc_grgo")
+
+    def calc_activity(self, n: int = 1) -> None:
+        check(_abi.hip_lib().cbm_step(self._h, n), "cbm_step")
+
+    def run_mf_train(self, train, n: int | None = None, err_drive=None) -> None:
+        """n steps over a device-resident uint8 [steps, numMF] tensor."""
+        ptr, pitch, rows = _device_rows(train)
+        n = rows if n is None else n
+        err = None
+        if err_drive is not None:
+            err_arr = np.ascontiguousarray(err_drive, dtype=np.float32)
+            err = err_arr.ctypes.data
+        check(_abi.hip_lib().cbm_run_mf_train(self._h, ptr, pitch, n, err), "cbm_run_mf_train")
+
+    def profile_steps(self, train, n: int | None = None) -> tuple[float, float]:
+        ptr, pitch, rows = _device_rows(train)
+        n = rows if n is None else n
+        gr, st = C.c_double(), C.c_double()
+        check(_abi.hip_lib().cbm_profile_steps(self._h, ptr, pitch, n, C.byref(gr), C.byref(st)),
+              "cbm_profile_steps")
+        return gr.value, st.value
+
+    def sync(self) -> None:
+        check(_abi.hip_lib().cbm_sync(self._h), "cbm_sync")
+
+    def device_bytes(self) -> int:
+        b = C.c_size_t()
+        check(_abi.hip_lib().cbm_device_bytes(self._h, C.byref(b)), "cbm_device_bytes")
+        return b.value
+
+    # ------------------------------------------------------------ exports
+    def export_ap_gr(self):
+        return self.read("AP_GR")
+
+    def export_ap_nc(self, zone: int = 0):
+        return self.read("AP_NC", zone)
+
+    def export_ap_pc(self, zone: int = 0):
+        return self.read("AP_PC", zone)
+
+    def export_ap_io(self, zone: int = 0):
+        return self.read("AP_IO", zone)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            _abi.hip_lib().cbm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _device_rows(train):
+    """(pointer, pitch, rows) of a 2-D uint8 device tensor."""
+    if hasattr(train, "data_ptr"):
+        assert train.dtype.itemsize == 1 and train.dim() == 2 and train.is_cuda
+        return train.data_ptr(), train.stride(0), train.shape[0]
+    raise TypeError("expected a torch uint8 tensor on the GPU")

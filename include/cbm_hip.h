@@ -1,0 +1,414 @@
+/*
+ * cbm_hip.h -- C-ABI of the MI355X cerebellar network step.
+ *
+ * One handle = one network (an input net plus numZones microzones) resident
+ * on one GPU.  The functions below replace the reference's device FFI and the
+ * host-side small-population loops of one 1 ms step:
+ *
+ *   cbm_create / cbm_commit   <- InNet::initCUDA + MZone::initCUDA
+ *                                (CBM_CORE_LIB_MGPU_DEV/src/innetmodules/innet.cpp:168-606,
+ *                                 CBM_CORE_LIB_MGPU_DEV/src/mzonemodules/mzone.cpp:71-137)
+ *   cbm_set_mf                <- CBMSimCore::updateMFInput
+ *                                (CBM_CORE_LIB_MGPU_DEV/src/interface/cbmsimcore.cpp:312-320)
+ *   cbm_set_err_drive         <- CBMSimCore::updateErrDrive / MZone::setErrDrive
+ *                                (cbmsimcore.cpp:322-325, mzone.cpp:167-173)
+ *   cbm_set_grpc_steps        <- MZone::setGRPCPlastSteps / resetGRPCPlastSteps (mzone.cpp:726-736)
+ *   cbm_set_ginc_grgo         <- InNet::setGIncGRtoGO / resetGIncGRtoGO (innet.cpp:669-677)
+ *   cbm_step                  <- CBMSimCore::calcActivity (cbmsimcore.cpp:139-310), i.e. the nine
+ *                                live kernels of CBM_CORE_LIB_MGPU_DEV/src/cuda/kernels.cu:79-376 and
+ *                                the host loops of innet.cpp:800-995 and mzone.cpp:187-605
+ *   cbm_run_mf_train          <- a driver loop of updateMFInput + calcActivity over a
+ *                                device-resident spike train (no counterpart; one launch sequence
+ *                                per ms without host round trips)
+ *   cbm_read                  <- InNet/MZone export* (innet.cpp:679-798, mzone.cpp:739-788)
+ *                                and writeToState (innet.cpp:608-667, mzone.cpp:139-165)
+ *   cbm_write                 <- state upload of the constructors (innet.cpp:406-481)
+ *
+ * All functions return 0 on success and a negative CBM_E_* code on failure;
+ * cbm_last_error() gives a message for the calling thread.  A handle is not
+ * thread safe: one host thread drives it, as in the reference.
+ *
+ * Arrays cross the boundary in the CBMState layout of the reference
+ * (cell-major 2-D arrays [cell][slot], unused slots = UINT32_MAX), so a
+ * CBMState can be handed over field by field.
+ */
+#ifndef CBM_HIP_H
+#define CBM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CBM_ABI_VERSION 1
+
+/* ---------------------------------------------------------------- errors */
+#define CBM_OK 0
+#define CBM_E_ARG (-1)      /* bad argument (null, wrong size, bad field)      */
+#define CBM_E_STATE (-2)    /* call not valid in this state (e.g. not committed) */
+#define CBM_E_HIP (-3)      /* HIP runtime error                                */
+#define CBM_E_UNSUPPORTED (-4) /* network outside what the kernels support      */
+#define CBM_E_NOMEM (-5)
+
+/* ---------------------------------------------------------------- sizes */
+/* Population sizes and fan-ins: the fields of ConnectivityParams that the step
+ * uses (CBM_STATE_LIB_DEV/src/params/connectivityparams.cpp:90-184). */
+typedef struct cbm_sizes {
+    uint32_t numGR, grX, grY;
+    uint32_t numGO, goX, goY;
+    uint32_t numMF;
+    uint32_t numSC, numBC, numPC, numNC, numIO;
+    uint32_t numZones;
+
+    uint32_t maxnumpGRfromMFtoGR;   /* slots per GR in P_GR_FROM_MF_TO_GR       */
+    uint32_t maxnumpGRfromGOtoGR;   /* slots per GR in P_GR_FROM_GO_TO_GR       */
+    uint32_t maxnumpGRfromGRtoGO;   /* slots per GR in P_GR_FROM_GR_TO_GO       */
+    uint32_t maxnumpMFfromMFtoGO;   /* slots per MF in P_MF_FROM_MF_TO_GO       */
+    uint32_t maxnumpGOGABAOutGOGO;  /* slots per GO in P_GO_GABA_OUT_GOGO       */
+    uint32_t maxnumpGOCoupInGOGO;   /* slots per GO in P_GO_COUP_IN_GOGO        */
+
+    uint32_t numpPCfromGRtoPC;      /* numGR/numPC: contiguous PF block per PC  */
+    uint32_t numpBCfromGRtoBC;      /* numGR/numBC                              */
+    uint32_t numpSCfromGRtoSC;      /* numGR/numSC                              */
+    uint32_t numpBCfromBCtoPC;      /* 4                                        */
+    uint32_t numpPCfromPCtoBC;      /* 16                                       */
+    uint32_t numpPCfromSCtoPC;      /* numSC/numPC                              */
+    uint32_t numpNCfromPCtoNC;      /* numPC/numNC*numpPCfromPCtoNC             */
+    uint32_t numpNCfromMFtoNC;      /* numMF/numNC                              */
+    uint32_t numpIOfromNCtoIO;      /* numNC (all to all)                       */
+    uint32_t numpIOInIOIO;          /* numIO-1                                  */
+    uint32_t numPopHistBinsPC;      /* tsPopHistPC/tsPerPopHistBinPC (8)        */
+} cbm_sizes;
+
+/* ---------------------------------------------------------------- params */
+/* Derived activity constants, exactly as ActivityParams::updateParams computes
+ * them (CBM_STATE_LIB_DEV/src/params/activityparams.cpp:436-636).  Computed
+ * once on the host and shared by every consumer. */
+typedef struct cbm_act_params {
+    float msPerTimeStep;
+    uint32_t numTSinMFHist;
+    /* golgi */
+    float eLeakGO, eGABAGO, threshMaxGO, threshRestGO;
+    float gIncMFtoGO, gIncGRtoGO, gGABAIncGOtoGO, coupleRiRjRatioGO;
+    float gMGluRScaleGRtoGO, gMGluRIncScaleGO, mGluRScaleGO, gluScaleGO, gLeakGO;
+    float goGABAGOGOSynRec, goGABAGOGOSynDepF;
+    float gDecMFtoGO, gDecGRtoGO, gGABADecGOtoGO;
+    float mGluRDecayGO, gMGluRIncDecayGO, gMGluRDecGRtoGO, gluDecayGO, threshDecGO;
+    /* granule */
+    float eLeakGR, eGOGR, threshMaxGR, threshRestGR, gIncMFtoGR, gIncGOtoGR;
+    float gDecMFtoGR, gDecGOtoGR, threshDecGR, gLeakGR;
+    uint32_t tsPerHistBinGR;
+    /* stellate */
+    float eLeakSC, gLeakSC, gDecGRtoSC, threshMaxSC, threshRestSC, threshDecSC, gIncGRtoSC;
+    /* basket */
+    float eLeakBC, ePCtoBC, gLeakBC, gDecGRtoBC, gDecPCtoBC, threshDecBC, threshRestBC,
+          threshMaxBC, gIncGRtoBC, gIncPCtoBC;
+    /* purkinje */
+    float initSynWofGRtoPC, eLeakPC, eBCtoPC, eSCtoPC, threshMaxPC, threshRestPC, threshDecPC,
+          gLeakPC, gDecGRtoPC, gDecBCtoPC, gDecSCtoPC, gIncSCtoPC, gIncGRtoPC, gIncBCtoPC;
+    uint32_t tsPopHistPC, tsPerPopHistBinPC, numPopHistBinsPC;
+    float synLTPStepSizeGRtoPC, synLTDStepSizeGRtoPC;
+    /* inferior olive */
+    float coupleRiRjRatioIO, eLeakIO, eNCtoIO, gLeakIO, gDecTSofNCtoIO, gDecTTofNCtoIO,
+          gDecT0ofNCtoIO, gIncNCtoIO, gIncTauNCtoIO, threshRestIO, threshMaxIO, threshDecIO;
+    uint32_t tsLTDDurationIO;
+    int32_t tsLTDStartAPIO, tsLTPStartAPIO;
+    uint32_t grPCHistCheckBinIO;
+    float maxExtIncVIO;
+    /* nucleus */
+    float eLeakNC, ePCtoNC, gmaxNMDADecMFtoNC, gmaxAMPADecMFtoNC, gNMDAIncMFtoNC, gAMPAIncMFtoNC,
+          gIncAvgPCtoNC, gDecPCtoNC, gLeakNC, threshDecNC, threshMaxNC, threshRestNC,
+          relPDecTSofNCtoIO, relPDecTTofNCtoIO, relPDecT0ofNCtoIO, relPIncNCtoIO, relPIncTauNCtoIO,
+          initSynWofMFtoNC, synLTDPCPopActThreshMFtoNC, synLTPPCPopActThreshMFtoNC,
+          synLTDStepSizeMFtoNC, synLTPStepSizeMFtoNC;
+} cbm_act_params;
+
+/* Options of this implementation (no reference counterpart). */
+typedef struct cbm_options {
+    /* Age, in steps, of the GR->GO input count that the Golgi update of step t
+     * reads: 1 = the count of apBuf after step t-1 ("fresh"), 2 = after step
+     * t-2 ("stale").  The reference reads a host buffer while an async copy
+     * into it is in flight (innet.cpp:1236-1237 vs innet.cpp:835-838), so both
+     * occur there; 2 is the default. */
+    uint32_t grgoLag;
+    /* HIP device ordinal to run on. */
+    int32_t device;
+    /* Granule cells per thread and threads per block of the GR kernel; 0 = pick. */
+    uint32_t grCellsPerThread, grBlockThreads;
+} cbm_options;
+
+/* ---------------------------------------------------------------- fields */
+/* Every array that crosses the boundary.  Shapes are given by
+ * cbm_field_shape() below.  Z = one array per microzone (zone argument). */
+enum cbm_field {
+    /* input-net connectivity (InNetConnectivityState, innetconnectivitystate.h:130-143) */
+    CBM_F_NUMP_GR_FROM_MF_TO_GR = 0, CBM_F_P_GR_FROM_MF_TO_GR,
+    CBM_F_NUMP_GR_FROM_GO_TO_GR, CBM_F_P_GR_FROM_GO_TO_GR,
+    CBM_F_NUMP_GR_FROM_GR_TO_GO, CBM_F_P_GR_FROM_GR_TO_GO,
+    CBM_F_P_GR_DELAYMASK_GR_TO_GO, CBM_F_P_GR_DELAYMASK_GR_TO_BSP,
+    CBM_F_NUMP_MF_FROM_MF_TO_GO, CBM_F_P_MF_FROM_MF_TO_GO,
+    CBM_F_NUMP_GO_GABA_OUT_GOGO, CBM_F_P_GO_GABA_OUT_GOGO,
+    CBM_F_NUMP_GO_COUP_IN_GOGO, CBM_F_P_GO_COUP_IN_GOGO,
+    /* microzone connectivity (Z) (MZoneConnectivityState) */
+    CBM_F_P_BC_FROM_BC_TO_PC, CBM_F_P_PC_FROM_PC_TO_BC, CBM_F_P_PC_FROM_SC_TO_PC,
+    CBM_F_P_NC_FROM_PC_TO_NC, CBM_F_P_NC_FROM_MF_TO_NC, CBM_F_P_IO_FROM_NC_TO_IO,
+    CBM_F_P_IO_IN_IOIO,
+    /* zone random generator (Z): seed, number of Random() draws discarded after
+     * seeding (MZone::testReduction draws numGR, mzone.cpp:846-849), and the
+     * full generator state (352 words + index) for snapshots */
+    CBM_F_ZONE_RNG_SEED, CBM_F_ZONE_RNG_SKIP, CBM_F_ZONE_RNG_STATE,
+    /* input-net activity (InNetActivityState, innetactivitystate.h:41-89) */
+    CBM_F_HIST_MF, CBM_F_APBUF_MF,
+    CBM_F_AP_GO, CBM_F_APBUF_GO, CBM_F_VM_GO, CBM_F_VCOUPLE_GO, CBM_F_THRESH_GO,
+    CBM_F_INPUT_MF_GO, CBM_F_INPUT_GO_GO, CBM_F_INPUT_GOGABA_SYNDEP_GO,
+    CBM_F_GO_GABA_OUT_SYNSCALE_GOGO,
+    CBM_F_G_MF_GO, CBM_F_G_GR_GO, CBM_F_G_GO_GO, CBM_F_G_MGLUR_GO, CBM_F_G_MGLURINC_GO,
+    CBM_F_MGLUR_GO, CBM_F_GLU_GO,
+    CBM_F_AP_GR, CBM_F_APBUF_GR, CBM_F_G_MF_SUM_GR, CBM_F_G_GO_SUM_GR, CBM_F_THRESH_GR,
+    CBM_F_VM_GR, CBM_F_GKCA_GR, CBM_F_HIST_GR,
+    CBM_F_AP_SC, CBM_F_APBUF_SC, CBM_F_G_PF_SC, CBM_F_THRESH_SC, CBM_F_VM_SC,
+    CBM_F_INPUT_SUM_PF_SC,
+    /* microzone activity (Z) (MZoneActivityState, mzoneactivitystate.h:47-100) */
+    CBM_F_AP_BC, CBM_F_APBUF_BC, CBM_F_INPUT_PC_BC, CBM_F_G_PF_BC, CBM_F_G_PC_BC,
+    CBM_F_THRESH_BC, CBM_F_VM_BC,
+    CBM_F_AP_PC, CBM_F_APBUF_PC, CBM_F_INPUT_BC_PC, CBM_F_INPUT_SC_PC, CBM_F_PFPC_SYNW,
+    CBM_F_INPUT_SUM_PF_PC, CBM_F_G_PF_PC, CBM_F_G_BC_PC, CBM_F_G_SC_PC, CBM_F_VM_PC,
+    CBM_F_THRESH_PC, CBM_F_HIST_PC_POPACT, CBM_F_PC_POPACT_CTRS,
+    CBM_F_AP_IO, CBM_F_APBUF_IO, CBM_F_INPUT_NC_IO, CBM_F_ERR_DRIVE, CBM_F_G_NC_IO,
+    CBM_F_THRESH_IO, CBM_F_VM_IO, CBM_F_VCOUPLE_IO, CBM_F_PFPC_PLAST_TIMER_IO,
+    CBM_F_MFNC_NOLTP_NOLTD,
+    CBM_F_AP_NC, CBM_F_APBUF_NC, CBM_F_INPUT_PC_NC, CBM_F_INPUT_MF_NC, CBM_F_G_PC_NC,
+    CBM_F_G_PC_SCALE_NC, CBM_F_MF_SYNW_NC, CBM_F_GMAX_NMDA_MF_NC, CBM_F_GMAX_AMPA_MF_NC,
+    CBM_F_G_MF_NMDA_NC, CBM_F_G_MF_AMPA_NC, CBM_F_THRESH_NC, CBM_F_VM_NC,
+    CBM_F_SYN_IO_PRELEASE_NC,
+    /* run-time values that live between steps but not in CBMState */
+    CBM_F_CUR_TIME,          /* u64: steps taken (CBMSimCore::curTime)                  */
+    CBM_F_MF_INPUT,          /* u8[numMF]: MF spikes used by the next step               */
+    CBM_F_GINC_GR_TO_GO,     /* f32: InNet::tempGIncGRtoGO                               */
+    CBM_F_GRPC_PLAST_STEPS,  /* f32[2] (Z): MZone::tempGRPCLTDStep, tempGRPCLTPStep      */
+    CBM_F_SUM_GR_INPUT_GO,   /* u32[numGO]: exportSumGRInputGO                           */
+    CBM_F_SUM_GO_INPUT_GO,   /* f32[numGO]: exportSumGOInputGO                           */
+    /* pipeline of population input sums; row 0 = sum over apBuf after the last
+     * step, row 1 = the step before (the two-step PF lag, cbmsimcore.cpp:139-310) */
+    CBM_F_PIPE_PFPC_SUM,     /* f32[2][numPC] (Z)                                        */
+    CBM_F_PIPE_PFBC_SUM,     /* u32[2][numBC]                                            */
+    CBM_F_PIPE_PFSC_SUM,     /* u32[2][numSC]                                            */
+    CBM_F_PIPE_GRGO_SUM,     /* u32[2][numGO]                                            */
+    CBM_F_COUNT
+};
+
+enum cbm_field_kind { CBM_KIND_CON = 0, CBM_KIND_STATE = 1, CBM_KIND_RUNTIME = 2 };
+
+/* Shape of a field: element size in bytes, rows x cols elements, whether it is
+ * per zone, and its kind.  Returns 0, or CBM_E_ARG for an unknown field. */
+static inline int cbm_field_shape(int f, const cbm_sizes *s, size_t *elem, size_t *rows,
+                                  size_t *cols, int *zoned, int *kind)
+{
+    size_t e = 4, r = 1, c = 1;
+    int z = 0, k = CBM_KIND_STATE;
+    switch (f) {
+    case CBM_F_NUMP_GR_FROM_MF_TO_GR: case CBM_F_NUMP_GR_FROM_GO_TO_GR:
+    case CBM_F_NUMP_GR_FROM_GR_TO_GO: case CBM_F_P_GR_DELAYMASK_GR_TO_BSP:
+        r = s->numGR; k = CBM_KIND_CON; break;
+    case CBM_F_P_GR_FROM_MF_TO_GR: r = s->numGR; c = s->maxnumpGRfromMFtoGR; k = CBM_KIND_CON; break;
+    case CBM_F_P_GR_FROM_GO_TO_GR: r = s->numGR; c = s->maxnumpGRfromGOtoGR; k = CBM_KIND_CON; break;
+    case CBM_F_P_GR_FROM_GR_TO_GO: case CBM_F_P_GR_DELAYMASK_GR_TO_GO:
+        r = s->numGR; c = s->maxnumpGRfromGRtoGO; k = CBM_KIND_CON; break;
+    case CBM_F_NUMP_MF_FROM_MF_TO_GO: r = s->numMF; k = CBM_KIND_CON; break;
+    case CBM_F_P_MF_FROM_MF_TO_GO: r = s->numMF; c = s->maxnumpMFfromMFtoGO; k = CBM_KIND_CON; break;
+    case CBM_F_NUMP_GO_GABA_OUT_GOGO: case CBM_F_NUMP_GO_COUP_IN_GOGO:
+        r = s->numGO; k = CBM_KIND_CON; break;
+    case CBM_F_P_GO_GABA_OUT_GOGO: r = s->numGO; c = s->maxnumpGOGABAOutGOGO; k = CBM_KIND_CON; break;
+    case CBM_F_P_GO_COUP_IN_GOGO: r = s->numGO; c = s->maxnumpGOCoupInGOGO; k = CBM_KIND_CON; break;
+    case CBM_F_P_BC_FROM_BC_TO_PC: r = s->numBC; c = s->numpBCfromBCtoPC; z = 1; k = CBM_KIND_CON; break;
+    case CBM_F_P_PC_FROM_PC_TO_BC: r = s->numPC; c = s->numpPCfromPCtoBC; z = 1; k = CBM_KIND_CON; break;
+    case CBM_F_P_PC_FROM_SC_TO_PC: r = s->numPC; c = s->numpPCfromSCtoPC; z = 1; k = CBM_KIND_CON; break;
+    case CBM_F_P_NC_FROM_PC_TO_NC: r = s->numNC; c = s->numpNCfromPCtoNC; z = 1; k = CBM_KIND_CON; break;
+    case CBM_F_P_NC_FROM_MF_TO_NC: r = s->numNC; c = s->numpNCfromMFtoNC; z = 1; k = CBM_KIND_CON; break;
+    case CBM_F_P_IO_FROM_NC_TO_IO: r = s->numIO; c = s->numpIOfromNCtoIO; z = 1; k = CBM_KIND_CON; break;
+    case CBM_F_P_IO_IN_IOIO: r = s->numIO; c = s->numpIOInIOIO; z = 1; k = CBM_KIND_CON; break;
+    case CBM_F_ZONE_RNG_SEED: z = 1; k = CBM_KIND_CON; break;
+    case CBM_F_ZONE_RNG_SKIP: e = 8; z = 1; k = CBM_KIND_CON; break;
+    case CBM_F_ZONE_RNG_STATE: r = 353; z = 1; k = CBM_KIND_RUNTIME; break;
+
+    case CBM_F_HIST_MF: e = 1; r = s->numMF; break;
+    case CBM_F_APBUF_MF: r = s->numMF; break;
+    case CBM_F_AP_GO: e = 1; r = s->numGO; break;
+    case CBM_F_APBUF_GO: case CBM_F_VM_GO: case CBM_F_VCOUPLE_GO: case CBM_F_THRESH_GO:
+    case CBM_F_INPUT_MF_GO: case CBM_F_INPUT_GO_GO: case CBM_F_INPUT_GOGABA_SYNDEP_GO:
+    case CBM_F_GO_GABA_OUT_SYNSCALE_GOGO: case CBM_F_G_MF_GO: case CBM_F_G_GR_GO:
+    case CBM_F_G_GO_GO: case CBM_F_G_MGLUR_GO: case CBM_F_G_MGLURINC_GO: case CBM_F_MGLUR_GO:
+    case CBM_F_GLU_GO:
+        r = s->numGO; break;
+    case CBM_F_AP_GR: e = 1; r = s->numGR; break;
+    case CBM_F_APBUF_GR: case CBM_F_G_MF_SUM_GR: case CBM_F_G_GO_SUM_GR: case CBM_F_THRESH_GR:
+    case CBM_F_VM_GR: case CBM_F_GKCA_GR:
+        r = s->numGR; break;
+    case CBM_F_HIST_GR: e = 8; r = s->numGR; break;
+    case CBM_F_AP_SC: e = 1; r = s->numSC; break;
+    case CBM_F_APBUF_SC: case CBM_F_G_PF_SC: case CBM_F_THRESH_SC: case CBM_F_VM_SC:
+    case CBM_F_INPUT_SUM_PF_SC:
+        r = s->numSC; break;
+
+    case CBM_F_AP_BC: e = 1; r = s->numBC; z = 1; break;
+    case CBM_F_APBUF_BC: case CBM_F_INPUT_PC_BC: case CBM_F_G_PF_BC: case CBM_F_G_PC_BC:
+    case CBM_F_THRESH_BC: case CBM_F_VM_BC:
+        r = s->numBC; z = 1; break;
+    case CBM_F_AP_PC: e = 1; r = s->numPC; z = 1; break;
+    case CBM_F_APBUF_PC: case CBM_F_INPUT_BC_PC: case CBM_F_INPUT_SUM_PF_PC: case CBM_F_G_PF_PC:
+    case CBM_F_G_BC_PC: case CBM_F_VM_PC: case CBM_F_THRESH_PC:
+        r = s->numPC; z = 1; break;
+    case CBM_F_INPUT_SC_PC: e = 1; r = s->numPC; c = s->numpPCfromSCtoPC; z = 1; break;
+    case CBM_F_G_SC_PC: r = s->numPC; c = s->numpPCfromSCtoPC; z = 1; break;
+    case CBM_F_PFPC_SYNW: r = s->numPC; c = s->numpPCfromGRtoPC; z = 1; break;
+    case CBM_F_HIST_PC_POPACT: r = s->numPopHistBinsPC; z = 1; break;
+    case CBM_F_PC_POPACT_CTRS: r = 3; z = 1; break;
+    case CBM_F_AP_IO: e = 1; r = s->numIO; z = 1; break;
+    case CBM_F_APBUF_IO: case CBM_F_THRESH_IO: case CBM_F_VM_IO: case CBM_F_VCOUPLE_IO:
+    case CBM_F_PFPC_PLAST_TIMER_IO:
+        r = s->numIO; z = 1; break;
+    case CBM_F_INPUT_NC_IO: e = 1; r = s->numIO; c = s->numpIOfromNCtoIO; z = 1; break;
+    case CBM_F_G_NC_IO: r = s->numIO; c = s->numpIOfromNCtoIO; z = 1; break;
+    case CBM_F_ERR_DRIVE: z = 1; break;
+    case CBM_F_MFNC_NOLTP_NOLTD: e = 1; r = 2; z = 1; break;
+    case CBM_F_AP_NC: e = 1; r = s->numNC; z = 1; break;
+    case CBM_F_APBUF_NC: case CBM_F_THRESH_NC: case CBM_F_VM_NC: case CBM_F_SYN_IO_PRELEASE_NC:
+        r = s->numNC; z = 1; break;
+    case CBM_F_INPUT_PC_NC: e = 1; r = s->numNC; c = s->numpNCfromPCtoNC; z = 1; break;
+    case CBM_F_G_PC_NC: case CBM_F_G_PC_SCALE_NC: r = s->numNC; c = s->numpNCfromPCtoNC; z = 1; break;
+    case CBM_F_INPUT_MF_NC: e = 1; r = s->numNC; c = s->numpNCfromMFtoNC; z = 1; break;
+    case CBM_F_MF_SYNW_NC: case CBM_F_GMAX_NMDA_MF_NC: case CBM_F_GMAX_AMPA_MF_NC:
+    case CBM_F_G_MF_NMDA_NC: case CBM_F_G_MF_AMPA_NC:
+        r = s->numNC; c = s->numpNCfromMFtoNC; z = 1; break;
+
+    case CBM_F_CUR_TIME: e = 8; k = CBM_KIND_RUNTIME; break;
+    case CBM_F_MF_INPUT: e = 1; r = s->numMF; k = CBM_KIND_RUNTIME; break;
+    case CBM_F_GINC_GR_TO_GO: k = CBM_KIND_RUNTIME; break;
+    case CBM_F_GRPC_PLAST_STEPS: r = 2; z = 1; k = CBM_KIND_RUNTIME; break;
+    case CBM_F_SUM_GR_INPUT_GO: case CBM_F_SUM_GO_INPUT_GO: r = s->numGO; k = CBM_KIND_RUNTIME; break;
+    case CBM_F_PIPE_PFPC_SUM: r = 2; c = s->numPC; z = 1; k = CBM_KIND_RUNTIME; break;
+    case CBM_F_PIPE_PFBC_SUM: r = 2; c = s->numBC; k = CBM_KIND_RUNTIME; break;
+    case CBM_F_PIPE_PFSC_SUM: r = 2; c = s->numSC; k = CBM_KIND_RUNTIME; break;
+    case CBM_F_PIPE_GRGO_SUM: r = 2; c = s->numGO; k = CBM_KIND_RUNTIME; break;
+    default: return CBM_E_ARG;
+    }
+    if (elem) *elem = e;
+    if (rows) *rows = r;
+    if (cols) *cols = c;
+    if (zoned) *zoned = z;
+    if (kind) *kind = k;
+    return CBM_OK;
+}
+
+/* 1 if the field holds IEEE binary32 values, 0 if integers. */
+static inline int cbm_field_is_float(int f)
+{
+    switch (f) {
+    case CBM_F_VM_GO: case CBM_F_VCOUPLE_GO: case CBM_F_THRESH_GO:
+    case CBM_F_INPUT_GOGABA_SYNDEP_GO: case CBM_F_GO_GABA_OUT_SYNSCALE_GOGO:
+    case CBM_F_G_MF_GO: case CBM_F_G_GR_GO: case CBM_F_G_GO_GO: case CBM_F_G_MGLUR_GO:
+    case CBM_F_G_MGLURINC_GO: case CBM_F_MGLUR_GO: case CBM_F_GLU_GO:
+    case CBM_F_G_MF_SUM_GR: case CBM_F_G_GO_SUM_GR: case CBM_F_THRESH_GR: case CBM_F_VM_GR:
+    case CBM_F_GKCA_GR: case CBM_F_G_PF_SC: case CBM_F_THRESH_SC: case CBM_F_VM_SC:
+    case CBM_F_G_PF_BC: case CBM_F_G_PC_BC: case CBM_F_THRESH_BC: case CBM_F_VM_BC:
+    case CBM_F_PFPC_SYNW: case CBM_F_INPUT_SUM_PF_PC: case CBM_F_G_PF_PC: case CBM_F_G_BC_PC:
+    case CBM_F_G_SC_PC: case CBM_F_VM_PC: case CBM_F_THRESH_PC: case CBM_F_ERR_DRIVE:
+    case CBM_F_G_NC_IO: case CBM_F_THRESH_IO: case CBM_F_VM_IO: case CBM_F_VCOUPLE_IO:
+    case CBM_F_G_PC_NC: case CBM_F_G_PC_SCALE_NC: case CBM_F_MF_SYNW_NC:
+    case CBM_F_GMAX_NMDA_MF_NC: case CBM_F_GMAX_AMPA_MF_NC: case CBM_F_G_MF_NMDA_NC:
+    case CBM_F_G_MF_AMPA_NC: case CBM_F_THRESH_NC: case CBM_F_VM_NC:
+    case CBM_F_SYN_IO_PRELEASE_NC: case CBM_F_GINC_GR_TO_GO: case CBM_F_GRPC_PLAST_STEPS:
+    case CBM_F_SUM_GO_INPUT_GO: case CBM_F_PIPE_PFPC_SUM:
+        return 1;
+    default:
+        return 0;
+    }
+}
+
+static inline size_t cbm_field_bytes(int f, const cbm_sizes *s)
+{
+    size_t e = 0, r = 0, c = 0;
+    if (cbm_field_shape(f, s, &e, &r, &c, 0, 0) != CBM_OK) return 0;
+    return e * r * c;
+}
+
+/* ---------------------------------------------------------------- handle */
+typedef struct cbm_handle cbm_handle;
+
+/* Version of the ABI the library was built with (CBM_ABI_VERSION). */
+int cbm_abi_version(void);
+
+/* Message for the last failing call on this thread ("" if none). */
+const char *cbm_last_error(void);
+
+/* Number of HIP devices visible (0 if none). */
+int cbm_device_count(void);
+
+/* Create a network of the given sizes and derived constants.  opts may be
+ * NULL (defaults).  Connectivity and initial state are then given with
+ * cbm_write() and the network is made runnable with cbm_commit(). */
+int cbm_create(const cbm_sizes *sizes, const cbm_act_params *params, const cbm_options *opts,
+               cbm_handle **out);
+int cbm_destroy(cbm_handle *h);
+
+/* Copy `bytes` bytes from host memory into a field (bytes must equal
+ * cbm_field_bytes).  Connectivity fields may only be written before
+ * cbm_commit(); state and run-time fields at any time. */
+int cbm_write(cbm_handle *h, int field, int zone, const void *src, size_t bytes);
+
+/* Validate connectivity, build the device layouts and upload everything. */
+int cbm_commit(cbm_handle *h);
+
+/* Synchronous copy of a field into host memory. */
+int cbm_read(cbm_handle *h, int field, int zone, void *dst, size_t bytes);
+
+/* updateMFInput: numMF bytes, nonzero = spike, used by the next step. */
+int cbm_set_mf(cbm_handle *h, const uint8_t *mf);
+/* updateErrDrive: errDrive of the zone = rel * maxExtIncVIO for the next step. */
+int cbm_set_err_drive(cbm_handle *h, int zone, float rel);
+/* setGRPCPlastSteps; cbm_reset_grpc_steps restores the parameter values. */
+int cbm_set_grpc_steps(cbm_handle *h, int zone, float ltd, float ltp);
+int cbm_reset_grpc_steps(cbm_handle *h, int zone);
+/* setGIncGRtoGO; cbm_reset_ginc_grgo restores gIncGRtoGO. */
+int cbm_set_ginc_grgo(cbm_handle *h, float inc);
+int cbm_reset_ginc_grgo(cbm_handle *h);
+
+/* n 1-ms steps with the MF input last set (calcActivity n times).  Returns
+ * after the work is queued; cbm_sync() waits for it. */
+int cbm_step(cbm_handle *h, int n);
+
+/* n steps where step k reads MF spikes from mf_dev + k*pitch (a DEVICE
+ * pointer to n rows of numMF bytes, e.g. a pre-generated spike train).
+ * err_drive_host, if not NULL, gives [n][numZones] relative error drives
+ * (0 = none) applied before each step. */
+int cbm_run_mf_train(cbm_handle *h, const uint8_t *mf_dev, size_t pitch, int n,
+                     const float *err_drive_host);
+
+/* Wait for all queued work of the handle. */
+int cbm_sync(cbm_handle *h);
+
+/* The HIP stream (hipStream_t) the handle launches on, as an opaque pointer. */
+void *cbm_stream(cbm_handle *h);
+
+/* Time the dominant (granule) kernel over the next `steps` steps with HIP
+ * events on the handle's stream: runs them with the MF train as in
+ * cbm_run_mf_train and reports the mean GR-kernel duration and the mean
+ * whole-step duration in microseconds. */
+int cbm_profile_steps(cbm_handle *h, const uint8_t *mf_dev, size_t pitch, int n,
+                      double *gr_kernel_us, double *step_us);
+
+/* cbm_field_shape / cbm_field_is_float as exported functions, for hosts that
+ * cannot use the inline versions (FFI bindings). */
+int cbm_field_info(int field, const cbm_sizes *s, size_t *elem, size_t *rows, size_t *cols,
+                   int *zoned, int *kind, int *is_float);
+
+/* Bytes of device memory held by the handle. */
+int cbm_device_bytes(cbm_handle *h, size_t *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CBM_HIP_H */

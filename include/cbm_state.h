@@ -1,0 +1,85 @@
+/*
+ * cbm_state.h -- C-ABI of the network-state library (CBMState side).
+ *
+ * Builds what CBMSimCore consumes: ConnectivityParams / ActivityParams parsed
+ * from the reference's key/value text files, connectivity, and initial
+ * activity state, all in the CBMState layout and with the field vocabulary of
+ * cbm_hip.h.  It replaces, for non-C++ hosts, the constructors of
+ *   CBMState                 CBM_STATE_LIB_DEV/src/interfaces/cbmstate.cpp:12-204
+ *   ConnectivityParams       CBM_STATE_LIB_DEV/src/params/connectivityparams.cpp:12-184
+ *   ActivityParams           CBM_STATE_LIB_DEV/src/params/activityparams.cpp:12-42,436-636
+ *   InNet/MZone activity     CBM_STATE_LIB_DEV/src/state/innetactivitystate.cpp:277-337,
+ *                            CBM_STATE_LIB_DEV/src/state/mzoneactivitystate.cpp:357-445
+ *   MZoneConnectivityState   CBM_STATE_LIB_DEV/src/state/mzoneconnectivitystate.cpp:190-417
+ * and adds the seeded synthetic input-net generator the benchmark networks use.
+ */
+#ifndef CBM_STATE_H
+#define CBM_STATE_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include "cbm_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct cbm_state cbm_state;
+
+/* Input-net generators. */
+enum cbm_gen_kind {
+    /* Each GR draws nMFperGR distinct MFs and nGOperGR distinct GOs uniformly,
+     * and projects to nGOoutPerGR distinct GOs (uniformly, or within a window
+     * of the GO grid when grgoWindowX > 0); each GO draws nMFperGO distinct MFs.
+     * Delay masks follow InNetConnectivityState::assignGRDelays
+     * (innetconnectivitystate.cpp:920-962); GO-GO wiring follows connectGOGO
+     * (innetconnectivitystate.cpp:852-918). */
+    CBM_GEN_UNIFORM = 0
+};
+
+typedef struct cbm_gen_spec {
+    int32_t kind;            /* cbm_gen_kind                                    */
+    uint32_t numZones;
+    int32_t innetSeed;       /* seed of the input-net generator                 */
+    uint32_t nMFperGR, nGOperGR, nGOoutPerGR, nMFperGO;
+    uint32_t grgoWindowX, grgoWindowY; /* 0 = uniform GR->GO targets; else a
+                                          window in GO-grid cells            */
+    int32_t mzoneConSeed[8]; /* MZoneConnectivityState seeds (PC->NC wiring)    */
+    int32_t mzoneRngSeed[8]; /* MZone generator seeds (NC->IO release draws)    */
+} cbm_gen_spec;
+
+/* Parse the text of an actParams file and derive the constants.  Unknown keys
+ * are kept; missing keys read as 0 (std::map semantics of the reference). */
+int cbm_act_params_from_text(const char *text, cbm_act_params *out);
+
+/* Build a complete network state from the texts of an actParams and a
+ * conParams file and a generator spec. */
+int cbm_state_create(const char *actText, const char *conText, const cbm_gen_spec *gen,
+                     cbm_state **out);
+int cbm_state_destroy(cbm_state *st);
+
+int cbm_state_sizes(const cbm_state *st, cbm_sizes *out);
+int cbm_state_params(const cbm_state *st, cbm_act_params *out);
+
+/* Pointer to the state's storage of a field (CBMState layout) and its size in
+ * bytes.  The pointer stays valid until cbm_state_destroy. */
+int cbm_state_field(cbm_state *st, int field, int zone, void **ptr, size_t *bytes);
+
+/* Hand every connectivity, state and generator-seed field of the state to a
+ * simulation handle (cbm_write for each) and commit it. */
+int cbm_state_load_into(cbm_state *st, cbm_handle *h);
+
+/* Outputs of the library's own SFMT generator (the CRandomSFMT0 contract the
+ * state and core code rely on), for checking against the reference's:
+ * kind 0 = BRandom() as uint32, 1 = Random() as double after `skip` Random()
+ * calls, 2 = IRandom(lo,hi) and 3 = IRandomX(lo,hi) as int32. */
+int cbm_sfmt_sample(int seed, uint64_t skip, int kind, int lo, int hi, int n, void *out);
+
+/* Message of the last failing call on this thread. */
+const char *cbm_state_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

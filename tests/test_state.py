@@ -1,0 +1,116 @@
+"""Network state built by libcbm_state: sizes (ConnectivityParams,
+connectivityparams.cpp:90-184), the seeded synthetic input net, delay masks
+(innetconnectivitystate.cpp:920-962), GO-GO wiring (:852-918), microzone
+wiring (mzoneconnectivitystate.cpp:213-417) and initial values
+(innetactivitystate.cpp:277-337, mzoneactivitystate.cpp:357-445)."""
+import numpy as np
+import pytest
+
+from cerebellum_simulation_amd import configs
+
+
+@pytest.fixture(scope="module")
+def c0():
+    return configs.build_state("c0", seed=0)
+
+
+@pytest.fixture(scope="module")
+def c1():
+    return configs.build_state("c1", seed=0)
+
+
+def test_sizes_c0_c1(c0, c1):
+    s = c0.sizes
+    assert (s.numGR, s.numGO, s.numMF, s.numPC, s.numBC, s.numSC, s.numIO, s.numNC) == \
+        (16384, 256, 256, 8, 32, 128, 1, 2)
+    assert s.numpPCfromGRtoPC == 2048 and s.numpBCfromGRtoBC == 512 and s.numpSCfromGRtoSC == 128
+    s = c1.sizes
+    assert (s.numGR, s.numGO, s.numMF, s.numPC, s.numBC, s.numSC, s.numIO, s.numNC) == \
+        (1048576, 1024, 1024, 32, 128, 512, 4, 8)
+    assert s.numpPCfromGRtoPC == 32768 and s.numpNCfromPCtoNC == 12 and s.numpNCfromMFtoNC == 128
+
+
+def test_gr_fan_in_out(c0):
+    s = c0.sizes
+    for nf, pf, k, n in [("NUMP_GR_FROM_MF_TO_GR", "P_GR_FROM_MF_TO_GR", 4, s.numMF),
+                         ("NUMP_GR_FROM_GO_TO_GR", "P_GR_FROM_GO_TO_GR", 3, s.numGO),
+                         ("NUMP_GR_FROM_GR_TO_GO", "P_GR_FROM_GR_TO_GO", 2, s.numGO)]:
+        cnt, p = c0.field(nf), c0.field(pf)
+        assert (cnt == k).all() and p.shape == (s.numGR, k)
+        assert p.max() < n
+        srt = np.sort(p, axis=1)
+        assert (np.diff(srt, axis=1) > 0).all(), "inputs of one GR are distinct"
+    # uniform draws: every MF is used about 4*numGR/numMF times
+    use = np.bincount(c0.field("P_GR_FROM_MF_TO_GR").ravel(), minlength=s.numMF)
+    assert abs(use.mean() - 256.0) < 1e-9 and use.std() < 3 * np.sqrt(256)
+
+
+def test_seed_determinism():
+    a = configs.build_state("t4k", seed=3).field("P_GR_FROM_MF_TO_GR").copy()
+    b = configs.build_state("t4k", seed=3).field("P_GR_FROM_MF_TO_GR").copy()
+    c = configs.build_state("t4k", seed=4).field("P_GR_FROM_MF_TO_GR").copy()
+    assert np.array_equal(a, b) and not np.array_equal(a, c)
+
+
+def test_delay_masks(c1):
+    s = c1.sizes
+    bsp = c1.field("P_GR_DELAYMASK_GR_TO_BSP")
+    assert (bsp & (bsp - 1) == 0).all() and (bsp > 0).all()
+    bit = np.log2(bsp).astype(int)
+    x = np.arange(s.numGR) % s.grX
+    # a GR at the centre column has distance 0: delay = grAFDelayInTStep = 1
+    assert (bit[x == s.grX // 2] == 1).all()
+    # delay grows with the distance from the centre, up to (grX/2)/vel + 1
+    dist = np.abs(s.grX // 2 - x)
+    order = np.argsort(dist, kind="stable")
+    assert (np.diff(bit[order]) >= 0).all()
+    assert bit.max() == (s.grX // 2) // 147 + 1
+    go = c1.field("P_GR_DELAYMASK_GR_TO_GO")
+    assert (go & (go - 1) == 0).all() and (go > 0).all()
+    # wrapped distance on the ring of GR columns is at most grX/2
+    assert np.log2(go).max() <= (s.grX // 2) // 147 + 1
+
+
+def test_gogo_wiring(c0):
+    s = c0.sizes
+    n, p = c0.field("NUMP_GO_GABA_OUT_GOGO"), c0.field("P_GO_GABA_OUT_GOGO")
+    assert (n == 4).all()
+    i = np.arange(s.numGO)
+    x, y = i % s.goX, i // s.goX
+    for j, (dx, dy) in enumerate([(0, -1), (-1, -1), (-1, 0), (-1, 1)]):
+        assert np.array_equal(p[:, j], ((y + dy) % s.goY) * s.goX + (x + dx) % s.goX)
+    assert (c0.field("NUMP_GO_COUP_IN_GOGO") == 0).all()  # coupling probability 0
+
+
+def test_microzone_wiring(c1):
+    s = c1.sizes
+    bcpc = c1.field("P_BC_FROM_BC_TO_PC", 0)
+    assert bcpc.shape == (128, 4) and list(bcpc[0]) == [1, 31, 2, 30] and list(bcpc[3]) == [4, 28, 9, 23]
+    pcbc = c1.field("P_PC_FROM_PC_TO_BC", 0)
+    assert list(pcbc[0]) == [(j - 6) % 128 for j in range(16)]
+    scpc = c1.field("P_PC_FROM_SC_TO_PC", 0)
+    assert np.array_equal(np.sort(scpc.ravel()), np.arange(s.numSC))
+    assert (scpc // (s.numSC // s.numPC) == np.arange(s.numPC)[:, None]).all()
+    ioio = c1.field("P_IO_IN_IOIO", 0)
+    assert all(sorted(ioio[i].tolist()) == [k for k in range(4) if k != i] for i in range(4))
+    assert np.array_equal(c1.field("P_IO_FROM_NC_TO_IO", 0), np.tile(np.arange(8), (4, 1)))
+    assert np.array_equal(c1.field("P_NC_FROM_MF_TO_NC", 0).ravel(), np.arange(s.numMF))
+    pcnc = c1.field("P_NC_FROM_PC_TO_NC", 0)
+    # regular block first, then every PC reaches exactly numpPCfromPCtoNC distinct NCs
+    assert np.array_equal(pcnc[:, :4], np.arange(32).reshape(8, 4))
+    assert (np.bincount(pcnc.ravel(), minlength=32) == 3).all()
+    # rows drawn with IRandomX hold distinct PCs; the last NC is filled with every
+    # PC still short of connections, which may repeat a PC (the reference's loop,
+    # mzoneconnectivitystate.cpp:337-357, does not check)
+    assert all(len(set(r)) == len(r) for r in pcnc[:-1].tolist())
+
+
+def test_initial_values(c0):
+    p = c0.params
+    assert (c0.field("VM_GR") == np.float32(p.eLeakGR)).all()
+    assert (c0.field("THRESH_GR") == np.float32(p.threshRestGR)).all()
+    assert (c0.field("GO_GABA_OUT_SYNSCALE_GOGO") == 1).all()
+    assert (c0.field("PFPC_SYNW", 0) == np.float32(0.5)).all()
+    assert (c0.field("MF_SYNW_NC", 0) == np.float32(p.initSynWofMFtoNC)).all()
+    assert (c0.field("G_PC_SCALE_NC", 0) == np.float32(p.gIncAvgPCtoNC)).all()
+    assert c0.field("ZONE_RNG_SKIP", 0)[0] == c0.sizes.numGR
