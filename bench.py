@@ -69,7 +69,20 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=400)
     ap.add_argument("--cpu-steps", type=int, default=0, help="0 = pick a ~15 s sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB, 5 times (the known-size kernel that scales "
+                         "the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        n = args.calibrate_copy * 2**20 // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes_read": n * 4, "calibrate_copy_bytes_written": n * 4}))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the network step runs "
               "on a single MI355X; see DESIGN.md for how it would shard)", file=sys.stderr)

@@ -96,9 +96,18 @@ def _load(name: str) -> C.CDLL:
 
 
 def hip_lib() -> C.CDLL:
-    """libcbm_hip.so with argument types declared."""
+    """libcbm_hip.so with argument types declared.
+
+    When PyTorch is installed it is imported first: its wheel carries its own
+    HIP runtime under the same soname as /opt/rocm's, and the process must use
+    one runtime for both (the one loaded first wins; torch only initialises
+    on its own)."""
     global _hip
     if _hip is None:
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = _load("libcbm_hip.so")
         P, I, S, V = C.c_void_p, C.c_int, C.c_size_t, C.c_void_p
         L.cbm_abi_version.restype = I

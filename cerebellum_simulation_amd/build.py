@@ -69,6 +69,18 @@ def build_core() -> str | None:
     return out
 
 
+def build_driver() -> str:
+    """The C++ driver test program (tests/cpp/cbm_driver.cpp), written against
+    the reference's CBMSimCore / CBMState API and linked to libcbm_core."""
+    out_dir = os.path.join(ROOT, "tests", "cpp", "build")
+    os.makedirs(out_dir, exist_ok=True)
+    out = os.path.join(out_dir, "cbm_driver")
+    _run([CXX, *CXX_FLAGS, f"-I{os.path.join(CSRC, 'core', 'include')}", "-o", out,
+          os.path.join(ROOT, "tests", "cpp", "cbm_driver.cpp"), f"-L{LIB}", "-lcbm_core", "-lcbm_state",
+          "-lcbm_hip", "-Wl,-rpath,$ORIGIN/../../../cerebellum_simulation_amd/lib"])
+    return out
+
+
 def build_oracle() -> None:
     """The CPU oracle (test infrastructure) and, where the reference tree is
     present, oracle/_ref."""
@@ -84,6 +96,7 @@ def build(all_parts: bool = True) -> None:
     build_state()
     build_core()
     if all_parts:
+        build_driver()
         build_oracle()
 
 

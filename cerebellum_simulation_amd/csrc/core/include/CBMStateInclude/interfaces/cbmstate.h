@@ -1,0 +1,42 @@
+// cbmstate.h -- CBMState with the constructors and accessors drivers use
+// (CBM_STATE_LIB_DEV/CBMStateInclude/interfaces/cbmstate.h:34-81), built on
+// the network-state C-ABI (include/cbm_state.h).
+#ifndef CBM_CBMSTATE_H
+#define CBM_CBMSTATE_H
+
+#include <fstream>
+#include <string>
+
+#include <CXXToolsInclude/stdDefinitions/pstdint.h>
+
+#include <cbm_state.h>
+
+class CBMState
+{
+public:
+    // Parameter files and zone count; seeds drawn from time(0) as in cbmstate.cpp:35-58.
+    CBMState(std::fstream &actPFile, std::fstream &conPFile, unsigned int nZones);
+    // Explicit seeds (cbmstate.cpp:60-64).  mzoneARSeed seeds each zone's
+    // generator for the NC->IO release draws.
+    CBMState(std::fstream &actPFile, std::fstream &conPFile, unsigned int nZones, int innetCRSeed,
+             int *mzoneCRSeed, int *mzoneARSeed);
+    virtual ~CBMState();
+
+    ct_uint32_t getNumZones();
+
+    // The underlying state handle and its arrays (CBMState layout).
+    cbm_state *handle() { return st; }
+    const cbm_sizes &sizes() const { return sz; }
+    const cbm_act_params &params() const { return ap; }
+    void *field(int f, int zone, size_t *bytes);
+
+private:
+    CBMState();
+    void build(const std::string &act, const std::string &con, unsigned int nZones, int innetCRSeed,
+               const int *mzoneCRSeed, const int *mzoneARSeed);
+    cbm_state *st;
+    cbm_sizes sz;
+    cbm_act_params ap;
+};
+
+#endif

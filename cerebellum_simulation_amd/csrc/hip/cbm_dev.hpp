@@ -14,7 +14,7 @@ constexpr int kMaxIO = 64;
 constexpr uint16_t kNoCell = 0xFFFFu;   // unused connectivity slot
 constexpr uint8_t kNever = 0xFFu;       // delay mask 0: never delivers
 constexpr int kRing = 3;                // input-sum ring slots (t, t-1, t-2)
-constexpr int kSmallThreads = 256;     // threads of the small-population kernel
+constexpr int kSmallThreads = 1024;     // threads of the small-population kernel
 
 // Granule-cell kernel arguments.  Per-GR arrays are structure-of-arrays of
 // numGR elements; connectivity is packed four 16-bit cell indices per GR

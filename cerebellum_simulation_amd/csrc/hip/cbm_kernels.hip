@@ -209,6 +209,7 @@ __device__ void sfmt_regen4(uint32_t *st, int k)
 {
     const uint32_t msk = k == 0 ? 0xeffff7fbu : k == 1 ? 0xffffffefu : k == 2 ? 0xdfdfbfffu : 0x7fffdbfdu;
     uint32_t r1 = st[4 * 86 + k], r2 = st[4 * 87 + k];
+#pragma unroll 1
     for (int i = 0; i < 88; i++) {
         int jj = i + 68;
         if (jj >= 88) jj -= 88;
@@ -248,15 +249,27 @@ __global__ void __launch_bounds__(kSmallThreads) small_step_kernel(const SmallAr
     __shared__ uint32_t s_rng[352];
     __shared__ int s_doLTD[kMaxZones], s_doLTP[kMaxZones];
 
-    // ======================= phase 1: independent cell updates
+    // LDS partials of the synapse-parallel sub-phase (offsets in floats)
+    extern __shared__ float s_syn[];
+    const uint32_t kSC = s.numpPCfromSCtoPC, kMF = s.numpNCfromMFtoNC, kPC = s.numpNCfromPCtoNC;
+    const uint32_t kNCIO = s.numpIOfromNCtoIO;
+    const uint32_t offPC = 0;
+    const uint32_t offN = offPC + nZ * s.numSC;
+    const uint32_t offA = offN + nZ * s.numNC * kMF;
+    const uint32_t offP = offA + nZ * s.numNC * kMF;
+    const uint32_t offIO = offP + nZ * s.numNC * kPC;
+
+    // ======================= phase 1a: MF history, PF->PC fold, synapses
     // InNet::updateMFActivties (innet.cpp:800-815)
     {
-        uint8_t *histMF = F<uint8_t>(a, CBM_F_HIST_MF);
-        uint32_t *apBufMF = F<uint32_t>(a, CBM_F_APBUF_MF);
+        uint8_t *__restrict__ histMF = F<uint8_t>(a, CBM_F_HIST_MF);
+        uint32_t *__restrict__ apBufMF = F<uint32_t>(a, CBM_F_APBUF_MF);
         for (uint32_t i = tid; i < s.numMF; i += nThr) {
             const uint8_t ap = st.mf[i] > 0;
-            histMF[i] = histMF[i] || ap;
-            apBufMF[i] = (apBufMF[i] << 1) | (ap * 0x00000001u);
+            const uint8_t h = histMF[i];
+            const uint32_t b = apBufMF[i];
+            histMF[i] = h || ap;
+            apBufMF[i] = (b << 1) | (ap * 0x00000001u);
         }
     }
     // second level of this step's PF->PC sums: one wave per (zone, PC)
@@ -264,190 +277,249 @@ __global__ void __launch_bounds__(kSmallThreads) small_step_kernel(const SmallAr
         const uint32_t nChunks = s.numGR >> 6, perPC = s.numpPCfromGRtoPC >> 6;
         for (uint32_t task = wave; task < nZ * s.numPC; task += nWave) {
             const uint32_t z = task / s.numPC, pc = task % s.numPC;
-            const float *c = a.chunk + (size_t)z * nChunks + (size_t)pc * perPC;
+            const float *__restrict__ c = a.chunk + (size_t)z * nChunks + (size_t)pc * perPC;
             float acc = 0.0f;
+#pragma unroll 8
             for (uint32_t q = lane; q < perPC; q += 64) acc = acc + c[q];
             acc = wave_sum_butterfly(acc);
             if (lane == 0) a.pfpcRing[((size_t)slotT * nZ + z) * s.numPC + pc] = acc;
         }
     }
+    // SC->PC synapses of calcPCActivities (mzone.cpp:201-207)
+    {
+        float *__restrict__ gSC = F<float>(a, CBM_F_G_SC_PC);
+        const uint8_t *__restrict__ inSC = F<uint8_t>(a, CBM_F_INPUT_SC_PC);
+        for (uint32_t q = tid; q < nZ * s.numSC; q += nThr) {
+            float g = gSC[q];
+            const uint8_t in = inSC[q];
+            g = g + p.gIncSCtoPC * (1 - g) * in;
+            g = g * p.gDecSCtoPC;
+            gSC[q] = g;
+            s_syn[offPC + q] = g;
+        }
+    }
+    // MF->NC and PC->NC synapses of calcNCActivities (mzone.cpp:358-383)
+    {
+        const uint8_t *__restrict__ inMF = F<uint8_t>(a, CBM_F_INPUT_MF_NC);
+        const float *__restrict__ wMF = F<float>(a, CBM_F_MF_SYNW_NC);
+        float *__restrict__ gmN = F<float>(a, CBM_F_GMAX_NMDA_MF_NC);
+        float *__restrict__ gmA = F<float>(a, CBM_F_GMAX_AMPA_MF_NC);
+        float *__restrict__ gN = F<float>(a, CBM_F_G_MF_NMDA_NC);
+        float *__restrict__ gA = F<float>(a, CBM_F_G_MF_AMPA_NC);
+        for (uint32_t q = tid; q < nZ * s.numNC * kMF; q += nThr) {
+            const uint8_t in = inMF[q];
+            const float w = wMF[q];
+            float mN = gmN[q], mA = gmA[q], n = gN[q], aa = gA[q];
+            mN = mN * p.gmaxNMDADecMFtoNC + in * w * (1 - mN);
+            n = n + p.gNMDAIncMFtoNC * (mN - n);
+            mA = mA * p.gmaxAMPADecMFtoNC + in * w * (1 - mA);
+            aa = aa + p.gAMPAIncMFtoNC * (mA - aa);
+            gmN[q] = mN;
+            gN[q] = n;
+            gmA[q] = mA;
+            gA[q] = aa;
+            s_syn[offN + q] = n;
+            s_syn[offA + q] = aa;
+        }
+        const uint8_t *__restrict__ inPC = F<uint8_t>(a, CBM_F_INPUT_PC_NC);
+        const float *__restrict__ scPC = F<float>(a, CBM_F_G_PC_SCALE_NC);
+        float *__restrict__ gPC = F<float>(a, CBM_F_G_PC_NC);
+        for (uint32_t q = tid; q < nZ * s.numNC * kPC; q += nThr) {
+            const uint8_t in = inPC[q];
+            const float sc = scPC[q];
+            float g = gPC[q];
+            g = g * p.gDecPCtoNC + in * sc * (1 - g);
+            gPC[q] = g;
+            s_syn[offP + q] = g;
+        }
+    }
+    // NC->IO synapses of calcIOActivities (mzone.cpp:272-280)
+    {
+        float *__restrict__ gNC = F<float>(a, CBM_F_G_NC_IO);
+        uint8_t *__restrict__ inNC = F<uint8_t>(a, CBM_F_INPUT_NC_IO);
+        const float ms = p.msPerTimeStep;
+        for (uint32_t q = tid; q < nZ * s.numIO * kNCIO; q += nThr) {
+            float g = gNC[q];
+            const uint8_t in = inNC[q];
+            g = g * expf(-ms / (-p.gDecTSofNCtoIO * expf(-g / p.gDecTTofNCtoIO) + p.gDecT0ofNCtoIO));
+            g = g + in * p.gIncNCtoIO * expf(-g / p.gIncTauNCtoIO);
+            gNC[q] = g;
+            inNC[q] = 0;
+            s_syn[offIO + q] = g;
+        }
+    }
+    __syncthreads();
+
+    // ======================= phase 1b: cell updates
     // MZone::calcPCActivities (mzone.cpp:187-226)
     for (uint32_t idx = tid; idx < nZ * s.numPC; idx += nThr) {
         const uint32_t z = idx / s.numPC, i = idx % s.numPC;
-        const size_t zi = (size_t)z * s.numPC + i;
+        float *__restrict__ gPFp = F<float>(a, CBM_F_G_PF_PC);
+        float *__restrict__ gBCp = F<float>(a, CBM_F_G_BC_PC);
+        float *__restrict__ vPC = F<float>(a, CBM_F_VM_PC);
+        float *__restrict__ thPC = F<float>(a, CBM_F_THRESH_PC);
+        uint32_t *__restrict__ apBuf = F<uint32_t>(a, CBM_F_APBUF_PC);
         const float pfSum = a.pfpcRing[((size_t)slotT2 * nZ + z) * s.numPC + i];
-        float *gPF = F<float>(a, CBM_F_G_PF_PC), *gBC = F<float>(a, CBM_F_G_BC_PC);
-        float *vPC = F<float>(a, CBM_F_VM_PC), *thPC = F<float>(a, CBM_F_THRESH_PC);
-        const uint32_t *inBC = F<uint32_t>(a, CBM_F_INPUT_BC_PC);
-        const uint32_t nSC = s.numpPCfromSCtoPC;
-        float *gSC = F<float>(a, CBM_F_G_SC_PC) + zi * nSC;
-        const uint8_t *inSC = F<uint8_t>(a, CBM_F_INPUT_SC_PC) + zi * nSC;
-        float gSCPCSum;
-        gPF[zi] = gPF[zi] + pfSum * p.gIncGRtoPC;
-        gPF[zi] = gPF[zi] * p.gDecGRtoPC;
-        gBC[zi] = gBC[zi] + inBC[zi] * p.gIncBCtoPC;
-        gBC[zi] = gBC[zi] * p.gDecBCtoPC;
-        gSCPCSum = 0;
-        for (uint32_t j = 0; j < nSC; j++) {
-            gSC[j] = gSC[j] + p.gIncSCtoPC * (1 - gSC[j]) * inSC[j];
-            gSC[j] = gSC[j] * p.gDecSCtoPC;
-            gSCPCSum += gSC[j];
-        }
-        float v = vPC[zi];
-        v = v + (p.gLeakPC * (p.eLeakPC - v)) - (gPF[zi] * v) + (gBC[zi] * (p.eBCtoPC - v)) +
+        const uint32_t inBC = F<uint32_t>(a, CBM_F_INPUT_BC_PC)[idx];
+        float gPF = gPFp[idx], gBC = gBCp[idx], v = vPC[idx], th = thPC[idx];
+        const uint32_t ab = apBuf[idx];
+        float gSCPCSum = 0;
+        for (uint32_t j = 0; j < kSC; j++) gSCPCSum += s_syn[offPC + idx * kSC + j];
+        gPF = gPF + pfSum * p.gIncGRtoPC;
+        gPF = gPF * p.gDecGRtoPC;
+        gBC = gBC + inBC * p.gIncBCtoPC;
+        gBC = gBC * p.gDecBCtoPC;
+        v = v + (p.gLeakPC * (p.eLeakPC - v)) - (gPF * v) + (gBC * (p.eBCtoPC - v)) +
             (gSCPCSum * (p.eSCtoPC - v));
-        vPC[zi] = v;
-        float th = thPC[zi];
         th = th + (p.threshDecPC * (p.threshRestPC - th));
         const uint8_t ap = v > th;
-        F<uint8_t>(a, CBM_F_AP_PC)[zi] = ap;
-        uint32_t *apBuf = F<uint32_t>(a, CBM_F_APBUF_PC);
-        apBuf[zi] = (apBuf[zi] << 1) | (ap * 0x00000001u);
-        thPC[zi] = ap * p.threshMaxPC + (!ap) * th;
+        gPFp[idx] = gPF;
+        gBCp[idx] = gBC;
+        vPC[idx] = v;
+        F<uint8_t>(a, CBM_F_AP_PC)[idx] = ap;
+        apBuf[idx] = (ab << 1) | (ap * 0x00000001u);
+        thPC[idx] = ap * p.threshMaxPC + (!ap) * th;
         if (ap) atomicAdd(&F<uint32_t>(a, CBM_F_PC_POPACT_CTRS)[z * 3 + 2], 1u);
     }
     // MZone::calcBCActivities (mzone.cpp:228-256)
     for (uint32_t idx = tid; idx < nZ * s.numBC; idx += nThr) {
         const uint32_t i = idx % s.numBC;
+        float *__restrict__ gPFp = F<float>(a, CBM_F_G_PF_BC);
+        float *__restrict__ gPCp = F<float>(a, CBM_F_G_PC_BC);
+        float *__restrict__ vBC = F<float>(a, CBM_F_VM_BC);
+        float *__restrict__ thBC = F<float>(a, CBM_F_THRESH_BC);
+        uint32_t *__restrict__ apBuf = F<uint32_t>(a, CBM_F_APBUF_BC);
         const uint32_t pf = a.bcRing[(size_t)slotT2 * s.numBC + i];
-        float *gPF = F<float>(a, CBM_F_G_PF_BC), *gPC = F<float>(a, CBM_F_G_PC_BC);
-        float *vBC = F<float>(a, CBM_F_VM_BC), *thBC = F<float>(a, CBM_F_THRESH_BC);
-        const uint32_t *inPC = F<uint32_t>(a, CBM_F_INPUT_PC_BC);
-        gPF[idx] = gPF[idx] + (pf * p.gIncGRtoBC);
-        gPF[idx] = gPF[idx] * p.gDecGRtoBC;
-        gPC[idx] = gPC[idx] + (inPC[idx] * p.gIncPCtoBC);
-        gPC[idx] = gPC[idx] * p.gDecPCtoBC;
-        float v = vBC[idx];
-        v = v + (p.gLeakBC * (p.eLeakBC - v)) - (gPF[idx] * v) + (gPC[idx] * (p.ePCtoBC - v));
-        vBC[idx] = v;
-        float th = thBC[idx];
+        const uint32_t inPC = F<uint32_t>(a, CBM_F_INPUT_PC_BC)[idx];
+        float gPF = gPFp[idx], gPC = gPCp[idx], v = vBC[idx], th = thBC[idx];
+        const uint32_t ab = apBuf[idx];
+        gPF = gPF + (pf * p.gIncGRtoBC);
+        gPF = gPF * p.gDecGRtoBC;
+        gPC = gPC + (inPC * p.gIncPCtoBC);
+        gPC = gPC * p.gDecPCtoBC;
+        v = v + (p.gLeakBC * (p.eLeakBC - v)) - (gPF * v) + (gPC * (p.ePCtoBC - v));
         th = th + p.threshDecBC * (p.threshRestBC - th);
         const uint8_t ap = v > th;
+        gPFp[idx] = gPF;
+        gPCp[idx] = gPC;
+        vBC[idx] = v;
         F<uint8_t>(a, CBM_F_AP_BC)[idx] = ap;
-        uint32_t *apBuf = F<uint32_t>(a, CBM_F_APBUF_BC);
-        apBuf[idx] = (apBuf[idx] << 1) | (ap * 0x00000001u);
+        apBuf[idx] = (ab << 1) | (ap * 0x00000001u);
         thBC[idx] = ap * p.threshMaxBC + (!ap) * (th);
     }
     // InNet::calcSCActivities (innet.cpp:892-911)
     for (uint32_t i = tid; i < s.numSC; i += nThr) {
+        float *__restrict__ gPFp = F<float>(a, CBM_F_G_PF_SC);
+        float *__restrict__ vSC = F<float>(a, CBM_F_VM_SC);
+        float *__restrict__ thSC = F<float>(a, CBM_F_THRESH_SC);
+        uint32_t *__restrict__ apBuf = F<uint32_t>(a, CBM_F_APBUF_SC);
         const uint32_t pf = a.scRing[(size_t)slotT2 * s.numSC + i];
-        float *gPF = F<float>(a, CBM_F_G_PF_SC), *vSC = F<float>(a, CBM_F_VM_SC),
-              *thSC = F<float>(a, CBM_F_THRESH_SC);
-        gPF[i] = gPF[i] + (pf * p.gIncGRtoSC);
-        gPF[i] = gPF[i] * p.gDecGRtoSC;
-        float v = vSC[i];
-        v = v + (p.gLeakSC * (p.eLeakSC - v)) - gPF[i] * v;
-        vSC[i] = v;
-        const uint8_t ap = v > thSC[i];
-        F<uint8_t>(a, CBM_F_AP_SC)[i] = ap;
-        uint32_t *apBuf = F<uint32_t>(a, CBM_F_APBUF_SC);
-        apBuf[i] = (apBuf[i] << 1) | (ap * 0x00000001u);
-        float th = thSC[i];
+        float gPF = gPFp[i], v = vSC[i], th = thSC[i];
+        const uint32_t ab = apBuf[i];
+        gPF = gPF + (pf * p.gIncGRtoSC);
+        gPF = gPF * p.gDecGRtoSC;
+        v = v + (p.gLeakSC * (p.eLeakSC - v)) - gPF * v;
+        const uint8_t ap = v > th;
         th = th + p.threshDecSC * (p.threshRestSC - th);
+        gPFp[i] = gPF;
+        vSC[i] = v;
+        F<uint8_t>(a, CBM_F_AP_SC)[i] = ap;
+        apBuf[i] = (ab << 1) | (ap * 0x00000001u);
         thSC[i] = ap * p.threshMaxSC + (!ap) * (th);
     }
     // InNet::calcGOActivities (innet.cpp:827-876)
     for (uint32_t i = tid; i < s.numGO; i += nThr) {
+        uint32_t *__restrict__ inMFp = F<uint32_t>(a, CBM_F_INPUT_MF_GO);
+        uint32_t *__restrict__ inGOp = F<uint32_t>(a, CBM_F_INPUT_GO_GO);
+        float *__restrict__ inDepp = F<float>(a, CBM_F_INPUT_GOGABA_SYNDEP_GO);
+        float *__restrict__ gMFp = F<float>(a, CBM_F_G_MF_GO);
+        float *__restrict__ gGRp = F<float>(a, CBM_F_G_GR_GO);
+        float *__restrict__ gGOp = F<float>(a, CBM_F_G_GO_GO);
+        float *__restrict__ glup = F<float>(a, CBM_F_GLU_GO);
+        float *__restrict__ mGp = F<float>(a, CBM_F_MGLUR_GO);
+        float *__restrict__ gIncp = F<float>(a, CBM_F_G_MGLURINC_GO);
+        float *__restrict__ gMGp = F<float>(a, CBM_F_G_MGLUR_GO);
+        float *__restrict__ thp = F<float>(a, CBM_F_THRESH_GO);
+        float *__restrict__ vGOp = F<float>(a, CBM_F_VM_GO);
+        const float *__restrict__ vCp = F<float>(a, CBM_F_VCOUPLE_GO);
+        uint32_t *__restrict__ apBuf = F<uint32_t>(a, CBM_F_APBUF_GO);
         const uint32_t totalGRIn = a.goRing[(size_t)slotG * s.numGO + i];
-        uint32_t *inMF = F<uint32_t>(a, CBM_F_INPUT_MF_GO), *inGO = F<uint32_t>(a, CBM_F_INPUT_GO_GO);
-        float *inDep = F<float>(a, CBM_F_INPUT_GOGABA_SYNDEP_GO);
-        float *gMF = F<float>(a, CBM_F_G_MF_GO), *gGR = F<float>(a, CBM_F_G_GR_GO),
-              *gGO = F<float>(a, CBM_F_G_GO_GO);
-        float *glu = F<float>(a, CBM_F_GLU_GO), *mG = F<float>(a, CBM_F_MGLUR_GO),
-              *gInc = F<float>(a, CBM_F_G_MGLURINC_GO), *gMG = F<float>(a, CBM_F_G_MGLUR_GO);
-        float *th = F<float>(a, CBM_F_THRESH_GO), *vGO = F<float>(a, CBM_F_VM_GO),
-              *vC = F<float>(a, CBM_F_VCOUPLE_GO);
-        F<uint32_t>(a, CBM_F_SUM_GR_INPUT_GO)[i] = totalGRIn;
-        F<float>(a, CBM_F_SUM_GO_INPUT_GO)[i] = inDep[i];
-        gMF[i] = inMF[i] * p.gIncMFtoGO + gMF[i] * p.gDecMFtoGO;
-        gGR[i] = totalGRIn * st.gIncGRtoGO + gGR[i] * p.gDecGRtoGO;
-        gGO[i] = inDep[i] * p.gGABAIncGOtoGO + gGO[i] * p.gGABADecGOtoGO;
-        glu[i] = glu[i] * p.gluDecayGO + totalGRIn * p.gluScaleGO * exp(-1.5 * glu[i]);
-        mG[i] = mG[i] * p.mGluRDecayGO + glu[i] * p.mGluRScaleGO * expf(-mG[i]);
-        gInc[i] = gInc[i] * p.gMGluRIncDecayGO + mG[i] * p.gMGluRIncScaleGO * expf(-gInc[i]);
-        gMG[i] = gMG[i] * p.gMGluRDecGRtoGO + gInc[i] * p.gMGluRScaleGRtoGO;
-        float thv = th[i];
+        const uint32_t inMF = inMFp[i];
+        const float inDep = inDepp[i];
+        float gMF = gMFp[i], gGR = gGRp[i], gGO = gGOp[i], glu = glup[i], mG = mGp[i], gInc = gIncp[i],
+              gMG = gMGp[i], thv = thp[i], v = vGOp[i];
+        const float vC = vCp[i];
+        const uint32_t ab = apBuf[i];
+        gMF = inMF * p.gIncMFtoGO + gMF * p.gDecMFtoGO;
+        gGR = totalGRIn * st.gIncGRtoGO + gGR * p.gDecGRtoGO;
+        gGO = inDep * p.gGABAIncGOtoGO + gGO * p.gGABADecGOtoGO;
+        glu = glu * p.gluDecayGO + totalGRIn * p.gluScaleGO * exp(-1.5 * glu);
+        mG = mG * p.mGluRDecayGO + glu * p.mGluRScaleGO * expf(-mG);
+        gInc = gInc * p.gMGluRIncDecayGO + mG * p.gMGluRIncScaleGO * expf(-gInc);
+        gMG = gMG * p.gMGluRDecGRtoGO + gInc * p.gMGluRScaleGRtoGO;
         thv = thv + (p.threshRestGO - thv) * p.threshDecGO;
-        float v = vGO[i];
-        v = v + (p.gLeakGO * (p.eLeakGO - v)) + (gGO[i] * (p.eGABAGO - v)) - (gMF[i] + gGR[i]) * v + vC[i];
-        vGO[i] = v;
+        v = v + (p.gLeakGO * (p.eLeakGO - v)) + (gGO * (p.eGABAGO - v)) - (gMF + gGR) * v + vC;
         const uint8_t ap = v > thv;
+        F<uint32_t>(a, CBM_F_SUM_GR_INPUT_GO)[i] = totalGRIn;
+        F<float>(a, CBM_F_SUM_GO_INPUT_GO)[i] = inDep;
+        gMFp[i] = gMF;
+        gGRp[i] = gGR;
+        gGOp[i] = gGO;
+        glup[i] = glu;
+        mGp[i] = mG;
+        gIncp[i] = gInc;
+        gMGp[i] = gMG;
+        vGOp[i] = v;
         F<uint8_t>(a, CBM_F_AP_GO)[i] = ap;
-        uint32_t *apBuf = F<uint32_t>(a, CBM_F_APBUF_GO);
-        apBuf[i] = (apBuf[i] << 1) | (ap * 0x00000001u);
-        th[i] = ap * p.threshMaxGO + (!ap) * thv;
-        inMF[i] = 0;
-        inGO[i] = 0;
-        inDep[i] = 0;
+        apBuf[i] = (ab << 1) | (ap * 0x00000001u);
+        thp[i] = ap * p.threshMaxGO + (!ap) * thv;
+        inMFp[i] = 0;
+        inGOp[i] = 0;
+        inDepp[i] = 0;
     }
-    // MZone::calcIOActivities (mzone.cpp:258-332)
+    // MZone::calcIOActivities (mzone.cpp:282-334)
     for (uint32_t idx = tid; idx < nZ * s.numIO; idx += nThr) {
         const uint32_t z = idx / s.numIO;
-        const uint32_t k = s.numpIOfromNCtoIO;
-        float *gNC = F<float>(a, CBM_F_G_NC_IO) + (size_t)idx * k;
-        uint8_t *inNC = F<uint8_t>(a, CBM_F_INPUT_NC_IO) + (size_t)idx * k;
-        float *vIO = F<float>(a, CBM_F_VM_IO), *thIO = F<float>(a, CBM_F_THRESH_IO);
-        const float *vCIO = F<float>(a, CBM_F_VCOUPLE_IO);
-        const float ms = p.msPerTimeStep;
+        float *__restrict__ vIO = F<float>(a, CBM_F_VM_IO);
+        float *__restrict__ thIO = F<float>(a, CBM_F_THRESH_IO);
+        uint32_t *__restrict__ apBuf = F<uint32_t>(a, CBM_F_APBUF_IO);
+        float v = vIO[idx], th = thIO[idx];
+        const float vC = F<float>(a, CBM_F_VCOUPLE_IO)[idx];
+        const uint32_t ab = apBuf[idx];
         float gNCSum = 0;
-        for (uint32_t j = 0; j < k; j++) {
-            float g = gNC[j];
-            g = g * expf(-ms / (-p.gDecTSofNCtoIO * expf(-g / p.gDecTTofNCtoIO) + p.gDecT0ofNCtoIO));
-            g = g + inNC[j] * p.gIncNCtoIO * expf(-g / p.gIncTauNCtoIO);
-            gNC[j] = g;
-            gNCSum = gNCSum + g;
-            inNC[j] = 0;
-        }
+        for (uint32_t j = 0; j < kNCIO; j++) gNCSum = gNCSum + s_syn[offIO + idx * kNCIO + j];
         gNCSum = gNCSum / 3.1;
-        float v = vIO[idx];
-        v = v + p.gLeakIO * (p.eLeakIO - v) + gNCSum * 1.5 * (p.eNCtoIO - v) + vCIO[idx] + st.errDrive[z];
+        v = v + p.gLeakIO * (p.eLeakIO - v) + gNCSum * 1.5 * (p.eNCtoIO - v) + vC + st.errDrive[z];
+        const uint8_t ap = v > th;
         vIO[idx] = v;
-        const uint8_t ap = v > thIO[idx];
         F<uint8_t>(a, CBM_F_AP_IO)[idx] = ap;
-        uint32_t *apBuf = F<uint32_t>(a, CBM_F_APBUF_IO);
-        apBuf[idx] = (apBuf[idx] << 1) | (ap * 0x00000001u);
-        thIO[idx] = p.threshMaxIO * ap + (!ap) * (thIO[idx] + p.threshDecIO * (p.threshRestIO - thIO[idx]));
+        apBuf[idx] = (ab << 1) | (ap * 0x00000001u);
+        thIO[idx] = p.threshMaxIO * ap + (!ap) * (th + p.threshDecIO * (p.threshRestIO - th));
     }
-    // MZone::calcNCActivities (mzone.cpp:341-397)
+    // MZone::calcNCActivities (mzone.cpp:371-396)
     for (uint32_t idx = tid; idx < nZ * s.numNC; idx += nThr) {
-        const uint32_t kMF = s.numpNCfromMFtoNC, kPC = s.numpNCfromPCtoNC;
-        const uint8_t *inMF = F<uint8_t>(a, CBM_F_INPUT_MF_NC) + (size_t)idx * kMF;
-        const float *wMF = F<float>(a, CBM_F_MF_SYNW_NC) + (size_t)idx * kMF;
-        float *gmN = F<float>(a, CBM_F_GMAX_NMDA_MF_NC) + (size_t)idx * kMF;
-        float *gmA = F<float>(a, CBM_F_GMAX_AMPA_MF_NC) + (size_t)idx * kMF;
-        float *gN = F<float>(a, CBM_F_G_MF_NMDA_NC) + (size_t)idx * kMF;
-        float *gA = F<float>(a, CBM_F_G_MF_AMPA_NC) + (size_t)idx * kMF;
-        const uint8_t *inPC = F<uint8_t>(a, CBM_F_INPUT_PC_NC) + (size_t)idx * kPC;
-        const float *scPC = F<float>(a, CBM_F_G_PC_SCALE_NC) + (size_t)idx * kPC;
-        float *gPC = F<float>(a, CBM_F_G_PC_NC) + (size_t)idx * kPC;
-        float *vNC = F<float>(a, CBM_F_VM_NC), *thNC = F<float>(a, CBM_F_THRESH_NC);
+        float *__restrict__ vNC = F<float>(a, CBM_F_VM_NC);
+        float *__restrict__ thNC = F<float>(a, CBM_F_THRESH_NC);
+        uint32_t *__restrict__ apBuf = F<uint32_t>(a, CBM_F_APBUF_NC);
+        float v = vNC[idx], th = thNC[idx];
+        const uint32_t ab = apBuf[idx];
         const float ms = p.msPerTimeStep;
         float gMFNMDASum = 0, gMFAMPASum = 0, gPCNCSum = 0;
         for (uint32_t j = 0; j < kMF; j++) {
-            gmN[j] = gmN[j] * p.gmaxNMDADecMFtoNC + inMF[j] * wMF[j] * (1 - gmN[j]);
-            gN[j] = gN[j] + p.gNMDAIncMFtoNC * (gmN[j] - gN[j]);
-            gMFNMDASum = gMFNMDASum + gN[j];
-            gmA[j] = gmA[j] * p.gmaxAMPADecMFtoNC + inMF[j] * wMF[j] * (1 - gmA[j]);
-            gA[j] = gA[j] + p.gAMPAIncMFtoNC * (gmA[j] - gA[j]);
-            gMFAMPASum = gMFAMPASum + gA[j];
+            gMFNMDASum = gMFNMDASum + s_syn[offN + idx * kMF + j];
+            gMFAMPASum = gMFAMPASum + s_syn[offA + idx * kMF + j];
         }
         gMFNMDASum = gMFNMDASum * ms / ((float)kMF);
         gMFAMPASum = gMFAMPASum * ms / ((float)kMF);
-        float v = vNC[idx];
         gMFNMDASum = gMFNMDASum * v / (-80.0f);
-        for (uint32_t j = 0; j < kPC; j++) {
-            gPC[j] = gPC[j] * p.gDecPCtoNC + inPC[j] * scPC[j] * (1 - gPC[j]);
-            gPCNCSum = gPCNCSum + gPC[j];
-        }
+        for (uint32_t j = 0; j < kPC; j++) gPCNCSum = gPCNCSum + s_syn[offP + idx * kPC + j];
         gPCNCSum = gPCNCSum * ms / ((float)kPC);
         v = v + p.gLeakNC * (p.eLeakNC - v) - (gMFNMDASum + gMFAMPASum) * v + gPCNCSum * (p.ePCtoNC - v);
-        vNC[idx] = v;
-        float th = thNC[idx];
         th = th + p.threshDecNC * (p.threshRestNC - th);
         const uint8_t ap = v > th;
+        vNC[idx] = v;
         F<uint8_t>(a, CBM_F_AP_NC)[idx] = ap;
-        uint32_t *apBuf = F<uint32_t>(a, CBM_F_APBUF_NC);
-        apBuf[idx] = (apBuf[idx] << 1) | (ap * 0x00000001u);
+        apBuf[idx] = (ab << 1) | (ap * 0x00000001u);
         thNC[idx] = ap * p.threshMaxNC + (!ap) * th;
     }
     __syncthreads();
@@ -609,30 +681,34 @@ __global__ void __launch_bounds__(kSmallThreads) small_step_kernel(const SmallAr
     }
     // MZone::updateNCOut draws (mzone.cpp:505-511): one Random() per (IO, NC) in
     // row order from the zone's stream
-    if (wave == 0 && lane < 4) {
+    if (wave == 0) {
         for (uint32_t z = 0; z < nZ; z++) {
             uint32_t *g = a.rngState + (size_t)z * 353;
-            for (int i = 0; i < 88; i++) s_rng[4 * i + lane] = g[4 * i + lane];
+            for (int i = lane; i < 352; i += 64) s_rng[i] = g[i];
             uint32_t ix = g[352];
             __threadfence_block();
-            const float *rel = F<float>(a, CBM_F_SYN_IO_PRELEASE_NC) + (size_t)z * s.numNC;
-            const uint32_t *pNC = F<uint32_t>(a, CBM_F_P_IO_FROM_NC_TO_IO) + (size_t)z * s.numIO * s.numNC;
-            uint8_t *inNC = F<uint8_t>(a, CBM_F_INPUT_NC_IO) + (size_t)z * s.numIO * s.numNC;
-            const uint32_t n = s.numIO * s.numpIOfromNCtoIO;
-            for (uint32_t q = 0; q < n; q++) {
-                if (ix >= 351) {
-                    __threadfence_block();
-                    sfmt_regen4(s_rng, lane);
-                    __threadfence_block();
-                    ix = 0;
+            if (lane < 4) {
+                const float *rel = F<float>(a, CBM_F_SYN_IO_PRELEASE_NC) + (size_t)z * s.numNC;
+                const uint32_t *pNC = F<uint32_t>(a, CBM_F_P_IO_FROM_NC_TO_IO) + (size_t)z * s.numIO * s.numNC;
+                uint8_t *inNC = F<uint8_t>(a, CBM_F_INPUT_NC_IO) + (size_t)z * s.numIO * s.numNC;
+                const uint32_t n = s.numIO * s.numpIOfromNCtoIO;
+#pragma unroll 1
+                for (uint32_t q = 0; q < n; q++) {
+                    if (ix >= 351) {
+                        __threadfence_block();
+                        sfmt_regen4(s_rng, lane);
+                        __threadfence_block();
+                        ix = 0;
+                    }
+                    const uint64_t r = (uint64_t)s_rng[ix] | ((uint64_t)s_rng[ix + 1] << 32);
+                    ix += 2;
+                    const double u = (double)(int64_t)(r >> 12) * (1.0 / 4503599627370496.0);
+                    if (lane == 0) inNC[q] = (u < rel[pNC[q]]);
                 }
-                const uint64_t r = (uint64_t)s_rng[ix] | ((uint64_t)s_rng[ix + 1] << 32);
-                ix += 2;
-                const double u = (double)(int64_t)(r >> 12) * (1.0 / 4503599627370496.0);
-                if (lane == 0) inNC[q] = (u < rel[pNC[q]]);
             }
+            ix = __shfl(ix, 0, 64);
             __threadfence_block();
-            for (int i = 0; i < 88; i++) g[4 * i + lane] = s_rng[4 * i + lane];
+            for (int i = lane; i < 352; i += 64) g[i] = s_rng[i];
             if (lane == 0) g[352] = ix;
             __threadfence_block();
         }

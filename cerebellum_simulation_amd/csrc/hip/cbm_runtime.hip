@@ -86,6 +86,7 @@ struct cbm_handle {
     Csr mfToGoIn{}, gabaIn{}, bcToPcIn{}, pcToBcIn{};
     uint32_t anyCoupling = 0;
     SmallArgs *smallArgs = nullptr;  // device copy of the constant small-kernel arguments
+    size_t smallLds = 0;             // dynamic LDS of the small kernel (synapse partials)
 
     uint64_t curTime = 0;
     float errDrive[kMaxZones] = {};
@@ -539,6 +540,15 @@ int cbm_commit(cbm_handle *h)
         a.p = h->p;
         a.grgoLag = h->o.grgoLag;
         a.anyCoupling = h->anyCoupling;
+        // synapse partials: SC->PC, MF->NC (NMDA, AMPA), PC->NC, NC->IO
+        h->smallLds = 4 * (size_t)nZ *
+                      (s.numSC + 2 * (size_t)s.numNC * s.numpNCfromMFtoNC + (size_t)s.numNC * s.numpNCfromPCtoNC +
+                       (size_t)s.numIO * s.numpIOfromNCtoIO);
+        if (h->smallLds > 150 * 1024)
+            return fail(CBM_E_UNSUPPORTED, "small-population synapse partials exceed the LDS of one workgroup");
+        if (h->smallLds > 64 * 1024)
+            HIP_TRY(hipFuncSetAttribute((const void *)small_step_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int)h->smallLds));
         if ((rc = h->dalloc((void **)&h->smallArgs, sizeof(SmallArgs)))) return rc;
         HIP_TRY(hipMemcpy(h->smallArgs, &a, sizeof(SmallArgs), hipMemcpyHostToDevice));
     }
@@ -735,7 +745,7 @@ static int launchStep(cbm_handle *h, const uint8_t *mf, bool timeGR)
     st.t = t;
     for (int z = 0; z < kMaxZones; z++) st.errDrive[z] = h->errDrive[z];
     st.gIncGRtoGO = h->gIncGRtoGO;
-    hipLaunchKernelGGL(small_step_kernel, dim3(1), dim3(kSmallThreads), 0, h->stream, h->smallArgs, st);
+    hipLaunchKernelGGL(small_step_kernel, dim3(1), dim3(kSmallThreads), h->smallLds, h->stream, h->smallArgs, st);
     HIP_TRY(hipGetLastError());
     // MZone::calcIOActivities clears errDrive after use (mzone.cpp:334)
     for (int z = 0; z < kMaxZones; z++) h->errDrive[z] = 0.0f;

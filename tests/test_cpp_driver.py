@@ -1,0 +1,69 @@
+"""A C++ driver written against the reference's CBMSimCore / CBMState API
+(tests/cpp/cbm_driver.cpp) links against libcbm_core and, on the GPU, gives
+the same exports as the Python mirror of the same network."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from cerebellum_simulation_amd import Network, NetState, configs, hip_lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "tests", "cpp", "build", "cbm_driver")
+CON = configs.conparams_text(**configs.CONFIGS["t8k2z"][0])
+
+
+def _files(tmp_path):
+    a, c = tmp_path / "act.txt", tmp_path / "con.txt"
+    a.write_text(configs.act_params_text())
+    c.write_text(CON)
+    return str(a), str(c)
+
+
+def test_driver_is_built_and_linked():
+    assert os.path.exists(DRIVER), "build the driver with __graft_entry__.build()"
+    out = subprocess.run(["ldd", DRIVER], capture_output=True, text=True).stdout
+    assert "libcbm_core.so" in out and "not found" not in out
+
+
+def test_driver_without_gpu_reports_error(tmp_path):
+    if hip_lib().cbm_device_count() > 0:
+        pytest.skip("a GPU is visible")
+    a, c = _files(tmp_path)
+    r = subprocess.run([DRIVER, a, c, "5", "3", str(tmp_path / "o.bin")], capture_output=True, text=True)
+    assert r.returncode == 1 and "no HIP device" in r.stderr
+
+
+@pytest.mark.gpu
+def test_driver_matches_python_mirror(gpu, tmp_path):
+    print("[case] C++ driver vs Python mirror", flush=True)
+    a, c = _files(tmp_path)
+    steps, seed = 60, 3
+    out = tmp_path / "o.bin"
+    r = subprocess.run([DRIVER, a, c, str(steps), str(seed), str(out)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    st = NetState(configs.act_params_text(), CON, num_zones=1, innet_seed=seed, n_mf_per_go=16)
+    net = Network.from_state(st)
+    s = st.sizes
+    idx = np.arange(s.numMF, dtype=np.uint64)
+    for t in range(steps):
+        mf = (((idx * 7 + t * 13) % 97) < 3).astype(np.uint8)
+        if t == steps // 2:
+            net.update_err_drive(0, 1.0)
+        if t == steps // 3:
+            net.set_grpc_plast_steps(0, -0.0009, 0.0005)
+        net.update_mf_input(mf)
+        net.calc_activity(1)
+    raw = np.fromfile(out, dtype=np.uint32)
+    parts = [("APBUF_GR", s.numGR), ("VM_GR", s.numGR), ("APBUF_GO", s.numGO), ("VM_GO", s.numGO),
+             ("APBUF_SC", s.numSC), ("APBUF_PC", s.numPC), ("VM_PC", s.numPC), ("APBUF_BC", s.numBC),
+             ("APBUF_IO", s.numIO), ("VM_IO", s.numIO), ("APBUF_NC", s.numNC), ("VM_NC", s.numNC)]
+    off = 0
+    for name, n in parts:
+        got = raw[off:off + n]
+        off += n
+        want = net.read(name).ravel().view(np.uint32)
+        assert np.array_equal(got, want), name
+    assert off == raw.size
