@@ -107,7 +107,7 @@ def _perturb(orc, st, rng, zones):
     orc.write("PIPE_GRGO_SUM", rng.integers(0, 40, (2, s.numGO), dtype=np.uint32))
 
 
-@pytest.mark.parametrize("cfg,warm", [("c0", 239), ("t8k2z", 119), ("t4k", 57)])
+@pytest.mark.parametrize("cfg,warm", [("c0", 239), ("t8k2z", 119), ("t4k", 59)])
 def test_one_step_from_identical_state(gpu, cfg, warm):
     import orc as O
     print(f"[case] one-step parity cfg={cfg} warm={warm}", flush=True)
