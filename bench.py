@@ -7,10 +7,14 @@ row of a synthetic MF spike train that is resident in HBM before the timed
 region (CS on a seeded 3% of MFs at 100 Hz in 500-1000 ms of every 2,000 ms
 trial, the US as error drive at 999 ms).  Prints one JSON line.
 
-roofline: the granule-cell kernel's algorithmic bytes per launch (numGR x
-104 B at fan-ins (4,3,2), 120 B at (4,3,4); the compulsory per-GR bytes with
-the reference's data types) divided by its mean duration, timed with HIP events
-on the stream it runs on, against MI355X's 8 TB/s.
+roofline: the step's dominant kernel -- the fused step kernel (one launch
+per step: the GR tiles plus one block for the small populations), or the GR
+kernel when the two-stream step is used -- with the GR algorithmic bytes per
+launch (numGR x 104 B at fan-ins (4,3,2), 120 B at (4,3,4); the compulsory
+per-GR bytes with the reference's data types) divided by its mean duration,
+timed with HIP events on the stream it runs on, against MI355X's 8 TB/s.  The
+GR and small-population kernels launched one after the other are timed too
+(gr_only_us, small_only_us).
 
 cpu_baseline: the CPU oracle (a C restatement of the same step, one thread),
 timed on this host over a bounded sample of the same workload.
@@ -69,6 +73,9 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=400)
     ap.add_argument("--cpu-steps", type=int, default=0, help="0 = pick a ~15 s sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--step-mode", type=int, default=0, help="0 pick, 1 fused launch, 2 two streams")
+    ap.add_argument("--cells-per-thread", type=int, default=0)
+    ap.add_argument("--block-threads", type=int, default=0)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB, 5 times (the known-size kernel that scales "
                          "the FETCH_SIZE / WRITE_SIZE counters)")
@@ -95,7 +102,8 @@ def main():
 
     t_build = time.perf_counter()
     st = configs.build_state(args.config, seed=0)
-    net = Network.from_state(st)
+    net = Network.from_state(st, step_mode=args.step_mode, cells_per_thread=args.cells_per_thread,
+                             block_threads=args.block_threads)
     s = st.sizes
     n = args.warmup + args.steps
     tr = configs.mf_train(s.numMF, n + args.profile_steps, seed=1)
@@ -116,15 +124,20 @@ def main():
     sim_ms_per_s = args.steps / elapsed
     print(f"[bench] timed {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
 
-    # GR-kernel duration with HIP events on the handle's stream
-    gr_us, step_us = net.profile_steps(dtr[n:n + args.profile_steps])
+    # dominant-kernel duration with HIP events on the stream it runs on
+    half = args.profile_steps // 2
+    k_us, step_us = net.profile_steps(dtr[n:n + half])
+    gr_only, small_only = net.profile_split(dtr[n + half:n + args.profile_steps])
     n_out = s.maxnumpGRfromGRtoGO
     alg_bytes = s.numGR * bytes_per_gr(n_out)
-    achieved = alg_bytes / (gr_us * 1e-6) / 1e9
+    achieved = alg_bytes / (k_us * 1e-6) / 1e9
+    fused = net.step_mode == 1
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "gr_step_kernel", "kernel_us": round(gr_us, 3),
-                "algorithmic_bytes_per_launch": alg_bytes, "step_us_events": round(step_us, 3)}
+                "kernel": "step_kernel" if fused else "gr_step_kernel", "kernel_us": round(k_us, 3),
+                "algorithmic_bytes_per_launch": alg_bytes, "step_us_events": round(step_us, 3),
+                "gr_only_us": round(gr_only, 3), "small_only_us": round(small_only, 3),
+                "gr_only_frac": round(alg_bytes / (gr_only * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)}
     traffic_file = os.path.join(ROOT, "profiles", f"traffic_{args.config}.json")
     if os.path.exists(traffic_file):
         with open(traffic_file) as f:
@@ -132,7 +145,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu_steps = args.cpu_steps or {"c0": 5000, "c1": 120, "c2": 30}.get(args.config, 100)
+        cpu_steps = args.cpu_steps or {"c0": 5000, "c1": 2000, "c2": 400}.get(args.config, 100)
         cpu = cpu_baseline(args.config, cpu_steps)
 
     out = {
@@ -142,7 +155,7 @@ def main():
         "data": "synthetic: seeded Bernoulli MF trains, seeded synthetic connectivity",
         "config": {"workload": WORKLOADS.get(args.config, args.config), "name": args.config,
                    "numGR": s.numGR, "numGO": s.numGO, "numMF": s.numMF, "numPC": s.numPC,
-                   "parallelism": "single GPU"},
+                   "parallelism": "single GPU", "step_mode": "fused" if fused else "two streams"},
         "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -130,11 +130,14 @@ def hip_lib() -> C.CDLL:
         L.cbm_stream.argtypes = [P]
         L.cbm_stream.restype = P
         L.cbm_profile_steps.argtypes = [P, V, S, I, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.cbm_profile_split.argtypes = [P, V, S, I, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.cbm_step_mode.argtypes = [P]
         L.cbm_device_bytes.argtypes = [P, C.POINTER(C.c_size_t)]
         L.cbm_field_info.argtypes = [I, C.POINTER(Sizes), C.POINTER(S), C.POINTER(S), C.POINTER(S),
                                      C.POINTER(I), C.POINTER(I), C.POINTER(I)]
-        if L.cbm_abi_version() != 1:
-            raise RuntimeError("libcbm_hip.so ABI version mismatch")
+        want = int(re.search(r"#define CBM_ABI_VERSION (\d+)", open(HEADER).read()).group(1))
+        if L.cbm_abi_version() != want:
+            raise RuntimeError(f"libcbm_hip.so ABI version {L.cbm_abi_version()} != header's {want}: rebuild")
         _hip = L
     return _hip
 

@@ -15,6 +15,7 @@ constexpr uint16_t kNoCell = 0xFFFFu;   // unused connectivity slot
 constexpr uint8_t kNever = 0xFFu;       // delay mask 0: never delivers
 constexpr int kRing = 3;                // input-sum ring slots (t, t-1, t-2)
 constexpr int kSmallThreads = 1024;     // threads of the small-population kernel
+constexpr uint32_t kSmallLdsMax = 152 * 1024;  // dynamic LDS cap of the small kernel
 
 // Granule-cell kernel arguments.  Per-GR arrays are structure-of-arrays of
 // numGR elements; connectivity is packed four 16-bit cell indices per GR
@@ -32,10 +33,12 @@ struct GrArgs {
     const uint8_t *bspDly;     // delay bit towards BC/PC/SC
     const uint8_t *mf;         // raw MF bytes of this step
     const uint8_t *apGOprev;   // GO spikes of the previous step
-    float *chunk;              // [numZones][numGR/64] PF->PC sums of 64-GR chunks
+    float *chunk;              // [numZones][numGR/64] PF->PC sums of 64-GR chunks, this step
+    const float *foldSrc;      // chunk sums of the previous step (nullptr: nothing to fold)
+    float *foldDst;            // their ring slot, [numZones][numPC]
     uint32_t *bcCnt, *scCnt, *goCnt;  // this step's ring slot (zeroed beforehand)
-    const int32_t *plastTimer;        // [numZones][numIO]
-    uint32_t numGR, numMF, numGO, numZones, numIO;
+    const int32_t *plastTimer;        // [numZones][numIO] as of the previous step
+    uint32_t numGR, numMF, numGO, numZones, numIO, numPC, perPC;  // perPC: 64-GR chunks per PC
     uint32_t log2Lbc, log2Lsc, log2GRperIO;
     uint32_t doPlast;          // t % tsPerHistBinGR == 0: plasticity + history this step
     float gLeak, eLeak, eGO, threshRest, threshMax, threshDec;
@@ -44,26 +47,51 @@ struct GrArgs {
     float ltd[kMaxZones], ltp[kMaxZones];
 };
 
-// CSR list: row r covers idx[ptr[r] .. ptr[r+1])
-struct Csr {
-    const uint32_t *ptr;
-    const uint32_t *idx;
+// Byte offsets of the small kernel's dynamic LDS regions (16-byte aligned),
+// and element offsets of the index lists inside the staged u16 index blob.
+struct SmallLds {
+    uint32_t mf, hmf;              // u8 [numMF]: raw MF bytes, MF history after this step
+    uint32_t goMF;                 // u32 [numGO]: MF->GO counts being scattered
+    uint32_t apGO, scaleGO, vGO;   // u8 / f32 / f32 [numGO]
+    uint32_t syn;                  // f32 synapse partials (offsets below, in floats)
+    uint32_t synPC, synN, synA, synP, synIO;
+    uint32_t apPC, apBC, apSC, apIO, apNC;  // u8 spikes of this step (zones stacked)
+    uint32_t vIO, rel;             // f32 [nZ*numIO], [nZ*numNC]
+    uint32_t rng;                  // u32 [nZ][353]: generator words + index
+    uint32_t pop;                  // u32 [nZ][popStride]: 3 counters, bins, 2 flags
+    uint32_t popStride;
+    uint32_t idx;                  // u16 blob (16-byte copy units below)
+    uint32_t iMFNC, iSCPC, iPCNC, iNCIO, iIOIO, iBCPC, iPCBC;  // u16 element offsets
+    uint32_t slotsBCPC, slotsPCBC; // ELL widths of the BC->PC and PC->BC in-lists
+    uint32_t idx16;                // blob length in 16-byte units
+    uint32_t total;                // bytes
+    // thread offsets: element i of a population (or synapse group) is handled
+    // by thread (i + offset) mod kSmallThreads, so that the small populations
+    // land on different waves instead of all on wave 0
+    uint32_t oSynSC, oSynMF, oSynPC, oSynIO;      // SC->PC, MF->NC, PC->NC, NC->IO synapses
+    uint32_t oPC, oBC, oSC, oIO, oNC;             // cell updates
+    uint32_t oRBCPC, oRSCPC, oRPCBC, oRPCNC, oRIO; // output routing
+    uint32_t l2PC, l2BC, l2IO, l2NC;              // log2 of the power-of-two populations
 };
 
 // Small-population kernel arguments.  Field storage is the CBMState layout;
-// per-zone fields hold numZones consecutive copies.
+// per-zone fields hold numZones consecutive copies.  Fields that both kernels
+// touch within one step are double-buffered by step parity.
 struct SmallArgs {
-    void *f[CBM_F_COUNT];      // device storage of every state field
-    const float *chunk;        // [numZones][numGR/64]
+    void *f[CBM_F_COUNT];      // device storage of every state field (nullptr: see below)
+    const float *chunk2;       // [2][numZones][numGR/64] chunk sums by step parity
     float *pfpcRing;           // [kRing][numZones][numPC]
     uint32_t *bcRing, *scRing, *goRing;  // [kRing][n]
     uint32_t *rngState;        // [numZones][353]
-    Csr mfToGoIn;              // per GO: MF sources (any order)
-    Csr gabaIn;                // per GO: GABA sources in the reference's accumulation order
-    Csr bcToPcIn;              // per PC: BC sources
-    Csr pcToBcIn;              // per BC: PC sources (with multiplicity)
+    uint8_t *apGO2;            // [2][numGO] GO spikes by step parity
+    int32_t *timer2;           // [2][numZones][numIO] PF->PC plasticity timers by step parity
+    const uint4 *idxBlob;      // u16 index lists of the microzone outputs, staged into LDS
+    const uint16_t *mfgoOut;   // [numMF][mfgoPad] GO targets of each MF (kNoCell-padded)
+    const uint16_t *gabaEll;   // [numGO][gabaPad] GABA sources in increasing order (kNoCell-padded)
     const int32_t *coupN;      // numpGOCoupInGOGO
     const uint32_t *coupP;     // pGOCoupInGOGO [numGO][maxCoup]
+    uint32_t mfgoPad, gabaPad;  // row widths of mfgoOut and gabaEll (multiples of 8 and 4)
+    SmallLds lds;
     cbm_sizes s;
     cbm_act_params p;
     uint32_t grgoLag;
@@ -71,13 +99,25 @@ struct SmallArgs {
 };
 
 // Per-step values of the small-population kernel (passed by value; the
-// constant SmallArgs block lives in device memory).
+// constant SmallArgs block lives in device memory).  Slots and flags are the
+// host's 64-bit step arithmetic, done once per step.
 struct SmallStep {
     const uint8_t *mf;         // raw MF bytes of this step
     uint64_t t;                // this step (curTime after the increment)
     float errDrive[kMaxZones];
     float gIncGRtoGO;
+    uint32_t par;              // t & 1: parity buffer written this step
+    uint32_t slotPrev, slotT2, slotG, slotNext;  // ring slots of t-1, t-2, t-grgoLag, t+1
+    uint32_t popHist;          // t % tsPerPopHistBinPC == 0: PC population histogram + MF->NC weights
+    uint32_t resetMFHist;      // t % numTSinMFHist == 0
 };
+
+// Bytes of dynamic LDS a GR tile of nt threads x cpt cells uses.
+__host__ __device__ constexpr uint32_t grTileLds(uint32_t nt, uint32_t cpt, uint32_t numGO, uint32_t numMF,
+                                                 uint32_t numZones, uint32_t numIO)
+{
+    return numGO * 4 + 2 * (nt * cpt / 64) * 4 + numZones * numIO * 4 + numMF + numGO;
+}
 
 __host__ __device__ inline uint32_t ilog2(uint32_t x)
 {

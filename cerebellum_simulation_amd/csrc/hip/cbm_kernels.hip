@@ -1,6 +1,8 @@
 // cbm_kernels.hip -- CDNA4 (gfx950) kernels of one 1 ms cerebellar network step.
 //
-// Two kernels per step, launched back to back on one stream:
+// Two kernels per step.  The small kernel of step t reads only what step t-1
+// produced, so it runs on a second stream concurrently with the GR kernel of
+// step t; fields both touch within one step are double-buffered by parity.
 //
 //  gr_step_kernel     every granule cell (GR), fused: the membrane/threshold/AHP
 //                     update (kernels.cu:79-110), the MF and GO input sums
@@ -11,14 +13,18 @@
 //                     over the per-GR arrays; spike vectors staged in LDS; PF sums
 //                     are wave ballots/popcounts and a fixed-order 64-lane
 //                     butterfly; the GR->GO histogram lives in LDS per block.
+//                     The first tiles also fold the previous step's chunk
+//                     sums into per-PC sums.
 //
 //  small_step_kernel  one workgroup: the small populations in the reference's
 //                     order -- PC/BC/SC (mzone.cpp:187-256, innet.cpp:892-911),
 //                     GO (innet.cpp:816-980), IO/NC and their outputs, the
 //                     NC->IO release draws from the zone's SFMT stream and
-//                     MF->NC plasticity (mzone.cpp:258-605) -- plus the second
-//                     level of the PF->PC sum.
+//                     MF->NC plasticity (mzone.cpp:258-605).
 //
+//  step_kernel        both in one launch: block 0 is the small step, the other
+//                     blocks are GR tiles.
+
 // Floating-point expressions keep the reference's operand order and types and
 // the file is built with -ffp-contract=off, so binary32 results round where the
 // reference's source rounds.
@@ -53,14 +59,27 @@ __device__ __forceinline__ void stage_flags(uint8_t *dst, const uint8_t *src, ui
     }
 }
 
-// ---------------------------------------------------------------- GR kernel
-constexpr int kGrThreads = 256;
-
-template <int CPT>
-__global__ void __launch_bounds__(kGrThreads) gr_step_kernel(const GrArgs a)
+// ---------------------------------------------------------------- PF->PC fold
+// Second level of the PF->PC sum of one (zone, PC), run by one full wave: lane
+// l adds the 64-GR chunk sums l, l+64, l+128, ... in order, then the lanes are
+// combined by the xor butterfly.
+__device__ __forceinline__ float fold_pc(const float *__restrict__ c, uint32_t perPC, int lane)
 {
-    extern __shared__ uint32_t smem[];
-    constexpr int kSeg = 4 * CPT;  // at most this many 64-GR segments per tile
+    float acc = 0.0f;
+#pragma unroll 8
+    for (uint32_t q = lane; q < perPC; q += 64) acc = acc + c[q];
+    return wave_sum_butterfly(acc);
+}
+
+// ---------------------------------------------------------------- GR tile
+// One tile of NT*CPT consecutive granule cells: thread tid owns cells
+// tileBase + j*NT + tid, j < CPT.  Every global load of the tile's cells is
+// issued before the LDS staging barrier, so all of a thread's requests are in
+// flight together.
+template <int CPT, int NT>
+__device__ __forceinline__ void gr_tile(const GrArgs &a, const uint32_t tile, uint32_t *smem)
+{
+    constexpr int kSeg = NT * CPT / 64;  // 64-GR segments per tile (PF blocks hold >= 64 GRs)
     uint32_t *s_goh = smem;                          // [numGO] GR->GO counts of the tile
     uint32_t *s_sc = s_goh + a.numGO;                // [kSeg]
     uint32_t *s_bc = s_sc + kSeg;                    // [kSeg]
@@ -70,90 +89,103 @@ __global__ void __launch_bounds__(kGrThreads) gr_step_kernel(const GrArgs a)
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
-    for (uint32_t i = tid; i < a.numGO; i += kGrThreads) s_goh[i] = 0;
+    const uint32_t tileBase = tile * (uint32_t)(NT * CPT);
+
+    float v[CPT], gk[CPT], th[CPT], gE[CPT], gI[CPT], w0[CPT];
+    uint32_t ab[CPT], gd[CPT], bsp[CPT];
+    uint2 mi[CPT], gi[CPT], go[CPT];
+    uint64_t hist[CPT];
+#pragma unroll
+    for (int j = 0; j < CPT; j++) {
+        const uint32_t gr = tileBase + j * NT + tid;
+        v[j] = a.v[gr];
+        gk[j] = a.gKCa[gr];
+        th[j] = a.thr[gr];
+        gE[j] = a.gE[gr];
+        gI[j] = a.gI[gr];
+        ab[j] = a.apBuf[gr];
+        mi[j] = a.mfIn[gr];
+        gi[j] = a.goIn[gr];
+        go[j] = a.goOut[gr];
+        gd[j] = a.goOutDly[gr];
+        bsp[j] = a.bspDly[gr];
+        w0[j] = a.w[gr];
+        hist[j] = a.doPlast ? a.hist[gr] : 0;
+    }
+
+    for (uint32_t i = tid; i < a.numGO; i += NT) s_goh[i] = 0;
     if (tid < 2 * kSeg) s_sc[tid] = 0;  // s_sc and s_bc are contiguous
     if (a.doPlast) {
-        for (uint32_t i = tid; i < a.numZones * a.numIO; i += kGrThreads) {
+        for (uint32_t i = tid; i < a.numZones * a.numIO; i += NT) {
             const uint32_t z = i / a.numIO;
             const int32_t tm = a.plastTimer[i];
             // MZone::runPFPCPlastCUDA step choice (mzone.cpp:660-674)
             s_step[i] = tm < a.ltdEnd ? a.ltd[z] : (tm > a.ltpStart ? a.ltp[z] : 0.0f);
         }
     }
-    stage_flags(s_mf, a.mf, a.numMF, tid, kGrThreads);
-    stage_flags(s_go, a.apGOprev, a.numGO, tid, kGrThreads);
+    stage_flags(s_mf, a.mf, a.numMF, tid, NT);
+    stage_flags(s_go, a.apGOprev, a.numGO, tid, NT);
     __syncthreads();
 
-    const uint32_t tileBase = blockIdx.x * (kGrThreads * CPT);
     const uint32_t firstSc = tileBase >> a.log2Lsc, firstBc = tileBase >> a.log2Lbc;
     const uint32_t nChunks = a.numGR >> 6;
 
 #pragma unroll
     for (int j = 0; j < CPT; j++) {
-        const uint32_t gr = tileBase + j * kGrThreads + tid;
-        float v = a.v[gr];
-        float gk = a.gKCa[gr];
-        float th = a.thr[gr];
-        float gE = a.gE[gr];
-        float gI = a.gI[gr];
-        uint32_t ab = a.apBuf[gr];
-        const uint2 mi = a.mfIn[gr];
-        const uint2 gi = a.goIn[gr];
-        const uint2 go = a.goOut[gr];
-        const uint32_t gd = a.goOutDly[gr];
-        const uint32_t bsp = a.bspDly[gr];
+        const uint32_t gr = tileBase + j * NT + tid;
+        float vv = v[j], g = gk[j], t = th[j], e = gE[j], i = gI[j];
+        uint32_t b = ab[j];
 
         // calcActivityGRGPU (kernels.cu:92-109)
-        v = v + (a.gLeak + gk * gk * gk * gk) * (a.eLeak - v) - (gE)*v + (gI) * (a.eGO - v);
-        th = th + (a.threshRest - th) * a.threshDec;
-        const uint32_t ap = v > th;
-        const float thN = ap * a.threshMax + (!ap) * th;
-        gk = gk * 0.9999f;
-        const float gkN = ap * (gk + 0.003f) + (!ap) * gk;
-        ab = (ab << 1) | (ap);
+        vv = vv + (a.gLeak + g * g * g * g) * (a.eLeak - vv) - (e)*vv + (i) * (a.eGO - vv);
+        t = t + (a.threshRest - t) * a.threshDec;
+        const uint32_t ap = vv > t;
+        const float thN = ap * a.threshMax + (!ap) * t;
+        g = g * 0.9999f;
+        const float gkN = ap * (g + 0.003f) + (!ap) * g;
+        b = (b << 1) | (ap);
 
         // updateGRInOPGPU for MF then GO inputs (kernels.cu:185-198)
         int cm = 0, cg = 0;
         {
-            const uint32_t m0 = lo16(mi.x), m1 = hi16(mi.x), m2 = lo16(mi.y), m3 = hi16(mi.y);
+            const uint32_t m0 = lo16(mi[j].x), m1 = hi16(mi[j].x), m2 = lo16(mi[j].y), m3 = hi16(mi[j].y);
             if (m0 != kNoCell) cm += s_mf[m0];
             if (m1 != kNoCell) cm += s_mf[m1];
             if (m2 != kNoCell) cm += s_mf[m2];
             if (m3 != kNoCell) cm += s_mf[m3];
-            const uint32_t g0 = lo16(gi.x), g1 = hi16(gi.x), g2 = lo16(gi.y), g3 = hi16(gi.y);
+            const uint32_t g0 = lo16(gi[j].x), g1 = hi16(gi[j].x), g2 = lo16(gi[j].y), g3 = hi16(gi[j].y);
             if (g0 != kNoCell) cg += s_go[g0];
             if (g1 != kNoCell) cg += s_go[g1];
             if (g2 != kNoCell) cg += s_go[g2];
             if (g3 != kNoCell) cg += s_go[g3];
         }
-        gE = gE * a.decMF + a.incMF * (cm);
-        gI = gI * a.decGO + a.incGO * (cg);
+        e = e * a.decMF + a.incMF * (cm);
+        i = i * a.decGO + a.incGO * (cg);
 
-        a.v[gr] = v;
+        a.v[gr] = vv;
         a.thr[gr] = thN;
         a.gKCa[gr] = gkN;
-        a.apBuf[gr] = ab;
-        a.gE[gr] = gE;
-        a.gI[gr] = gI;
+        a.apBuf[gr] = b;
+        a.gE[gr] = e;
+        a.gI[gr] = i;
 
         uint32_t bit39 = 0;
         if (a.doPlast) {
-            const uint64_t h = a.hist[gr];
+            const uint64_t h = hist[j];
             bit39 = (uint32_t)(h >> 39) & 1u;
             // updateGRHistory (kernels.cu:244-245): mask 0x1F is hard-coded there
-            a.hist[gr] = (h << 1) | (uint64_t)(((ab & 0x0000001Fu) > 0) * 0x00000001u);
+            a.hist[gr] = (h << 1) | (uint64_t)(((b & 0x0000001Fu) > 0) * 0x00000001u);
         }
 
         // delayed spike towards BC/PC/SC (kernels.cu:256,269)
-        const uint32_t d = (bsp == kNever) ? 0u : ((ab >> bsp) & 1u);
+        const uint32_t d = (bsp[j] == kNever) ? 0u : ((b >> bsp[j]) & 1u);
         const uint64_t bal = __ballot(d);
 
         // PF->PC: plasticity, then this step's 64-GR chunk sum, per zone
         const uint32_t io = gr >> a.log2GRperIO;
         for (uint32_t z = 0; z < a.numZones; z++) {
-            float* wz = a.w + (size_t)z * a.numGR;
-            float wv = 0.0f;
-            if (a.doPlast || bal != 0) wv = wz[gr];
+            float *wz = a.w + (size_t)z * a.numGR;
+            float wv = z == 0 ? w0[j] : wz[gr];
             if (a.doPlast) {
                 const float plastStep = s_step[z * a.numIO + io];
                 wv = wv + (bit39 > 0) * plastStep;
@@ -174,31 +206,54 @@ __global__ void __launch_bounds__(kGrThreads) gr_step_kernel(const GrArgs a)
         }
 
         // GR->GO outputs (kernels.cu:133-144)
-        if (ab != 0) {
-            const uint32_t t0 = lo16(go.x), t1 = hi16(go.x), t2 = lo16(go.y), t3 = hi16(go.y);
-            const uint32_t d0 = gd & 0xFFu, d1 = (gd >> 8) & 0xFFu, d2 = (gd >> 16) & 0xFFu, d3 = gd >> 24;
-            if (t0 != kNoCell && d0 != kNever && ((ab >> d0) & 1u)) atomicAdd(&s_goh[t0], 1u);
-            if (t1 != kNoCell && d1 != kNever && ((ab >> d1) & 1u)) atomicAdd(&s_goh[t1], 1u);
-            if (t2 != kNoCell && d2 != kNever && ((ab >> d2) & 1u)) atomicAdd(&s_goh[t2], 1u);
-            if (t3 != kNoCell && d3 != kNever && ((ab >> d3) & 1u)) atomicAdd(&s_goh[t3], 1u);
+        if (b != 0) {
+            const uint32_t t0 = lo16(go[j].x), t1 = hi16(go[j].x), t2 = lo16(go[j].y), t3 = hi16(go[j].y);
+            const uint32_t dl = gd[j];
+            const uint32_t d0 = dl & 0xFFu, d1 = (dl >> 8) & 0xFFu, d2 = (dl >> 16) & 0xFFu, d3 = dl >> 24;
+            if (t0 != kNoCell && d0 != kNever && ((b >> d0) & 1u)) atomicAdd(&s_goh[t0], 1u);
+            if (t1 != kNoCell && d1 != kNever && ((b >> d1) & 1u)) atomicAdd(&s_goh[t1], 1u);
+            if (t2 != kNoCell && d2 != kNever && ((b >> d2) & 1u)) atomicAdd(&s_goh[t2], 1u);
+            if (t3 != kNoCell && d3 != kNever && ((b >> d3) & 1u)) atomicAdd(&s_goh[t3], 1u);
         }
     }
     __syncthreads();
 
-    for (uint32_t i = tid; i < a.numGO; i += kGrThreads) {
+    for (uint32_t i = tid; i < a.numGO; i += NT) {
         const uint32_t c = s_goh[i];
         if (c) atomicAdd(&a.goCnt[i], c);
     }
-    const uint32_t tileGR = kGrThreads * CPT;
+    const uint32_t tileGR = NT * CPT;
     const uint32_t nSc = (tileGR >> a.log2Lsc) ? (tileGR >> a.log2Lsc) : 1;
     const uint32_t nBc = (tileGR >> a.log2Lbc) ? (tileGR >> a.log2Lbc) : 1;
     if (tid < (int)nSc && s_sc[tid]) atomicAdd(&a.scCnt[firstSc + tid], s_sc[tid]);
     if (tid < (int)nBc && s_bc[tid]) atomicAdd(&a.bcCnt[firstBc + tid], s_bc[tid]);
+
+    // second level of the previous step's PF->PC sums, one wave per (zone,
+    // PC) on the first tiles: the small populations read them one step later
+    if (a.foldSrc) {
+        const uint32_t waves = (a.numGR / (NT * CPT)) * (uint32_t)(NT / 64);
+        for (uint32_t task = tile * (uint32_t)(NT / 64) + (uint32_t)(tid >> 6); task < a.numZones * a.numPC;
+             task += waves) {  // wave-uniform
+            const uint32_t z = task / a.numPC, pc = task % a.numPC;
+            const float acc = fold_pc(a.foldSrc + (size_t)z * nChunks + (size_t)pc * a.perPC, a.perPC, lane);
+            if (lane == 0) a.foldDst[task] = acc;
+        }
+    }
 }
 
-template __global__ void gr_step_kernel<1>(const GrArgs);
-template __global__ void gr_step_kernel<2>(const GrArgs);
-template __global__ void gr_step_kernel<4>(const GrArgs);
+// GR tiles alone (the two-stream step, and GR-only timing)
+template <int CPT, int NT>
+__global__ void __launch_bounds__(NT) gr_step_kernel(const GrArgs a)
+{
+    extern __shared__ uint32_t smem[];
+    gr_tile<CPT, NT>(a, blockIdx.x, smem);
+}
+
+template __global__ void gr_step_kernel<1, 256>(const GrArgs);
+template __global__ void gr_step_kernel<2, 256>(const GrArgs);
+template __global__ void gr_step_kernel<4, 256>(const GrArgs);
+template __global__ void gr_step_kernel<1, 1024>(const GrArgs);
+template __global__ void gr_step_kernel<2, 1024>(const GrArgs);
 
 // ---------------------------------------------------------------- SFMT on device
 // The zone generator of MZone (CRandomSFMT0, sfmt.cpp:210-246,337-355),
@@ -209,520 +264,672 @@ __device__ void sfmt_regen4(uint32_t *st, int k)
 {
     const uint32_t msk = k == 0 ? 0xeffff7fbu : k == 1 ? 0xffffffefu : k == 2 ? 0xdfdfbfffu : 0x7fffdbfdu;
     uint32_t r1 = st[4 * 86 + k], r2 = st[4 * 87 + k];
+    // Eight words per batch, their inputs read before any of them is written:
+    // word i reads words i and (i+68) mod 88, and for i >= 20 the latter was
+    // regenerated at least 13 words earlier, i.e. in a previous batch.
 #pragma unroll 1
-    for (int i = 0; i < 88; i++) {
-        int jj = i + 68;
-        if (jj >= 88) jj -= 88;
-        const uint32_t av = st[4 * i + k], bv = st[4 * jj + k];
-        const uint32_t aPrev = __shfl(av, k == 0 ? 0 : k - 1, 4);
-        const uint32_t r1Next = __shfl(r1, k == 3 ? 3 : k + 1, 4);
-        const uint32_t sa = (av << 24) | (k == 0 ? 0u : (aPrev >> 8));
-        const uint32_t sc = (r1 >> 24) | (k == 3 ? 0u : (r1Next << 8));
-        const uint32_t r = av ^ sa ^ ((bv >> 7) & msk) ^ sc ^ (r2 << 14);
-        st[4 * i + k] = r;
-        r1 = r2;
-        r2 = r;
+    for (int i0 = 0; i0 < 88; i0 += 8) {
+        uint32_t av[8], bv[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            int jj = i0 + j + 68;
+            if (jj >= 88) jj -= 88;
+            av[j] = st[4 * (i0 + j) + k];
+            bv[j] = st[4 * jj + k];
+        }
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const uint32_t aPrev = __shfl(av[j], k == 0 ? 0 : k - 1, 4);
+            const uint32_t r1Next = __shfl(r1, k == 3 ? 3 : k + 1, 4);
+            const uint32_t sa = (av[j] << 24) | (k == 0 ? 0u : (aPrev >> 8));
+            const uint32_t sc = (r1 >> 24) | (k == 3 ? 0u : (r1Next << 8));
+            const uint32_t r = av[j] ^ sa ^ ((bv[j] >> 7) & msk) ^ sc ^ (r2 << 14);
+            st[4 * (i0 + j) + k] = r;
+            r1 = r2;
+            r2 = r;
+        }
     }
+}
+
+// The same fold as a stand-alone launch (four waves per block, one task per
+// wave), for reading the last step's PF->PC sums between steps.
+__global__ void __launch_bounds__(256) pfpc_fold_kernel(const float *__restrict__ chunk, float *__restrict__ out,
+                                                        uint32_t nChunks, uint32_t perPC, uint32_t numPC,
+                                                        uint32_t tasks)
+{
+    const uint32_t task = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (task >= tasks) return;  // wave-uniform
+    const uint32_t z = task / numPC, pc = task % numPC;
+    const float acc = fold_pc(chunk + (size_t)z * nChunks + (size_t)pc * perPC, perPC, lane);
+    if (lane == 0) out[task] = acc;
 }
 
 // ---------------------------------------------------------------- small kernel
+// Device pointers read from the argument block are generic; casting them to
+// the global address space gives global_load/global_store, which do not share
+// the LDS wait counter the way flat accesses do.
 template <typename T>
-__device__ __forceinline__ T *F(const SmallArgs &a, int f)
+using G = __attribute__((address_space(1))) T;
+
+template <typename T>
+__device__ __forceinline__ G<T> *gp(T *p)
 {
-    return reinterpret_cast<T *>(a.f[f]);
+    return (G<T> *)(p);
 }
 
-__global__ void __launch_bounds__(kSmallThreads) small_step_kernel(const SmallArgs *__restrict__ args,
-                                                                   const SmallStep st)
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
+template <typename T>
+__device__ __forceinline__ G<T> *F(const SmallArgs &a, int f)
 {
-    const SmallArgs &a = *args;
+    return (G<T> *)(a.f[f]);
+}
+
+template <typename T>
+__device__ __forceinline__ T *at(uint8_t *base, uint32_t off)
+{
+    return reinterpret_cast<T *>(base + off);
+}
+
+// Sequential sum of n LDS floats in index order (16-byte reads when aligned).
+__device__ __forceinline__ float lds_sum_seq(const float *x, uint32_t n, float acc)
+{
+    if ((n & 3u) == 0 && ((uintptr_t)x & 15u) == 0) {
+        const float4 *x4 = reinterpret_cast<const float4 *>(x);
+#pragma unroll 4
+        for (uint32_t j = 0; j < n / 4; j++) {
+            const float4 v = x4[j];
+            acc = acc + v.x;
+            acc = acc + v.y;
+            acc = acc + v.z;
+            acc = acc + v.w;
+        }
+        return acc;
+    }
+    for (uint32_t j = 0; j < n; j++) acc = acc + x[j];
+    return acc;
+}
+
+// One workgroup steps every population other than GR.  The phases and what
+// each reads:
+//   0  stage MF bytes, output index lists, generator words and the PC
+//      population histogram into LDS; MF history; MF->GO scatter of this
+//      step's spiking MFs; per-synapse conductances (SC->PC, MF->NC, PC->NC,
+//      NC->IO) into LDS
+//   1  MF->NC routing; PC, BC, SC, GO, IO and NC updates (spikes to LDS);
+//      PC/BC/SC read the PF sums of t-2, GO the GR->GO count of t-grgoLag
+//   2  output routing from the LDS spikes; population histogram; clear the
+//      ring slot of step t+1
+//   3  GO->GO depression scale; NC->IO release draws (wave 0); MF->NC weights
+// Within a phase every loop iteration issues all of its global loads before
+// its first store, so each phase costs about one memory round trip, and each
+// population or synapse group starts on its own wave, so that no wave runs
+// the updates of every small population in turn.
+__device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &st, uint8_t *lds)
+{
     const cbm_sizes &s = a.s;
     const cbm_act_params &p = a.p;
+    const SmallLds &L = a.lds;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     constexpr int nThr = kSmallThreads, nWave = kSmallThreads / 64;
-    const uint64_t t = st.t;
-    const uint32_t slotT = (uint32_t)(t % kRing), slotT2 = (uint32_t)((t + kRing - 2) % kRing);
-    const uint32_t slotG = (uint32_t)((t + kRing - a.grgoLag) % kRing);
+    const uint32_t par = st.par, parPrev = par ^ 1u;
     const uint32_t nZ = s.numZones;
+    const uint32_t kSC = s.numpPCfromSCtoPC, kMF = s.numpNCfromMFtoNC, kPC = s.numpNCfromPCtoNC;
+    const uint32_t kNCIO = s.numpIOfromNCtoIO, kIOIO = s.numpIOInIOIO;
+    const uint32_t bins = s.numPopHistBinsPC;
 
-    __shared__ uint32_t s_rng[352];
+    uint8_t *s_mf = lds + L.mf;
+    uint8_t *s_hmf = lds + L.hmf;
+    uint32_t *s_goMF = at<uint32_t>(lds, L.goMF);
+    uint8_t *s_apGO = lds + L.apGO;
+    float *s_scale = at<float>(lds, L.scaleGO);
+    float *s_vGO = at<float>(lds, L.vGO);
+    float *s_syn = at<float>(lds, L.syn);
+    uint8_t *s_apPC = lds + L.apPC, *s_apBC = lds + L.apBC, *s_apSC = lds + L.apSC;
+    uint8_t *s_apIO = lds + L.apIO, *s_apNC = lds + L.apNC;
+    float *s_vIO = at<float>(lds, L.vIO), *s_rel = at<float>(lds, L.rel);
+    uint32_t *s_rng = at<uint32_t>(lds, L.rng);
+    uint32_t *s_pop = at<uint32_t>(lds, L.pop);
+    const uint16_t *s_idx = at<uint16_t>(lds, L.idx);
     __shared__ int s_doLTD[kMaxZones], s_doLTP[kMaxZones];
 
-    // LDS partials of the synapse-parallel sub-phase (offsets in floats)
-    extern __shared__ float s_syn[];
-    const uint32_t kSC = s.numpPCfromSCtoPC, kMF = s.numpNCfromMFtoNC, kPC = s.numpNCfromPCtoNC;
-    const uint32_t kNCIO = s.numpIOfromNCtoIO;
-    const uint32_t offPC = 0;
-    const uint32_t offN = offPC + nZ * s.numSC;
-    const uint32_t offA = offN + nZ * s.numNC * kMF;
-    const uint32_t offP = offA + nZ * s.numNC * kMF;
-    const uint32_t offIO = offP + nZ * s.numNC * kPC;
-
-    // ======================= phase 1a: MF history, PF->PC fold, synapses
-    // InNet::updateMFActivties (innet.cpp:800-815)
+    // ======================= phase 0
+    for (uint32_t i = tid; i < s.numGO; i += nThr) s_goMF[i] = 0;
+    __syncthreads();
+    // staging, InNet::updateMFActivties (innet.cpp:800-815) and
+    // InNet::updateMFtoGOOut (innet.cpp:912-931): each spiking MF adds one to
+    // every GO it targets; the counts are integers, so the order is free
     {
-        uint8_t *__restrict__ histMF = F<uint8_t>(a, CBM_F_HIST_MF);
-        uint32_t *__restrict__ apBufMF = F<uint32_t>(a, CBM_F_APBUF_MF);
-        for (uint32_t i = tid; i < s.numMF; i += nThr) {
-            const uint8_t ap = st.mf[i] > 0;
-            const uint8_t h = histMF[i];
-            const uint32_t b = apBufMF[i];
-            histMF[i] = h || ap;
-            apBufMF[i] = (b << 1) | (ap * 0x00000001u);
+        const uint32_t nMFw = s.numMF / 4, nRng = nZ * 353, nPop = nZ * L.popStride;
+        const uint32_t n = max(max(max(nMFw, L.idx16), max(nRng, nPop)), s.numMF);
+        const G<uint32_t> *mfw = (const G<uint32_t> *)st.mf;
+        uint32_t *mfl = at<uint32_t>(lds, L.mf);
+        u32x4 *bd = at<u32x4>(lds, L.idx);
+        auto histMF = F<uint8_t>(a, CBM_F_HIST_MF);
+        auto apBufMF = F<uint32_t>(a, CBM_F_APBUF_MF);
+        for (uint32_t i = tid; i < n; i += nThr) {
+            // every array is read at a clamped index (in bounds, no branch),
+            // so all of the iteration's loads are issued before the first wait
+            const uint32_t im = min(i, s.numMF - 1), ip = min(i, nPop - 1);
+            const uint32_t w = mfw[min(i, nMFw - 1)];
+            const u32x4 bl = ((const G<u32x4> *)a.idxBlob)[min(i, L.idx16 - 1)];
+            const uint32_t r = gp(a.rngState)[min(i, nRng - 1)];
+            const uint32_t pz = ip / L.popStride, pk = ip % L.popStride;
+            const G<uint32_t> *pw = pk < 3 ? F<uint32_t>(a, CBM_F_PC_POPACT_CTRS) + pz * 3 + pk
+                                           : F<uint32_t>(a, CBM_F_HIST_PC_POPACT) + (size_t)pz * bins +
+                                                 min(pk - 3, bins - 1);
+            const uint32_t pv32 = *pw;
+            const uint8_t pv8 = F<uint8_t>(a, CBM_F_MFNC_NOLTP_NOLTD)[pz * 2 + (pk >= 3 + bins ? pk - 3 - bins : 0)];
+            const uint8_t m = gp(st.mf)[im];
+            const uint8_t h = histMF[im];
+            const uint32_t b = apBufMF[im];
+            const uint32_t pv = pk < 3 + bins ? pv32 : pv8;
+            if (i < nMFw) mfl[i] = w;
+            if (i < L.idx16) bd[i] = bl;
+            if (i < nRng) s_rng[i] = r;
+            if (i < nPop) s_pop[i] = pv;
+            if (i < s.numMF) {
+                const uint8_t ap = m > 0;
+                s_hmf[i] = h || ap;
+                apBufMF[i] = (b << 1) | (ap * 0x00000001u);
+                if (ap) {
+                    const G<u32x4> *row = (const G<u32x4> *)(a.mfgoOut + (size_t)i * a.mfgoPad);
+                    const uint32_t nv = a.mfgoPad / 8;
+                    for (uint32_t c0 = 0; c0 < nv; c0 += 4) {
+                        u32x4 wv[4];
+#pragma unroll
+                        for (int k = 0; k < 4; k++) wv[k] = row[min(c0 + k, nv - 1)];
+#pragma unroll
+                        for (int k = 0; k < 4; k++) {
+                            if (c0 + k >= nv) break;
+                            const uint32_t v4[4] = {wv[k].x, wv[k].y, wv[k].z, wv[k].w};
+#pragma unroll
+                            for (int e = 0; e < 4; e++) {
+                                const uint32_t lo = v4[e] & 0xFFFFu, hi = v4[e] >> 16;
+                                if (lo != kNoCell) atomicAdd(&s_goMF[lo], 1u);
+                                if (hi != kNoCell) atomicAdd(&s_goMF[hi], 1u);
+                            }
+                        }
+                    }
+                }
+            }
         }
     }
-    // second level of this step's PF->PC sums: one wave per (zone, PC)
+    // synapses: SC->PC of calcPCActivities (mzone.cpp:201-207), MF->NC and
+    // PC->NC of calcNCActivities (mzone.cpp:358-383), NC->IO of
+    // calcIOActivities (mzone.cpp:272-280)
     {
-        const uint32_t nChunks = s.numGR >> 6, perPC = s.numpPCfromGRtoPC >> 6;
-        for (uint32_t task = wave; task < nZ * s.numPC; task += nWave) {
-            const uint32_t z = task / s.numPC, pc = task % s.numPC;
-            const float *__restrict__ c = a.chunk + (size_t)z * nChunks + (size_t)pc * perPC;
-            float acc = 0.0f;
-#pragma unroll 8
-            for (uint32_t q = lane; q < perPC; q += 64) acc = acc + c[q];
-            acc = wave_sum_butterfly(acc);
-            if (lane == 0) a.pfpcRing[((size_t)slotT * nZ + z) * s.numPC + pc] = acc;
-        }
-    }
-    // SC->PC synapses of calcPCActivities (mzone.cpp:201-207)
-    {
-        float *__restrict__ gSC = F<float>(a, CBM_F_G_SC_PC);
-        const uint8_t *__restrict__ inSC = F<uint8_t>(a, CBM_F_INPUT_SC_PC);
-        for (uint32_t q = tid; q < nZ * s.numSC; q += nThr) {
-            float g = gSC[q];
-            const uint8_t in = inSC[q];
-            g = g + p.gIncSCtoPC * (1 - g) * in;
-            g = g * p.gDecSCtoPC;
-            gSC[q] = g;
-            s_syn[offPC + q] = g;
-        }
-    }
-    // MF->NC and PC->NC synapses of calcNCActivities (mzone.cpp:358-383)
-    {
-        const uint8_t *__restrict__ inMF = F<uint8_t>(a, CBM_F_INPUT_MF_NC);
-        const float *__restrict__ wMF = F<float>(a, CBM_F_MF_SYNW_NC);
-        float *__restrict__ gmN = F<float>(a, CBM_F_GMAX_NMDA_MF_NC);
-        float *__restrict__ gmA = F<float>(a, CBM_F_GMAX_AMPA_MF_NC);
-        float *__restrict__ gN = F<float>(a, CBM_F_G_MF_NMDA_NC);
-        float *__restrict__ gA = F<float>(a, CBM_F_G_MF_AMPA_NC);
-        for (uint32_t q = tid; q < nZ * s.numNC * kMF; q += nThr) {
-            const uint8_t in = inMF[q];
-            const float w = wMF[q];
-            float mN = gmN[q], mA = gmA[q], n = gN[q], aa = gA[q];
-            mN = mN * p.gmaxNMDADecMFtoNC + in * w * (1 - mN);
-            n = n + p.gNMDAIncMFtoNC * (mN - n);
-            mA = mA * p.gmaxAMPADecMFtoNC + in * w * (1 - mA);
-            aa = aa + p.gAMPAIncMFtoNC * (mA - aa);
-            gmN[q] = mN;
-            gN[q] = n;
-            gmA[q] = mA;
-            gA[q] = aa;
-            s_syn[offN + q] = n;
-            s_syn[offA + q] = aa;
-        }
-        const uint8_t *__restrict__ inPC = F<uint8_t>(a, CBM_F_INPUT_PC_NC);
-        const float *__restrict__ scPC = F<float>(a, CBM_F_G_PC_SCALE_NC);
-        float *__restrict__ gPC = F<float>(a, CBM_F_G_PC_NC);
-        for (uint32_t q = tid; q < nZ * s.numNC * kPC; q += nThr) {
-            const uint8_t in = inPC[q];
-            const float sc = scPC[q];
-            float g = gPC[q];
-            g = g * p.gDecPCtoNC + in * sc * (1 - g);
-            gPC[q] = g;
-            s_syn[offP + q] = g;
-        }
-    }
-    // NC->IO synapses of calcIOActivities (mzone.cpp:272-280)
-    {
-        float *__restrict__ gNC = F<float>(a, CBM_F_G_NC_IO);
-        uint8_t *__restrict__ inNC = F<uint8_t>(a, CBM_F_INPUT_NC_IO);
+        auto gSC = F<float>(a, CBM_F_G_SC_PC);
+        auto inSC = F<uint8_t>(a, CBM_F_INPUT_SC_PC);
+        auto inMF = F<uint8_t>(a, CBM_F_INPUT_MF_NC);
+        auto wMF = F<float>(a, CBM_F_MF_SYNW_NC);
+        auto gmN = F<float>(a, CBM_F_GMAX_NMDA_MF_NC);
+        auto gmA = F<float>(a, CBM_F_GMAX_AMPA_MF_NC);
+        auto gN = F<float>(a, CBM_F_G_MF_NMDA_NC);
+        auto gA = F<float>(a, CBM_F_G_MF_AMPA_NC);
+        auto inPC = F<uint8_t>(a, CBM_F_INPUT_PC_NC);
+        auto scPC = F<float>(a, CBM_F_G_PC_SCALE_NC);
+        auto gPC = F<float>(a, CBM_F_G_PC_NC);
+        auto gNC = F<float>(a, CBM_F_G_NC_IO);
+        auto inNC = F<uint8_t>(a, CBM_F_INPUT_NC_IO);
+        const uint32_t nS = nZ * s.numSC, nM = nZ * s.numNC * kMF, nP = nZ * s.numNC * kPC,
+                       nI = nZ * s.numIO * kNCIO;
+        const uint32_t rounds = (max(max(nS, nM), max(nP, nI)) + nThr - 1) / nThr;
         const float ms = p.msPerTimeStep;
-        for (uint32_t q = tid; q < nZ * s.numIO * kNCIO; q += nThr) {
-            float g = gNC[q];
-            const uint8_t in = inNC[q];
-            g = g * expf(-ms / (-p.gDecTSofNCtoIO * expf(-g / p.gDecTTofNCtoIO) + p.gDecT0ofNCtoIO));
-            g = g + in * p.gIncNCtoIO * expf(-g / p.gIncTauNCtoIO);
-            gNC[q] = g;
-            inNC[q] = 0;
-            s_syn[offIO + q] = g;
+        for (uint32_t rd = 0; rd < rounds; rd++) {
+            // group-local indices (each group starts on its own wave), clamped for the loads
+            const uint32_t jS = rd * nThr + ((tid - L.oSynSC) & (nThr - 1)), jM = rd * nThr + ((tid - L.oSynMF) & (nThr - 1));
+            const uint32_t jP = rd * nThr + ((tid - L.oSynPC) & (nThr - 1)), jI = rd * nThr + ((tid - L.oSynIO) & (nThr - 1));
+            const uint32_t qs = min(jS, nS - 1), qm = min(jM, nM - 1), qp = min(jP, nP - 1), qi = min(jI, nI - 1);
+            float g1 = gSC[qs];
+            const uint8_t i1 = inSC[qs];
+            const uint8_t i2 = inMF[qm];
+            const float w = wMF[qm];
+            float mN = gmN[qm], mA = gmA[qm], nn = gN[qm], aa = gA[qm];
+            const uint8_t i3 = inPC[qp];
+            const float sc = scPC[qp];
+            float g3 = gPC[qp];
+            float g4 = gNC[qi];
+            const uint8_t i4 = inNC[qi];
+            if (jS < nS) {
+                const uint32_t q = jS;
+                g1 = g1 + p.gIncSCtoPC * (1 - g1) * i1;
+                g1 = g1 * p.gDecSCtoPC;
+                gSC[q] = g1;
+                s_syn[L.synPC + q] = g1;
+            }
+            if (jM < nM) {
+                const uint32_t q = jM;
+                mN = mN * p.gmaxNMDADecMFtoNC + i2 * w * (1 - mN);
+                nn = nn + p.gNMDAIncMFtoNC * (mN - nn);
+                mA = mA * p.gmaxAMPADecMFtoNC + i2 * w * (1 - mA);
+                aa = aa + p.gAMPAIncMFtoNC * (mA - aa);
+                gmN[q] = mN;
+                gN[q] = nn;
+                gmA[q] = mA;
+                gA[q] = aa;
+                s_syn[L.synN + q] = nn;
+                s_syn[L.synA + q] = aa;
+            }
+            if (jP < nP) {
+                const uint32_t q = jP;
+                g3 = g3 * p.gDecPCtoNC + i3 * sc * (1 - g3);
+                gPC[q] = g3;
+                s_syn[L.synP + q] = g3;
+            }
+            if (jI < nI) {
+                const uint32_t q = jI;
+                g4 = g4 * expf(-ms / (-p.gDecTSofNCtoIO * expf(-g4 / p.gDecTTofNCtoIO) + p.gDecT0ofNCtoIO));
+                g4 = g4 + i4 * p.gIncNCtoIO * expf(-g4 / p.gIncTauNCtoIO);
+                gNC[q] = g4;
+                s_syn[L.synIO + q] = g4;
+            }
         }
     }
     __syncthreads();
 
-    // ======================= phase 1b: cell updates
-    // MZone::calcPCActivities (mzone.cpp:187-226)
-    for (uint32_t idx = tid; idx < nZ * s.numPC; idx += nThr) {
-        const uint32_t z = idx / s.numPC, i = idx % s.numPC;
-        float *__restrict__ gPFp = F<float>(a, CBM_F_G_PF_PC);
-        float *__restrict__ gBCp = F<float>(a, CBM_F_G_BC_PC);
-        float *__restrict__ vPC = F<float>(a, CBM_F_VM_PC);
-        float *__restrict__ thPC = F<float>(a, CBM_F_THRESH_PC);
-        uint32_t *__restrict__ apBuf = F<uint32_t>(a, CBM_F_APBUF_PC);
-        const float pfSum = a.pfpcRing[((size_t)slotT2 * nZ + z) * s.numPC + i];
-        const uint32_t inBC = F<uint32_t>(a, CBM_F_INPUT_BC_PC)[idx];
-        float gPF = gPFp[idx], gBC = gBCp[idx], v = vPC[idx], th = thPC[idx];
-        const uint32_t ab = apBuf[idx];
-        float gSCPCSum = 0;
-        for (uint32_t j = 0; j < kSC; j++) gSCPCSum += s_syn[offPC + idx * kSC + j];
-        gPF = gPF + pfSum * p.gIncGRtoPC;
-        gPF = gPF * p.gDecGRtoPC;
-        gBC = gBC + inBC * p.gIncBCtoPC;
-        gBC = gBC * p.gDecBCtoPC;
-        v = v + (p.gLeakPC * (p.eLeakPC - v)) - (gPF * v) + (gBC * (p.eBCtoPC - v)) +
-            (gSCPCSum * (p.eSCtoPC - v));
-        th = th + (p.threshDecPC * (p.threshRestPC - th));
-        const uint8_t ap = v > th;
-        gPFp[idx] = gPF;
-        gBCp[idx] = gBC;
-        vPC[idx] = v;
-        F<uint8_t>(a, CBM_F_AP_PC)[idx] = ap;
-        apBuf[idx] = (ab << 1) | (ap * 0x00000001u);
-        thPC[idx] = ap * p.threshMaxPC + (!ap) * th;
-        if (ap) atomicAdd(&F<uint32_t>(a, CBM_F_PC_POPACT_CTRS)[z * 3 + 2], 1u);
+    // ======================= phase 1
+    // MZone::updateMFNCOut (mzone.cpp:514-524): the caller's raw MF bytes
+    {
+        auto inMF = F<uint8_t>(a, CBM_F_INPUT_MF_NC);
+        for (uint32_t q = tid; q < nZ * s.numNC * kMF; q += nThr) inMF[q] = s_mf[s_idx[L.iMFNC + q]];
     }
-    // MZone::calcBCActivities (mzone.cpp:228-256)
-    for (uint32_t idx = tid; idx < nZ * s.numBC; idx += nThr) {
-        const uint32_t i = idx % s.numBC;
-        float *__restrict__ gPFp = F<float>(a, CBM_F_G_PF_BC);
-        float *__restrict__ gPCp = F<float>(a, CBM_F_G_PC_BC);
-        float *__restrict__ vBC = F<float>(a, CBM_F_VM_BC);
-        float *__restrict__ thBC = F<float>(a, CBM_F_THRESH_BC);
-        uint32_t *__restrict__ apBuf = F<uint32_t>(a, CBM_F_APBUF_BC);
-        const uint32_t pf = a.bcRing[(size_t)slotT2 * s.numBC + i];
-        const uint32_t inPC = F<uint32_t>(a, CBM_F_INPUT_PC_BC)[idx];
-        float gPF = gPFp[idx], gPC = gPCp[idx], v = vBC[idx], th = thBC[idx];
-        const uint32_t ab = apBuf[idx];
-        gPF = gPF + (pf * p.gIncGRtoBC);
-        gPF = gPF * p.gDecGRtoBC;
-        gPC = gPC + (inPC * p.gIncPCtoBC);
-        gPC = gPC * p.gDecPCtoBC;
-        v = v + (p.gLeakBC * (p.eLeakBC - v)) - (gPF * v) + (gPC * (p.ePCtoBC - v));
-        th = th + p.threshDecBC * (p.threshRestBC - th);
-        const uint8_t ap = v > th;
-        gPFp[idx] = gPF;
-        gPCp[idx] = gPC;
-        vBC[idx] = v;
-        F<uint8_t>(a, CBM_F_AP_BC)[idx] = ap;
-        apBuf[idx] = (ab << 1) | (ap * 0x00000001u);
-        thBC[idx] = ap * p.threshMaxBC + (!ap) * (th);
-    }
-    // InNet::calcSCActivities (innet.cpp:892-911)
-    for (uint32_t i = tid; i < s.numSC; i += nThr) {
-        float *__restrict__ gPFp = F<float>(a, CBM_F_G_PF_SC);
-        float *__restrict__ vSC = F<float>(a, CBM_F_VM_SC);
-        float *__restrict__ thSC = F<float>(a, CBM_F_THRESH_SC);
-        uint32_t *__restrict__ apBuf = F<uint32_t>(a, CBM_F_APBUF_SC);
-        const uint32_t pf = a.scRing[(size_t)slotT2 * s.numSC + i];
-        float gPF = gPFp[i], v = vSC[i], th = thSC[i];
-        const uint32_t ab = apBuf[i];
-        gPF = gPF + (pf * p.gIncGRtoSC);
-        gPF = gPF * p.gDecGRtoSC;
-        v = v + (p.gLeakSC * (p.eLeakSC - v)) - gPF * v;
-        const uint8_t ap = v > th;
-        th = th + p.threshDecSC * (p.threshRestSC - th);
-        gPFp[i] = gPF;
-        vSC[i] = v;
-        F<uint8_t>(a, CBM_F_AP_SC)[i] = ap;
-        apBuf[i] = (ab << 1) | (ap * 0x00000001u);
-        thSC[i] = ap * p.threshMaxSC + (!ap) * (th);
-    }
-    // InNet::calcGOActivities (innet.cpp:827-876)
-    for (uint32_t i = tid; i < s.numGO; i += nThr) {
-        uint32_t *__restrict__ inMFp = F<uint32_t>(a, CBM_F_INPUT_MF_GO);
-        uint32_t *__restrict__ inGOp = F<uint32_t>(a, CBM_F_INPUT_GO_GO);
-        float *__restrict__ inDepp = F<float>(a, CBM_F_INPUT_GOGABA_SYNDEP_GO);
-        float *__restrict__ gMFp = F<float>(a, CBM_F_G_MF_GO);
-        float *__restrict__ gGRp = F<float>(a, CBM_F_G_GR_GO);
-        float *__restrict__ gGOp = F<float>(a, CBM_F_G_GO_GO);
-        float *__restrict__ glup = F<float>(a, CBM_F_GLU_GO);
-        float *__restrict__ mGp = F<float>(a, CBM_F_MGLUR_GO);
-        float *__restrict__ gIncp = F<float>(a, CBM_F_G_MGLURINC_GO);
-        float *__restrict__ gMGp = F<float>(a, CBM_F_G_MGLUR_GO);
-        float *__restrict__ thp = F<float>(a, CBM_F_THRESH_GO);
-        float *__restrict__ vGOp = F<float>(a, CBM_F_VM_GO);
-        const float *__restrict__ vCp = F<float>(a, CBM_F_VCOUPLE_GO);
-        uint32_t *__restrict__ apBuf = F<uint32_t>(a, CBM_F_APBUF_GO);
-        const uint32_t totalGRIn = a.goRing[(size_t)slotG * s.numGO + i];
-        const uint32_t inMF = inMFp[i];
-        const float inDep = inDepp[i];
-        float gMF = gMFp[i], gGR = gGRp[i], gGO = gGOp[i], glu = glup[i], mG = mGp[i], gInc = gIncp[i],
-              gMG = gMGp[i], thv = thp[i], v = vGOp[i];
-        const float vC = vCp[i];
-        const uint32_t ab = apBuf[i];
-        gMF = inMF * p.gIncMFtoGO + gMF * p.gDecMFtoGO;
-        gGR = totalGRIn * st.gIncGRtoGO + gGR * p.gDecGRtoGO;
-        gGO = inDep * p.gGABAIncGOtoGO + gGO * p.gGABADecGOtoGO;
-        glu = glu * p.gluDecayGO + totalGRIn * p.gluScaleGO * exp(-1.5 * glu);
-        mG = mG * p.mGluRDecayGO + glu * p.mGluRScaleGO * expf(-mG);
-        gInc = gInc * p.gMGluRIncDecayGO + mG * p.gMGluRIncScaleGO * expf(-gInc);
-        gMG = gMG * p.gMGluRDecGRtoGO + gInc * p.gMGluRScaleGRtoGO;
-        thv = thv + (p.threshRestGO - thv) * p.threshDecGO;
-        v = v + (p.gLeakGO * (p.eLeakGO - v)) + (gGO * (p.eGABAGO - v)) - (gMF + gGR) * v + vC;
-        const uint8_t ap = v > thv;
-        F<uint32_t>(a, CBM_F_SUM_GR_INPUT_GO)[i] = totalGRIn;
-        F<float>(a, CBM_F_SUM_GO_INPUT_GO)[i] = inDep;
-        gMFp[i] = gMF;
-        gGRp[i] = gGR;
-        gGOp[i] = gGO;
-        glup[i] = glu;
-        mGp[i] = mG;
-        gIncp[i] = gInc;
-        gMGp[i] = gMG;
-        vGOp[i] = v;
-        F<uint8_t>(a, CBM_F_AP_GO)[i] = ap;
-        apBuf[i] = (ab << 1) | (ap * 0x00000001u);
-        thp[i] = ap * p.threshMaxGO + (!ap) * thv;
-        inMFp[i] = 0;
-        inGOp[i] = 0;
-        inDepp[i] = 0;
-    }
-    // MZone::calcIOActivities (mzone.cpp:282-334)
-    for (uint32_t idx = tid; idx < nZ * s.numIO; idx += nThr) {
-        const uint32_t z = idx / s.numIO;
-        float *__restrict__ vIO = F<float>(a, CBM_F_VM_IO);
-        float *__restrict__ thIO = F<float>(a, CBM_F_THRESH_IO);
-        uint32_t *__restrict__ apBuf = F<uint32_t>(a, CBM_F_APBUF_IO);
-        float v = vIO[idx], th = thIO[idx];
-        const float vC = F<float>(a, CBM_F_VCOUPLE_IO)[idx];
-        const uint32_t ab = apBuf[idx];
-        float gNCSum = 0;
-        for (uint32_t j = 0; j < kNCIO; j++) gNCSum = gNCSum + s_syn[offIO + idx * kNCIO + j];
-        gNCSum = gNCSum / 3.1;
-        v = v + p.gLeakIO * (p.eLeakIO - v) + gNCSum * 1.5 * (p.eNCtoIO - v) + vC + st.errDrive[z];
-        const uint8_t ap = v > th;
-        vIO[idx] = v;
-        F<uint8_t>(a, CBM_F_AP_IO)[idx] = ap;
-        apBuf[idx] = (ab << 1) | (ap * 0x00000001u);
-        thIO[idx] = p.threshMaxIO * ap + (!ap) * (th + p.threshDecIO * (p.threshRestIO - th));
-    }
-    // MZone::calcNCActivities (mzone.cpp:371-396)
-    for (uint32_t idx = tid; idx < nZ * s.numNC; idx += nThr) {
-        float *__restrict__ vNC = F<float>(a, CBM_F_VM_NC);
-        float *__restrict__ thNC = F<float>(a, CBM_F_THRESH_NC);
-        uint32_t *__restrict__ apBuf = F<uint32_t>(a, CBM_F_APBUF_NC);
-        float v = vNC[idx], th = thNC[idx];
-        const uint32_t ab = apBuf[idx];
-        const float ms = p.msPerTimeStep;
-        float gMFNMDASum = 0, gMFAMPASum = 0, gPCNCSum = 0;
-        for (uint32_t j = 0; j < kMF; j++) {
-            gMFNMDASum = gMFNMDASum + s_syn[offN + idx * kMF + j];
-            gMFAMPASum = gMFAMPASum + s_syn[offA + idx * kMF + j];
+    // cell updates: MZone::calcPCActivities (mzone.cpp:187-226),
+    // calcBCActivities (mzone.cpp:228-256), InNet::calcSCActivities
+    // (innet.cpp:892-911), InNet::calcGOActivities (innet.cpp:827-876),
+    // MZone::calcIOActivities (mzone.cpp:282-334) with the plasticity timer of
+    // MZone::updateIOOut (mzone.cpp:484), and MZone::calcNCActivities
+    // (mzone.cpp:371-396) with the release probability of MZone::updateNCOut
+    // (mzone.cpp:497-503); the last two depend only on the cell's own spike.
+    // Cell i of every population but GO is handled by the same thread; GO
+    // follows in its own pass to bound the registers held at once.
+    {
+        const uint32_t nPCc = nZ * s.numPC, nBCc = nZ * s.numBC, nSCc = s.numSC, nGOc = s.numGO;
+        const uint32_t nIOc = nZ * s.numIO, nNCc = nZ * s.numNC;
+        const uint32_t rounds = (max(max(nPCc, nBCc), max(max(nSCc, nIOc), nNCc)) + nThr - 1) / nThr;
+        for (uint32_t rd = 0; rd < rounds; rd++) {
+            // population-local cell indices: each population starts on its own wave
+            const uint32_t cP = rd * nThr + ((tid - L.oPC) & (nThr - 1)), cB = rd * nThr + ((tid - L.oBC) & (nThr - 1));
+            const uint32_t cS = rd * nThr + ((tid - L.oSC) & (nThr - 1)), cI = rd * nThr + ((tid - L.oIO) & (nThr - 1));
+            const uint32_t cN = rd * nThr + ((tid - L.oNC) & (nThr - 1));
+            const bool hPC = cP < nPCc, hBC = cB < nBCc, hSC = cS < nSCc, hIO = cI < nIOc, hNC = cN < nNCc;
+            // ---- loads, at clamped indices (in bounds, no branch)
+            const uint32_t iP = min(cP, nPCc - 1), iB = min(cB, nBCc - 1), iS = min(cS, nSCc - 1);
+            const uint32_t iI = min(cI, nIOc - 1), iN = min(cN, nNCc - 1);
+            const float pcPF = gp(a.pfpcRing)[(size_t)st.slotT2 * nPCc + iP];
+            const uint32_t pcIn = F<uint32_t>(a, CBM_F_INPUT_BC_PC)[iP];
+            const float pcGPF = F<float>(a, CBM_F_G_PF_PC)[iP], pcGBC = F<float>(a, CBM_F_G_BC_PC)[iP];
+            const float pcV = F<float>(a, CBM_F_VM_PC)[iP], pcTh = F<float>(a, CBM_F_THRESH_PC)[iP];
+            const uint32_t pcAb = F<uint32_t>(a, CBM_F_APBUF_PC)[iP];
+            const uint32_t bcPF = gp(a.bcRing)[(size_t)st.slotT2 * s.numBC + (iB & (s.numBC - 1))];
+            const uint32_t bcIn = F<uint32_t>(a, CBM_F_INPUT_PC_BC)[iB];
+            const float bcGPF = F<float>(a, CBM_F_G_PF_BC)[iB], bcGPC = F<float>(a, CBM_F_G_PC_BC)[iB];
+            const float bcV = F<float>(a, CBM_F_VM_BC)[iB], bcTh = F<float>(a, CBM_F_THRESH_BC)[iB];
+            const uint32_t bcAb = F<uint32_t>(a, CBM_F_APBUF_BC)[iB];
+            const uint32_t scPF = gp(a.scRing)[(size_t)st.slotT2 * s.numSC + iS];
+            const float scGPF = F<float>(a, CBM_F_G_PF_SC)[iS], scV = F<float>(a, CBM_F_VM_SC)[iS];
+            const float scTh = F<float>(a, CBM_F_THRESH_SC)[iS];
+            const uint32_t scAb = F<uint32_t>(a, CBM_F_APBUF_SC)[iS];
+            const float ioV = F<float>(a, CBM_F_VM_IO)[iI], ioTh = F<float>(a, CBM_F_THRESH_IO)[iI];
+            const float ioVC = F<float>(a, CBM_F_VCOUPLE_IO)[iI];
+            const uint32_t ioAb = F<uint32_t>(a, CBM_F_APBUF_IO)[iI];
+            const int32_t ioTm = gp(a.timer2)[(size_t)parPrev * nIOc + iI];
+            const float ncV = F<float>(a, CBM_F_VM_NC)[iN], ncTh = F<float>(a, CBM_F_THRESH_NC)[iN];
+            const float ncR = F<float>(a, CBM_F_SYN_IO_PRELEASE_NC)[iN];
+            const uint32_t ncAb = F<uint32_t>(a, CBM_F_APBUF_NC)[iN];
+            // ---- PC
+            if (hPC) {
+                const uint32_t i = iP, z = i >> L.l2PC;
+                float gPF = pcGPF, gBC = pcGBC, vv = pcV, th = pcTh;
+                const float gSCPCSum = lds_sum_seq(s_syn + L.synPC + i * kSC, kSC, 0.0f);
+                gPF = gPF + pcPF * p.gIncGRtoPC;
+                gPF = gPF * p.gDecGRtoPC;
+                gBC = gBC + pcIn * p.gIncBCtoPC;
+                gBC = gBC * p.gDecBCtoPC;
+                vv = vv + (p.gLeakPC * (p.eLeakPC - vv)) - (gPF * vv) + (gBC * (p.eBCtoPC - vv)) +
+                     (gSCPCSum * (p.eSCtoPC - vv));
+                th = th + (p.threshDecPC * (p.threshRestPC - th));
+                const uint8_t ap = vv > th;
+                F<float>(a, CBM_F_G_PF_PC)[i] = gPF;
+                F<float>(a, CBM_F_G_BC_PC)[i] = gBC;
+                F<float>(a, CBM_F_VM_PC)[i] = vv;
+                F<uint8_t>(a, CBM_F_AP_PC)[i] = ap;
+                s_apPC[i] = ap;
+                F<uint32_t>(a, CBM_F_APBUF_PC)[i] = (pcAb << 1) | (ap * 0x00000001u);
+                F<float>(a, CBM_F_THRESH_PC)[i] = ap * p.threshMaxPC + (!ap) * th;
+                if (ap) atomicAdd(&s_pop[z * L.popStride + 2], 1u);
+            }
+            // ---- BC
+            if (hBC) {
+                const uint32_t i = iB;
+                float gPF = bcGPF, gPC = bcGPC, vv = bcV, th = bcTh;
+                gPF = gPF + (bcPF * p.gIncGRtoBC);
+                gPF = gPF * p.gDecGRtoBC;
+                gPC = gPC + (bcIn * p.gIncPCtoBC);
+                gPC = gPC * p.gDecPCtoBC;
+                vv = vv + (p.gLeakBC * (p.eLeakBC - vv)) - (gPF * vv) + (gPC * (p.ePCtoBC - vv));
+                th = th + p.threshDecBC * (p.threshRestBC - th);
+                const uint8_t ap = vv > th;
+                F<float>(a, CBM_F_G_PF_BC)[i] = gPF;
+                F<float>(a, CBM_F_G_PC_BC)[i] = gPC;
+                F<float>(a, CBM_F_VM_BC)[i] = vv;
+                F<uint8_t>(a, CBM_F_AP_BC)[i] = ap;
+                s_apBC[i] = ap;
+                F<uint32_t>(a, CBM_F_APBUF_BC)[i] = (bcAb << 1) | (ap * 0x00000001u);
+                F<float>(a, CBM_F_THRESH_BC)[i] = ap * p.threshMaxBC + (!ap) * (th);
+            }
+            // ---- SC
+            if (hSC) {
+                const uint32_t i = iS;
+                float gPF = scGPF, vv = scV, th = scTh;
+                gPF = gPF + (scPF * p.gIncGRtoSC);
+                gPF = gPF * p.gDecGRtoSC;
+                vv = vv + (p.gLeakSC * (p.eLeakSC - vv)) - gPF * vv;
+                const uint8_t ap = vv > th;
+                th = th + p.threshDecSC * (p.threshRestSC - th);
+                F<float>(a, CBM_F_G_PF_SC)[i] = gPF;
+                F<float>(a, CBM_F_VM_SC)[i] = vv;
+                F<uint8_t>(a, CBM_F_AP_SC)[i] = ap;
+                s_apSC[i] = ap;
+                F<uint32_t>(a, CBM_F_APBUF_SC)[i] = (scAb << 1) | (ap * 0x00000001u);
+                F<float>(a, CBM_F_THRESH_SC)[i] = ap * p.threshMaxSC + (!ap) * (th);
+            }
+            // ---- IO
+            if (hIO) {
+                const uint32_t i = iI, z = i >> L.l2IO;
+                float vv = ioV;
+                float gNCSum = lds_sum_seq(s_syn + L.synIO + i * kNCIO, kNCIO, 0.0f);
+                gNCSum = gNCSum / 3.1;
+                vv = vv + p.gLeakIO * (p.eLeakIO - vv) + gNCSum * 1.5 * (p.eNCtoIO - vv) + ioVC + st.errDrive[z];
+                const uint8_t ap = vv > ioTh;
+                F<float>(a, CBM_F_VM_IO)[i] = vv;
+                s_vIO[i] = vv;
+                F<uint8_t>(a, CBM_F_AP_IO)[i] = ap;
+                s_apIO[i] = ap;
+                F<uint32_t>(a, CBM_F_APBUF_IO)[i] = (ioAb << 1) | (ap * 0x00000001u);
+                F<float>(a, CBM_F_THRESH_IO)[i] =
+                    p.threshMaxIO * ap + (!ap) * (ioTh + p.threshDecIO * (p.threshRestIO - ioTh));
+                gp(a.timer2)[(size_t)par * nIOc + i] = (!ap) * (ioTm + 1) + ap * p.tsLTDStartAPIO;
+            }
+            // ---- NC
+            if (hNC) {
+                const uint32_t i = iN;
+                float vv = ncV, th = ncTh, r = ncR;
+                const float ms = p.msPerTimeStep;
+                float gMFNMDASum = lds_sum_seq(s_syn + L.synN + i * kMF, kMF, 0.0f);
+                float gMFAMPASum = lds_sum_seq(s_syn + L.synA + i * kMF, kMF, 0.0f);
+                float gPCNCSum = lds_sum_seq(s_syn + L.synP + i * kPC, kPC, 0.0f);
+                gMFNMDASum = gMFNMDASum * ms / ((float)kMF);
+                gMFAMPASum = gMFAMPASum * ms / ((float)kMF);
+                gMFNMDASum = gMFNMDASum * vv / (-80.0f);
+                gPCNCSum = gPCNCSum * ms / ((float)kPC);
+                vv = vv + p.gLeakNC * (p.eLeakNC - vv) - (gMFNMDASum + gMFAMPASum) * vv + gPCNCSum * (p.ePCtoNC - vv);
+                th = th + p.threshDecNC * (p.threshRestNC - th);
+                const uint8_t ap = vv > th;
+                F<float>(a, CBM_F_VM_NC)[i] = vv;
+                F<uint8_t>(a, CBM_F_AP_NC)[i] = ap;
+                s_apNC[i] = ap;
+                F<uint32_t>(a, CBM_F_APBUF_NC)[i] = (ncAb << 1) | (ap * 0x00000001u);
+                F<float>(a, CBM_F_THRESH_NC)[i] = ap * p.threshMaxNC + (!ap) * th;
+                r = r * expf(-ms / (p.relPDecTSofNCtoIO * expf(-r / p.relPDecTTofNCtoIO) + p.relPDecT0ofNCtoIO));
+                r = r + ap * p.relPIncNCtoIO * expf(-r / p.relPIncTauNCtoIO);
+                F<float>(a, CBM_F_SYN_IO_PRELEASE_NC)[i] = r;
+                s_rel[i] = r;
+            }
         }
-        gMFNMDASum = gMFNMDASum * ms / ((float)kMF);
-        gMFAMPASum = gMFAMPASum * ms / ((float)kMF);
-        gMFNMDASum = gMFNMDASum * v / (-80.0f);
-        for (uint32_t j = 0; j < kPC; j++) gPCNCSum = gPCNCSum + s_syn[offP + idx * kPC + j];
-        gPCNCSum = gPCNCSum * ms / ((float)kPC);
-        v = v + p.gLeakNC * (p.eLeakNC - v) - (gMFNMDASum + gMFAMPASum) * v + gPCNCSum * (p.ePCtoNC - v);
-        th = th + p.threshDecNC * (p.threshRestNC - th);
-        const uint8_t ap = v > th;
-        vNC[idx] = v;
-        F<uint8_t>(a, CBM_F_AP_NC)[idx] = ap;
-        apBuf[idx] = (ab << 1) | (ap * 0x00000001u);
-        thNC[idx] = ap * p.threshMaxNC + (!ap) * th;
+        for (uint32_t i = tid; i < nGOc; i += nThr) {
+            float gMF = 0, gGR = 0, gGO = 0, glu = 0, mG = 0, gInc = 0, gMG = 0, thv = 0, v = 0, vC = 0, sc = 0, inDep = 0;
+            uint32_t totalGRIn = 0, inMF = 0, goAb = 0;
+            {
+                totalGRIn = gp(a.goRing)[(size_t)st.slotG * s.numGO + i];
+                inMF = F<uint32_t>(a, CBM_F_INPUT_MF_GO)[i];
+                inDep = F<float>(a, CBM_F_INPUT_GOGABA_SYNDEP_GO)[i];
+                gMF = F<float>(a, CBM_F_G_MF_GO)[i];
+                gGR = F<float>(a, CBM_F_G_GR_GO)[i];
+                gGO = F<float>(a, CBM_F_G_GO_GO)[i];
+                glu = F<float>(a, CBM_F_GLU_GO)[i];
+                mG = F<float>(a, CBM_F_MGLUR_GO)[i];
+                gInc = F<float>(a, CBM_F_G_MGLURINC_GO)[i];
+                gMG = F<float>(a, CBM_F_G_MGLUR_GO)[i];
+                thv = F<float>(a, CBM_F_THRESH_GO)[i];
+                v = F<float>(a, CBM_F_VM_GO)[i];
+                vC = F<float>(a, CBM_F_VCOUPLE_GO)[i];
+                sc = F<float>(a, CBM_F_GO_GABA_OUT_SYNSCALE_GOGO)[i];
+                goAb = F<uint32_t>(a, CBM_F_APBUF_GO)[i];
+            }
+            {
+                gMF = inMF * p.gIncMFtoGO + gMF * p.gDecMFtoGO;
+                gGR = totalGRIn * st.gIncGRtoGO + gGR * p.gDecGRtoGO;
+                gGO = inDep * p.gGABAIncGOtoGO + gGO * p.gGABADecGOtoGO;
+                glu = glu * p.gluDecayGO + totalGRIn * p.gluScaleGO * exp(-1.5 * glu);
+                mG = mG * p.mGluRDecayGO + glu * p.mGluRScaleGO * expf(-mG);
+                gInc = gInc * p.gMGluRIncDecayGO + mG * p.gMGluRIncScaleGO * expf(-gInc);
+                gMG = gMG * p.gMGluRDecGRtoGO + gInc * p.gMGluRScaleGRtoGO;
+                thv = thv + (p.threshRestGO - thv) * p.threshDecGO;
+                v = v + (p.gLeakGO * (p.eLeakGO - v)) + (gGO * (p.eGABAGO - v)) - (gMF + gGR) * v + vC;
+                const uint8_t ap = v > thv;
+                F<uint32_t>(a, CBM_F_SUM_GR_INPUT_GO)[i] = totalGRIn;
+                F<float>(a, CBM_F_SUM_GO_INPUT_GO)[i] = inDep;
+                F<float>(a, CBM_F_G_MF_GO)[i] = gMF;
+                F<float>(a, CBM_F_G_GR_GO)[i] = gGR;
+                F<float>(a, CBM_F_G_GO_GO)[i] = gGO;
+                F<float>(a, CBM_F_GLU_GO)[i] = glu;
+                F<float>(a, CBM_F_MGLUR_GO)[i] = mG;
+                F<float>(a, CBM_F_G_MGLURINC_GO)[i] = gInc;
+                F<float>(a, CBM_F_G_MGLUR_GO)[i] = gMG;
+                F<float>(a, CBM_F_VM_GO)[i] = v;
+                gp(a.apGO2)[(size_t)par * s.numGO + i] = ap;
+                s_apGO[i] = ap;
+                s_scale[i] = sc;
+                if (a.anyCoupling) s_vGO[i] = v;
+                F<uint32_t>(a, CBM_F_APBUF_GO)[i] = (goAb << 1) | (ap * 0x00000001u);
+                F<float>(a, CBM_F_THRESH_GO)[i] = ap * p.threshMaxGO + (!ap) * thv;
+            }
+        }
     }
     __syncthreads();
 
-    // ======================= phase 2: output routing (reads phase-1 spikes)
-    // InNet::updateMFtoGOOut (innet.cpp:912-931), as a gather over each GO's MFs
-    for (uint32_t g = tid; g < s.numGO; g += nThr) {
-        uint32_t c = 0;
-        for (uint32_t q = a.mfToGoIn.ptr[g]; q < a.mfToGoIn.ptr[g + 1]; q++) c += st.mf[a.mfToGoIn.idx[q]] > 0;
-        F<uint32_t>(a, CBM_F_INPUT_MF_GO)[g] = c;
-    }
-    // InNet::updateGOtoGOOut (innet.cpp:937-950): each GO gathers its GABA
-    // inputs in increasing source order, which is the order the reference adds them
+    // ======================= phase 2
+    // MF->GO counts; InNet::updateGOtoGOOut (innet.cpp:937-950): each GO
+    // gathers its GABA inputs in increasing source order, which is the order
+    // the reference adds them
     {
-        const uint8_t *apGO = F<uint8_t>(a, CBM_F_AP_GO);
-        const float *scale = F<float>(a, CBM_F_GO_GABA_OUT_SYNSCALE_GOGO);
+        auto inMFGO = F<uint32_t>(a, CBM_F_INPUT_MF_GO);
+        auto inGO = F<uint32_t>(a, CBM_F_INPUT_GO_GO);
+        auto inDep = F<float>(a, CBM_F_INPUT_GOGABA_SYNDEP_GO);
         for (uint32_t g = tid; g < s.numGO; g += nThr) {
             uint32_t c = 0;
             float dep = 0;
-            for (uint32_t q = a.gabaIn.ptr[g]; q < a.gabaIn.ptr[g + 1]; q++) {
-                const uint32_t src = a.gabaIn.idx[q];
-                if (apGO[src]) {
-                    c++;
-                    dep += scale[src];
+            const G<u32x2> *row = (const G<u32x2> *)(a.gabaEll + (size_t)g * a.gabaPad);
+            for (uint32_t k0 = 0; k0 < a.gabaPad / 4; k0++) {
+                const u32x2 w = row[k0];
+                const uint32_t src[4] = {w.x & 0xFFFFu, w.x >> 16, w.y & 0xFFFFu, w.y >> 16};
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    if (src[k] != kNoCell && s_apGO[src[k]]) {
+                        c++;
+                        dep += s_scale[src[k]];
+                    }
                 }
             }
-            F<uint32_t>(a, CBM_F_INPUT_GO_GO)[g] = c;
-            F<float>(a, CBM_F_INPUT_GOGABA_SYNDEP_GO)[g] = dep;
+            inMFGO[g] = s_goMF[g];
+            inGO[g] = c;
+            inDep[g] = dep;
         }
     }
     // GO electrical coupling (innet.cpp:961-978): vCoupleGO is a Jacobi sweep;
-    // the threshold update is in place in cell order, so it runs on one lane
+    // the threshold update is in place in cell order, so it runs on one lane.
+    // Without coupling inputs (or with a zero ratio) both terms are zero.
     {
-        const float *vGO = F<float>(a, CBM_F_VM_GO);
-        float *vC = F<float>(a, CBM_F_VCOUPLE_GO);
-        const uint32_t kc = s.maxnumpGOCoupInGOGO;
-        const float ratio = p.coupleRiRjRatioGO;
-        for (uint32_t i = tid; i < s.numGO; i += nThr) {
-            float acc = 0;
-            for (int j = 0; j < a.coupN[i]; j++) acc = acc + ratio * (vGO[a.coupP[(size_t)i * kc + j]] - vGO[i]);
-            vC[i] = acc;
-        }
-        if (a.anyCoupling && tid == 0) {
-            float *th = F<float>(a, CBM_F_THRESH_GO);
-            for (uint32_t i = 0; i < s.numGO; i++) {
-                float tc = 0;
-                for (int j = 0; j < a.coupN[i]; j++) tc = tc + ratio * (th[a.coupP[(size_t)i * kc + j]] - th[i]);
-                th[i] = th[i] + tc;
+        auto vC = F<float>(a, CBM_F_VCOUPLE_GO);
+        if (a.anyCoupling) {
+            const uint32_t kc = s.maxnumpGOCoupInGOGO;
+            const float ratio = p.coupleRiRjRatioGO;
+            for (uint32_t i = tid; i < s.numGO; i += nThr) {
+                float acc = 0;
+                for (int j = 0; j < gp(a.coupN)[i]; j++) acc = acc + ratio * (s_vGO[gp(a.coupP)[(size_t)i * kc + j]] - s_vGO[i]);
+                vC[i] = acc;
             }
-        }
-    }
-    for (uint32_t z = 0; z < nZ; z++) {
-        // MZone::updateMFNCOut (mzone.cpp:514-524): the caller's raw MF bytes
-        {
-            const size_t n = (size_t)s.numNC * s.numpNCfromMFtoNC;
-            const uint32_t *pMF = F<uint32_t>(a, CBM_F_P_NC_FROM_MF_TO_NC) + z * n;
-            uint8_t *inMF = F<uint8_t>(a, CBM_F_INPUT_MF_NC) + z * n;
-            for (size_t q = tid; q < n; q += nThr) inMF[q] = st.mf[pMF[q]];
-        }
-        // MZone::updateBCPCOut (mzone.cpp:449-465)
-        {
-            const uint8_t *apBC = F<uint8_t>(a, CBM_F_AP_BC) + (size_t)z * s.numBC;
-            uint32_t *inBCPC = F<uint32_t>(a, CBM_F_INPUT_BC_PC) + (size_t)z * s.numPC;
-            const uint32_t *ptr = a.bcToPcIn.ptr + (size_t)z * (s.numPC + 1);
-            for (uint32_t i = tid; i < s.numPC; i += nThr) {
-                uint32_t c = 0;
-                for (uint32_t q = ptr[i]; q < ptr[i + 1]; q++) c += apBC[a.bcToPcIn.idx[q]] != 0;
-                inBCPC[i] = c;
-            }
-        }
-        // MZone::updateSCPCOut (mzone.cpp:467-477)
-        {
-            const size_t n = (size_t)s.numPC * s.numpPCfromSCtoPC;
-            const uint32_t *pSC = F<uint32_t>(a, CBM_F_P_PC_FROM_SC_TO_PC) + z * n;
-            uint8_t *inSC = F<uint8_t>(a, CBM_F_INPUT_SC_PC) + z * n;
-            const uint8_t *apSC = F<uint8_t>(a, CBM_F_AP_SC);
-            for (size_t q = tid; q < n; q += nThr) inSC[q] = apSC[pSC[q]];
-        }
-        // MZone::updatePCOut (mzone.cpp:409-447)
-        {
-            const uint8_t *apPC = F<uint8_t>(a, CBM_F_AP_PC) + (size_t)z * s.numPC;
-            uint32_t *inPCBC = F<uint32_t>(a, CBM_F_INPUT_PC_BC) + (size_t)z * s.numBC;
-            const uint32_t *ptr = a.pcToBcIn.ptr + (size_t)z * (s.numBC + 1);
-            for (uint32_t i = tid; i < s.numBC; i += nThr) {
-                uint32_t c = 0;
-                for (uint32_t q = ptr[i]; q < ptr[i + 1]; q++) c += apPC[a.pcToBcIn.idx[q]];
-                inPCBC[i] = c;
-            }
-            const size_t n = (size_t)s.numNC * s.numpNCfromPCtoNC;
-            const uint32_t *pNC = F<uint32_t>(a, CBM_F_P_NC_FROM_PC_TO_NC) + z * n;
-            uint8_t *inPCNC = F<uint8_t>(a, CBM_F_INPUT_PC_NC) + z * n;
-            for (size_t q = tid; q < n; q += nThr) inPCNC[q] = apPC[pNC[q]];
-        }
-        // MZone::updateIOOut (mzone.cpp:479-493)
-        {
-            int32_t *timer = F<int32_t>(a, CBM_F_PFPC_PLAST_TIMER_IO) + (size_t)z * s.numIO;
-            const uint8_t *apIO = F<uint8_t>(a, CBM_F_AP_IO) + (size_t)z * s.numIO;
-            const float *vIO = F<float>(a, CBM_F_VM_IO) + (size_t)z * s.numIO;
-            float *vCIO = F<float>(a, CBM_F_VCOUPLE_IO) + (size_t)z * s.numIO;
-            const uint32_t *pIO = F<uint32_t>(a, CBM_F_P_IO_IN_IOIO) + (size_t)z * s.numIO * s.numpIOInIOIO;
-            for (uint32_t i = tid; i < s.numIO; i += nThr) {
-                timer[i] = (!apIO[i]) * (timer[i] + 1) + apIO[i] * p.tsLTDStartAPIO;
-                float vc = 0;
-                for (uint32_t j = 0; j < s.numpIOInIOIO; j++)
-                    vc = vc + p.coupleRiRjRatioIO * (vIO[pIO[i * s.numpIOInIOIO + j]] - vIO[i]);
-                vCIO[i] = vc;
-            }
-        }
-        // MZone::updateNCOut release probability (mzone.cpp:497-503)
-        {
-            float *rel = F<float>(a, CBM_F_SYN_IO_PRELEASE_NC) + (size_t)z * s.numNC;
-            const uint8_t *apNC = F<uint8_t>(a, CBM_F_AP_NC) + (size_t)z * s.numNC;
-            const float ms = p.msPerTimeStep;
-            for (uint32_t i = tid; i < s.numNC; i += nThr) {
-                float r = rel[i];
-                r = r * expf(-ms / (p.relPDecTSofNCtoIO * expf(-r / p.relPDecTTofNCtoIO) + p.relPDecT0ofNCtoIO));
-                r = r + apNC[i] * p.relPIncNCtoIO * expf(-r / p.relPIncTauNCtoIO);
-                rel[i] = r;
-            }
-        }
-        // MZone::updateMFNCSyn population histogram (mzone.cpp:536-580)
-        if (tid == 0) {
-            s_doLTD[z] = 0;
-            s_doLTP[z] = 0;
-            if (t % p.tsPerPopHistBinPC == 0) {
-                uint32_t *ctrs = F<uint32_t>(a, CBM_F_PC_POPACT_CTRS) + z * 3;
-                uint32_t *hist = F<uint32_t>(a, CBM_F_HIST_PC_POPACT) + (size_t)z * s.numPopHistBinsPC;
-                uint8_t *flags = F<uint8_t>(a, CBM_F_MFNC_NOLTP_NOLTD) + z * 2;
-                ctrs[0] = ctrs[0] - hist[ctrs[1]] + ctrs[2];
-                hist[ctrs[1]] = ctrs[2];
-                ctrs[2] = 0;
-                ctrs[1]++;
-                ctrs[1] = ctrs[1] % p.numPopHistBinsPC;
-                const float avg = ((float)ctrs[0]) / p.numPopHistBinsPC;
-                if (avg >= p.synLTDPCPopActThreshMFtoNC && !flags[1]) {
-                    s_doLTD[z] = 1;
-                    flags[1] = 1;
-                } else if (avg < p.synLTDPCPopActThreshMFtoNC) {
-                    flags[1] = 0;
-                }
-                if (avg <= p.synLTPPCPopActThreshMFtoNC && !flags[0]) {
-                    s_doLTP[z] = 1;
-                    flags[0] = 1;
-                } else if (avg > p.synLTPPCPopActThreshMFtoNC) {
-                    flags[0] = 0;
+            if (tid == 0) {
+                auto th = F<float>(a, CBM_F_THRESH_GO);
+                for (uint32_t i = 0; i < s.numGO; i++) {
+                    float tc = 0;
+                    for (int j = 0; j < gp(a.coupN)[i]; j++) tc = tc + ratio * (th[gp(a.coupP)[(size_t)i * kc + j]] - th[i]);
+                    th[i] = th[i] + tc;
                 }
             }
+        } else {
+            for (uint32_t i = tid; i < s.numGO; i += nThr) vC[i] = 0.0f;
         }
     }
+    // microzone outputs: MZone::updateBCPCOut (mzone.cpp:449-465),
+    // updateSCPCOut (mzone.cpp:467-477), updatePCOut (mzone.cpp:409-447) and
+    // the IO coupling of updateIOOut (mzone.cpp:486-489), from LDS
+    {
+        auto inBCPC = F<uint32_t>(a, CBM_F_INPUT_BC_PC);
+        for (uint32_t i = (tid - L.oRBCPC) & (nThr - 1); i < nZ * s.numPC; i += nThr) {
+            const uint32_t z = i >> L.l2PC, pc = i & (s.numPC - 1);
+            const uint16_t *ell = s_idx + L.iBCPC + (size_t)z * L.slotsBCPC * s.numPC + pc;
+            uint32_t c = 0;
+            for (uint32_t k0 = 0; k0 < L.slotsBCPC; k0 += 8) {
+                uint32_t b[8];
+#pragma unroll
+                for (int k = 0; k < 8; k++) b[k] = k0 + k < L.slotsBCPC ? ell[(size_t)(k0 + k) * s.numPC] : kNoCell;
+#pragma unroll
+                for (int k = 0; k < 8; k++)
+                    if (b[k] != kNoCell) c += s_apBC[z * s.numBC + b[k]] != 0;
+            }
+            inBCPC[i] = c;
+        }
+        auto inSC = F<uint8_t>(a, CBM_F_INPUT_SC_PC);
+        for (uint32_t q = (tid - L.oRSCPC) & (nThr - 1); q < nZ * s.numPC * kSC; q += nThr)
+            inSC[q] = s_apSC[s_idx[L.iSCPC + q]];
+        auto inPCBC = F<uint32_t>(a, CBM_F_INPUT_PC_BC);
+        for (uint32_t i = (tid - L.oRPCBC) & (nThr - 1); i < nZ * s.numBC; i += nThr) {
+            const uint32_t z = i >> L.l2BC, bc = i & (s.numBC - 1);
+            const uint16_t *ell = s_idx + L.iPCBC + (size_t)z * L.slotsPCBC * s.numBC + bc;
+            uint32_t c = 0;
+            for (uint32_t k0 = 0; k0 < L.slotsPCBC; k0 += 8) {
+                uint32_t pc[8];
+#pragma unroll
+                for (int k = 0; k < 8; k++) pc[k] = k0 + k < L.slotsPCBC ? ell[(size_t)(k0 + k) * s.numBC] : kNoCell;
+#pragma unroll
+                for (int k = 0; k < 8; k++)
+                    if (pc[k] != kNoCell) c += s_apPC[z * s.numPC + pc[k]];
+            }
+            inPCBC[i] = c;
+        }
+        auto inPCNC = F<uint8_t>(a, CBM_F_INPUT_PC_NC);
+        const uint32_t per = s.numNC * kPC;
+        for (uint32_t q = (tid - L.oRPCNC) & (nThr - 1); q < nZ * per; q += nThr)
+            inPCNC[q] = s_apPC[(q / per) * s.numPC + s_idx[L.iPCNC + q]];
+        auto vCIO = F<float>(a, CBM_F_VCOUPLE_IO);
+        for (uint32_t i = (tid - L.oRIO) & (nThr - 1); i < nZ * s.numIO; i += nThr) {
+            const uint32_t z = i >> L.l2IO;
+            float vc = 0;
+            for (uint32_t j = 0; j < kIOIO; j++)
+                vc = vc + p.coupleRiRjRatioIO * (s_vIO[z * s.numIO + s_idx[L.iIOIO + i * kIOIO + j]] - s_vIO[i]);
+            vCIO[i] = vc;
+        }
+    }
+    // MZone::updateMFNCSyn population histogram (mzone.cpp:536-580), one lane per zone
+    if (tid < (int)nZ) {
+        const uint32_t z = tid;
+        uint32_t *ctrs = s_pop + z * L.popStride, *hist = ctrs + 3, *flags = ctrs + 3 + bins;
+        int doLTD = 0, doLTP = 0;
+        if (st.popHist) {
+            const uint32_t b = ctrs[1] < bins ? ctrs[1] : 0;  // commit keeps ctrs[1] < bins
+            ctrs[0] = ctrs[0] - hist[b] + ctrs[2];
+            hist[b] = ctrs[2];
+            ctrs[2] = 0;
+            ctrs[1]++;
+            ctrs[1] = ctrs[1] % p.numPopHistBinsPC;
+            const float avg = ((float)ctrs[0]) / p.numPopHistBinsPC;
+            if (avg >= p.synLTDPCPopActThreshMFtoNC && !flags[1]) {
+                doLTD = 1;
+                flags[1] = 1;
+            } else if (avg < p.synLTDPCPopActThreshMFtoNC) {
+                flags[1] = 0;
+            }
+            if (avg <= p.synLTPPCPopActThreshMFtoNC && !flags[0]) {
+                doLTP = 1;
+                flags[0] = 1;
+            } else if (avg > p.synLTPPCPopActThreshMFtoNC) {
+                flags[0] = 0;
+            }
+        }
+        s_doLTD[z] = doLTD;
+        s_doLTP[z] = doLTP;
+        for (uint32_t k = 0; k < 3; k++) F<uint32_t>(a, CBM_F_PC_POPACT_CTRS)[z * 3 + k] = ctrs[k];
+        for (uint32_t k = 0; k < bins; k++) F<uint32_t>(a, CBM_F_HIST_PC_POPACT)[(size_t)z * bins + k] = hist[k];
+        for (uint32_t k = 0; k < 2; k++) F<uint8_t>(a, CBM_F_MFNC_NOLTP_NOLTD)[z * 2 + k] = (uint8_t)flags[k];
+    }
+    // the ring slot of step t+1 is the slot PC/BC/SC read above (t-2): clear it now
+    for (uint32_t i = tid; i < s.numBC; i += nThr) gp(a.bcRing)[(size_t)st.slotNext * s.numBC + i] = 0;
+    for (uint32_t i = tid; i < s.numSC; i += nThr) gp(a.scRing)[(size_t)st.slotNext * s.numSC + i] = 0;
+    for (uint32_t i = tid; i < s.numGO; i += nThr) gp(a.goRing)[(size_t)st.slotNext * s.numGO + i] = 0;
     __syncthreads();
 
-    // ======================= phase 3: scale update, draws, MF->NC weights
-    {
-        const uint8_t *apGO = F<uint8_t>(a, CBM_F_AP_GO);
-        float *scale = F<float>(a, CBM_F_GO_GABA_OUT_SYNSCALE_GOGO);
-        const float rec = p.goGABAGOGOSynRec, depF = p.goGABAGOGOSynDepF;
-        for (uint32_t i = tid; i < s.numGO; i += nThr)
-            scale[i] = scale[i] + (!apGO[i]) * (1 - scale[i]) * rec - apGO[i] * (scale[i] * (1 - depF));
-    }
-    // MZone::updateNCOut draws (mzone.cpp:505-511): one Random() per (IO, NC) in
-    // row order from the zone's stream
+    // ======================= phase 3
+    // MZone::updateNCOut draws (mzone.cpp:505-511): one Random() per (IO, NC)
+    // in row order from the zone's stream.  Draw k of a run uses generator
+    // words ix+2k, ix+2k+1, so up to 64 draws that fit before the next
+    // regeneration are taken at once, one per lane.
     if (wave == 0) {
+        auto inNC = F<uint8_t>(a, CBM_F_INPUT_NC_IO);
+        const uint32_t n = s.numIO * kNCIO;
         for (uint32_t z = 0; z < nZ; z++) {
-            uint32_t *g = a.rngState + (size_t)z * 353;
-            for (int i = lane; i < 352; i += 64) s_rng[i] = g[i];
+            uint32_t *g = s_rng + (size_t)z * 353;
             uint32_t ix = g[352];
-            __threadfence_block();
-            if (lane < 4) {
-                const float *rel = F<float>(a, CBM_F_SYN_IO_PRELEASE_NC) + (size_t)z * s.numNC;
-                const uint32_t *pNC = F<uint32_t>(a, CBM_F_P_IO_FROM_NC_TO_IO) + (size_t)z * s.numIO * s.numNC;
-                uint8_t *inNC = F<uint8_t>(a, CBM_F_INPUT_NC_IO) + (size_t)z * s.numIO * s.numNC;
-                const uint32_t n = s.numIO * s.numpIOfromNCtoIO;
+            const uint16_t *pn = s_idx + L.iNCIO + (size_t)z * n;
+            const float *rel = s_rel + (size_t)z * s.numNC;
 #pragma unroll 1
-                for (uint32_t q = 0; q < n; q++) {
-                    if (ix >= 351) {
-                        __threadfence_block();
-                        sfmt_regen4(s_rng, lane);
-                        __threadfence_block();
-                        ix = 0;
-                    }
-                    const uint64_t r = (uint64_t)s_rng[ix] | ((uint64_t)s_rng[ix + 1] << 32);
-                    ix += 2;
-                    const double u = (double)(int64_t)(r >> 12) * (1.0 / 4503599627370496.0);
-                    if (lane == 0) inNC[q] = (u < rel[pNC[q]]);
+            for (uint32_t done = 0; done < n;) {
+                if (ix >= 351) {
+                    __threadfence_block();
+                    if (lane < 4) sfmt_regen4(g, lane);
+                    __threadfence_block();
+                    ix = 0;
                 }
+                uint32_t k = (352u - ix) >> 1;
+                k = k < 64u ? k : 64u;
+                k = k < n - done ? k : n - done;
+                if ((uint32_t)lane < k) {
+                    const uint32_t q = done + lane, w = ix + 2 * lane;
+                    const uint64_t r = (uint64_t)g[w] | ((uint64_t)g[w + 1] << 32);
+                    const double u = (double)(int64_t)(r >> 12) * (1.0 / 4503599627370496.0);
+                    inNC[(size_t)z * n + q] = (u < rel[pn[q]]);
+                }
+                ix += 2 * k;
+                done += k;
             }
-            ix = __shfl(ix, 0, 64);
-            __threadfence_block();
-            for (int i = lane; i < 352; i += 64) g[i] = s_rng[i];
             if (lane == 0) g[352] = ix;
-            __threadfence_block();
+        }
+        __threadfence_block();
+        for (uint32_t i = lane; i < nZ * 353; i += 64) gp(a.rngState)[i] = s_rng[i];
+    }
+    // GO->GO synaptic depression scale (innet.cpp:952-954), after the GABA
+    // gather above has used the previous scale
+    {
+        auto scale = F<float>(a, CBM_F_GO_GABA_OUT_SYNSCALE_GOGO);
+        const float rec = p.goGABAGOGOSynRec, depF = p.goGABAGOGOSynDepF;
+        for (uint32_t i = tid; i < s.numGO; i += nThr) {
+            const float sc = s_scale[i];
+            const uint8_t ap = s_apGO[i];
+            scale[i] = sc + (!ap) * (1 - sc) * rec - ap * (sc * (1 - depF));
         }
     }
     // MZone::updateMFNCSyn weights (mzone.cpp:586-600)
-    for (uint32_t z = 0; z < nZ; z++) {
-        if (t % p.tsPerPopHistBinPC != 0) continue;
-        const int doLTD = s_doLTD[z], doLTP = s_doLTP[z];
-        const size_t n = (size_t)s.numNC * s.numpNCfromMFtoNC;
-        const uint32_t *pMF = F<uint32_t>(a, CBM_F_P_NC_FROM_MF_TO_NC) + z * n;
-        float *w = F<float>(a, CBM_F_MF_SYNW_NC) + z * n;
-        const uint8_t *histMF = F<uint8_t>(a, CBM_F_HIST_MF);
-        for (size_t q = tid; q < n; q += nThr) {
-            const float delta = histMF[pMF[q]] * (doLTD * p.synLTDStepSizeMFtoNC + doLTP * p.synLTPStepSizeMFtoNC);
+    if (st.popHist) {
+        const uint32_t per = s.numNC * kMF;
+        auto w = F<float>(a, CBM_F_MF_SYNW_NC);
+        for (uint32_t q = tid; q < nZ * per; q += nThr) {
+            const uint32_t z = q / per;
+            const int doLTD = s_doLTD[z], doLTP = s_doLTP[z];
+            const float delta = s_hmf[s_idx[L.iMFNC + q]] * (doLTD * p.synLTDStepSizeMFtoNC + doLTP * p.synLTPStepSizeMFtoNC);
             float wv = w[q];
             wv = wv + delta;
             wv = (wv > 0) * wv;
@@ -730,19 +937,35 @@ __global__ void __launch_bounds__(kSmallThreads) small_step_kernel(const SmallAr
             w[q] = wv;
         }
     }
-    __syncthreads();
-
-    // ======================= phase 4: MF history reset, next ring slot
-    if (t % p.numTSinMFHist == 0) {
-        uint8_t *histMF = F<uint8_t>(a, CBM_F_HIST_MF);
-        for (uint32_t i = tid; i < s.numMF; i += nThr) histMF[i] = 0;
-    }
+    // MF history, cleared every numTSinMFHist steps (innet.cpp:982-995)
     {
-        const uint32_t nx = (uint32_t)((t + 1) % kRing);
-        for (uint32_t i = tid; i < s.numBC; i += nThr) a.bcRing[(size_t)nx * s.numBC + i] = 0;
-        for (uint32_t i = tid; i < s.numSC; i += nThr) a.scRing[(size_t)nx * s.numSC + i] = 0;
-        for (uint32_t i = tid; i < s.numGO; i += nThr) a.goRing[(size_t)nx * s.numGO + i] = 0;
+        auto histMF = F<uint8_t>(a, CBM_F_HIST_MF);
+        for (uint32_t i = tid; i < s.numMF; i += nThr) histMF[i] = st.resetMFHist ? 0 : s_hmf[i];
     }
 }
+
+// the small populations alone (the two-stream step, and per-kernel timing)
+__global__ void __launch_bounds__(kSmallThreads) small_step_kernel(const SmallArgs *__restrict__ args,
+                                                                   const SmallStep st)
+{
+    extern __shared__ __align__(16) uint8_t lds[];
+    small_step(*args, st, lds);
+}
+
+// One launch per step: block 0 steps the small populations, blocks 1.. are GR
+// tiles of 1024 threads (one block per CU: the small step needs up to 128
+// VGPRs per thread).
+template <int CPT>
+__global__ void __launch_bounds__(kSmallThreads) step_kernel(const SmallArgs *__restrict__ args, const SmallStep st, const GrArgs g)
+{
+    extern __shared__ __align__(16) uint8_t lds[];
+    if (blockIdx.x == 0)
+        small_step(*args, st, lds);
+    else
+        gr_tile<CPT, kSmallThreads>(g, blockIdx.x - 1, reinterpret_cast<uint32_t *>(lds));
+}
+
+template __global__ void step_kernel<1>(const SmallArgs *, const SmallStep, const GrArgs);
+template __global__ void step_kernel<2>(const SmallArgs *, const SmallStep, const GrArgs);
 
 }  // namespace cbmdev

@@ -17,9 +17,13 @@
 #include "cbm_dev.hpp"
 
 namespace cbmdev {
-template <int CPT>
+template <int CPT, int NT>
 __global__ void gr_step_kernel(const GrArgs a);
+template <int CPT>
+__global__ void step_kernel(const SmallArgs *args, const SmallStep st, const GrArgs g);
 __global__ void small_step_kernel(const SmallArgs *args, const SmallStep st);
+__global__ void pfpc_fold_kernel(const float *chunk, float *out, uint32_t nChunks, uint32_t perPC, uint32_t numPC,
+                                 uint32_t tasks);
 }  // namespace cbmdev
 
 using namespace cbmdev;
@@ -56,6 +60,7 @@ bool deviceField(int f)
     case CBM_F_P_GO_COUP_IN_GOGO: case CBM_F_SUM_GR_INPUT_GO: case CBM_F_SUM_GO_INPUT_GO:
         return true;
     case CBM_F_AP_GR: case CBM_F_INPUT_SUM_PF_PC: case CBM_F_INPUT_SUM_PF_SC: case CBM_F_ERR_DRIVE:
+    case CBM_F_AP_GO: case CBM_F_PFPC_PLAST_TIMER_IO:  // double-buffered by step parity
         return false;
     default:
         return kind == CBM_KIND_STATE;
@@ -68,9 +73,15 @@ struct cbm_handle {
     cbm_sizes s{};
     cbm_act_params p{};
     cbm_options o{};
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;   // GR kernel, host copies; joined with stream2 after every call
+    hipStream_t stream2 = nullptr;  // small-population kernel
+    hipEvent_t evG = nullptr;       // recorded on stream after each GR kernel (and host copies)
+    hipEvent_t evS = nullptr;       // recorded on stream2 after each small kernel
     bool committed = false;
-    int cpt = 4;
+    int cpt = 4, nt = 256;   // GR kernel shape of the two-stream step
+    int fusedCpt = 1;        // GR cells per thread of the fused step
+    int mode = 0;            // 1 fused, 2 two streams (decided at commit)
+    size_t fusedLds = 0;     // dynamic LDS of the fused step kernel
 
     std::vector<uint8_t> host[CBM_F_COUNT];  // staged fields, zones stacked
     bool written[CBM_F_COUNT] = {};
@@ -79,14 +90,16 @@ struct cbm_handle {
     uint2 *mfIn = nullptr, *goIn = nullptr, *goOut = nullptr;
     uint32_t *goOutDly = nullptr;
     uint8_t *bspDly = nullptr;
-    float *chunk = nullptr, *pfpcRing = nullptr;
+    float *chunk2 = nullptr, *pfpcRing = nullptr;  // chunk sums [2][nZ][numGR/64] by step parity
     uint32_t *bcRing = nullptr, *scRing = nullptr, *goRing = nullptr, *rngState = nullptr;
+    uint8_t *apGO2 = nullptr;   // [2][numGO]
+    int32_t *timer2 = nullptr;  // [2][nZ][numIO]
+    bool skipFold = true;       // the ring slot of the last step holds host-written PF->PC sums
     uint8_t *dmf = nullptr;
     uint8_t *pinnedMF = nullptr;
-    Csr mfToGoIn{}, gabaIn{}, bcToPcIn{}, pcToBcIn{};
     uint32_t anyCoupling = 0;
     SmallArgs *smallArgs = nullptr;  // device copy of the constant small-kernel arguments
-    size_t smallLds = 0;             // dynamic LDS of the small kernel (synapse partials)
+    size_t smallLds = 0;             // dynamic LDS of the small kernel
 
     uint64_t curTime = 0;
     float errDrive[kMaxZones] = {};
@@ -95,7 +108,7 @@ struct cbm_handle {
 
     std::vector<void *> allocs;
     size_t devBytes = 0;
-    hipEvent_t ev[2] = {};  // GR-kernel timing pair used by cbm_profile_steps
+    hipEvent_t ev[4] = {};  // timing events of cbm_profile_steps / cbm_profile_split
 
     size_t zoneBytes(int f) const { return cbm_field_bytes(f, &s); }
     int zonesOf(int f) const
@@ -149,19 +162,95 @@ int maskBit(uint32_t m, uint8_t *out)
     return CBM_OK;
 }
 
-int buildCsr(cbm_handle *h, uint32_t rows, const std::vector<std::vector<uint32_t>> &lists, Csr *out,
-             std::vector<uint32_t> *ptrAcc, std::vector<uint32_t> *idxAcc)
+uint32_t roundUp(uint32_t x, uint32_t m) { return (x + m - 1) / m * m; }
+
+uint32_t maxLen(const std::vector<std::vector<uint32_t>> &lists)
 {
-    (void)h;
-    (void)out;
-    uint32_t base = (uint32_t)idxAcc->size();
-    for (uint32_t r = 0; r < rows; r++) {
-        ptrAcc->push_back(base);
-        for (uint32_t v : lists[r]) idxAcc->push_back(v);
-        base = (uint32_t)idxAcc->size();
-    }
-    ptrAcc->push_back(base);
-    return CBM_OK;
+    size_t m = 0;
+    for (const auto &l : lists) m = std::max(m, l.size());
+    return (uint32_t)m;
+}
+
+// Lists of `rows` rows padded to `width` slots, slot-major (out[k*rows + r]),
+// kNoCell in unused slots.
+void appendEll(std::vector<uint16_t> *out, const std::vector<std::vector<uint32_t>> &lists, uint32_t width)
+{
+    const size_t rows = lists.size(), base = out->size();
+    out->resize(base + (size_t)width * rows, kNoCell);
+    for (size_t r = 0; r < rows; r++)
+        for (size_t k = 0; k < lists[r].size(); k++) (*out)[base + k * rows + r] = (uint16_t)lists[r][k];
+}
+
+// Dynamic LDS layout of the small kernel.  Every region starts on 16 bytes;
+// the u16 index blob holds, in order, P_NC_FROM_MF_TO_NC, P_PC_FROM_SC_TO_PC,
+// P_NC_FROM_PC_TO_NC, P_IO_FROM_NC_TO_IO, P_IO_IN_IOIO and the BC->PC and
+// PC->BC in-lists, each padded to 8 elements.
+SmallLds smallLayout(const cbm_sizes &s, uint32_t slotsBCPC, uint32_t slotsPCBC, bool coupling)
+{
+    SmallLds L{};
+    const uint32_t nZ = s.numZones;
+    uint32_t o = 0;
+    auto take = [&](uint32_t bytes) {
+        const uint32_t at = o;
+        o = roundUp(o + bytes, 16);
+        return at;
+    };
+    L.mf = take(s.numMF);
+    L.hmf = take(s.numMF);
+    L.goMF = take(4 * s.numGO);
+    L.apGO = take(s.numGO);
+    L.scaleGO = take(4 * s.numGO);
+    L.vGO = take(coupling ? 4 * s.numGO : 0);
+    uint32_t f = 0;  // floats
+    L.synPC = f; f = roundUp(f + nZ * s.numSC, 4);
+    L.synN = f;  f = roundUp(f + nZ * s.numNC * s.numpNCfromMFtoNC, 4);
+    L.synA = f;  f = roundUp(f + nZ * s.numNC * s.numpNCfromMFtoNC, 4);
+    L.synP = f;  f = roundUp(f + nZ * s.numNC * s.numpNCfromPCtoNC, 4);
+    L.synIO = f; f = roundUp(f + nZ * s.numIO * s.numpIOfromNCtoIO, 4);
+    L.syn = take(4 * f);
+    L.apPC = take(nZ * s.numPC);
+    L.apBC = take(nZ * s.numBC);
+    L.apSC = take(s.numSC);
+    L.apIO = take(nZ * s.numIO);
+    L.apNC = take(nZ * s.numNC);
+    L.vIO = take(4 * nZ * s.numIO);
+    L.rel = take(4 * nZ * s.numNC);
+    L.rng = take(4 * nZ * 353);
+    L.popStride = 3 + s.numPopHistBinsPC + 2;
+    L.pop = take(4 * nZ * L.popStride);
+    uint32_t e = 0;  // u16 elements
+    L.iMFNC = e; e = roundUp(e + nZ * s.numNC * s.numpNCfromMFtoNC, 8);
+    L.iSCPC = e; e = roundUp(e + nZ * s.numPC * s.numpPCfromSCtoPC, 8);
+    L.iPCNC = e; e = roundUp(e + nZ * s.numNC * s.numpNCfromPCtoNC, 8);
+    L.iNCIO = e; e = roundUp(e + nZ * s.numIO * s.numpIOfromNCtoIO, 8);
+    L.iIOIO = e; e = roundUp(e + nZ * s.numIO * s.numpIOInIOIO, 8);
+    L.iBCPC = e; e = roundUp(e + nZ * slotsBCPC * s.numPC, 8);
+    L.iPCBC = e; e = roundUp(e + nZ * slotsPCBC * s.numBC, 8);
+    L.slotsBCPC = slotsBCPC;
+    L.slotsPCBC = slotsPCBC;
+    L.idx16 = e / 8;
+    L.idx = take(2 * e);
+    L.total = o;
+    // thread offsets: consecutive wave-aligned ranges, smallest groups first,
+    // wrapping around the block (a group larger than the block covers it all)
+    auto place = [](std::initializer_list<std::pair<uint32_t *, uint32_t>> groups) {
+        uint32_t cur = 0;
+        for (auto &g : groups) {
+            *g.first = cur % kSmallThreads;
+            cur += roundUp(std::max<uint32_t>(g.second, 1), 64);
+        }
+    };
+    place({{&L.oSynIO, nZ * s.numIO * s.numpIOfromNCtoIO}, {&L.oSynPC, nZ * s.numNC * s.numpNCfromPCtoNC},
+           {&L.oSynSC, nZ * s.numSC}, {&L.oSynMF, nZ * s.numNC * s.numpNCfromMFtoNC}});
+    place({{&L.oIO, nZ * s.numIO}, {&L.oNC, nZ * s.numNC}, {&L.oPC, nZ * s.numPC}, {&L.oBC, nZ * s.numBC},
+           {&L.oSC, s.numSC}});
+    place({{&L.oRIO, nZ * s.numIO}, {&L.oRBCPC, nZ * s.numPC}, {&L.oRPCBC, nZ * s.numBC},
+           {&L.oRPCNC, nZ * s.numNC * s.numpNCfromPCtoNC}, {&L.oRSCPC, nZ * s.numPC * s.numpPCfromSCtoPC}});
+    L.l2PC = ilog2(s.numPC);
+    L.l2BC = ilog2(s.numBC);
+    L.l2IO = ilog2(s.numIO);
+    L.l2NC = ilog2(s.numNC);
+    return L;
 }
 
 int validateSizes(const cbm_sizes &s)
@@ -222,13 +311,23 @@ int cbm_create(const cbm_sizes *sizes, const cbm_act_params *params, const cbm_o
         delete h;
         return fail(CBM_E_ARG, "device ordinal out of range");
     }
-    h->cpt = h->o.grCellsPerThread ? (int)h->o.grCellsPerThread : 4;
-    if (h->cpt != 1 && h->cpt != 2 && h->cpt != 4) {
+    h->cpt = h->o.grCellsPerThread ? (int)h->o.grCellsPerThread : 1;
+    h->nt = h->o.grBlockThreads ? (int)h->o.grBlockThreads : 256;
+    // fused GR tiles: 2 cells per thread unless 1 is asked for (or the net is one tile)
+    h->fusedCpt = (h->o.grCellsPerThread == 1 || sizes->numGR < 2048) ? 1 : 2;
+    const bool shapeOk = (h->nt == 256 && (h->cpt == 1 || h->cpt == 2 || h->cpt == 4)) ||
+                         (h->nt == 1024 && (h->cpt == 1 || h->cpt == 2));
+    if (!shapeOk || h->o.stepMode > 2) {
         delete h;
-        return fail(CBM_E_ARG, "grCellsPerThread must be 1, 2 or 4");
+        return fail(CBM_E_ARG, "GR kernel shape must be 256 threads x 1, 2 or 4 cells or 1024 x 1 or 2; stepMode 0..2");
     }
     hipError_t e = hipSetDevice(h->o.device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&h->evG, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&h->evS, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventRecord(h->evG, h->stream);
+    if (e == hipSuccess) e = hipEventRecord(h->evS, h->stream2);
     if (e == hipSuccess) e = hipHostMalloc((void **)&h->pinnedMF, h->s.numMF, hipHostMallocDefault);
     if (e != hipSuccess) {
         std::string m = hipGetErrorString(e);
@@ -253,9 +352,13 @@ int cbm_destroy(cbm_handle *h)
     if (!h) return fail(CBM_E_ARG, "null handle");
     (void)hipSetDevice(h->o.device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
+    if (h->stream2) (void)hipStreamSynchronize(h->stream2);
     for (void *p : h->allocs) (void)hipFree(p);
     if (h->pinnedMF) (void)hipHostFree(h->pinnedMF);
+    if (h->evG) (void)hipEventDestroy(h->evG);
+    if (h->evS) (void)hipEventDestroy(h->evS);
     if (h->stream) (void)hipStreamDestroy(h->stream);
+    if (h->stream2) (void)hipStreamDestroy(h->stream2);
     delete h;
     return CBM_OK;
 }
@@ -275,6 +378,40 @@ static int fieldCheck(cbm_handle *h, int field, int zone, size_t bytes, int *zon
 static uint32_t ringSlot(const cbm_handle *h, int r)
 {
     return (uint32_t)((h->curTime + (uint64_t)kRing * 4 - (uint64_t)r) % kRing);
+}
+
+// wait for both streams
+static int syncAll(cbm_handle *h)
+{
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream2));
+    return CBM_OK;
+}
+
+// device address of a parity-buffered field as of the last completed step
+static uint8_t *parityField(cbm_handle *h, int field, int zone)
+{
+    const uint32_t par = (uint32_t)(h->curTime & 1u);
+    if (field == CBM_F_AP_GO) return h->apGO2 + (size_t)par * h->s.numGO;
+    if (field == CBM_F_PFPC_PLAST_TIMER_IO)
+        return (uint8_t *)(h->timer2 + ((size_t)par * h->s.numZones + zone) * h->s.numIO);
+    return nullptr;
+}
+
+// PF->PC sums of the last step still sit as chunk sums until the next small
+// kernel folds them; fold them into their ring slot now (same order as that fold)
+static int foldLastStep(cbm_handle *h)
+{
+    if (h->skipFold) return CBM_OK;  // the slot holds host-written sums
+    const cbm_sizes &s = h->s;
+    const uint32_t nChunks = s.numGR / 64, tasks = s.numZones * s.numPC;
+    const float *c = h->chunk2 + (size_t)(h->curTime & 1u) * s.numZones * nChunks;
+    float *out = h->pfpcRing + (size_t)ringSlot(h, 0) * s.numZones * s.numPC;
+    hipLaunchKernelGGL(pfpc_fold_kernel, dim3((tasks + 3) / 4), dim3(256), 0, h->stream, c, out, nChunks,
+                       s.numpPCfromGRtoPC / 64, s.numPC, tasks);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return CBM_OK;
 }
 
 int cbm_write(cbm_handle *h, int field, int zone, const void *src, size_t bytes)
@@ -300,7 +437,11 @@ int cbm_write(cbm_handle *h, int field, int zone, const void *src, size_t bytes)
     }
     if (kind == CBM_KIND_CON && field != CBM_F_ZONE_RNG_STATE)
         return fail(CBM_E_STATE, "connectivity cannot change after cbm_commit");
-    HIP_TRY(hipStreamSynchronize(h->stream));
+    if ((rc = syncAll(h))) return rc;
+    if (uint8_t *d = parityField(h, field, zone)) {
+        HIP_TRY(hipMemcpy(d, src, bytes, hipMemcpyHostToDevice));
+        return CBM_OK;
+    }
     switch (field) {
     case CBM_F_AP_GR: case CBM_F_INPUT_SUM_PF_PC: case CBM_F_INPUT_SUM_PF_SC:
         return CBM_OK;  // derived from other state; accepted and ignored
@@ -315,7 +456,15 @@ int cbm_write(cbm_handle *h, int field, int zone, const void *src, size_t bytes)
         if ((rc = cbm_read(h, CBM_F_PIPE_PFBC_SUM, 0, bc.data(), bc.size()))) return rc;
         if ((rc = cbm_read(h, CBM_F_PIPE_PFSC_SUM, 0, sc.data(), sc.size()))) return rc;
         if ((rc = cbm_read(h, CBM_F_PIPE_GRGO_SUM, 0, go.data(), go.size()))) return rc;
+        std::vector<uint8_t> apgo(h->s.numGO), tim(h->host[CBM_F_PFPC_PLAST_TIMER_IO].size());
+        const size_t tz = h->zoneBytes(CBM_F_PFPC_PLAST_TIMER_IO);
+        if ((rc = cbm_read(h, CBM_F_AP_GO, 0, apgo.data(), apgo.size()))) return rc;
+        for (uint32_t z = 0; z < h->s.numZones; z++)
+            if ((rc = cbm_read(h, CBM_F_PFPC_PLAST_TIMER_IO, (int)z, tim.data() + z * tz, tz))) return rc;
         std::memcpy(&h->curTime, src, 8);
+        if ((rc = cbm_write(h, CBM_F_AP_GO, 0, apgo.data(), apgo.size()))) return rc;
+        for (uint32_t z = 0; z < h->s.numZones; z++)
+            if ((rc = cbm_write(h, CBM_F_PFPC_PLAST_TIMER_IO, (int)z, tim.data() + z * tz, tz))) return rc;
         for (uint32_t z = 0; z < h->s.numZones; z++)
             if ((rc = cbm_write(h, CBM_F_PIPE_PFPC_SUM, (int)z, pf.data() + z * pfz, pfz))) return rc;
         if ((rc = cbm_write(h, CBM_F_PIPE_PFBC_SUM, 0, bc.data(), bc.size()))) return rc;
@@ -329,15 +478,21 @@ int cbm_write(cbm_handle *h, int field, int zone, const void *src, size_t bytes)
         std::memcpy(&h->ltp[zone], (const uint8_t *)src + 4, 4);
         return CBM_OK;
     case CBM_F_MF_INPUT: HIP_TRY(hipMemcpy(h->dmf, src, bytes, hipMemcpyHostToDevice)); return CBM_OK;
+    case CBM_F_PC_POPACT_CTRS:
+        if (((const uint32_t *)src)[1] >= h->s.numPopHistBinsPC)
+            return fail(CBM_E_ARG, "PC_POPACT_CTRS[1] must be below numPopHistBinsPC");
+        break;
     case CBM_F_ZONE_RNG_STATE:
         if (((const uint32_t *)src)[352] > 352) return fail(CBM_E_ARG, "generator index above 352");
         HIP_TRY(hipMemcpy(h->rngState + (size_t)zone * 353, src, bytes, hipMemcpyHostToDevice));
         return CBM_OK;
     case CBM_F_PIPE_PFPC_SUM: {
         const size_t row = (size_t)h->s.numPC * 4;
+        if ((rc = foldLastStep(h))) return rc;  // other zones' row 0 keeps its value
         for (int r = 0; r < 2; r++)
             HIP_TRY(hipMemcpy(h->pfpcRing + ((size_t)ringSlot(h, r) * h->s.numZones + zone) * h->s.numPC,
                               (const uint8_t *)src + r * row, row, hipMemcpyHostToDevice));
+        h->skipFold = true;
         return CBM_OK;
     }
     case CBM_F_PIPE_PFBC_SUM: case CBM_F_PIPE_PFSC_SUM: case CBM_F_PIPE_GRGO_SUM: {
@@ -365,6 +520,8 @@ int cbm_commit(cbm_handle *h)
     HIP_TRY(hipSetDevice(h->o.device));
     int rc;
 
+    if (s.numGR % (uint32_t)(h->nt * h->cpt) || s.numGR % (1024u * (uint32_t)h->fusedCpt))
+        return fail(CBM_E_UNSUPPORTED, "numGR must be a multiple of the GR tile");
     // ---- granule connectivity: validate and pack
     const uint32_t nGR = s.numGR;
     std::vector<uint2> mfIn(nGR), goIn(nGR), goOut(nGR);
@@ -408,20 +565,26 @@ int cbm_commit(cbm_handle *h)
         }
     }
 
-    // ---- gather lists of the small kernel
-    std::vector<uint32_t> mfgPtr, mfgIdx, gabPtr, gabIdx, bcpPtr, bcpIdx, pcbPtr, pcbIdx;
+    // ---- connectivity of the small kernel
+    const uint32_t nZ = s.numZones;
+    std::vector<uint16_t> mfgoOut, gabaEll, blob;
+    uint32_t mfgoPad = 8, gabaPad = 4, slotsBCPC = 0, slotsPCBC = 0;
     {
-        std::vector<std::vector<uint32_t>> in(s.numGO), gab(s.numGO);
+        // MF -> GO targets per MF, row-major, padded to a multiple of 8 slots
         const int32_t *n = h->H<int32_t>(CBM_F_NUMP_MF_FROM_MF_TO_GO);
         const uint32_t *pp = h->H<uint32_t>(CBM_F_P_MF_FROM_MF_TO_GO);
+        mfgoPad = roundUp(std::max<uint32_t>(s.maxnumpMFfromMFtoGO, 1), 8);
+        mfgoOut.assign((size_t)s.numMF * mfgoPad, kNoCell);
         for (uint32_t m = 0; m < s.numMF; m++) {
             if (n[m] < 0 || (uint32_t)n[m] > s.maxnumpMFfromMFtoGO) return fail(CBM_E_ARG, "MF->GO count above maximum");
             for (int j = 0; j < n[m]; j++) {
                 uint32_t g = pp[(size_t)m * s.maxnumpMFfromMFtoGO + j];
                 if ((rc = checkIndex("P_MF_FROM_MF_TO_GO", g, s.numGO))) return rc;
-                in[g].push_back(m);
+                mfgoOut[(size_t)m * mfgoPad + j] = (uint16_t)g;
             }
         }
+        // GABA sources per GO, ascending source then slot: the reference's order
+        std::vector<std::vector<uint32_t>> gab(s.numGO);
         const int32_t *ng = h->H<int32_t>(CBM_F_NUMP_GO_GABA_OUT_GOGO);
         const uint32_t *pg = h->H<uint32_t>(CBM_F_P_GO_GABA_OUT_GOGO);
         for (uint32_t i = 0; i < s.numGO; i++) {
@@ -429,11 +592,13 @@ int cbm_commit(cbm_handle *h)
             for (int j = 0; j < ng[i]; j++) {
                 uint32_t g = pg[(size_t)i * s.maxnumpGOGABAOutGOGO + j];
                 if ((rc = checkIndex("P_GO_GABA_OUT_GOGO", g, s.numGO))) return rc;
-                gab[g].push_back(i);  // i ascending, j ascending: the reference's order
+                gab[g].push_back(i);
             }
         }
-        buildCsr(h, s.numGO, in, nullptr, &mfgPtr, &mfgIdx);
-        buildCsr(h, s.numGO, gab, nullptr, &gabPtr, &gabIdx);
+        gabaPad = roundUp(std::max<uint32_t>(maxLen(gab), 1), 4);
+        gabaEll.assign((size_t)s.numGO * gabaPad, kNoCell);
+        for (uint32_t g = 0; g < s.numGO; g++)
+            for (size_t k = 0; k < gab[g].size(); k++) gabaEll[(size_t)g * gabaPad + k] = (uint16_t)gab[g][k];
         const int32_t *nc = h->H<int32_t>(CBM_F_NUMP_GO_COUP_IN_GOGO);
         const uint32_t *pc = h->H<uint32_t>(CBM_F_P_GO_COUP_IN_GOGO);
         h->anyCoupling = 0;
@@ -444,58 +609,75 @@ int cbm_commit(cbm_handle *h)
             if (nc[i] > 0 && h->p.coupleRiRjRatioGO != 0.0f) h->anyCoupling = 1;
         }
     }
-    for (uint32_t z = 0; z < s.numZones; z++) {
-        std::vector<std::vector<uint32_t>> bp(s.numPC), pb(s.numBC);
-        const uint32_t *b = h->H<uint32_t>(CBM_F_P_BC_FROM_BC_TO_PC, z);
-        for (uint32_t i = 0; i < s.numBC; i++)
-            for (uint32_t j = 0; j < s.numpBCfromBCtoPC; j++) {
-                uint32_t pc = b[(size_t)i * s.numpBCfromBCtoPC + j];
-                if ((rc = checkIndex("P_BC_FROM_BC_TO_PC", pc, s.numPC))) return rc;
-                bp[pc].push_back(i);
+    {
+        // microzone lists: validate, then build the u16 blob in the LDS order
+        std::vector<std::vector<std::vector<uint32_t>>> bp(nZ), pb(nZ);
+        for (uint32_t z = 0; z < nZ; z++) {
+            bp[z].resize(s.numPC);
+            pb[z].resize(s.numBC);
+            const uint32_t *b = h->H<uint32_t>(CBM_F_P_BC_FROM_BC_TO_PC, z);
+            for (uint32_t i = 0; i < s.numBC; i++)
+                for (uint32_t j = 0; j < s.numpBCfromBCtoPC; j++) {
+                    uint32_t pc = b[(size_t)i * s.numpBCfromBCtoPC + j];
+                    if ((rc = checkIndex("P_BC_FROM_BC_TO_PC", pc, s.numPC))) return rc;
+                    bp[z][pc].push_back(i);
+                }
+            const uint32_t *q = h->H<uint32_t>(CBM_F_P_PC_FROM_PC_TO_BC, z);
+            for (uint32_t i = 0; i < s.numPC; i++)
+                for (uint32_t j = 0; j < s.numpPCfromPCtoBC; j++) {
+                    uint32_t bc = q[(size_t)i * s.numpPCfromPCtoBC + j];
+                    if ((rc = checkIndex("P_PC_FROM_PC_TO_BC", bc, s.numBC))) return rc;
+                    pb[z][bc].push_back(i);
+                }
+            slotsBCPC = std::max(slotsBCPC, maxLen(bp[z]));
+            slotsPCBC = std::max(slotsPCBC, maxLen(pb[z]));
+            struct { int f; uint32_t n; } chk[] = {
+                {CBM_F_P_PC_FROM_SC_TO_PC, s.numSC}, {CBM_F_P_NC_FROM_PC_TO_NC, s.numPC},
+                {CBM_F_P_NC_FROM_MF_TO_NC, s.numMF}, {CBM_F_P_IO_FROM_NC_TO_IO, s.numNC},
+                {CBM_F_P_IO_IN_IOIO, s.numIO}};
+            for (auto &c : chk) {
+                const uint32_t *v = h->H<uint32_t>(c.f, z);
+                size_t cnt = h->zoneBytes(c.f) / 4;
+                for (size_t k = 0; k < cnt; k++)
+                    if ((rc = checkIndex("microzone connectivity", v[k], c.n))) return rc;
             }
-        const uint32_t *q = h->H<uint32_t>(CBM_F_P_PC_FROM_PC_TO_BC, z);
-        for (uint32_t i = 0; i < s.numPC; i++)
-            for (uint32_t j = 0; j < s.numpPCfromPCtoBC; j++) {
-                uint32_t bc = q[(size_t)i * s.numpPCfromPCtoBC + j];
-                if ((rc = checkIndex("P_PC_FROM_PC_TO_BC", bc, s.numBC))) return rc;
-                pb[bc].push_back(i);
-            }
-        buildCsr(h, s.numPC, bp, nullptr, &bcpPtr, &bcpIdx);
-        buildCsr(h, s.numBC, pb, nullptr, &pcbPtr, &pcbIdx);
-        struct { int f; uint32_t n; } chk[] = {
-            {CBM_F_P_PC_FROM_SC_TO_PC, s.numSC}, {CBM_F_P_NC_FROM_PC_TO_NC, s.numPC},
-            {CBM_F_P_NC_FROM_MF_TO_NC, s.numMF}, {CBM_F_P_IO_FROM_NC_TO_IO, s.numNC},
-            {CBM_F_P_IO_IN_IOIO, s.numIO}};
-        for (auto &c : chk) {
-            const uint32_t *v = h->H<uint32_t>(c.f, z);
-            size_t cnt = h->zoneBytes(c.f) / 4;
-            for (size_t k = 0; k < cnt; k++)
-                if ((rc = checkIndex("microzone connectivity", v[k], c.n))) return rc;
         }
+        const uint32_t *ctr = h->H<uint32_t>(CBM_F_PC_POPACT_CTRS);
+        for (uint32_t z = 0; z < nZ; z++)
+            if (ctr[z * 3 + 1] >= s.numPopHistBinsPC) return fail(CBM_E_ARG, "PC_POPACT_CTRS[1] must be below numPopHistBinsPC");
+        const SmallLds L = smallLayout(s, slotsBCPC, slotsPCBC, h->anyCoupling != 0);
+        blob.assign((size_t)L.idx16 * 8, kNoCell);
+        auto put = [&](uint32_t off, int f) {
+            const size_t per = h->zoneBytes(f) / 4;
+            for (uint32_t z = 0; z < nZ; z++) {
+                const uint32_t *v = h->H<uint32_t>(f, z);
+                for (size_t k = 0; k < per; k++) blob[off + z * per + k] = (uint16_t)v[k];
+            }
+        };
+        put(L.iMFNC, CBM_F_P_NC_FROM_MF_TO_NC);
+        put(L.iSCPC, CBM_F_P_PC_FROM_SC_TO_PC);
+        put(L.iPCNC, CBM_F_P_NC_FROM_PC_TO_NC);
+        put(L.iNCIO, CBM_F_P_IO_FROM_NC_TO_IO);
+        put(L.iIOIO, CBM_F_P_IO_IN_IOIO);
+        for (uint32_t z = 0; z < nZ; z++) {
+            std::vector<uint16_t> e1, e2;
+            appendEll(&e1, bp[z], slotsBCPC);
+            appendEll(&e2, pb[z], slotsPCBC);
+            std::copy(e1.begin(), e1.end(), blob.begin() + L.iBCPC + (size_t)z * slotsBCPC * s.numPC);
+            std::copy(e2.begin(), e2.end(), blob.begin() + L.iPCBC + (size_t)z * slotsPCBC * s.numBC);
+        }
+        if (L.total > kSmallLdsMax)
+            return fail(CBM_E_UNSUPPORTED, "small-population state exceeds the LDS of one workgroup");
+        h->smallLds = L.total;
     }
 
     // ---- device storage
     if ((rc = h->upload(&h->mfIn, mfIn)) || (rc = h->upload(&h->goIn, goIn)) || (rc = h->upload(&h->goOut, goOut)) ||
         (rc = h->upload(&h->goOutDly, goOutDly)) || (rc = h->upload(&h->bspDly, bsp)))
         return rc;
-    uint32_t *tmp;
-    if ((rc = h->upload(&tmp, mfgPtr))) return rc;
-    h->mfToGoIn.ptr = tmp;
-    if ((rc = h->upload(&tmp, mfgIdx))) return rc;
-    h->mfToGoIn.idx = tmp;
-    if ((rc = h->upload(&tmp, gabPtr))) return rc;
-    h->gabaIn.ptr = tmp;
-    if ((rc = h->upload(&tmp, gabIdx))) return rc;
-    h->gabaIn.idx = tmp;
-    // per-zone CSR pointers are stored with a per-zone index base
-    if ((rc = h->upload(&tmp, bcpPtr))) return rc;
-    h->bcToPcIn.ptr = tmp;
-    if ((rc = h->upload(&tmp, bcpIdx))) return rc;
-    h->bcToPcIn.idx = tmp;
-    if ((rc = h->upload(&tmp, pcbPtr))) return rc;
-    h->pcToBcIn.ptr = tmp;
-    if ((rc = h->upload(&tmp, pcbIdx))) return rc;
-    h->pcToBcIn.idx = tmp;
+    uint16_t *dMfgo = nullptr, *dGaba = nullptr, *dBlob = nullptr;
+    if ((rc = h->upload(&dMfgo, mfgoOut)) || (rc = h->upload(&dGaba, gabaEll)) || (rc = h->upload(&dBlob, blob)))
+        return rc;
 
     for (int f = 0; f < CBM_F_COUNT; f++) {
         if (!deviceField(f)) continue;
@@ -503,8 +685,14 @@ int cbm_commit(cbm_handle *h)
         if ((rc = h->dalloc(&h->dfield[f], b))) return rc;
         if (b) HIP_TRY(hipMemcpy(h->dfield[f], h->host[f].data(), b, hipMemcpyHostToDevice));
     }
-    const uint32_t nZ = s.numZones;
-    if ((rc = h->dalloc((void **)&h->chunk, (size_t)nZ * (nGR / 64) * 4))) return rc;
+    const uint32_t par = (uint32_t)(h->curTime & 1u);
+    if ((rc = h->dalloc((void **)&h->apGO2, 2 * (size_t)s.numGO))) return rc;
+    HIP_TRY(hipMemcpy(h->apGO2 + (size_t)par * s.numGO, h->host[CBM_F_AP_GO].data(), s.numGO, hipMemcpyHostToDevice));
+    const size_t timerBytes = h->host[CBM_F_PFPC_PLAST_TIMER_IO].size();
+    if ((rc = h->dalloc((void **)&h->timer2, 2 * timerBytes))) return rc;
+    HIP_TRY(hipMemcpy((uint8_t *)h->timer2 + par * timerBytes, h->host[CBM_F_PFPC_PLAST_TIMER_IO].data(), timerBytes,
+                      hipMemcpyHostToDevice));
+    if ((rc = h->dalloc((void **)&h->chunk2, 2 * (size_t)nZ * (nGR / 64) * 4))) return rc;
     if ((rc = h->dalloc((void **)&h->pfpcRing, (size_t)kRing * nZ * s.numPC * 4))) return rc;
     if ((rc = h->dalloc((void **)&h->bcRing, (size_t)kRing * s.numBC * 4))) return rc;
     if ((rc = h->dalloc((void **)&h->scRing, (size_t)kRing * s.numSC * 4))) return rc;
@@ -524,34 +712,41 @@ int cbm_commit(cbm_handle *h)
     {
         SmallArgs a;
         for (int f = 0; f < CBM_F_COUNT; f++) a.f[f] = h->dfield[f];
-        a.chunk = h->chunk;
+        a.chunk2 = h->chunk2;
         a.pfpcRing = h->pfpcRing;
         a.bcRing = h->bcRing;
         a.scRing = h->scRing;
         a.goRing = h->goRing;
         a.rngState = h->rngState;
-        a.mfToGoIn = h->mfToGoIn;
-        a.gabaIn = h->gabaIn;
-        a.bcToPcIn = h->bcToPcIn;
-        a.pcToBcIn = h->pcToBcIn;
+        a.apGO2 = h->apGO2;
+        a.timer2 = h->timer2;
+        a.idxBlob = reinterpret_cast<const uint4 *>(dBlob);
+        a.mfgoOut = dMfgo;
+        a.gabaEll = dGaba;
         a.coupN = (const int32_t *)h->dfield[CBM_F_NUMP_GO_COUP_IN_GOGO];
         a.coupP = (const uint32_t *)h->dfield[CBM_F_P_GO_COUP_IN_GOGO];
+        a.mfgoPad = mfgoPad;
+        a.gabaPad = gabaPad;
+        a.lds = smallLayout(s, slotsBCPC, slotsPCBC, h->anyCoupling != 0);
         a.s = s;
         a.p = h->p;
         a.grgoLag = h->o.grgoLag;
         a.anyCoupling = h->anyCoupling;
-        // synapse partials: SC->PC, MF->NC (NMDA, AMPA), PC->NC, NC->IO
-        h->smallLds = 4 * (size_t)nZ *
-                      (s.numSC + 2 * (size_t)s.numNC * s.numpNCfromMFtoNC + (size_t)s.numNC * s.numpNCfromPCtoNC +
-                       (size_t)s.numIO * s.numpIOfromNCtoIO);
-        if (h->smallLds > 150 * 1024)
-            return fail(CBM_E_UNSUPPORTED, "small-population synapse partials exceed the LDS of one workgroup");
         if (h->smallLds > 64 * 1024)
             HIP_TRY(hipFuncSetAttribute((const void *)small_step_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                         (int)h->smallLds));
+        // fused step: every block gets max(small, GR tile) LDS (one 1024-thread
+        // block per CU either way, the small step's registers decide that)
+        h->fusedLds = std::max<size_t>(h->smallLds, grTileLds(1024, h->fusedCpt, s.numGO, s.numMF, nZ, s.numIO));
+        h->mode = h->o.stepMode ? (int)h->o.stepMode : 1;
+        if (h->mode == 1 && h->fusedLds > 64 * 1024) {
+            const void *k = h->fusedCpt == 2 ? (const void *)step_kernel<2> : (const void *)step_kernel<1>;
+            HIP_TRY(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->fusedLds));
+        }
         if ((rc = h->dalloc((void **)&h->smallArgs, sizeof(SmallArgs)))) return rc;
         HIP_TRY(hipMemcpy(h->smallArgs, &a, sizeof(SmallArgs), hipMemcpyHostToDevice));
     }
+    h->skipFold = true;
     h->committed = true;
     // pipeline rows given before commit
     if (h->written[CBM_F_PIPE_PFPC_SUM]) {
@@ -581,7 +776,11 @@ int cbm_read(cbm_handle *h, int field, int zone, void *dst, size_t bytes)
         return CBM_OK;
     }
     HIP_TRY(hipSetDevice(h->o.device));
-    HIP_TRY(hipStreamSynchronize(h->stream));
+    if ((rc = syncAll(h))) return rc;
+    if (const uint8_t *d = parityField(h, field, zone)) {
+        HIP_TRY(hipMemcpy(dst, d, bytes, hipMemcpyDeviceToHost));
+        return CBM_OK;
+    }
     switch (field) {
     case CBM_F_CUR_TIME: std::memcpy(dst, &h->curTime, 8); return CBM_OK;
     case CBM_F_GINC_GR_TO_GO: std::memcpy(dst, &h->gIncGRtoGO, 4); return CBM_OK;
@@ -604,6 +803,7 @@ int cbm_read(cbm_handle *h, int field, int zone, void *dst, size_t bytes)
     case CBM_F_PIPE_PFPC_SUM: case CBM_F_INPUT_SUM_PF_PC: {
         const size_t row = (size_t)h->s.numPC * 4;
         const int r0 = field == CBM_F_PIPE_PFPC_SUM ? 0 : 1, nr = field == CBM_F_PIPE_PFPC_SUM ? 2 : 1;
+        if (r0 == 0 && (rc = foldLastStep(h))) return rc;
         for (int r = 0; r < nr; r++)
             HIP_TRY(hipMemcpy((uint8_t *)dst + r * row,
                               h->pfpcRing + ((size_t)ringSlot(h, r0 + r) * h->s.numZones + zone) * h->s.numPC, row,
@@ -633,9 +833,11 @@ int cbm_set_mf(cbm_handle *h, const uint8_t *mf)
     if (!h || !mf) return fail(CBM_E_ARG, "null argument");
     if (!h->committed) return cbm_write(h, CBM_F_MF_INPUT, 0, mf, h->s.numMF);
     HIP_TRY(hipSetDevice(h->o.device));
-    HIP_TRY(hipStreamSynchronize(h->stream));  // the pinned stage may still be in flight
+    int rc = syncAll(h);  // the pinned stage and the previous step may still be in flight
+    if (rc) return rc;
     std::memcpy(h->pinnedMF, mf, h->s.numMF);
     HIP_TRY(hipMemcpyAsync(h->dmf, h->pinnedMF, h->s.numMF, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipEventRecord(h->evG, h->stream));  // the next small kernel waits for the copy too
     return CBM_OK;
 }
 
@@ -674,12 +876,43 @@ int cbm_reset_ginc_grgo(cbm_handle *h)
     return CBM_OK;
 }
 
-static int launchStep(cbm_handle *h, const uint8_t *mf, bool timeGR)
+// Launch the GR tiles alone (nt x cpt shape) on `st`.
+static void launchGr(cbm_handle *h, const GrArgs &g, int nt, int cpt, hipStream_t st)
+{
+    const cbm_sizes &s = h->s;
+    const uint32_t blocks = s.numGR / (uint32_t)(nt * cpt);
+    const size_t lds = grTileLds(nt, cpt, s.numGO, s.numMF, s.numZones, s.numIO);
+    if (nt == 1024) {
+        if (cpt == 2) hipLaunchKernelGGL((gr_step_kernel<2, 1024>), dim3(blocks), dim3(1024), lds, st, g);
+        else hipLaunchKernelGGL((gr_step_kernel<1, 1024>), dim3(blocks), dim3(1024), lds, st, g);
+    } else if (cpt == 4) {
+        hipLaunchKernelGGL((gr_step_kernel<4, 256>), dim3(blocks), dim3(256), lds, st, g);
+    } else if (cpt == 2) {
+        hipLaunchKernelGGL((gr_step_kernel<2, 256>), dim3(blocks), dim3(256), lds, st, g);
+    } else {
+        hipLaunchKernelGGL((gr_step_kernel<1, 256>), dim3(blocks), dim3(256), lds, st, g);
+    }
+}
+
+enum Timing { kNoTiming = 0, kTimeKernel = 1, kSplit = 2 };
+
+// One step t.  The GR tiles of t need the GO spikes and plasticity timers of
+// t-1 (small populations of t-1); the small populations of t need the chunk
+// sums and ring slots of t-1 (GR tiles of t-1).  So the two halves of step t
+// are independent and run concurrently:
+//  fused (mode 1): one launch; block 0 is the small step, the rest GR tiles;
+//    consecutive launches on one stream are ordered by the stream
+//  two streams (mode 2):
+//    stream : wait(evS = small t-1) -> GR t -> record evG
+//    stream2: wait(evG = GR t-1)    -> small t -> record evS
+//  split timing: GR t then small t, in order on `stream`
+static int launchStep(cbm_handle *h, const uint8_t *mf, int timing)
 {
     const cbm_sizes &s = h->s;
     const cbm_act_params &p = h->p;
     const uint64_t t = ++h->curTime;
     const uint32_t slot = (uint32_t)(t % kRing);
+    const uint32_t par = (uint32_t)(t & 1u), parPrev = par ^ 1u;
 
     GrArgs g;
     g.v = (float *)h->dfield[CBM_F_VM_GR];
@@ -696,17 +929,23 @@ static int launchStep(cbm_handle *h, const uint8_t *mf, bool timeGR)
     g.goOutDly = h->goOutDly;
     g.bspDly = h->bspDly;
     g.mf = mf;
-    g.apGOprev = (const uint8_t *)h->dfield[CBM_F_AP_GO];
-    g.chunk = h->chunk;
+    g.apGOprev = h->apGO2 + (size_t)parPrev * s.numGO;
+    g.chunk = h->chunk2 + (size_t)par * s.numZones * (s.numGR / 64);
+    // the GR launch of step t folds the chunk sums of t-1 into their ring slot,
+    // unless the host wrote that slot
+    g.foldSrc = h->skipFold ? nullptr : h->chunk2 + (size_t)parPrev * s.numZones * (s.numGR / 64);
+    g.foldDst = h->pfpcRing + (size_t)((t + kRing - 1) % kRing) * s.numZones * s.numPC;
     g.bcCnt = h->bcRing + (size_t)slot * s.numBC;
     g.scCnt = h->scRing + (size_t)slot * s.numSC;
     g.goCnt = h->goRing + (size_t)slot * s.numGO;
-    g.plastTimer = (const int32_t *)h->dfield[CBM_F_PFPC_PLAST_TIMER_IO];
+    g.plastTimer = h->timer2 + (size_t)parPrev * s.numZones * s.numIO;
     g.numGR = s.numGR;
     g.numMF = s.numMF;
     g.numGO = s.numGO;
     g.numZones = s.numZones;
     g.numIO = s.numIO;
+    g.numPC = s.numPC;
+    g.perPC = s.numpPCfromGRtoPC / 64;
     g.log2Lbc = ilog2(s.numpBCfromGRtoBC);
     g.log2Lsc = ilog2(s.numpSCfromGRtoSC);
     g.log2GRperIO = ilog2(s.numGR / s.numIO);
@@ -727,28 +966,63 @@ static int launchStep(cbm_handle *h, const uint8_t *mf, bool timeGR)
         g.ltd[z] = h->ltd[z];
         g.ltp[z] = h->ltp[z];
     }
-    const int tileGR = 256 * h->cpt;
-    const uint32_t blocks = s.numGR / tileGR;
-    const size_t lds = (size_t)s.numGO * 4 + 2 * 4 * h->cpt * 4 + (size_t)s.numZones * s.numIO * 4 + s.numMF + s.numGO;
-    if (timeGR) HIP_TRY(hipEventRecord(h->ev[0], h->stream));
-    if (h->cpt == 4)
-        hipLaunchKernelGGL(gr_step_kernel<4>, dim3(blocks), dim3(256), lds, h->stream, g);
-    else if (h->cpt == 2)
-        hipLaunchKernelGGL(gr_step_kernel<2>, dim3(blocks), dim3(256), lds, h->stream, g);
-    else
-        hipLaunchKernelGGL(gr_step_kernel<1>, dim3(blocks), dim3(256), lds, h->stream, g);
-    HIP_TRY(hipGetLastError());
-    if (timeGR) HIP_TRY(hipEventRecord(h->ev[1], h->stream));
 
     SmallStep st;
     st.mf = mf;
     st.t = t;
     for (int z = 0; z < kMaxZones; z++) st.errDrive[z] = h->errDrive[z];
     st.gIncGRtoGO = h->gIncGRtoGO;
-    hipLaunchKernelGGL(small_step_kernel, dim3(1), dim3(kSmallThreads), h->smallLds, h->stream, h->smallArgs, st);
-    HIP_TRY(hipGetLastError());
+    st.par = par;
+    st.slotPrev = (uint32_t)((t + kRing - 1) % kRing);
+    st.slotT2 = (uint32_t)((t + kRing - 2) % kRing);
+    st.slotG = (uint32_t)((t + kRing - h->o.grgoLag) % kRing);
+    st.slotNext = (uint32_t)((t + 1) % kRing);
+    st.popHist = (t % p.tsPerPopHistBinPC) == 0;
+    st.resetMFHist = (t % p.numTSinMFHist) == 0;
+    h->skipFold = false;
+
+    if (timing == kSplit) {
+        HIP_TRY(hipEventRecord(h->ev[0], h->stream));
+        launchGr(h, g, h->mode == 1 ? 1024 : h->nt, h->mode == 1 ? h->fusedCpt : h->cpt, h->stream);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventRecord(h->ev[1], h->stream));
+        HIP_TRY(hipEventRecord(h->ev[2], h->stream));
+        hipLaunchKernelGGL(small_step_kernel, dim3(1), dim3(kSmallThreads), h->smallLds, h->stream, h->smallArgs, st);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventRecord(h->ev[3], h->stream));
+    } else if (h->mode == 1) {
+        const uint32_t tiles = s.numGR / (1024u * (uint32_t)h->fusedCpt);
+        if (timing == kTimeKernel) HIP_TRY(hipEventRecord(h->ev[0], h->stream));
+        if (h->fusedCpt == 2)
+            hipLaunchKernelGGL(step_kernel<2>, dim3(1 + tiles), dim3(kSmallThreads), h->fusedLds, h->stream,
+                               h->smallArgs, st, g);
+        else
+            hipLaunchKernelGGL(step_kernel<1>, dim3(1 + tiles), dim3(kSmallThreads), h->fusedLds, h->stream,
+                               h->smallArgs, st, g);
+        HIP_TRY(hipGetLastError());
+        if (timing == kTimeKernel) HIP_TRY(hipEventRecord(h->ev[1], h->stream));
+    } else {
+        HIP_TRY(hipStreamWaitEvent(h->stream, h->evS, 0));
+        if (timing == kTimeKernel) HIP_TRY(hipEventRecord(h->ev[0], h->stream));
+        launchGr(h, g, h->nt, h->cpt, h->stream);
+        HIP_TRY(hipGetLastError());
+        if (timing == kTimeKernel) HIP_TRY(hipEventRecord(h->ev[1], h->stream));
+        HIP_TRY(hipStreamWaitEvent(h->stream2, h->evG, 0));
+        hipLaunchKernelGGL(small_step_kernel, dim3(1), dim3(kSmallThreads), h->smallLds, h->stream2, h->smallArgs, st);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventRecord(h->evG, h->stream));
+        HIP_TRY(hipEventRecord(h->evS, h->stream2));
+    }
     // MZone::calcIOActivities clears errDrive after use (mzone.cpp:334)
     for (int z = 0; z < kMaxZones; z++) h->errDrive[z] = 0.0f;
+    return CBM_OK;
+}
+
+// make `stream` wait for the last small kernel, so that work queued on it
+// afterwards (and cbm_stream() users) sees the whole step
+static int join(cbm_handle *h)
+{
+    HIP_TRY(hipStreamWaitEvent(h->stream, h->evS, 0));
     return CBM_OK;
 }
 
@@ -758,10 +1032,10 @@ int cbm_step(cbm_handle *h, int n)
     if (!h->committed) return fail(CBM_E_STATE, "cbm_commit has not been called");
     HIP_TRY(hipSetDevice(h->o.device));
     for (int k = 0; k < n; k++) {
-        int rc = launchStep(h, h->dmf, false);
+        int rc = launchStep(h, h->dmf, kNoTiming);
         if (rc) return rc;
     }
-    return CBM_OK;
+    return join(h);
 }
 
 static int checkTrain(cbm_handle *h, const uint8_t *mf_dev, size_t pitch, int n)
@@ -789,10 +1063,13 @@ int cbm_run_mf_train(cbm_handle *h, const uint8_t *mf_dev, size_t pitch, int n, 
                 const float e = err[(size_t)k * h->s.numZones + z];
                 if (e != 0.0f) h->errDrive[z] = e * h->p.maxExtIncVIO;
             }
-        if ((rc = launchStep(h, mf_dev + (size_t)k * pitch, false))) return rc;
+        if ((rc = launchStep(h, mf_dev + (size_t)k * pitch, kNoTiming))) return rc;
     }
-    if (n > 0)
+    if ((rc = join(h))) return rc;
+    if (n > 0) {
         HIP_TRY(hipMemcpyAsync(h->dmf, mf_dev + (size_t)(n - 1) * pitch, h->s.numMF, hipMemcpyDeviceToDevice, h->stream));
+        HIP_TRY(hipEventRecord(h->evG, h->stream));
+    }
     return CBM_OK;
 }
 
@@ -800,49 +1077,82 @@ int cbm_sync(cbm_handle *h)
 {
     if (!h) return fail(CBM_E_ARG, "null handle");
     HIP_TRY(hipSetDevice(h->o.device));
-    HIP_TRY(hipStreamSynchronize(h->stream));
-    return CBM_OK;
+    return syncAll(h);
 }
 
 void *cbm_stream(cbm_handle *h) { return h ? (void *)h->stream : nullptr; }
 
-int cbm_profile_steps(cbm_handle *h, const uint8_t *mf_dev, size_t pitch, int n, double *grUs, double *stepUs)
+// n steps with event pairs around the timed launches (one pair per step, or
+// two in split mode), read back after the whole run so that timing inserts no
+// host waits between steps.  us[0], us[1]: mean durations of the pairs;
+// *stepUs: mean step time over the run.
+static int profileRun(cbm_handle *h, const uint8_t *mf_dev, size_t pitch, int n, int timing, double us[2],
+                      double *stepUs)
 {
     int rc = checkTrain(h, mf_dev, pitch, n);
     if (rc) return rc;
     if (n == 0) return fail(CBM_E_ARG, "n must be positive");
     HIP_TRY(hipSetDevice(h->o.device));
-    HIP_TRY(hipStreamSynchronize(h->stream));
-    // one event pair around every GR launch, read back after the whole run so
-    // that timing does not insert host waits between steps
-    std::vector<hipEvent_t> ev(2 * (size_t)n + 2);
+    if ((rc = syncAll(h))) return rc;
+    const int per = timing == kSplit ? 4 : 2;
+    std::vector<hipEvent_t> ev((size_t)per * n + 2);
     for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
-    HIP_TRY(hipEventRecord(ev[2 * n], h->stream));
+    HIP_TRY(hipStreamWaitEvent(h->stream, h->evS, 0));
+    HIP_TRY(hipEventRecord(ev[(size_t)per * n], h->stream));
     for (int k = 0; k < n; k++) {
-        h->ev[0] = ev[2 * k];
-        h->ev[1] = ev[2 * k + 1];
-        if ((rc = launchStep(h, mf_dev + (size_t)k * pitch, true))) break;
+        for (int e = 0; e < per; e++) h->ev[e] = ev[(size_t)per * k + e];
+        if ((rc = launchStep(h, mf_dev + (size_t)k * pitch, timing))) break;
     }
-    h->ev[0] = h->ev[1] = nullptr;
+    for (auto &e : h->ev) e = nullptr;
+    if (!rc) rc = join(h);
     if (!rc) {
-        HIP_TRY(hipEventRecord(ev[2 * n + 1], h->stream));
-        HIP_TRY(hipEventSynchronize(ev[2 * n + 1]));
-        double grTotal = 0;
-        for (int k = 0; k < n; k++) {
-            float ms = 0;
-            HIP_TRY(hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]));
-            grTotal += ms;
-        }
+        HIP_TRY(hipEventRecord(ev[(size_t)per * n + 1], h->stream));
+        HIP_TRY(hipEventSynchronize(ev[(size_t)per * n + 1]));
+        double tot[2] = {0, 0};
+        for (int k = 0; k < n; k++)
+            for (int j = 0; j < per / 2; j++) {
+                float ms = 0;
+                HIP_TRY(hipEventElapsedTime(&ms, ev[(size_t)per * k + 2 * j], ev[(size_t)per * k + 2 * j + 1]));
+                tot[j] += ms;
+            }
         float total = 0;
-        HIP_TRY(hipEventElapsedTime(&total, ev[2 * n], ev[2 * n + 1]));
-        if (grUs) *grUs = grTotal * 1000.0 / n;
+        HIP_TRY(hipEventElapsedTime(&total, ev[(size_t)per * n], ev[(size_t)per * n + 1]));
+        us[0] = tot[0] * 1000.0 / n;
+        us[1] = tot[1] * 1000.0 / n;
         if (stepUs) *stepUs = (double)total * 1000.0 / n;
         HIP_TRY(hipMemcpyAsync(h->dmf, mf_dev + (size_t)(n - 1) * pitch, h->s.numMF, hipMemcpyDeviceToDevice,
                                h->stream));
+        HIP_TRY(hipEventRecord(h->evG, h->stream));
     }
     HIP_TRY(hipStreamSynchronize(h->stream));
     for (auto &e : ev) (void)hipEventDestroy(e);
     return rc;
+}
+
+int cbm_profile_steps(cbm_handle *h, const uint8_t *mf_dev, size_t pitch, int n, double *kernelUs, double *stepUs)
+{
+    double us[2] = {0, 0};
+    const int rc = profileRun(h, mf_dev, pitch, n, kTimeKernel, us, stepUs);
+    if (!rc && kernelUs) *kernelUs = us[0];
+    return rc;
+}
+
+int cbm_profile_split(cbm_handle *h, const uint8_t *mf_dev, size_t pitch, int n, double *grUs, double *smallUs)
+{
+    double us[2] = {0, 0};
+    const int rc = profileRun(h, mf_dev, pitch, n, kSplit, us, nullptr);
+    if (!rc) {
+        if (grUs) *grUs = us[0];
+        if (smallUs) *smallUs = us[1];
+    }
+    return rc;
+}
+
+int cbm_step_mode(cbm_handle *h)
+{
+    if (!h) return fail(CBM_E_ARG, "null handle");
+    if (!h->committed) return fail(CBM_E_STATE, "cbm_commit has not been called");
+    return h->mode;
 }
 
 int cbm_field_info(int field, const cbm_sizes *s, size_t *elem, size_t *rows, size_t *cols, int *zoned,

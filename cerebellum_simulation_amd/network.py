@@ -110,10 +110,13 @@ class Network:
     """One network on one GPU behind the cbm_* C-ABI (CBMSimCore's role)."""
 
     def __init__(self, sizes: Sizes, params: ActParams, *, grgo_lag: int = 2, device: int = 0,
-                 cells_per_thread: int = 0):
+                 cells_per_thread: int = 0, block_threads: int = 0, step_mode: int = 0):
+        """step_mode: 0 = pick, 1 = one fused launch per step, 2 = GR and
+        small-population kernels on two streams (cbm_options.stepMode)."""
         L = _abi.hip_lib()
         o = Options()
         o.grgoLag, o.device, o.grCellsPerThread = grgo_lag, device, cells_per_thread
+        o.grBlockThreads, o.stepMode = block_threads, step_mode
         h = C.c_void_p()
         check(L.cbm_create(C.byref(sizes), C.byref(params), C.byref(o), C.byref(h)), "cbm_create")
         self._h = h
@@ -182,12 +185,29 @@ class Network:
         check(_abi.hip_lib().cbm_run_mf_train(self._h, ptr, pitch, n, err), "cbm_run_mf_train")
 
     def profile_steps(self, train, n: int | None = None) -> tuple[float, float]:
+        """(mean duration of the step's dominant kernel, mean step time) in us."""
         ptr, pitch, rows = _device_rows(train)
         n = rows if n is None else n
         gr, st = C.c_double(), C.c_double()
         check(_abi.hip_lib().cbm_profile_steps(self._h, ptr, pitch, n, C.byref(gr), C.byref(st)),
               "cbm_profile_steps")
         return gr.value, st.value
+
+    def profile_split(self, train, n: int | None = None) -> tuple[float, float]:
+        """(GR kernel, small-population kernel) mean durations in us, run one
+        after the other on one stream."""
+        ptr, pitch, rows = _device_rows(train)
+        n = rows if n is None else n
+        gr, sm = C.c_double(), C.c_double()
+        check(_abi.hip_lib().cbm_profile_split(self._h, ptr, pitch, n, C.byref(gr), C.byref(sm)),
+              "cbm_profile_split")
+        return gr.value, sm.value
+
+    @property
+    def step_mode(self) -> int:
+        m = _abi.hip_lib().cbm_step_mode(self._h)
+        check(min(m, 0), "cbm_step_mode")
+        return m
 
     def sync(self) -> None:
         check(_abi.hip_lib().cbm_sync(self._h), "cbm_sync")

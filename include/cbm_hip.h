@@ -42,7 +42,7 @@
 extern "C" {
 #endif
 
-#define CBM_ABI_VERSION 1
+#define CBM_ABI_VERSION 2
 
 /* ---------------------------------------------------------------- errors */
 #define CBM_OK 0
@@ -135,8 +135,14 @@ typedef struct cbm_options {
     uint32_t grgoLag;
     /* HIP device ordinal to run on. */
     int32_t device;
-    /* Granule cells per thread and threads per block of the GR kernel; 0 = pick. */
+    /* Granule cells per thread and threads per block of the GR kernel in the
+     * two-stream step; 0 = pick (256 x 1).  The fused step uses 1024-thread
+     * blocks with 2 cells per thread, or 1 when grCellsPerThread is 1. */
     uint32_t grCellsPerThread, grBlockThreads;
+    /* How a step is launched: 0 = pick (fused), 1 = one fused launch per step (block
+     * 0 steps the small populations, the other blocks are GR tiles), 2 = the GR
+     * and small-population kernels on two streams, ordered by events. */
+    uint32_t stepMode;
 } cbm_options;
 
 /* ---------------------------------------------------------------- fields */
@@ -392,12 +398,23 @@ int cbm_sync(cbm_handle *h);
 /* The HIP stream (hipStream_t) the handle launches on, as an opaque pointer. */
 void *cbm_stream(cbm_handle *h);
 
-/* Time the dominant (granule) kernel over the next `steps` steps with HIP
- * events on the handle's stream: runs them with the MF train as in
- * cbm_run_mf_train and reports the mean GR-kernel duration and the mean
- * whole-step duration in microseconds. */
+/* Time the dominant kernel over the next n steps with HIP events on the
+ * stream it is launched on: runs them with the MF train as in
+ * cbm_run_mf_train and reports the mean duration of the step kernel (fused
+ * mode) or of the GR kernel (two-stream mode), and the mean whole-step
+ * duration, in microseconds. */
 int cbm_profile_steps(cbm_handle *h, const uint8_t *mf_dev, size_t pitch, int n,
-                      double *gr_kernel_us, double *step_us);
+                      double *kernel_us, double *step_us);
+
+/* Run the next n steps as a GR-kernel launch followed by a small-population
+ * launch, in order on one stream, and report each kernel's mean duration in
+ * microseconds (HIP events).  The results equal those of the normal step. */
+int cbm_profile_split(cbm_handle *h, const uint8_t *mf_dev, size_t pitch, int n,
+                      double *gr_kernel_us, double *small_kernel_us);
+
+/* The launch mode the handle uses (1 = fused, 2 = two streams), or a negative
+ * CBM_E_* code. */
+int cbm_step_mode(cbm_handle *h);
 
 /* cbm_field_shape / cbm_field_is_float as exported functions, for hosts that
  * cannot use the inline versions (FFI bindings). */

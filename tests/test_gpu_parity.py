@@ -107,10 +107,11 @@ def _perturb(orc, st, rng, zones):
     orc.write("PIPE_GRGO_SUM", rng.integers(0, 40, (2, s.numGO), dtype=np.uint32))
 
 
+@pytest.mark.parametrize("mode", [1, 2])
 @pytest.mark.parametrize("cfg,warm", [("c0", 239), ("t8k2z", 119), ("t4k", 59)])
-def test_one_step_from_identical_state(gpu, cfg, warm):
+def test_one_step_from_identical_state(gpu, cfg, warm, mode):
     import orc as O
-    print(f"[case] one-step parity cfg={cfg} warm={warm}", flush=True)
+    print(f"[case] one-step parity cfg={cfg} warm={warm} mode={mode}", flush=True)
     st = configs.build_state(cfg, seed=0)
     z = st.num_zones
     rng = np.random.default_rng(7)
@@ -118,7 +119,8 @@ def test_one_step_from_identical_state(gpu, cfg, warm):
     tr = configs.mf_train(st.sizes.numMF, warm + 1, seed=2)
     o32.run_mf_train(tr[:warm])
     _perturb(o32, st, rng, z)
-    net = Network.from_state(st)
+    net = Network.from_state(st, step_mode=mode)
+    assert net.step_mode == mode
     transfer(o32, net, z)
     rep = {}
     assert_parity(compare(net, o32, z, rep), where="after transfer")
@@ -154,11 +156,12 @@ def test_one_step_from_identical_state(gpu, cfg, warm):
     assert diff <= max(2, sp_gpu.size // 100000)
 
 
-def test_multistep_from_initial_state(gpu):
+@pytest.mark.parametrize("mode", [1, 2])
+def test_multistep_from_initial_state(gpu, mode):
     import orc as O
-    print("[case] multistep c0 600 steps", flush=True)
+    print(f"[case] multistep c0 600 steps mode={mode}", flush=True)
     st = configs.build_state("c0", seed=0)
-    net = Network.from_state(st)
+    net = Network.from_state(st, step_mode=mode)
     o32 = O.Oracle.from_state(st, precision=32)
     assert np.array_equal(net.read("ZONE_RNG_STATE"), o32.read("ZONE_RNG_STATE"))
     steps = 600
@@ -180,11 +183,12 @@ def test_multistep_from_initial_state(gpu):
     assert net.read("APBUF_GO").any()
 
 
-def test_1m_gr_steps(gpu):
+@pytest.mark.parametrize("mode", [1, 2])
+def test_1m_gr_steps(gpu, mode):
     import orc as O
-    print("[case] c1 parity 12 steps", flush=True)
+    print(f"[case] c1 parity 12 steps mode={mode}", flush=True)
     st = configs.build_state("c1", seed=0)
-    net = Network.from_state(st)
+    net = Network.from_state(st, step_mode=mode)
     o32 = O.Oracle.from_state(st, precision=32)
     tr = configs.mf_train(st.sizes.numMF, 12, seed=3)
     # warm the GR population so the outputs are not all zero
@@ -219,11 +223,12 @@ def _direct_counts(net, st):
     return bc, sc, go, pc
 
 
-def test_4m_gr_properties(gpu):
-    print("[case] c2 full-size properties", flush=True)
+@pytest.mark.parametrize("mode", [0, 1])
+def test_4m_gr_properties(gpu, mode):
+    print(f"[case] c2 full-size properties mode={mode}", flush=True)
     import torch
     st = configs.build_state("c2", seed=0)
-    net = Network.from_state(st)
+    net = Network.from_state(st, step_mode=mode)
     tr = configs.mf_train(st.sizes.numMF, 64, seed=4)
     v = net.read("VM_GR")
     v[::13] = -30.0
@@ -236,12 +241,46 @@ def test_4m_gr_properties(gpu):
     assert np.array_equal(net.read("PIPE_GRGO_SUM")[0], go)
     assert go.sum() > 0 and bc.sum() > 0
     assert np.allclose(net.read("PIPE_PFPC_SUM", 0)[0], pc, rtol=1e-5, atol=1e-6)
-    # determinism: a second network from the same state gives identical results
-    net2 = Network.from_state(st)
+    # determinism: a second network from the same state, launched the other
+    # way, gives identical results
+    net2 = Network.from_state(st, step_mode=2 if net.step_mode == 1 else 1)
     net2.write("VM_GR", v)
     net2.run_mf_train(torch.from_numpy(tr).cuda())
     for name in ("APBUF_GR", "VM_GR", "VM_GO", "PIPE_PFPC_SUM", "HIST_GR"):
         assert np.array_equal(net.read(name), net2.read(name)), name
+
+
+SHAPES = [dict(step_mode=1), dict(step_mode=1, cells_per_thread=1), dict(step_mode=2),
+          dict(step_mode=2, cells_per_thread=2), dict(step_mode=2, cells_per_thread=4),
+          dict(step_mode=2, block_threads=1024, cells_per_thread=1),
+          dict(step_mode=2, block_threads=1024, cells_per_thread=2)]
+
+
+def test_launch_shapes_agree(gpu):
+    """Every launch mode and GR tile shape, and the split-timing path, give
+    bit-identical networks (the chunk sums and counts do not depend on the
+    tiling)."""
+    print("[case] launch shapes", flush=True)
+    import torch
+    st = configs.build_state("t8k2z", seed=3)
+    tr = torch.from_numpy(configs.mf_train(st.sizes.numMF, 120, seed=9)).cuda()
+    err = configs.us_err_drive(120, st.num_zones, t0=0, trial_ms=120, us_ms=30)
+    ref = None
+    for shape in SHAPES + ["split"]:
+        net = Network.from_state(st, **(shape if shape != "split" else {}))
+        if shape == "split":
+            net.run_mf_train(tr[:60], err_drive=err[:60])
+            net.profile_split(tr[60:])
+        else:
+            net.run_mf_train(tr, err_drive=err)
+        got = {(f, z): net.read(f, z) for f in ("APBUF_GR", "VM_GR", "PFPC_SYNW", "VM_GO", "APBUF_NC",
+                                                 "PIPE_PFPC_SUM", "PIPE_GRGO_SUM", "ZONE_RNG_STATE", "VM_IO")
+               for z in range(st.num_zones if net.info(f)["zoned"] else 1)}
+        if ref is None:
+            ref = got
+            continue
+        for k, v in got.items():
+            assert np.array_equal(v, ref[k]), (shape, k)
 
 
 def test_grgo_lag_fresh(gpu):
