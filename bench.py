@@ -18,6 +18,11 @@ GR and small-population kernels launched one after the other are timed too
 
 cpu_baseline: the CPU oracle (a C restatement of the same step, one thread),
 timed on this host over a bounded sample of the same workload.
+
+batched: the 256-network ragged pack (64 x 16K + 96 x 65K + 64 x 262K + 32 x
+1M GR, each with its own seed, CS subset and GO-GO inhibition on or off)
+stepped by one launch per ms, in network-ms per wall-second, against the same
+networks run one at a time on the same GPU ("plain").
 """
 import argparse
 import json
@@ -64,6 +69,57 @@ def cpu_baseline(cfg: str, sample_steps: int):
                       f" with {os.cpu_count()} logical CPUs"}
 
 
+def batch_bench(which: str, steps: int) -> dict:
+    """Network-ms/s of the ragged batch (one launch per step for all networks)
+    and of the same networks stepped one at a time."""
+    import torch
+
+    from cerebellum_simulation_amd import Batch, Network, configs
+
+    mix = configs.BATCH_MIX if which == "full" else [(c, max(1, n // 8)) for c, n in configs.BATCH_MIX]
+    plan = configs.batch_plan(mix)
+    t_build = time.perf_counter()
+    nets, trains = [], []
+    warm, plain_steps = 10, max(20, steps // 4)
+    total = warm + 2 * 8 + steps + plain_steps
+    for i, (cls, seed, inh) in enumerate(plan):
+        st = configs.build_state(cls, seed=seed, gogo_inhibition=inh)
+        nets.append(Network.from_state(st))
+        trains.append(torch.from_numpy(configs.mf_train(st.sizes.numMF, total, seed=seed)).to("cuda:0"))
+    gr_total = sum(n.sizes.numGR for n in nets)
+    print(f"[bench] batch of {len(nets)} networks ({gr_total} GR) built in {time.perf_counter() - t_build:.1f} s",
+          file=sys.stderr, flush=True)
+    b = Batch(nets)
+    b.run_mf_train([t[:warm] for t in trains])
+    cpt = b.autotune([t[warm:warm + 8] for t in trains], steps=8)
+    k0 = warm + 8
+    b.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.run_mf_train([t[k0:k0 + steps] for t in trains])
+    b.sync()
+    el = time.perf_counter() - t0
+    batched = len(nets) * steps / el
+    k1 = k0 + steps
+    # the same networks, one at a time (each network's own fused launch per step)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for n, t in zip(nets, trains):
+        n.run_mf_train(t[k1:k1 + plain_steps])
+        n.sync()
+    el_plain = time.perf_counter() - t0
+    plain = len(nets) * plain_steps / el_plain
+    alg = sum(n.sizes.numGR * bytes_per_gr(n.sizes.maxnumpGRfromGRtoGO) for n in nets)
+    step_us = el / steps * 1e6
+    b.close()
+    return {"networks": len(nets), "gr_total": gr_total, "mix": [list(m) for m in mix],
+            "value": round(batched, 1), "unit": "network-ms/s", "plain": round(plain, 1),
+            "speedup_vs_plain": round(batched / plain, 2), "steps": steps, "plain_steps": plain_steps,
+            "step_us": round(step_us, 2), "cells_per_thread": cpt,
+            "roofline_frac": round(alg / (step_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
+            "algorithmic_bytes_per_step": alg}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -76,6 +132,9 @@ def main():
     ap.add_argument("--step-mode", type=int, default=0, help="0 pick, 1 fused launch, 2 two streams")
     ap.add_argument("--cells-per-thread", type=int, default=0)
     ap.add_argument("--block-threads", type=int, default=0)
+    ap.add_argument("--batch", default="full", choices=["full", "small", "none"],
+                    help="ragged batch to measure beside the single network (small: 1/8 of each class)")
+    ap.add_argument("--batch-steps", type=int, default=200)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB, 5 times (the known-size kernel that scales "
                          "the FETCH_SIZE / WRITE_SIZE counters)")
@@ -143,6 +202,11 @@ def main():
         with open(traffic_file) as f:
             roofline["traffic"] = json.load(f).get("hbm_bytes_per_launch")
 
+    batched = None
+    if args.batch != "none":
+        del dtr
+        batched = batch_bench(args.batch, args.batch_steps)
+
     cpu = None
     if not args.no_cpu:
         cpu_steps = args.cpu_steps or {"c0": 5000, "c1": 2000, "c2": 400}.get(args.config, 100)
@@ -156,7 +220,7 @@ def main():
         "config": {"workload": WORKLOADS.get(args.config, args.config), "name": args.config,
                    "numGR": s.numGR, "numGO": s.numGO, "numMF": s.numMF, "numPC": s.numPC,
                    "parallelism": "single GPU", "step_mode": "fused" if fused else "two streams"},
-        "roofline": roofline, "cpu_baseline": cpu,
+        "roofline": roofline, "cpu_baseline": cpu, "batched": batched,
     }
     print(json.dumps(out), flush=True)
 

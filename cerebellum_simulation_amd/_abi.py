@@ -71,6 +71,10 @@ class Options(C.Structure):
     _fields_ = _parse_struct(_TEXT, "cbm_options")
 
 
+class BatchOptions(C.Structure):
+    _fields_ = _parse_struct(_TEXT, "cbm_batch_options")
+
+
 class GenSpec(C.Structure):
     _fields_ = [("kind", C.c_int32), ("numZones", C.c_uint32), ("innetSeed", C.c_int32),
                 ("nMFperGR", C.c_uint32), ("nGOperGR", C.c_uint32), ("nGOoutPerGR", C.c_uint32),
@@ -132,6 +136,15 @@ def hip_lib() -> C.CDLL:
         L.cbm_profile_steps.argtypes = [P, V, S, I, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.cbm_profile_split.argtypes = [P, V, S, I, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.cbm_step_mode.argtypes = [P]
+        PP = C.POINTER(C.c_void_p)
+        SP = C.POINTER(C.c_size_t)
+        L.cbm_batch_create.argtypes = [PP, I, C.POINTER(BatchOptions), C.POINTER(P)]
+        L.cbm_batch_destroy.argtypes = [P]
+        L.cbm_batch_run_mf_train.argtypes = [P, PP, SP, I, PP]
+        L.cbm_batch_sync.argtypes = [P]
+        L.cbm_batch_profile.argtypes = [P, PP, SP, I, C.POINTER(C.c_double)]
+        L.cbm_batch_autotune.argtypes = [P, PP, SP, I, C.POINTER(I)]
+        L.cbm_batch_info.argtypes = [P, C.POINTER(I), C.POINTER(I), C.POINTER(I)]
         L.cbm_device_bytes.argtypes = [P, C.POINTER(C.c_size_t)]
         L.cbm_field_info.argtypes = [I, C.POINTER(Sizes), C.POINTER(S), C.POINTER(S), C.POINTER(S),
                                      C.POINTER(I), C.POINTER(I), C.POINTER(I)]

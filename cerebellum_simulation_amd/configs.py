@@ -183,12 +183,45 @@ CONFIGS = {
 }
 
 
-def build_state(name: str, seed: int = 0, act_overrides: dict | None = None, **gen_overrides):
-    """NetState of a named configuration (GR inputs drawn with `seed`)."""
+# Size classes of the ragged batch: 16K, 65K, 262K and 1M GR, with GO, MF
+# and PC growing from the 16K-GR network (c0) towards the 1M-GR one (c1).
+BATCH_CLASSES = {
+    "b16k": CONFIGS["c0"],
+    # 65,536 GR; 512 GO; 512 MF; 16 PC; 64 BC; 256 SC; 2 IO; 4 NC
+    "b65k": (dict(grXP2=9, grYP2=7, goXP2=5, goYP2=4, numMFP2=9, numSCP2=8, numPCP2=4, numNCP2=2,
+                  numIOP2=1, numpPCfromPCtoNC=3, grPFVelInGRXPerTStep=70),
+             dict(n_mf_per_gr=4, n_go_per_gr=3, n_go_out_per_gr=2, n_mf_per_go=16)),
+    # 262,144 GR; 1,024 GO; 1,024 MF; 32 PC; 128 BC; 512 SC; 4 IO; 8 NC
+    "b262k": (dict(grXP2=10, grYP2=8, goXP2=5, goYP2=5, numMFP2=10, numSCP2=9, numPCP2=5, numNCP2=3,
+                   numIOP2=2, numpPCfromPCtoNC=3, grPFVelInGRXPerTStep=100),
+              dict(n_mf_per_gr=4, n_go_per_gr=3, n_go_out_per_gr=2, n_mf_per_go=16)),
+    "b1m": CONFIGS["c1"],
+}
+# the 256-network ragged batch: 64 x 16K + 96 x 65K + 64 x 262K + 32 x 1M GR
+BATCH_MIX = [("b16k", 64), ("b65k", 96), ("b262k", 64), ("b1m", 32)]
+
+
+def build_state(name: str, seed: int = 0, act_overrides: dict | None = None, *, gogo_inhibition: bool = True,
+                **gen_overrides):
+    """NetState of a named configuration or batch size class (GR inputs
+    drawn with `seed`); gogo_inhibition=False leaves the GO-GO GABA matrix empty."""
     from .network import NetState
-    con, gen = CONFIGS[name]
+    con, gen = CONFIGS[name] if name in CONFIGS else BATCH_CLASSES[name]
     gen = {**gen, **gen_overrides}
-    return NetState(act_params_text(act_overrides), conparams_text(**con), innet_seed=seed, **gen)
+    con_text = conparams_text(**con, gogo_gaba=None if gogo_inhibition else [])
+    return NetState(act_params_text(act_overrides), con_text, innet_seed=seed, **gen)
+
+
+def batch_plan(mix=None, base_seed: int = 1000):
+    """[(size class, seed, gogo_inhibition)] of a ragged batch: every network
+    has its own seed (which also picks its CS MF subset) and GO-GO inhibition
+    on or off, half and half within each class."""
+    plan = []
+    for cls, count in (BATCH_MIX if mix is None else mix):
+        for i in range(count):
+            seed = base_seed + len(plan) * 7919
+            plan.append((cls, seed, i % 2 == 0))
+    return plan
 
 
 def mf_train(num_mf: int, steps: int, seed: int = 0, *, t0: int = 0, trial_ms: int = 2000,

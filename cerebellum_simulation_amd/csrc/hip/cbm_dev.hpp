@@ -17,10 +17,10 @@ constexpr int kRing = 3;                // input-sum ring slots (t, t-1, t-2)
 constexpr int kSmallThreads = 1024;     // threads of the small-population kernel
 constexpr uint32_t kSmallLdsMax = 152 * 1024;  // dynamic LDS cap of the small kernel
 
-// Granule-cell kernel arguments.  Per-GR arrays are structure-of-arrays of
-// numGR elements; connectivity is packed four 16-bit cell indices per GR
-// (kNoCell = empty slot) and one 8-bit delay (bit index of the spike register)
-// per output slot.
+// Granule-cell kernel arguments that do not change from step to step.
+// Per-GR arrays are structure-of-arrays of numGR elements; connectivity is
+// packed four 16-bit cell indices per GR (kNoCell = empty slot) and one 8-bit
+// delay (bit index of the spike register) per output slot.
 struct GrArgs {
     float *v, *thr, *gKCa, *gE, *gI;
     uint32_t *apBuf;
@@ -31,6 +31,16 @@ struct GrArgs {
     const uint2 *goOut;        // 4 x u16 GO targets
     const uint32_t *goOutDly;  // 4 x u8 delay bit per target
     const uint8_t *bspDly;     // delay bit towards BC/PC/SC
+    uint32_t numGR, numMF, numGO, numZones, numIO, numPC, perPC;  // perPC: 64-GR chunks per PC
+    uint32_t log2Lbc, log2Lsc, log2GRperIO;
+    float gLeak, eLeak, eGO, threshRest, threshMax, threshDec;
+    float decMF, incMF, decGO, incGO;
+    int32_t ltdEnd, ltpStart;  // tsLTDStartAPIO + tsLTDDurationIO, tsLTPStartAPIO
+    float ltd[kMaxZones], ltp[kMaxZones];
+};
+
+// Granule-cell arguments of one step.
+struct GrStep {
     const uint8_t *mf;         // raw MF bytes of this step
     const uint8_t *apGOprev;   // GO spikes of the previous step
     float *chunk;              // [numZones][numGR/64] PF->PC sums of 64-GR chunks, this step
@@ -38,13 +48,7 @@ struct GrArgs {
     float *foldDst;            // their ring slot, [numZones][numPC]
     uint32_t *bcCnt, *scCnt, *goCnt;  // this step's ring slot (zeroed beforehand)
     const int32_t *plastTimer;        // [numZones][numIO] as of the previous step
-    uint32_t numGR, numMF, numGO, numZones, numIO, numPC, perPC;  // perPC: 64-GR chunks per PC
-    uint32_t log2Lbc, log2Lsc, log2GRperIO;
     uint32_t doPlast;          // t % tsPerHistBinGR == 0: plasticity + history this step
-    float gLeak, eLeak, eGO, threshRest, threshMax, threshDec;
-    float decMF, incMF, decGO, incGO;
-    int32_t ltdEnd, ltpStart;  // tsLTDStartAPIO + tsLTDDurationIO, tsLTPStartAPIO
-    float ltd[kMaxZones], ltp[kMaxZones];
 };
 
 // Byte offsets of the small kernel's dynamic LDS regions (16-byte aligned),
@@ -110,6 +114,27 @@ struct SmallStep {
     uint32_t slotPrev, slotT2, slotG, slotNext;  // ring slots of t-1, t-2, t-grgoLag, t+1
     uint32_t popHist;          // t % tsPerPopHistBinPC == 0: PC population histogram + MF->NC weights
     uint32_t resetMFHist;      // t % numTSinMFHist == 0
+};
+
+// One network of a batch: its constant arguments, the buffers the per-step
+// arguments point into, and the run the batch is executing.  Kernels derive
+// step t = t0 + k + 1 of run step k from these.
+struct BatchNet {
+    const SmallArgs *small;    // the network's device argument block
+    GrArgs gr;                 // step-independent GR arguments
+    float *chunk2, *pfpcRing;
+    uint32_t *bcRing, *scRing, *goRing;
+    uint8_t *apGO2;
+    int32_t *timer2;
+    const uint8_t *mf;         // MF row k at mf + k * pitch (device memory)
+    uint64_t pitch;
+    const float *err;          // [n][numZones] relative error drive, or nullptr
+    uint64_t t0;               // curTime before the run
+    float errDrive0[kMaxZones];  // drive pending from the host for the first step
+    float maxExtIncVIO;
+    float gIncGRtoGO;
+    uint32_t skipFold0;        // the host wrote the PF->PC sums of step t0
+    uint32_t grgoLag, tsPerHistBinGR, tsPerPopHistBinPC, numTSinMFHist;
 };
 
 // Bytes of dynamic LDS a GR tile of nt threads x cpt cells uses.

@@ -1,0 +1,114 @@
+// cbm_handle.hpp -- the network handle behind the cbm_* C-ABI, shared by the
+// single-network runtime (cbm_runtime.hip) and the batch (cbm_batch.hip).
+// Not part of the public interface.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "cbm_dev.hpp"
+
+namespace cbmdev {
+template <int CPT, int NT>
+__global__ void gr_step_kernel(const GrArgs a, const GrStep st);
+template <int CPT>
+__global__ void step_kernel(const SmallArgs *args, const SmallStep st, const GrArgs g, const GrStep gs);
+__global__ void small_step_kernel(const SmallArgs *args, const SmallStep st);
+__global__ void pfpc_fold_kernel(const float *chunk, float *out, uint32_t nChunks, uint32_t perPC, uint32_t numPC,
+                                 uint32_t tasks);
+
+// records the message returned by cbm_last_error() and returns code
+int fail(int code, const std::string &m);
+}  // namespace cbmdev
+
+#define HIP_TRY(expr)                                                                        \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return cbmdev::fail(CBM_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+using namespace cbmdev;  // private header: the runtime's own names
+
+struct cbm_handle {
+    cbm_batch *batch = nullptr;      // the batch the network belongs to, if any
+    cbm_sizes s{};
+    cbm_act_params p{};
+    cbm_options o{};
+    hipStream_t stream = nullptr;   // GR kernel, host copies; joined with stream2 after every call
+    hipStream_t stream2 = nullptr;  // small-population kernel
+    hipEvent_t evG = nullptr;       // recorded on stream after each GR kernel (and host copies)
+    hipEvent_t evS = nullptr;       // recorded on stream2 after each small kernel
+    bool committed = false;
+    int cpt = 4, nt = 256;   // GR kernel shape of the two-stream step
+    int fusedCpt = 1;        // GR cells per thread of the fused step
+    int mode = 0;            // 1 fused, 2 two streams (decided at commit)
+    size_t fusedLds = 0;     // dynamic LDS of the fused step kernel
+
+    std::vector<uint8_t> host[CBM_F_COUNT];  // staged fields, zones stacked
+    bool written[CBM_F_COUNT] = {};
+    void *dfield[CBM_F_COUNT] = {};
+
+    uint2 *mfIn = nullptr, *goIn = nullptr, *goOut = nullptr;
+    uint32_t *goOutDly = nullptr;
+    uint8_t *bspDly = nullptr;
+    float *chunk2 = nullptr, *pfpcRing = nullptr;  // chunk sums [2][nZ][numGR/64] by step parity
+    uint32_t *bcRing = nullptr, *scRing = nullptr, *goRing = nullptr, *rngState = nullptr;
+    uint8_t *apGO2 = nullptr;   // [2][numGO]
+    int32_t *timer2 = nullptr;  // [2][nZ][numIO]
+    bool skipFold = true;       // the ring slot of the last step holds host-written PF->PC sums
+    uint8_t *dmf = nullptr;
+    uint8_t *pinnedMF = nullptr;
+    uint32_t anyCoupling = 0;
+    SmallArgs *smallArgs = nullptr;  // device copy of the constant small-kernel arguments
+    size_t smallLds = 0;             // dynamic LDS of the small kernel
+
+    uint64_t curTime = 0;
+    float errDrive[kMaxZones] = {};
+    float ltd[kMaxZones] = {}, ltp[kMaxZones] = {};
+    float gIncGRtoGO = 0;
+
+    std::vector<void *> allocs;
+    size_t devBytes = 0;
+    hipEvent_t ev[4] = {};  // timing events of cbm_profile_steps / cbm_profile_split
+
+    size_t zoneBytes(int f) const { return cbm_field_bytes(f, &s); }
+    int zonesOf(int f) const
+    {
+        int zoned = 0;
+        cbm_field_shape(f, &s, nullptr, nullptr, nullptr, &zoned, nullptr);
+        return zoned ? (int)s.numZones : 1;
+    }
+    template <typename T>
+    T *H(int f, int z = 0) { return reinterpret_cast<T *>(host[f].data() + (size_t)z * zoneBytes(f)); }
+
+    int dalloc(void **p, size_t bytes)
+    {
+        if (bytes == 0) bytes = 4;
+        hipError_t e = hipMalloc(p, bytes);
+        if (e != hipSuccess) return fail(CBM_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+        allocs.push_back(*p);
+        devBytes += bytes;
+        HIP_TRY(hipMemset(*p, 0, bytes));
+        return CBM_OK;
+    }
+    template <typename T>
+    int upload(T **dst, const std::vector<T> &v)
+    {
+        int rc = dalloc((void **)dst, v.size() * sizeof(T));
+        if (rc) return rc;
+        if (!v.empty()) HIP_TRY(hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+        return CBM_OK;
+    }
+};
+
+
+namespace cbmdev {
+// wait for both streams of the handle
+int syncAll(cbm_handle *h);
+// the step-independent part of the GR kernel arguments
+void grConstArgs(const cbm_handle *h, GrArgs *g);
+// device MF train checks shared by cbm_run_mf_train and the batch
+int checkTrain(cbm_handle *h, const uint8_t *mf_dev, size_t pitch, int n);
+}  // namespace cbmdev

@@ -77,7 +77,8 @@ __device__ __forceinline__ float fold_pc(const float *__restrict__ c, uint32_t p
 // issued before the LDS staging barrier, so all of a thread's requests are in
 // flight together.
 template <int CPT, int NT>
-__device__ __forceinline__ void gr_tile(const GrArgs &a, const uint32_t tile, uint32_t *smem)
+__device__ __forceinline__ void gr_tile(const GrArgs &a, const GrStep &st, const uint32_t tile,
+                                        uint32_t *smem)
 {
     constexpr int kSeg = NT * CPT / 64;  // 64-GR segments per tile (PF blocks hold >= 64 GRs)
     uint32_t *s_goh = smem;                          // [numGO] GR->GO counts of the tile
@@ -110,21 +111,21 @@ __device__ __forceinline__ void gr_tile(const GrArgs &a, const uint32_t tile, ui
         gd[j] = a.goOutDly[gr];
         bsp[j] = a.bspDly[gr];
         w0[j] = a.w[gr];
-        hist[j] = a.doPlast ? a.hist[gr] : 0;
+        hist[j] = st.doPlast ? a.hist[gr] : 0;
     }
 
     for (uint32_t i = tid; i < a.numGO; i += NT) s_goh[i] = 0;
     if (tid < 2 * kSeg) s_sc[tid] = 0;  // s_sc and s_bc are contiguous
-    if (a.doPlast) {
+    if (st.doPlast) {
         for (uint32_t i = tid; i < a.numZones * a.numIO; i += NT) {
             const uint32_t z = i / a.numIO;
-            const int32_t tm = a.plastTimer[i];
+            const int32_t tm = st.plastTimer[i];
             // MZone::runPFPCPlastCUDA step choice (mzone.cpp:660-674)
             s_step[i] = tm < a.ltdEnd ? a.ltd[z] : (tm > a.ltpStart ? a.ltp[z] : 0.0f);
         }
     }
-    stage_flags(s_mf, a.mf, a.numMF, tid, NT);
-    stage_flags(s_go, a.apGOprev, a.numGO, tid, NT);
+    stage_flags(s_mf, st.mf, a.numMF, tid, NT);
+    stage_flags(s_go, st.apGOprev, a.numGO, tid, NT);
     __syncthreads();
 
     const uint32_t firstSc = tileBase >> a.log2Lsc, firstBc = tileBase >> a.log2Lbc;
@@ -170,7 +171,7 @@ __device__ __forceinline__ void gr_tile(const GrArgs &a, const uint32_t tile, ui
         a.gI[gr] = i;
 
         uint32_t bit39 = 0;
-        if (a.doPlast) {
+        if (st.doPlast) {
             const uint64_t h = hist[j];
             bit39 = (uint32_t)(h >> 39) & 1u;
             // updateGRHistory (kernels.cu:244-245): mask 0x1F is hard-coded there
@@ -186,7 +187,7 @@ __device__ __forceinline__ void gr_tile(const GrArgs &a, const uint32_t tile, ui
         for (uint32_t z = 0; z < a.numZones; z++) {
             float *wz = a.w + (size_t)z * a.numGR;
             float wv = z == 0 ? w0[j] : wz[gr];
-            if (a.doPlast) {
+            if (st.doPlast) {
                 const float plastStep = s_step[z * a.numIO + io];
                 wv = wv + (bit39 > 0) * plastStep;
                 wv = (wv > 0) * wv;
@@ -195,7 +196,7 @@ __device__ __forceinline__ void gr_tile(const GrArgs &a, const uint32_t tile, ui
             }
             float x = 0.0f;
             if (bal != 0) x = wave_sum_butterfly(wv * d);
-            if (lane == 0) a.chunk[(size_t)z * nChunks + (gr >> 6)] = x;
+            if (lane == 0) st.chunk[(size_t)z * nChunks + (gr >> 6)] = x;
         }
 
         // PF->BC and PF->SC spike counts of this wave's 64 GRs
@@ -220,40 +221,40 @@ __device__ __forceinline__ void gr_tile(const GrArgs &a, const uint32_t tile, ui
 
     for (uint32_t i = tid; i < a.numGO; i += NT) {
         const uint32_t c = s_goh[i];
-        if (c) atomicAdd(&a.goCnt[i], c);
+        if (c) atomicAdd(&st.goCnt[i], c);
     }
     const uint32_t tileGR = NT * CPT;
     const uint32_t nSc = (tileGR >> a.log2Lsc) ? (tileGR >> a.log2Lsc) : 1;
     const uint32_t nBc = (tileGR >> a.log2Lbc) ? (tileGR >> a.log2Lbc) : 1;
-    if (tid < (int)nSc && s_sc[tid]) atomicAdd(&a.scCnt[firstSc + tid], s_sc[tid]);
-    if (tid < (int)nBc && s_bc[tid]) atomicAdd(&a.bcCnt[firstBc + tid], s_bc[tid]);
+    if (tid < (int)nSc && s_sc[tid]) atomicAdd(&st.scCnt[firstSc + tid], s_sc[tid]);
+    if (tid < (int)nBc && s_bc[tid]) atomicAdd(&st.bcCnt[firstBc + tid], s_bc[tid]);
 
     // second level of the previous step's PF->PC sums, one wave per (zone,
     // PC) on the first tiles: the small populations read them one step later
-    if (a.foldSrc) {
+    if (st.foldSrc) {
         const uint32_t waves = (a.numGR / (NT * CPT)) * (uint32_t)(NT / 64);
         for (uint32_t task = tile * (uint32_t)(NT / 64) + (uint32_t)(tid >> 6); task < a.numZones * a.numPC;
              task += waves) {  // wave-uniform
             const uint32_t z = task / a.numPC, pc = task % a.numPC;
-            const float acc = fold_pc(a.foldSrc + (size_t)z * nChunks + (size_t)pc * a.perPC, a.perPC, lane);
-            if (lane == 0) a.foldDst[task] = acc;
+            const float acc = fold_pc(st.foldSrc + (size_t)z * nChunks + (size_t)pc * a.perPC, a.perPC, lane);
+            if (lane == 0) st.foldDst[task] = acc;
         }
     }
 }
 
 // GR tiles alone (the two-stream step, and GR-only timing)
 template <int CPT, int NT>
-__global__ void __launch_bounds__(NT) gr_step_kernel(const GrArgs a)
+__global__ void __launch_bounds__(NT) gr_step_kernel(const GrArgs a, const GrStep st)
 {
     extern __shared__ uint32_t smem[];
-    gr_tile<CPT, NT>(a, blockIdx.x, smem);
+    gr_tile<CPT, NT>(a, st, blockIdx.x, smem);
 }
 
-template __global__ void gr_step_kernel<1, 256>(const GrArgs);
-template __global__ void gr_step_kernel<2, 256>(const GrArgs);
-template __global__ void gr_step_kernel<4, 256>(const GrArgs);
-template __global__ void gr_step_kernel<1, 1024>(const GrArgs);
-template __global__ void gr_step_kernel<2, 1024>(const GrArgs);
+template __global__ void gr_step_kernel<1, 256>(const GrArgs, const GrStep);
+template __global__ void gr_step_kernel<2, 256>(const GrArgs, const GrStep);
+template __global__ void gr_step_kernel<4, 256>(const GrArgs, const GrStep);
+template __global__ void gr_step_kernel<1, 1024>(const GrArgs, const GrStep);
+template __global__ void gr_step_kernel<2, 1024>(const GrArgs, const GrStep);
 
 // ---------------------------------------------------------------- SFMT on device
 // The zone generator of MZone (CRandomSFMT0, sfmt.cpp:210-246,337-355),
@@ -367,7 +368,8 @@ __device__ __forceinline__ float lds_sum_seq(const float *x, uint32_t n, float a
 // its first store, so each phase costs about one memory round trip, and each
 // population or synapse group starts on its own wave, so that no wave runs
 // the updates of every small population in turn.
-__device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &st, uint8_t *lds)
+__device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &st, const float *errDrive,
+                                           uint8_t *lds)
 {
     const cbm_sizes &s = a.s;
     const cbm_act_params &p = a.p;
@@ -643,7 +645,7 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
                 float vv = ioV;
                 float gNCSum = lds_sum_seq(s_syn + L.synIO + i * kNCIO, kNCIO, 0.0f);
                 gNCSum = gNCSum / 3.1;
-                vv = vv + p.gLeakIO * (p.eLeakIO - vv) + gNCSum * 1.5 * (p.eNCtoIO - vv) + ioVC + st.errDrive[z];
+                vv = vv + p.gLeakIO * (p.eLeakIO - vv) + gNCSum * 1.5 * (p.eNCtoIO - vv) + ioVC + errDrive[z];
                 const uint8_t ap = vv > ioTh;
                 F<float>(a, CBM_F_VM_IO)[i] = vv;
                 s_vIO[i] = vv;
@@ -949,23 +951,94 @@ __global__ void __launch_bounds__(kSmallThreads) small_step_kernel(const SmallAr
                                                                    const SmallStep st)
 {
     extern __shared__ __align__(16) uint8_t lds[];
-    small_step(*args, st, lds);
+    small_step(*args, st, st.errDrive, lds);
 }
 
 // One launch per step: block 0 steps the small populations, blocks 1.. are GR
 // tiles of 1024 threads (one block per CU: the small step needs up to 128
 // VGPRs per thread).
 template <int CPT>
-__global__ void __launch_bounds__(kSmallThreads) step_kernel(const SmallArgs *__restrict__ args, const SmallStep st, const GrArgs g)
+__global__ void __launch_bounds__(kSmallThreads) step_kernel(const SmallArgs *__restrict__ args, const SmallStep st, const GrArgs g, const GrStep gs)
 {
     extern __shared__ __align__(16) uint8_t lds[];
     if (blockIdx.x == 0)
-        small_step(*args, st, lds);
+        small_step(*args, st, st.errDrive, lds);
     else
-        gr_tile<CPT, kSmallThreads>(g, blockIdx.x - 1, reinterpret_cast<uint32_t *>(lds));
+        gr_tile<CPT, kSmallThreads>(g, gs, blockIdx.x - 1, reinterpret_cast<uint32_t *>(lds));
 }
 
-template __global__ void step_kernel<1>(const SmallArgs *, const SmallStep, const GrArgs);
-template __global__ void step_kernel<2>(const SmallArgs *, const SmallStep, const GrArgs);
+template __global__ void step_kernel<1>(const SmallArgs *, const SmallStep, const GrArgs, const GrStep);
+template __global__ void step_kernel<2>(const SmallArgs *, const SmallStep, const GrArgs, const GrStep);
+
+// ---------------------------------------------------------------- batch
+// One launch per step for a pack of networks: block n < nNets steps network
+// n's small populations, the other blocks are GR tiles (network, tile) from
+// the tile table.  Run step kk = *kbase + k (kbase: device counter advanced by
+// a captured graph, or nullptr).
+template <int CPT>
+__global__ void __launch_bounds__(kSmallThreads) batch_step_kernel(const BatchNet *__restrict__ nets,
+                                                                   const uint2 *__restrict__ tiles, uint32_t nNets,
+                                                                   const uint32_t *kbase, uint32_t k)
+{
+    extern __shared__ __align__(16) uint8_t lds[];
+    __shared__ float s_err[kMaxZones];  // in LDS: the small step indexes it by zone
+    const uint32_t kk = (kbase ? *kbase : 0u) + k;
+    const bool small = blockIdx.x < nNets;
+    const uint2 tt = small ? make_uint2(blockIdx.x, 0u) : tiles[blockIdx.x - nNets];
+    const BatchNet &d = nets[tt.x];
+    const uint64_t t = d.t0 + kk + 1;
+    const uint32_t par = (uint32_t)(t & 1u), parPrev = par ^ 1u;
+    const uint8_t *mf = d.mf + (size_t)kk * d.pitch;
+    const uint32_t nZ = d.gr.numZones;
+    if (small) {
+        SmallStep st;
+        st.mf = mf;
+        st.t = t;
+        if (threadIdx.x < (uint32_t)kMaxZones) {
+            const uint32_t z = threadIdx.x;
+            float e = kk == 0 ? d.errDrive0[z] : 0.0f;
+            if (d.err && z < nZ) {
+                const float r = d.err[(size_t)kk * nZ + z];
+                if (r != 0.0f) e = r * d.maxExtIncVIO;
+            }
+            s_err[z] = e;
+        }
+        st.gIncGRtoGO = d.gIncGRtoGO;
+        st.par = par;
+        st.slotPrev = (uint32_t)((t + kRing - 1) % kRing);
+        st.slotT2 = (uint32_t)((t + kRing - 2) % kRing);
+        st.slotG = (uint32_t)((t + kRing - d.grgoLag) % kRing);
+        st.slotNext = (uint32_t)((t + 1) % kRing);
+        st.popHist = (t % d.tsPerPopHistBinPC) == 0;
+        st.resetMFHist = (t % d.numTSinMFHist) == 0;
+        __syncthreads();
+        small_step(*d.small, st, s_err, lds);
+    } else {
+        const GrArgs &g = d.gr;
+        const uint32_t slot = (uint32_t)(t % kRing);
+        const size_t nChunks = g.numGR / 64;
+        GrStep gs;
+        gs.mf = mf;
+        gs.apGOprev = d.apGO2 + (size_t)parPrev * g.numGO;
+        gs.chunk = d.chunk2 + (size_t)par * nZ * nChunks;
+        gs.foldSrc = (kk == 0 && d.skipFold0) ? nullptr : d.chunk2 + (size_t)parPrev * nZ * nChunks;
+        gs.foldDst = d.pfpcRing + (size_t)((t + kRing - 1) % kRing) * nZ * g.numPC;
+        gs.bcCnt = d.bcRing + (size_t)slot * d.small->s.numBC;
+        gs.scCnt = d.scRing + (size_t)slot * d.small->s.numSC;
+        gs.goCnt = d.goRing + (size_t)slot * g.numGO;
+        gs.plastTimer = d.timer2 + (size_t)parPrev * nZ * g.numIO;
+        gs.doPlast = (t % d.tsPerHistBinGR) == 0;
+        gr_tile<CPT, kSmallThreads>(g, gs, tt.y, reinterpret_cast<uint32_t *>(lds));
+    }
+}
+
+template __global__ void batch_step_kernel<1>(const BatchNet *, const uint2 *, uint32_t, const uint32_t *, uint32_t);
+template __global__ void batch_step_kernel<2>(const BatchNet *, const uint2 *, uint32_t, const uint32_t *, uint32_t);
+
+// advances the run-step counter of a captured batch graph
+__global__ void batch_advance_kernel(uint32_t *kbase, uint32_t steps)
+{
+    if (threadIdx.x == 0) *kbase += steps;
+}
 
 }  // namespace cbmdev

@@ -3,8 +3,8 @@
 // Owns one network on one GPU: stages the CBMState-layout fields written by
 // the caller, validates them, packs granule connectivity into the device
 // layout of the GR kernel (four 16-bit indices per GR, 8-bit delay bits),
-// builds the gather lists of the small-population kernel, and launches the
-// two kernels of each step on the handle's stream.
+// builds the gather lists of the small-population kernel, and launches each
+// step (one fused launch, or two kernels on two streams).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -14,36 +14,21 @@
 #include <vector>
 
 #include "../state/sfmt.hpp"
-#include "cbm_dev.hpp"
-
-namespace cbmdev {
-template <int CPT, int NT>
-__global__ void gr_step_kernel(const GrArgs a);
-template <int CPT>
-__global__ void step_kernel(const SmallArgs *args, const SmallStep st, const GrArgs g);
-__global__ void small_step_kernel(const SmallArgs *args, const SmallStep st);
-__global__ void pfpc_fold_kernel(const float *chunk, float *out, uint32_t nChunks, uint32_t perPC, uint32_t numPC,
-                                 uint32_t tasks);
-}  // namespace cbmdev
+#include "cbm_handle.hpp"
 
 using namespace cbmdev;
 
 namespace {
-
 thread_local std::string g_err;
+}  // namespace
 
-int fail(int code, const std::string &m)
+int cbmdev::fail(int code, const std::string &m)
 {
     g_err = m;
     return code;
 }
 
-#define HIP_TRY(expr)                                                                        \
-    do {                                                                                     \
-        hipError_t e_ = (expr);                                                              \
-        if (e_ != hipSuccess)                                                                \
-            return fail(CBM_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));     \
-    } while (0)
+namespace {
 
 bool isPow2(uint32_t x) { return x && !(x & (x - 1)); }
 
@@ -68,77 +53,6 @@ bool deviceField(int f)
 }
 
 }  // namespace
-
-struct cbm_handle {
-    cbm_sizes s{};
-    cbm_act_params p{};
-    cbm_options o{};
-    hipStream_t stream = nullptr;   // GR kernel, host copies; joined with stream2 after every call
-    hipStream_t stream2 = nullptr;  // small-population kernel
-    hipEvent_t evG = nullptr;       // recorded on stream after each GR kernel (and host copies)
-    hipEvent_t evS = nullptr;       // recorded on stream2 after each small kernel
-    bool committed = false;
-    int cpt = 4, nt = 256;   // GR kernel shape of the two-stream step
-    int fusedCpt = 1;        // GR cells per thread of the fused step
-    int mode = 0;            // 1 fused, 2 two streams (decided at commit)
-    size_t fusedLds = 0;     // dynamic LDS of the fused step kernel
-
-    std::vector<uint8_t> host[CBM_F_COUNT];  // staged fields, zones stacked
-    bool written[CBM_F_COUNT] = {};
-    void *dfield[CBM_F_COUNT] = {};
-
-    uint2 *mfIn = nullptr, *goIn = nullptr, *goOut = nullptr;
-    uint32_t *goOutDly = nullptr;
-    uint8_t *bspDly = nullptr;
-    float *chunk2 = nullptr, *pfpcRing = nullptr;  // chunk sums [2][nZ][numGR/64] by step parity
-    uint32_t *bcRing = nullptr, *scRing = nullptr, *goRing = nullptr, *rngState = nullptr;
-    uint8_t *apGO2 = nullptr;   // [2][numGO]
-    int32_t *timer2 = nullptr;  // [2][nZ][numIO]
-    bool skipFold = true;       // the ring slot of the last step holds host-written PF->PC sums
-    uint8_t *dmf = nullptr;
-    uint8_t *pinnedMF = nullptr;
-    uint32_t anyCoupling = 0;
-    SmallArgs *smallArgs = nullptr;  // device copy of the constant small-kernel arguments
-    size_t smallLds = 0;             // dynamic LDS of the small kernel
-
-    uint64_t curTime = 0;
-    float errDrive[kMaxZones] = {};
-    float ltd[kMaxZones] = {}, ltp[kMaxZones] = {};
-    float gIncGRtoGO = 0;
-
-    std::vector<void *> allocs;
-    size_t devBytes = 0;
-    hipEvent_t ev[4] = {};  // timing events of cbm_profile_steps / cbm_profile_split
-
-    size_t zoneBytes(int f) const { return cbm_field_bytes(f, &s); }
-    int zonesOf(int f) const
-    {
-        int zoned = 0;
-        cbm_field_shape(f, &s, nullptr, nullptr, nullptr, &zoned, nullptr);
-        return zoned ? (int)s.numZones : 1;
-    }
-    template <typename T>
-    T *H(int f, int z = 0) { return reinterpret_cast<T *>(host[f].data() + (size_t)z * zoneBytes(f)); }
-
-    int dalloc(void **p, size_t bytes)
-    {
-        if (bytes == 0) bytes = 4;
-        hipError_t e = hipMalloc(p, bytes);
-        if (e != hipSuccess) return fail(CBM_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
-        allocs.push_back(*p);
-        devBytes += bytes;
-        HIP_TRY(hipMemset(*p, 0, bytes));
-        return CBM_OK;
-    }
-    template <typename T>
-    int upload(T **dst, const std::vector<T> &v)
-    {
-        int rc = dalloc((void **)dst, v.size() * sizeof(T));
-        if (rc) return rc;
-        if (!v.empty()) HIP_TRY(hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
-        return CBM_OK;
-    }
-};
 
 namespace {
 
@@ -350,6 +264,7 @@ int cbm_create(const cbm_sizes *sizes, const cbm_act_params *params, const cbm_o
 int cbm_destroy(cbm_handle *h)
 {
     if (!h) return fail(CBM_E_ARG, "null handle");
+    if (h->batch) return fail(CBM_E_STATE, "the network belongs to a batch: destroy the batch first");
     (void)hipSetDevice(h->o.device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     if (h->stream2) (void)hipStreamSynchronize(h->stream2);
@@ -380,13 +295,16 @@ static uint32_t ringSlot(const cbm_handle *h, int r)
     return (uint32_t)((h->curTime + (uint64_t)kRing * 4 - (uint64_t)r) % kRing);
 }
 
-// wait for both streams
-static int syncAll(cbm_handle *h)
+}  // extern "C"
+
+int cbmdev::syncAll(cbm_handle *h)
 {
     HIP_TRY(hipStreamSynchronize(h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream2));
     return CBM_OK;
 }
+
+extern "C" {
 
 // device address of a parity-buffered field as of the last completed step
 static uint8_t *parityField(cbm_handle *h, int field, int zone)
@@ -877,22 +795,73 @@ int cbm_reset_ginc_grgo(cbm_handle *h)
 }
 
 // Launch the GR tiles alone (nt x cpt shape) on `st`.
-static void launchGr(cbm_handle *h, const GrArgs &g, int nt, int cpt, hipStream_t st)
+static void launchGr(cbm_handle *h, const GrArgs &g, const GrStep &gs, int nt, int cpt, hipStream_t st)
 {
     const cbm_sizes &s = h->s;
     const uint32_t blocks = s.numGR / (uint32_t)(nt * cpt);
     const size_t lds = grTileLds(nt, cpt, s.numGO, s.numMF, s.numZones, s.numIO);
     if (nt == 1024) {
-        if (cpt == 2) hipLaunchKernelGGL((gr_step_kernel<2, 1024>), dim3(blocks), dim3(1024), lds, st, g);
-        else hipLaunchKernelGGL((gr_step_kernel<1, 1024>), dim3(blocks), dim3(1024), lds, st, g);
+        if (cpt == 2) hipLaunchKernelGGL((gr_step_kernel<2, 1024>), dim3(blocks), dim3(1024), lds, st, g, gs);
+        else hipLaunchKernelGGL((gr_step_kernel<1, 1024>), dim3(blocks), dim3(1024), lds, st, g, gs);
     } else if (cpt == 4) {
-        hipLaunchKernelGGL((gr_step_kernel<4, 256>), dim3(blocks), dim3(256), lds, st, g);
+        hipLaunchKernelGGL((gr_step_kernel<4, 256>), dim3(blocks), dim3(256), lds, st, g, gs);
     } else if (cpt == 2) {
-        hipLaunchKernelGGL((gr_step_kernel<2, 256>), dim3(blocks), dim3(256), lds, st, g);
+        hipLaunchKernelGGL((gr_step_kernel<2, 256>), dim3(blocks), dim3(256), lds, st, g, gs);
     } else {
-        hipLaunchKernelGGL((gr_step_kernel<1, 256>), dim3(blocks), dim3(256), lds, st, g);
+        hipLaunchKernelGGL((gr_step_kernel<1, 256>), dim3(blocks), dim3(256), lds, st, g, gs);
     }
 }
+
+}  // extern "C"
+
+// the step-independent part of the GR kernel arguments
+void cbmdev::grConstArgs(const cbm_handle *h, GrArgs *gp_)
+{
+    const cbm_sizes &s = h->s;
+    const cbm_act_params &p = h->p;
+    GrArgs &g = *gp_;
+    g.v = (float *)h->dfield[CBM_F_VM_GR];
+    g.thr = (float *)h->dfield[CBM_F_THRESH_GR];
+    g.gKCa = (float *)h->dfield[CBM_F_GKCA_GR];
+    g.gE = (float *)h->dfield[CBM_F_G_MF_SUM_GR];
+    g.gI = (float *)h->dfield[CBM_F_G_GO_SUM_GR];
+    g.apBuf = (uint32_t *)h->dfield[CBM_F_APBUF_GR];
+    g.hist = (uint64_t *)h->dfield[CBM_F_HIST_GR];
+    g.w = (float *)h->dfield[CBM_F_PFPC_SYNW];
+    g.mfIn = h->mfIn;
+    g.goIn = h->goIn;
+    g.goOut = h->goOut;
+    g.goOutDly = h->goOutDly;
+    g.bspDly = h->bspDly;
+    g.numGR = s.numGR;
+    g.numMF = s.numMF;
+    g.numGO = s.numGO;
+    g.numZones = s.numZones;
+    g.numIO = s.numIO;
+    g.numPC = s.numPC;
+    g.perPC = s.numpPCfromGRtoPC / 64;
+    g.log2Lbc = ilog2(s.numpBCfromGRtoBC);
+    g.log2Lsc = ilog2(s.numpSCfromGRtoSC);
+    g.log2GRperIO = ilog2(s.numGR / s.numIO);
+    g.gLeak = p.gLeakGR;
+    g.eLeak = p.eLeakGR;
+    g.eGO = p.eGOGR;
+    g.threshRest = p.threshRestGR;
+    g.threshMax = p.threshMaxGR;
+    g.threshDec = p.threshDecGR;
+    g.decMF = p.gDecMFtoGR;
+    g.incMF = p.gIncMFtoGR;
+    g.decGO = p.gDecGOtoGR;
+    g.incGO = p.gIncGOtoGR;
+    g.ltdEnd = p.tsLTDStartAPIO + (int)p.tsLTDDurationIO;
+    g.ltpStart = p.tsLTPStartAPIO;
+    for (int z = 0; z < kMaxZones; z++) {
+        g.ltd[z] = h->ltd[z];
+        g.ltp[z] = h->ltp[z];
+    }
+}
+
+extern "C" {
 
 enum Timing { kNoTiming = 0, kTimeKernel = 1, kSplit = 2 };
 
@@ -915,58 +884,20 @@ static int launchStep(cbm_handle *h, const uint8_t *mf, int timing)
     const uint32_t par = (uint32_t)(t & 1u), parPrev = par ^ 1u;
 
     GrArgs g;
-    g.v = (float *)h->dfield[CBM_F_VM_GR];
-    g.thr = (float *)h->dfield[CBM_F_THRESH_GR];
-    g.gKCa = (float *)h->dfield[CBM_F_GKCA_GR];
-    g.gE = (float *)h->dfield[CBM_F_G_MF_SUM_GR];
-    g.gI = (float *)h->dfield[CBM_F_G_GO_SUM_GR];
-    g.apBuf = (uint32_t *)h->dfield[CBM_F_APBUF_GR];
-    g.hist = (uint64_t *)h->dfield[CBM_F_HIST_GR];
-    g.w = (float *)h->dfield[CBM_F_PFPC_SYNW];
-    g.mfIn = h->mfIn;
-    g.goIn = h->goIn;
-    g.goOut = h->goOut;
-    g.goOutDly = h->goOutDly;
-    g.bspDly = h->bspDly;
-    g.mf = mf;
-    g.apGOprev = h->apGO2 + (size_t)parPrev * s.numGO;
-    g.chunk = h->chunk2 + (size_t)par * s.numZones * (s.numGR / 64);
+    grConstArgs(h, &g);
+    GrStep gs;
+    gs.mf = mf;
+    gs.apGOprev = h->apGO2 + (size_t)parPrev * s.numGO;
+    gs.chunk = h->chunk2 + (size_t)par * s.numZones * (s.numGR / 64);
     // the GR launch of step t folds the chunk sums of t-1 into their ring slot,
     // unless the host wrote that slot
-    g.foldSrc = h->skipFold ? nullptr : h->chunk2 + (size_t)parPrev * s.numZones * (s.numGR / 64);
-    g.foldDst = h->pfpcRing + (size_t)((t + kRing - 1) % kRing) * s.numZones * s.numPC;
-    g.bcCnt = h->bcRing + (size_t)slot * s.numBC;
-    g.scCnt = h->scRing + (size_t)slot * s.numSC;
-    g.goCnt = h->goRing + (size_t)slot * s.numGO;
-    g.plastTimer = h->timer2 + (size_t)parPrev * s.numZones * s.numIO;
-    g.numGR = s.numGR;
-    g.numMF = s.numMF;
-    g.numGO = s.numGO;
-    g.numZones = s.numZones;
-    g.numIO = s.numIO;
-    g.numPC = s.numPC;
-    g.perPC = s.numpPCfromGRtoPC / 64;
-    g.log2Lbc = ilog2(s.numpBCfromGRtoBC);
-    g.log2Lsc = ilog2(s.numpSCfromGRtoSC);
-    g.log2GRperIO = ilog2(s.numGR / s.numIO);
-    g.doPlast = (t % p.tsPerHistBinGR) == 0;
-    g.gLeak = p.gLeakGR;
-    g.eLeak = p.eLeakGR;
-    g.eGO = p.eGOGR;
-    g.threshRest = p.threshRestGR;
-    g.threshMax = p.threshMaxGR;
-    g.threshDec = p.threshDecGR;
-    g.decMF = p.gDecMFtoGR;
-    g.incMF = p.gIncMFtoGR;
-    g.decGO = p.gDecGOtoGR;
-    g.incGO = p.gIncGOtoGR;
-    g.ltdEnd = p.tsLTDStartAPIO + (int)p.tsLTDDurationIO;
-    g.ltpStart = p.tsLTPStartAPIO;
-    for (int z = 0; z < kMaxZones; z++) {
-        g.ltd[z] = h->ltd[z];
-        g.ltp[z] = h->ltp[z];
-    }
-
+    gs.foldSrc = h->skipFold ? nullptr : h->chunk2 + (size_t)parPrev * s.numZones * (s.numGR / 64);
+    gs.foldDst = h->pfpcRing + (size_t)((t + kRing - 1) % kRing) * s.numZones * s.numPC;
+    gs.bcCnt = h->bcRing + (size_t)slot * s.numBC;
+    gs.scCnt = h->scRing + (size_t)slot * s.numSC;
+    gs.goCnt = h->goRing + (size_t)slot * s.numGO;
+    gs.plastTimer = h->timer2 + (size_t)parPrev * s.numZones * s.numIO;
+    gs.doPlast = (t % p.tsPerHistBinGR) == 0;
     SmallStep st;
     st.mf = mf;
     st.t = t;
@@ -983,7 +914,7 @@ static int launchStep(cbm_handle *h, const uint8_t *mf, int timing)
 
     if (timing == kSplit) {
         HIP_TRY(hipEventRecord(h->ev[0], h->stream));
-        launchGr(h, g, h->mode == 1 ? 1024 : h->nt, h->mode == 1 ? h->fusedCpt : h->cpt, h->stream);
+        launchGr(h, g, gs, h->mode == 1 ? 1024 : h->nt, h->mode == 1 ? h->fusedCpt : h->cpt, h->stream);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventRecord(h->ev[1], h->stream));
         HIP_TRY(hipEventRecord(h->ev[2], h->stream));
@@ -995,16 +926,16 @@ static int launchStep(cbm_handle *h, const uint8_t *mf, int timing)
         if (timing == kTimeKernel) HIP_TRY(hipEventRecord(h->ev[0], h->stream));
         if (h->fusedCpt == 2)
             hipLaunchKernelGGL(step_kernel<2>, dim3(1 + tiles), dim3(kSmallThreads), h->fusedLds, h->stream,
-                               h->smallArgs, st, g);
+                               h->smallArgs, st, g, gs);
         else
             hipLaunchKernelGGL(step_kernel<1>, dim3(1 + tiles), dim3(kSmallThreads), h->fusedLds, h->stream,
-                               h->smallArgs, st, g);
+                               h->smallArgs, st, g, gs);
         HIP_TRY(hipGetLastError());
         if (timing == kTimeKernel) HIP_TRY(hipEventRecord(h->ev[1], h->stream));
     } else {
         HIP_TRY(hipStreamWaitEvent(h->stream, h->evS, 0));
         if (timing == kTimeKernel) HIP_TRY(hipEventRecord(h->ev[0], h->stream));
-        launchGr(h, g, h->nt, h->cpt, h->stream);
+        launchGr(h, g, gs, h->nt, h->cpt, h->stream);
         HIP_TRY(hipGetLastError());
         if (timing == kTimeKernel) HIP_TRY(hipEventRecord(h->ev[1], h->stream));
         HIP_TRY(hipStreamWaitEvent(h->stream2, h->evG, 0));
@@ -1038,7 +969,9 @@ int cbm_step(cbm_handle *h, int n)
     return join(h);
 }
 
-static int checkTrain(cbm_handle *h, const uint8_t *mf_dev, size_t pitch, int n)
+}  // extern "C"
+
+int cbmdev::checkTrain(cbm_handle *h, const uint8_t *mf_dev, size_t pitch, int n)
 {
     if (!h || !mf_dev || n < 0) return fail(CBM_E_ARG, "bad argument");
     if (!h->committed) return fail(CBM_E_STATE, "cbm_commit has not been called");
@@ -1051,6 +984,8 @@ static int checkTrain(cbm_handle *h, const uint8_t *mf_dev, size_t pitch, int n)
     }
     return CBM_OK;
 }
+
+extern "C" {
 
 int cbm_run_mf_train(cbm_handle *h, const uint8_t *mf_dev, size_t pitch, int n, const float *err)
 {

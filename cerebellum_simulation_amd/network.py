@@ -248,3 +248,70 @@ def _device_rows(train):
         assert train.dtype.itemsize == 1 and train.dim() == 2 and train.is_cuda
         return train.data_ptr(), train.stride(0), train.shape[0]
     raise TypeError("expected a torch uint8 tensor on the GPU")
+
+
+class Batch:
+    """A ragged pack of networks stepped together, one launch per step
+    (the cbm_batch_* C-ABI).  The networks stay usable on their own between
+    batched runs and must outlive the batch."""
+
+    def __init__(self, nets: list, *, cells_per_thread: int = 0, graph_steps: int = 0):
+        L = _abi.hip_lib()
+        o = _abi.BatchOptions()
+        o.cellsPerThread, o.graphSteps = cells_per_thread, graph_steps
+        arr = (C.c_void_p * len(nets))(*[n._h.value for n in nets])
+        b = C.c_void_p()
+        check(L.cbm_batch_create(arr, len(nets), C.byref(o), C.byref(b)), "cbm_batch_create")
+        self._b = b
+        self.nets = list(nets)
+
+    def _trains(self, trains, n):
+        if len(trains) != len(self.nets):
+            raise ValueError("one MF train per network")
+        rows = [_device_rows(t) for t in trains]
+        n = min(r[2] for r in rows) if n is None else n
+        ptrs = (C.c_void_p * len(rows))(*[r[0] for r in rows])
+        pitch = (C.c_size_t * len(rows))(*[r[1] for r in rows])
+        return ptrs, pitch, n
+
+    def run_mf_train(self, trains, n: int | None = None, err_drive=None) -> None:
+        """trains[i]: uint8 [steps, numMF_i] device tensor; err_drive[i]: None
+        or float32 [steps, numZones_i] relative drive."""
+        ptrs, pitch, n = self._trains(trains, n)
+        err = None
+        if err_drive is not None:
+            keep = [None if e is None else np.ascontiguousarray(e, dtype=np.float32) for e in err_drive]
+            err = (C.c_void_p * len(keep))(*[None if e is None else e.ctypes.data for e in keep])
+        check(_abi.hip_lib().cbm_batch_run_mf_train(self._b, ptrs, pitch, n, err), "cbm_batch_run_mf_train")
+
+    def profile(self, trains, n: int | None = None) -> float:
+        """Mean time of one batched step in us over n steps."""
+        ptrs, pitch, n = self._trains(trains, n)
+        us = C.c_double()
+        check(_abi.hip_lib().cbm_batch_profile(self._b, ptrs, pitch, n, C.byref(us)), "cbm_batch_profile")
+        return us.value
+
+    def autotune(self, trains, steps: int = 20) -> int:
+        ptrs, pitch, _ = self._trains(trains, steps)
+        c = C.c_int()
+        check(_abi.hip_lib().cbm_batch_autotune(self._b, ptrs, pitch, steps, C.byref(c)), "cbm_batch_autotune")
+        return c.value
+
+    def info(self) -> dict:
+        c, blocks, g = C.c_int(), C.c_int(), C.c_int()
+        check(_abi.hip_lib().cbm_batch_info(self._b, C.byref(c), C.byref(blocks), C.byref(g)), "cbm_batch_info")
+        return {"cells_per_thread": c.value, "blocks": blocks.value, "graph_steps": g.value}
+
+    def sync(self) -> None:
+        check(_abi.hip_lib().cbm_batch_sync(self._b), "cbm_batch_sync")
+
+    def close(self) -> None:
+        if getattr(self, "_b", None) is not None and self._b.value:
+            _abi.hip_lib().cbm_batch_destroy(self._b)
+            self._b = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -42,7 +42,7 @@
 extern "C" {
 #endif
 
-#define CBM_ABI_VERSION 2
+#define CBM_ABI_VERSION 3
 
 /* ---------------------------------------------------------------- errors */
 #define CBM_OK 0
@@ -415,6 +415,57 @@ int cbm_profile_split(cbm_handle *h, const uint8_t *mf_dev, size_t pitch, int n,
 /* The launch mode the handle uses (1 = fused, 2 = two streams), or a negative
  * CBM_E_* code. */
 int cbm_step_mode(cbm_handle *h);
+
+/* ---------------------------------------------------------------- batches */
+/* A ragged pack of committed networks stepped together, one launch per step
+ * for all of them: block n < count steps network n's small populations, the
+ * other blocks are the GR tiles of every network from a tile table built
+ * largest network first.  Each network keeps its own sizes, parameters, seeds
+ * and state, and a batched run leaves every network exactly as its own
+ * cbm_run_mf_train would.  No reference counterpart: the reference steps one
+ * network per process. */
+typedef struct cbm_batch cbm_batch;
+
+typedef struct cbm_batch_options {
+    /* GR cells per thread of the batched tiles: 0 = 2, or 1 or 2 (see
+     * cbm_batch_autotune). */
+    uint32_t cellsPerThread;
+    /* Steps per captured HIP graph (0 = plain launches): runs of at least this
+     * many steps replay one graph per window. */
+    uint32_t graphSteps;
+} cbm_batch_options;
+
+/* The networks must be committed, share a device, have numGR a multiple of
+ * 2048 and belong to no other batch.  They stay owned by the caller and must
+ * outlive the batch (cbm_destroy refuses a network still in a batch).  Between
+ * batched runs each network can be read, written and stepped on its own. */
+int cbm_batch_create(cbm_handle *const *nets, int count, const cbm_batch_options *opts, cbm_batch **out);
+int cbm_batch_destroy(cbm_batch *b);
+
+/* n steps of every network.  Network i reads MF row k at mf_dev[i] + k *
+ * pitch[i] (device memory, as cbm_run_mf_train) and, when err_host and
+ * err_host[i] are non-null, the relative error drive err_host[i][k *
+ * numZones + z] (0 = none). */
+int cbm_batch_run_mf_train(cbm_batch *b, const uint8_t *const *mf_dev, const size_t *pitch, int n,
+                           const float *const *err_host);
+
+/* Wait for the batch's queued work. */
+int cbm_batch_sync(cbm_batch *b);
+
+/* n batched steps (as cbm_batch_run_mf_train without error drive), reporting
+ * the mean time per batched step in microseconds (HIP events around the
+ * launches on the batch's stream). */
+int cbm_batch_profile(cbm_batch *b, const uint8_t *const *mf_dev, const size_t *pitch, int n, double *step_us);
+
+/* Autotuner of the GR tile shape: times `steps` batched steps with 1 and with
+ * 2 GR cells per thread on the batch's own networks (each trial starts from a
+ * device snapshot of every network), keeps the faster shape, and restores
+ * every network to its state before the call. */
+int cbm_batch_autotune(cbm_batch *b, const uint8_t *const *mf_dev, const size_t *pitch, int steps,
+                       int *chosen_cells_per_thread);
+
+/* Current GR cells per thread, launch blocks per step and graph window. */
+int cbm_batch_info(cbm_batch *b, int *cells_per_thread, int *blocks, int *graph_steps);
 
 /* cbm_field_shape / cbm_field_is_float as exported functions, for hosts that
  * cannot use the inline versions (FFI bindings). */
