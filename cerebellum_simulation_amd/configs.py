@@ -224,6 +224,27 @@ def batch_plan(mix=None, base_seed: int = 1000):
     return plan
 
 
+def cs_mfs(num_mf: int, seed: int = 0, cs_frac: float = 0.03, ctx_frac: float = 0.10) -> np.ndarray:
+    """Indices of the CS MFs that mf_train(num_mf, ..., seed=seed) drives."""
+    rng = np.random.default_rng(seed)
+    n_ctx = int(round(ctx_frac * num_mf))
+    rng.permutation(num_mf)
+    rng.uniform(1.0, 10.0, num_mf)
+    rng.uniform(30.0, 60.0, n_ctx)
+    return np.sort(rng.choice(num_mf, max(1, int(cs_frac * num_mf)), replace=False))
+
+
+def trained_pfpc_weights(state, cs: np.ndarray, w_cs: float = 0.0, w_other: float = 0.5) -> np.ndarray:
+    """PF->PC weights of a network that has learned the CS: GRs with at least
+    one CS MF among their inputs at w_cs, the others at w_other."""
+    s = state.sizes
+    pm = state.field("P_GR_FROM_MF_TO_GR")
+    nm = state.field("NUMP_GR_FROM_MF_TO_GR")
+    valid = np.arange(pm.shape[1])[None, :] < nm[:, None]
+    hit = (np.isin(pm, cs) & valid).any(axis=1)
+    return np.where(hit, np.float32(w_cs), np.float32(w_other)).astype(np.float32)
+
+
 def mf_train(num_mf: int, steps: int, seed: int = 0, *, t0: int = 0, trial_ms: int = 2000,
              cs: bool = True, cs_frac: float = 0.03, cs_on: int = 500, cs_off: int = 1000,
              cs_hz: float = 100.0, ctx_frac: float = 0.10) -> np.ndarray:

@@ -100,6 +100,12 @@ struct SmallArgs {
     cbm_act_params p;
     uint32_t grgoLag;
     uint32_t anyCoupling;      // some GO has a coupling input and ratio != 0
+    // spike recorder: row (t mod recCap) of rec holds the recorded populations'
+    // spikes of step t; recOff[k] is population k's offset in the row
+    // (CBM_REC_* order: PC, BC, SC, IO, NC, GO; UINT32_MAX = not recorded)
+    uint8_t *rec;
+    uint32_t recRow, recCap;
+    uint32_t recOff[6];
 };
 
 // Per-step values of the small-population kernel (passed by value; the

@@ -62,6 +62,9 @@ struct cbm_handle {
     uint8_t *pinnedMF = nullptr;
     uint32_t anyCoupling = 0;
     SmallArgs *smallArgs = nullptr;  // device copy of the constant small-kernel arguments
+    SmallArgs smallHost{};           // host copy of the same (re-uploaded when the recorder changes)
+    uint8_t *rec = nullptr;          // spike recorder ring (owned; freed on change and destroy)
+    uint64_t recFrom = 0;            // first step written to the current recorder
     size_t smallLds = 0;             // dynamic LDS of the small kernel
 
     uint64_t curTime = 0;

@@ -269,6 +269,7 @@ int cbm_destroy(cbm_handle *h)
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     if (h->stream2) (void)hipStreamSynchronize(h->stream2);
     for (void *p : h->allocs) (void)hipFree(p);
+    if (h->rec) (void)hipFree(h->rec);
     if (h->pinnedMF) (void)hipHostFree(h->pinnedMF);
     if (h->evG) (void)hipEventDestroy(h->evG);
     if (h->evS) (void)hipEventDestroy(h->evS);
@@ -650,6 +651,10 @@ int cbm_commit(cbm_handle *h)
         a.p = h->p;
         a.grgoLag = h->o.grgoLag;
         a.anyCoupling = h->anyCoupling;
+        a.rec = nullptr;
+        a.recRow = a.recCap = 0;
+        for (int k = 0; k < 6; k++) a.recOff[k] = 0xFFFFFFFFu;
+        h->smallHost = a;
         if (h->smallLds > 64 * 1024)
             HIP_TRY(hipFuncSetAttribute((const void *)small_step_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                         (int)h->smallLds));
@@ -743,6 +748,78 @@ int cbm_read(cbm_handle *h, int field, int zone, void *dst, size_t bytes)
     }
     if (!h->dfield[field]) return fail(CBM_E_ARG, "field is not readable");
     HIP_TRY(hipMemcpy(dst, (uint8_t *)h->dfield[field] + off, bytes, hipMemcpyDeviceToHost));
+    return CBM_OK;
+}
+
+// cells of recorder population k, zones stacked
+static uint32_t recCells(const cbm_sizes &s, int k)
+{
+    const uint32_t nZ = s.numZones;
+    switch (k) {
+    case CBM_REC_PC: return nZ * s.numPC;
+    case CBM_REC_BC: return nZ * s.numBC;
+    case CBM_REC_SC: return s.numSC;
+    case CBM_REC_IO: return nZ * s.numIO;
+    case CBM_REC_NC: return nZ * s.numNC;
+    case CBM_REC_GO: return s.numGO;
+    default: return 0;
+    }
+}
+
+int cbm_set_recorder(cbm_handle *h, uint32_t mask, uint32_t capacity)
+{
+    if (!h) return fail(CBM_E_ARG, "null handle");
+    if (!h->committed) return fail(CBM_E_STATE, "cbm_commit has not been called");
+    if (mask >> 6) return fail(CBM_E_ARG, "unknown recorder population");
+    HIP_TRY(hipSetDevice(h->o.device));
+    int rc = syncAll(h);
+    if (rc) return rc;
+    SmallArgs &a = h->smallHost;
+    if (h->rec) (void)hipFree(h->rec);
+    h->rec = nullptr;
+    a.rec = nullptr;
+    a.recRow = a.recCap = 0;
+    for (int k = 0; k < 6; k++) a.recOff[k] = 0xFFFFFFFFu;
+    if (mask && capacity) {
+        uint32_t row = 0;
+        for (int k = 0; k < 6; k++)
+            if (mask & (1u << k)) {
+                a.recOff[k] = row;
+                row += recCells(h->s, k);
+            }
+        HIP_TRY(hipMalloc((void **)&h->rec, (size_t)row * capacity));
+        HIP_TRY(hipMemset(h->rec, 0, (size_t)row * capacity));
+        a.rec = h->rec;
+        a.recRow = row;
+        a.recCap = capacity;
+        h->recFrom = h->curTime + 1;
+    }
+    HIP_TRY(hipMemcpy(h->smallArgs, &a, sizeof(SmallArgs), hipMemcpyHostToDevice));
+    return CBM_OK;
+}
+
+int cbm_read_recorded(cbm_handle *h, int population, uint64_t t0, int n, uint8_t *dst)
+{
+    if (!h || (n > 0 && !dst) || n < 0) return fail(CBM_E_ARG, "bad argument");
+    const SmallArgs &a = h->smallHost;
+    if (population < 0 || population >= 6 || !a.rec || a.recOff[population] == 0xFFFFFFFFu)
+        return fail(CBM_E_ARG, "population is not recorded");
+    if (n == 0) return CBM_OK;
+    if (t0 < h->recFrom || t0 + (uint64_t)n - 1 > h->curTime || h->curTime - t0 >= a.recCap)
+        return fail(CBM_E_ARG, "steps outside the recorded window");
+    HIP_TRY(hipSetDevice(h->o.device));
+    int rc = syncAll(h);
+    if (rc) return rc;
+    const uint32_t cells = recCells(h->s, population);
+    int done = 0;
+    while (done < n) {
+        const uint64_t t = t0 + (uint64_t)done;
+        const uint32_t r = (uint32_t)(t % a.recCap);
+        const int run = (int)std::min<uint64_t>((uint64_t)(n - done), (uint64_t)(a.recCap - r));
+        HIP_TRY(hipMemcpy2D(dst + (size_t)done * cells, cells, h->rec + (size_t)r * a.recRow + a.recOff[population],
+                            a.recRow, cells, run, hipMemcpyDeviceToHost));
+        done += run;
+    }
     return CBM_OK;
 }
 

@@ -203,6 +203,26 @@ class Network:
               "cbm_profile_split")
         return gr.value, sm.value
 
+    REC = {"PC": 0, "BC": 1, "SC": 2, "IO": 3, "NC": 4, "GO": 5}
+
+    def set_recorder(self, populations, capacity: int) -> None:
+        """Record the spikes of the named populations ("PC", "NC", ...) of
+        every following step into a device ring of `capacity` steps."""
+        mask = 0
+        for p in populations:
+            mask |= 1 << self.REC[p]
+        check(_abi.hip_lib().cbm_set_recorder(self._h, mask, capacity), "cbm_set_recorder")
+
+    def read_recorded(self, population: str, t0: int, n: int) -> np.ndarray:
+        """uint8 [n, cells] spikes of steps t0 .. t0+n-1 (zones stacked)."""
+        s, z = self.sizes, self.sizes.numZones
+        cells = {"PC": z * s.numPC, "BC": z * s.numBC, "SC": s.numSC, "IO": z * s.numIO, "NC": z * s.numNC,
+                 "GO": s.numGO}[population]
+        out = np.empty((n, cells), np.uint8)
+        check(_abi.hip_lib().cbm_read_recorded(self._h, self.REC[population], t0, n, out.ctypes.data),
+              "cbm_read_recorded")
+        return out
+
     @property
     def step_mode(self) -> int:
         m = _abi.hip_lib().cbm_step_mode(self._h)

@@ -42,7 +42,7 @@
 extern "C" {
 #endif
 
-#define CBM_ABI_VERSION 3
+#define CBM_ABI_VERSION 4
 
 /* ---------------------------------------------------------------- errors */
 #define CBM_OK 0
@@ -415,6 +415,25 @@ int cbm_profile_split(cbm_handle *h, const uint8_t *mf_dev, size_t pitch, int n,
 /* The launch mode the handle uses (1 = fused, 2 = two streams), or a negative
  * CBM_E_* code. */
 int cbm_step_mode(cbm_handle *h);
+
+/* ---------------------------------------------------------------- recorder */
+/* Populations the spike recorder can keep (bit masks for cbm_set_recorder, and
+ * the population argument of cbm_read_recorded). */
+enum cbm_rec_population {
+    CBM_REC_PC = 0, CBM_REC_BC = 1, CBM_REC_SC = 2, CBM_REC_IO = 3, CBM_REC_NC = 4, CBM_REC_GO = 5
+};
+
+/* Spike recorder for trial readouts (the per-ms exportAP* calls of the
+ * reference's trial drivers, e.g. Eyelid_Sim_CBM_Dev/src/ectrial/
+ * ecmanagementdelay.cpp:233-266, without a host round trip per step): from
+ * the next step on, step t writes the spike bytes of every population whose
+ * bit (1 << CBM_REC_x) is in `mask` into row t mod `capacity` of a device
+ * ring.  capacity 0 (or mask 0) turns recording off. */
+int cbm_set_recorder(cbm_handle *h, uint32_t mask, uint32_t capacity);
+
+/* Copy the recorded spikes of one population for steps t0 .. t0+n-1 (all
+ * still in the ring) into dst, [n][cells] bytes with zones stacked. */
+int cbm_read_recorded(cbm_handle *h, int population, uint64_t t0, int n, uint8_t *dst);
 
 /* ---------------------------------------------------------------- batches */
 /* A ragged pack of committed networks stepped together, one launch per step

@@ -121,6 +121,9 @@ class Oracle:
     def set_grpc_plast_steps(self, zone, ltd, ltp):
         _check(self.L.orc_set_grpc_steps(self._h, zone, ltd, ltp), "set_grpc_steps")
 
+    def reset_grpc_plast_steps(self, zone):
+        self.set_grpc_plast_steps(zone, self.params.synLTDStepSizeGRtoPC, self.params.synLTPStepSizeGRtoPC)
+
     def set_ginc_gr_to_go(self, inc):
         _check(self.L.orc_set_ginc_grgo(self._h, inc), "set_ginc_grgo")
 

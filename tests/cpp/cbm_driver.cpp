@@ -7,7 +7,9 @@
 //   cbm_driver <actParams> <conParams> <steps> <seed> <out.bin>
 //
 // Writes the final exports as raw arrays (apBufGR, vmGR, apBufGO, vmGO,
-// apBufSC, apBufPC, vmPC, apBufBC, apBufIO, vmIO, apBufNC, vmNC) for comparison.
+// apBufSC, apBufPC, vmPC, apBufBC, apBufIO, vmIO, apBufNC, vmNC) and the eyelid
+// trace of the EyelidIntegrator on zone 0's NC spikes (one float per step) for
+// comparison.
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -15,6 +17,7 @@
 
 #include <CBMCoreInclude/interface/cbmsimcore.h>
 #include <CBMStateInclude/interfaces/cbmstate.h>
+#include <CBMToolsInclude/eyelidintegrator.h>
 
 int main(int argc, char **argv)
 {
@@ -31,12 +34,16 @@ int main(int argc, char **argv)
         CBMSimCore sim(&state, &rSeed);
         const unsigned numMF = state.sizes().numMF;
         std::vector<ct_uint8_t> mf(numMF);
+        // the delay protocol's readout (ecmanagementdelay.cpp:92-93)
+        EyelidIntegrator eyelid(state.sizes().numNC, 1, 10.5, 0.012, -0.13, 0.13, 100);
+        std::vector<float> lid;
         for (int t = 0; t < steps; t++) {
             for (unsigned i = 0; i < numMF; i++) mf[i] = ((i * 7u + (unsigned)t * 13u) % 97u) < 3u;
             if (t == steps / 2) sim.updateErrDrive(0, 1.0f);
             if (t == steps / 3) sim.getMZoneList()[0]->setGRPCPlastSteps(-0.0009f, 0.0005f);
             sim.updateMFInput(mf.data());
             sim.calcActivity();
+            lid.push_back(eyelid.calcStep(sim.getMZoneList()[0]->exportAPNC()));
         }
         InNetInterface *in = sim.getInputNet();
         MZoneInterface *z = sim.getMZoneList()[0];
@@ -55,6 +62,7 @@ int main(int argc, char **argv)
         std::fwrite(z->exportVmIO(), 4, s.numIO, f);
         std::fwrite(z->exportAPBufNC(), 4, s.numNC, f);
         std::fwrite(z->exportVmNC(), 4, s.numNC, f);
+        std::fwrite(lid.data(), 4, lid.size(), f);
         std::fclose(f);
         std::printf("cbm_driver: %d steps of %u GR done\n", steps, s.numGR);
     } catch (const std::exception &e) {

@@ -47,6 +47,9 @@ def test_driver_matches_python_mirror(gpu, tmp_path):
     st = NetState(configs.act_params_text(), CON, num_zones=1, innet_seed=seed, n_mf_per_go=16)
     net = Network.from_state(st)
     s = st.sizes
+    from cerebellum_simulation_amd.eyelid import EyelidIntegrator
+    eye = EyelidIntegrator(s.numNC)
+    lid = []
     idx = np.arange(s.numMF, dtype=np.uint64)
     for t in range(steps):
         mf = (((idx * 7 + t * 13) % 97) < 3).astype(np.uint8)
@@ -56,6 +59,7 @@ def test_driver_matches_python_mirror(gpu, tmp_path):
             net.set_grpc_plast_steps(0, -0.0009, 0.0005)
         net.update_mf_input(mf)
         net.calc_activity(1)
+        lid.append(eye.step(net.read("AP_NC", 0)))
     raw = np.fromfile(out, dtype=np.uint32)
     parts = [("APBUF_GR", s.numGR), ("VM_GR", s.numGR), ("APBUF_GO", s.numGO), ("VM_GO", s.numGO),
              ("APBUF_SC", s.numSC), ("APBUF_PC", s.numPC), ("VM_PC", s.numPC), ("APBUF_BC", s.numBC),
@@ -66,4 +70,5 @@ def test_driver_matches_python_mirror(gpu, tmp_path):
         off += n
         want = net.read(name).ravel().view(np.uint32)
         assert np.array_equal(got, want), name
-    assert off == raw.size
+    assert np.array_equal(raw[off:].view(np.float32), np.array(lid, np.float32)), "eyelid trace"
+    assert raw.size == off + steps

@@ -243,3 +243,43 @@ def test_binary32_mimic_tracks_binary64_replay():
     assert spk < 1e-3
     same = o32.read("APBUF_GR") == o64.read("APBUF_GR")
     assert np.allclose(v32[same], v64[same], rtol=1e-4, atol=1e-4)
+
+
+def test_eyelid_integrator_closed_forms():
+    """EyelidIntegrator (eyelidintegrator.cpp:13-68): no input keeps the lid
+    open; a constant spike train drives gIn to gInc/(1-gDecay) per cell and the
+    lid to gSum*evMax/(gLeak+gSum); binary32 stays within 1e-5 of a binary64
+    run of the same recurrence."""
+    import numpy as np
+    from cerebellum_simulation_amd.eyelid import EyelidIntegrator, cr_onset
+    e = EyelidIntegrator(8)
+    assert all(e.step(np.zeros(8, np.uint8)) == 0 for _ in range(50))
+    e = EyelidIntegrator(8)
+    ones = np.ones(8, np.uint8)
+    for _ in range(3000):
+        v = e.step(ones)
+    gd, gi, gs, gl, em = (float(x) for x in (e.gDecay, e.gInc, e.gShift, e.gLeak, e.evMax))
+    g_ss = gi / (1 - gd)
+    assert abs(float(e.gIn[0]) - g_ss) <= 1e-5 * g_ss
+    gsum = 8 * g_ss + gs
+    v_ss = gsum * em / (gl + gsum)
+    assert abs(float(v) - v_ss) <= 1e-4 * v_ss
+    # binary64 run of the same recurrence on a random raster
+    rng = np.random.default_rng(3)
+    raster = (rng.random((400, 8)) < 0.4).astype(np.uint8)
+    e = EyelidIntegrator(8)
+    tr32 = e.trace(raster).astype(np.float64)
+    g = np.zeros(8)
+    vm = 0.0
+    tr64 = []
+    for row in raster:
+        g = g * gd + row * gi
+        s = max(g.sum() + gs, 0.0)
+        vm = vm + gl * (0 - vm) + s * (em - vm)
+        tr64.append(vm)
+    assert np.max(np.abs(tr32 - np.array(tr64))) <= 1e-3
+    # onset: first CS-window ms above baseline + 5
+    tr = np.zeros(2000, np.float32)
+    tr[640:900] = 20.0
+    assert cr_onset(tr, 500, 1000) == 640
+    assert cr_onset(np.zeros(2000, np.float32), 500, 1000) is None
