@@ -31,6 +31,9 @@ struct GrArgs {
     const uint2 *goOut;        // 4 x u16 GO targets
     const uint32_t *goOutDly;  // 4 x u8 delay bit per target
     const uint8_t *bspDly;     // delay bit towards BC/PC/SC
+    // compact format (nullptr: the arrays above): one 16-byte record per GR,
+    // see packGrConn
+    const uint4 *conn;
     uint32_t numGR, numMF, numGO, numZones, numIO, numPC, perPC;  // perPC: 64-GR chunks per PC
     uint32_t log2Lbc, log2Lsc, log2GRperIO;
     float gLeak, eLeak, eGO, threshRest, threshMax, threshDec;
@@ -142,6 +145,45 @@ struct BatchNet {
     uint32_t skipFold0;        // the host wrote the PF->PC sums of step t0
     uint32_t grgoLag, tsPerHistBinGR, tsPerPopHistBinPC, numTSinMFHist;
 };
+
+// Compact GR connectivity: up to 4 MF inputs, 3 GO inputs and 2 GO targets as
+// 11-bit cell indices (kNoCell11 = empty) and the three delays as 6-bit bit
+// indices (kNever6 = never), in two 64-bit halves:
+//   lo: m0 m1 m2 m3 g0 (11 bits each, from bit 0) | PF delay (bits 55..60)
+//   hi: g1 g2 o0 o1 (11 bits each, from bit 0) | delay o0 (44..49) | delay o1 (50..55)
+// Usable when numMF and numGO are at most 2047.
+constexpr uint32_t kNoCell11 = 0x7FFu;
+constexpr uint32_t kNever6 = 0x3Fu;
+
+__host__ __device__ inline uint4 packGrConn(const uint16_t m[4], const uint16_t g[3], const uint16_t o[2],
+                                            uint8_t dBsp, uint8_t dO0, uint8_t dO1)
+{
+    auto ix = [](uint16_t v) -> uint64_t { return v == kNoCell ? kNoCell11 : v; };
+    auto dl = [](uint8_t v) -> uint64_t { return v == kNever ? kNever6 : v; };
+    const uint64_t lo = ix(m[0]) | ix(m[1]) << 11 | ix(m[2]) << 22 | ix(m[3]) << 33 | ix(g[0]) << 44 | dl(dBsp) << 55;
+    const uint64_t hi = ix(g[1]) | ix(g[2]) << 11 | ix(o[0]) << 22 | ix(o[1]) << 33 | dl(dO0) << 44 | dl(dO1) << 50;
+    return make_uint4((uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32));
+}
+
+// the record back in the general format's registers (16-bit indices with
+// kNoCell, 8-bit delays with kNever)
+__host__ __device__ inline void unpackGrConn(const uint4 c, uint2 *mi, uint2 *gi, uint2 *go, uint32_t *gd, uint32_t *bsp)
+{
+    const uint64_t lo = (uint64_t)c.x | (uint64_t)c.y << 32, hi = (uint64_t)c.z | (uint64_t)c.w << 32;
+    auto ix = [](uint64_t w, int sh) -> uint32_t {
+        const uint32_t v = (uint32_t)(w >> sh) & kNoCell11;
+        return v == kNoCell11 ? (uint32_t)kNoCell : v;
+    };
+    auto dl = [](uint64_t w, int sh) -> uint32_t {
+        const uint32_t v = (uint32_t)(w >> sh) & kNever6;
+        return v == kNever6 ? (uint32_t)kNever : v;
+    };
+    *mi = make_uint2(ix(lo, 0) | ix(lo, 11) << 16, ix(lo, 22) | ix(lo, 33) << 16);
+    *gi = make_uint2(ix(lo, 44) | ix(hi, 0) << 16, ix(hi, 11) | (uint32_t)kNoCell << 16);
+    *go = make_uint2(ix(hi, 22) | ix(hi, 33) << 16, (uint32_t)kNoCell | (uint32_t)kNoCell << 16);
+    *gd = dl(hi, 44) | dl(hi, 50) << 8 | (uint32_t)kNever << 16 | (uint32_t)kNever << 24;
+    *bsp = dl(lo, 55);
+}
 
 // Bytes of dynamic LDS a GR tile of nt threads x cpt cells uses.
 __host__ __device__ constexpr uint32_t grTileLds(uint32_t nt, uint32_t cpt, uint32_t numGO, uint32_t numMF,

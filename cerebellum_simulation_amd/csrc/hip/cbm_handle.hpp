@@ -51,6 +51,7 @@ struct cbm_handle {
     void *dfield[CBM_F_COUNT] = {};
 
     uint2 *mfIn = nullptr, *goIn = nullptr, *goOut = nullptr;
+    uint4 *conn = nullptr;     // compact GR connectivity (nullptr: the five arrays)
     uint32_t *goOutDly = nullptr;
     uint8_t *bspDly = nullptr;
     float *chunk2 = nullptr, *pfpcRing = nullptr;  // chunk sums [2][nZ][numGR/64] by step parity

@@ -105,11 +105,15 @@ __device__ __forceinline__ void gr_tile(const GrArgs &a, const GrStep &st, const
         gE[j] = a.gE[gr];
         gI[j] = a.gI[gr];
         ab[j] = a.apBuf[gr];
-        mi[j] = a.mfIn[gr];
-        gi[j] = a.goIn[gr];
-        go[j] = a.goOut[gr];
-        gd[j] = a.goOutDly[gr];
-        bsp[j] = a.bspDly[gr];
+        if (a.conn) {
+            unpackGrConn(a.conn[gr], &mi[j], &gi[j], &go[j], &gd[j], &bsp[j]);
+        } else {
+            mi[j] = a.mfIn[gr];
+            gi[j] = a.goIn[gr];
+            go[j] = a.goOut[gr];
+            gd[j] = a.goOutDly[gr];
+            bsp[j] = a.bspDly[gr];
+        }
         w0[j] = a.w[gr];
         hist[j] = st.doPlast ? a.hist[gr] : 0;
     }

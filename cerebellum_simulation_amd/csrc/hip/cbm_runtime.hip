@@ -591,9 +591,24 @@ int cbm_commit(cbm_handle *h)
     }
 
     // ---- device storage
-    if ((rc = h->upload(&h->mfIn, mfIn)) || (rc = h->upload(&h->goIn, goIn)) || (rc = h->upload(&h->goOut, goOut)) ||
-        (rc = h->upload(&h->goOutDly, goOutDly)) || (rc = h->upload(&h->bspDly, bsp)))
+    const bool compact = h->o.connFormat != 1 && s.numMF <= 2047 && s.numGO <= 2047 && s.maxnumpGRfromMFtoGR <= 4 &&
+                         s.maxnumpGRfromGOtoGR <= 3 && s.maxnumpGRfromGRtoGO <= 2;
+    if (compact) {
+        std::vector<uint4> conn(nGR);
+        for (uint32_t i = 0; i < nGR; i++) {
+            const uint16_t m[4] = {(uint16_t)(mfIn[i].x & 0xFFFF), (uint16_t)(mfIn[i].x >> 16),
+                                   (uint16_t)(mfIn[i].y & 0xFFFF), (uint16_t)(mfIn[i].y >> 16)};
+            const uint16_t g[3] = {(uint16_t)(goIn[i].x & 0xFFFF), (uint16_t)(goIn[i].x >> 16),
+                                   (uint16_t)(goIn[i].y & 0xFFFF)};
+            const uint16_t o[2] = {(uint16_t)(goOut[i].x & 0xFFFF), (uint16_t)(goOut[i].x >> 16)};
+            conn[i] = packGrConn(m, g, o, bsp[i], (uint8_t)(goOutDly[i] & 0xFF), (uint8_t)((goOutDly[i] >> 8) & 0xFF));
+        }
+        if ((rc = h->upload(&h->conn, conn))) return rc;
+    } else if ((rc = h->upload(&h->mfIn, mfIn)) || (rc = h->upload(&h->goIn, goIn)) ||
+               (rc = h->upload(&h->goOut, goOut)) || (rc = h->upload(&h->goOutDly, goOutDly)) ||
+               (rc = h->upload(&h->bspDly, bsp))) {
         return rc;
+    }
     uint16_t *dMfgo = nullptr, *dGaba = nullptr, *dBlob = nullptr;
     if ((rc = h->upload(&dMfgo, mfgoOut)) || (rc = h->upload(&dGaba, gabaEll)) || (rc = h->upload(&dBlob, blob)))
         return rc;
@@ -910,6 +925,7 @@ void cbmdev::grConstArgs(const cbm_handle *h, GrArgs *gp_)
     g.goOut = h->goOut;
     g.goOutDly = h->goOutDly;
     g.bspDly = h->bspDly;
+    g.conn = h->conn;
     g.numGR = s.numGR;
     g.numMF = s.numMF;
     g.numGO = s.numGO;

@@ -110,13 +110,13 @@ class Network:
     """One network on one GPU behind the cbm_* C-ABI (CBMSimCore's role)."""
 
     def __init__(self, sizes: Sizes, params: ActParams, *, grgo_lag: int = 2, device: int = 0,
-                 cells_per_thread: int = 0, block_threads: int = 0, step_mode: int = 0):
+                 cells_per_thread: int = 0, block_threads: int = 0, step_mode: int = 0, conn_format: int = 0):
         """step_mode: 0 = pick, 1 = one fused launch per step, 2 = GR and
         small-population kernels on two streams (cbm_options.stepMode)."""
         L = _abi.hip_lib()
         o = Options()
         o.grgoLag, o.device, o.grCellsPerThread = grgo_lag, device, cells_per_thread
-        o.grBlockThreads, o.stepMode = block_threads, step_mode
+        o.grBlockThreads, o.stepMode, o.connFormat = block_threads, step_mode, conn_format
         h = C.c_void_p()
         check(L.cbm_create(C.byref(sizes), C.byref(params), C.byref(o), C.byref(h)), "cbm_create")
         self._h = h

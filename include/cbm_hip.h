@@ -42,7 +42,7 @@
 extern "C" {
 #endif
 
-#define CBM_ABI_VERSION 4
+#define CBM_ABI_VERSION 5
 
 /* ---------------------------------------------------------------- errors */
 #define CBM_OK 0
@@ -143,6 +143,11 @@ typedef struct cbm_options {
      * 0 steps the small populations, the other blocks are GR tiles), 2 = the GR
      * and small-population kernels on two streams, ordered by events. */
     uint32_t stepMode;
+    /* Device layout of the GR connectivity: 0 = pick (one 16-byte record per
+     * GR when numMF and numGO are at most 2047 and the fan-ins are at most 4
+     * MF, 3 GO and 2 GO targets; else 29 bytes in five arrays), 1 = the
+     * five arrays always.  Both give the same bits. */
+    uint32_t connFormat;
 } cbm_options;
 
 /* ---------------------------------------------------------------- fields */

@@ -253,7 +253,8 @@ def test_4m_gr_properties(gpu, mode):
 SHAPES = [dict(step_mode=1), dict(step_mode=1, cells_per_thread=1), dict(step_mode=2),
           dict(step_mode=2, cells_per_thread=2), dict(step_mode=2, cells_per_thread=4),
           dict(step_mode=2, block_threads=1024, cells_per_thread=1),
-          dict(step_mode=2, block_threads=1024, cells_per_thread=2)]
+          dict(step_mode=2, block_threads=1024, cells_per_thread=2), dict(step_mode=1, conn_format=1),
+          dict(step_mode=2, conn_format=1, cells_per_thread=2)]
 
 
 def test_launch_shapes_agree(gpu):
