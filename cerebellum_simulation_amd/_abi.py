@@ -92,6 +92,22 @@ _hip = None
 _state = None
 
 
+class SgDesc(C.Structure):
+    """cbm_sg_desc (include/cbm_surrogate.h)."""
+    _fields_ = [("numIn", C.c_uint32), ("hidden", C.c_uint32), ("numOut", C.c_uint32), ("taps", C.c_uint32),
+                ("dil", C.c_uint32 * 3), ("maxRows", C.c_uint32), ("device", C.c_int)]
+
+
+class SgBatch(C.Structure):
+    """cbm_sg_batch (include/cbm_surrogate.h)."""
+    _fields_ = [("mf", C.c_void_p), ("mfPitch", C.c_size_t), ("segStart", C.c_void_p), ("numSeg", C.c_int),
+                ("outputs", C.c_int), ("target", C.c_void_p), ("targetPitch", C.c_size_t)]
+
+
+class SgAdam(C.Structure):
+    _fields_ = [("lr", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float)]
+
+
 def _load(name: str) -> C.CDLL:
     path = os.path.join(LIBDIR, name)
     if not os.path.exists(path):
@@ -150,6 +166,25 @@ def hip_lib() -> C.CDLL:
         L.cbm_device_bytes.argtypes = [P, C.POINTER(C.c_size_t)]
         L.cbm_field_info.argtypes = [I, C.POINTER(Sizes), C.POINTER(S), C.POINTER(S), C.POINTER(S),
                                      C.POINTER(I), C.POINTER(I), C.POINTER(I)]
+        # learned surrogate (cbm_surrogate.h)
+        F = C.POINTER(C.c_float)
+        L.cbm_sg_create.argtypes = [C.POINTER(SgDesc), C.POINTER(P)]
+        L.cbm_sg_destroy.argtypes = [P]
+        L.cbm_sg_lead.argtypes = [P]
+        L.cbm_sg_num_params.argtypes = [P, C.POINTER(S)]
+        L.cbm_sg_get_params.argtypes = [P, V]
+        L.cbm_sg_set_params.argtypes = [P, V]
+        L.cbm_sg_init_params.argtypes = [P, C.c_uint64, C.c_float]
+        L.cbm_sg_forward.argtypes = [P, C.POINTER(SgBatch), V, S]
+        L.cbm_sg_loss_grad.argtypes = [P, C.POINTER(SgBatch), F, V]
+        L.cbm_sg_train_step.argtypes = [P, C.POINTER(SgBatch), C.POINTER(SgAdam), V]
+        L.cbm_sg_read_hidden.argtypes = [P, I, V, S]
+        L.cbm_sg_targets_from_pf.argtypes = [P, V, S, I, C.c_uint32, C.c_uint32, V, S]
+        L.cbm_sg_sync.argtypes = [P]
+        L.cbm_sg_last_timing.argtypes = [P, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.cbm_batch_sweep.argtypes = [P, V, S, I, V, S, PP, C.c_uint32]
+        L.cbm_batch_sweep_weights.argtypes = [P, I, V]
+        L.cbm_batch_sweep_reset.argtypes = [P]
         want = int(re.search(r"#define CBM_ABI_VERSION (\d+)", open(HEADER).read()).group(1))
         if L.cbm_abi_version() != want:
             raise RuntimeError(f"libcbm_hip.so ABI version {L.cbm_abi_version()} != header's {want}: rebuild")

@@ -44,7 +44,8 @@ def _run(cmd: list[str]) -> None:
 
 def build_hip() -> str:
     out = os.path.join(LIB, "libcbm_hip.so")
-    srcs = [os.path.join(CSRC, "hip", f) for f in ("cbm_runtime.hip", "cbm_batch.hip", "cbm_kernels.hip")]
+    srcs = [os.path.join(CSRC, "hip", f) for f in ("cbm_runtime.hip", "cbm_batch.hip", "cbm_kernels.hip",
+                                                      "cbm_surrogate.hip")]
     _run([HIPCC, *HIP_FLAGS, "-shared", "-o", out, *srcs])
     return out
 

@@ -247,11 +247,13 @@ def trained_pfpc_weights(state, cs: np.ndarray, w_cs: float = 0.0, w_other: floa
 
 def mf_train(num_mf: int, steps: int, seed: int = 0, *, t0: int = 0, trial_ms: int = 2000,
              cs: bool = True, cs_frac: float = 0.03, cs_on: int = 500, cs_off: int = 1000,
-             cs_hz: float = 100.0, ctx_frac: float = 0.10) -> np.ndarray:
+             cs_hz: float = 100.0, ctx_frac: float = 0.10, spike_seed: int | None = None) -> np.ndarray:
     """uint8 [steps, num_mf] Bernoulli spike train.  Rates per MF are drawn
     once per seed: background MFs U[1,10] Hz, context MFs U[30,60] Hz; when
     cs is set a seeded subset of cs_frac of the MFs fires at cs_hz during
-    [cs_on, cs_off) of every trial.  Step k is trial time (t0 + k) % trial_ms."""
+    [cs_on, cs_off) of every trial.  Step k is trial time (t0 + k) % trial_ms.
+    spike_seed, when given, draws the spikes from a generator of its own, so
+    trials of one network (same rates and CS subset) get fresh spikes."""
     rng = np.random.default_rng(seed)
     n_ctx = int(round(ctx_frac * num_mf))
     perm = rng.permutation(num_mf)
@@ -264,7 +266,19 @@ def mf_train(num_mf: int, steps: int, seed: int = 0, *, t0: int = 0, trial_ms: i
         tt = (t0 + np.arange(steps)) % trial_ms
         in_cs = (tt >= cs_on) & (tt < cs_off)
         p[np.ix_(in_cs, cs_idx)] = cs_hz * 1e-3
-    return (rng.random((steps, num_mf)) < p).astype(np.uint8)
+    draw = rng if spike_seed is None else np.random.default_rng(spike_seed)
+    return (draw.random((steps, num_mf)) < p).astype(np.uint8)
+
+
+def mf_rates(num_mf: int, seed: int = 0, *, cs_frac: float = 0.03, ctx_frac: float = 0.10):
+    """(rate in Hz per MF, CS MF indices) that mf_train(num_mf, ..., seed=seed) uses."""
+    rng = np.random.default_rng(seed)
+    n_ctx = int(round(ctx_frac * num_mf))
+    perm = rng.permutation(num_mf)
+    rate = rng.uniform(1.0, 10.0, num_mf)
+    rate[perm[:n_ctx]] = rng.uniform(30.0, 60.0, n_ctx)
+    cs_idx = rng.choice(num_mf, max(1, int(cs_frac * num_mf)), replace=False)
+    return rate, np.sort(cs_idx)
 
 
 def us_err_drive(steps: int, num_zones: int = 1, *, t0: int = 0, trial_ms: int = 2000,

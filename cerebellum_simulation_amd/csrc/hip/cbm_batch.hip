@@ -23,6 +23,9 @@ template <int CPT>
 __global__ void batch_step_kernel(const BatchNet *nets, const uint2 *tiles, uint32_t nNets, const uint32_t *kbase,
                                   uint32_t k);
 __global__ void batch_advance_kernel(uint32_t *kbase, uint32_t steps);
+__global__ void sweep_step_kernel(const BatchNet *nets, const SweepNet *sw, uint32_t nNets, const float *prob,
+                                  uint64_t pitch, uint32_t k);
+__global__ void cluster_mean_kernel(const float *w, uint32_t numGR, uint32_t nZ, uint32_t log2Cells, float *W);
 }  // namespace cbmdev
 
 using namespace cbmdev;
@@ -44,6 +47,11 @@ struct cbm_batch {
     size_t derrCap = 0;
     hipGraphExec_t graph = nullptr;  // captured window of graphSteps launches
     int graphCpt = 0;
+    // surrogate sweep: cluster weights per network (cbm_batch_sweep)
+    std::vector<SweepNet> hsw;
+    SweepNet *dsw = nullptr;
+    size_t swLds = 0;
+    uint32_t swCells = 0;
 };
 
 namespace {
@@ -185,7 +193,7 @@ int restoreSnapshot(cbm_batch *b, Snapshot *sn)
 
 // Run n batched steps; time the launches when evs is given (one pair).
 int run(cbm_batch *b, const uint8_t *const *mf, const size_t *pitch, int n, const float *const *err,
-        hipEvent_t *evs)
+        hipEvent_t *evs, const float *prob = nullptr, size_t probPitch = 0)
 {
     if (!mf || !pitch || n < 0) return fail(CBM_E_ARG, "bad argument");
     HIP_TRY(hipSetDevice(b->device));
@@ -231,7 +239,14 @@ int run(cbm_batch *b, const uint8_t *const *mf, const size_t *pitch, int n, cons
 
     if (evs) HIP_TRY(hipEventRecord(evs[0], b->stream));
     int k = 0;
-    if (b->graphSteps && (uint32_t)n >= b->graphSteps) {
+    if (prob) {
+        for (; k < n; k++) {
+            hipLaunchKernelGGL(sweep_step_kernel, dim3(2 * (uint32_t)N), dim3(kSmallThreads), b->swLds, b->stream,
+                               b->dnet, b->dsw, (uint32_t)N, prob, (uint64_t)probPitch, (uint32_t)k);
+            HIP_TRY(hipGetLastError());
+        }
+    }
+    if (b->graphSteps && (uint32_t)n >= b->graphSteps && !prob) {
         if ((rc = captureGraph(b))) return rc;
         const uint32_t zero = 0;
         HIP_TRY(hipMemcpyAsync(b->kbase, &zero, 4, hipMemcpyHostToDevice, b->stream));
@@ -248,7 +263,9 @@ int run(cbm_batch *b, const uint8_t *const *mf, const size_t *pitch, int n, cons
         cbm_handle *h = b->nets[i];
         h->curTime += (uint64_t)n;
         for (int z = 0; z < kMaxZones; z++) h->errDrive[z] = 0.0f;
-        h->skipFold = false;
+        // after a sweep the ring slot of the last step holds the cluster
+        // tile's PF->PC sums and the chunk sums are stale: nothing to fold
+        h->skipFold = prob != nullptr;
         HIP_TRY(hipMemcpyAsync(h->dmf, mf[i] + (size_t)(n - 1) * pitch[i], h->s.numMF, hipMemcpyDeviceToDevice,
                                b->stream));
     }
@@ -259,6 +276,49 @@ int run(cbm_batch *b, const uint8_t *const *mf, const size_t *pitch, int n, cons
         HIP_TRY(hipEventRecord(h->evG, h->stream));
         HIP_TRY(hipEventRecord(h->evS, h->stream2));
     }
+    return CBM_OK;
+}
+
+// Cluster weights of every network from its per-GR PF->PC weights.
+int sweepSetup(cbm_batch *b, uint32_t cells)
+{
+    if (!cells || (cells & (cells - 1)) || cells < 64) return fail(CBM_E_ARG, "cellsPerCluster must be a power of two >= 64");
+    uint32_t log2 = 0;
+    while ((1u << log2) < cells) log2++;
+    size_t lds = b->lds;
+    for (cbm_handle *h : b->nets) {
+        const cbm_sizes &sz = h->s;
+        const uint32_t nCl = sz.numGR / cells;
+        if (sz.numGR % cells || nCl % sz.numPC || nCl % sz.numBC || nCl % sz.numSC || nCl % sz.numIO || nCl == 0)
+            return fail(CBM_E_UNSUPPORTED, "clusters must tile every PC, BC, SC and IO block of GRs");
+        lds = std::max(lds, (size_t)4 * (sz.numZones * sz.numIO + (size_t)nCl * (1 + sz.numZones)));
+    }
+    HIP_TRY(hipSetDevice(b->device));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    for (SweepNet &w : b->hsw)
+        if (w.W) (void)hipFree(w.W);
+    b->hsw.assign(b->nets.size(), SweepNet{nullptr, 0, 0});
+    for (size_t i = 0; i < b->nets.size(); i++) {
+        cbm_handle *h = b->nets[i];
+        SweepNet &w = b->hsw[i];
+        w.nCl = h->s.numGR / cells;
+        w.log2Cells = log2;
+        const size_t n = (size_t)h->s.numZones * w.nCl;
+        HIP_TRY(hipMalloc((void **)&w.W, n * 4));
+        int rc = syncAll(h);
+        if (rc) return rc;
+        hipLaunchKernelGGL(cluster_mean_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, b->stream,
+                           b->hnet[i].gr.w, h->s.numGR, h->s.numZones, log2, w.W);
+        HIP_TRY(hipGetLastError());
+    }
+    if (!b->dsw) HIP_TRY(hipMalloc((void **)&b->dsw, b->nets.size() * sizeof(SweepNet)));
+    HIP_TRY(hipMemcpy(b->dsw, b->hsw.data(), b->hsw.size() * sizeof(SweepNet), hipMemcpyHostToDevice));
+    if (lds > 64 * 1024)
+        HIP_TRY(hipFuncSetAttribute((const void *)sweep_step_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)lds));
+    b->swLds = lds;
+    b->swCells = cells;
+    HIP_TRY(hipStreamSynchronize(b->stream));
     return CBM_OK;
 }
 
@@ -333,6 +393,9 @@ int cbm_batch_destroy(cbm_batch *b)
     if (b->dtiles) (void)hipFree(b->dtiles);
     if (b->kbase) (void)hipFree(b->kbase);
     if (b->derr) (void)hipFree(b->derr);
+    for (SweepNet &w : b->hsw)
+        if (w.W) (void)hipFree(w.W);
+    if (b->dsw) (void)hipFree(b->dsw);
     if (b->done) (void)hipEventDestroy(b->done);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
@@ -421,6 +484,48 @@ int cbm_batch_info(cbm_batch *b, int *cells_per_thread, int *blocks, int *graph_
     if (blocks) *blocks = (int)(b->nets.size() + b->ntiles);
     if (graph_steps) *graph_steps = (int)b->graphSteps;
     return CBM_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int cbm_batch_sweep(cbm_batch *b, const float *prob, size_t probPitch, int n, const uint8_t *mf_dev, size_t mfPitch,
+                    const float *const *err_host, uint32_t cellsPerCluster)
+{
+    if (!b || !prob || !mf_dev || n < 0) return fail(CBM_E_ARG, "bad argument");
+    const size_t N = b->nets.size();
+    uint32_t maxCl = 0;
+    for (cbm_handle *h : b->nets) maxCl = std::max(maxCl, h->s.numGR / std::max(cellsPerCluster, 1u));
+    if (probPitch < maxCl || ((uintptr_t)prob & 3)) return fail(CBM_E_ARG, "probability rows shorter than the cluster count");
+    hipPointerAttribute_t attr;
+    if (hipPointerGetAttributes(&attr, prob) != hipSuccess || attr.type != hipMemoryTypeDevice) {
+        (void)hipGetLastError();
+        return fail(CBM_E_ARG, "probabilities must be device memory");
+    }
+    int rc;
+    if (b->swCells != cellsPerCluster && (rc = sweepSetup(b, cellsPerCluster))) return rc;
+    std::vector<const uint8_t *> mf(N, mf_dev);
+    std::vector<size_t> pitch(N, mfPitch);
+    return run(b, mf.data(), pitch.data(), n, err_host, nullptr, prob, probPitch);
+}
+
+int cbm_batch_sweep_weights(cbm_batch *b, int net, float *host)
+{
+    if (!b || !host || net < 0 || net >= (int)b->nets.size()) return fail(CBM_E_ARG, "bad argument");
+    if (!b->swCells) return fail(CBM_E_STATE, "no sweep has run");
+    HIP_TRY(hipSetDevice(b->device));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    const SweepNet &w = b->hsw[net];
+    HIP_TRY(hipMemcpy(host, w.W, (size_t)b->nets[net]->s.numZones * w.nCl * 4, hipMemcpyDeviceToHost));
+    return CBM_OK;
+}
+
+int cbm_batch_sweep_reset(cbm_batch *b)
+{
+    if (!b) return fail(CBM_E_ARG, "null batch");
+    if (!b->swCells) return CBM_OK;
+    return sweepSetup(b, b->swCells);
 }
 
 }  // extern "C"

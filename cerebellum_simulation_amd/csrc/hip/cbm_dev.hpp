@@ -52,6 +52,7 @@ struct GrStep {
     uint32_t *bcCnt, *scCnt, *goCnt;  // this step's ring slot (zeroed beforehand)
     const int32_t *plastTimer;        // [numZones][numIO] as of the previous step
     uint32_t doPlast;          // t % tsPerHistBinGR == 0: plasticity + history this step
+    uint8_t *pfCnt;            // [numGR/64] delayed PF spikes per 64-GR chunk (nullptr: not recorded)
 };
 
 // Byte offsets of the small kernel's dynamic LDS regions (16-byte aligned),
@@ -144,6 +145,13 @@ struct BatchNet {
     float gIncGRtoGO;
     uint32_t skipFold0;        // the host wrote the PF->PC sums of step t0
     uint32_t grgoLag, tsPerHistBinGR, tsPerPopHistBinPC, numTSinMFHist;
+};
+
+// Per network of a surrogate sweep: cluster weights [numZones][nCl] and the
+// cluster size (2^log2Cells consecutive GRs).
+struct SweepNet {
+    float *W;
+    uint32_t nCl, log2Cells;
 };
 
 // Compact GR connectivity: up to 4 MF inputs, 3 GO inputs and 2 GO targets as

@@ -66,6 +66,7 @@ struct cbm_handle {
     SmallArgs smallHost{};           // host copy of the same (re-uploaded when the recorder changes)
     uint8_t *rec = nullptr;          // spike recorder ring (owned; freed on change and destroy)
     uint64_t recFrom = 0;            // first step written to the current recorder
+    uint32_t recOffPF = 0xFFFFFFFFu; // offset of the PF chunk counts in a recorder row
     size_t smallLds = 0;             // dynamic LDS of the small kernel
 
     uint64_t curTime = 0;

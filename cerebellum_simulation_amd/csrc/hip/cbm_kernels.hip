@@ -204,6 +204,7 @@ __device__ __forceinline__ void gr_tile(const GrArgs &a, const GrStep &st, const
         }
 
         // PF->BC and PF->SC spike counts of this wave's 64 GRs
+        if (st.pfCnt && lane == 0) st.pfCnt[gr >> 6] = (uint8_t)__popcll(bal);
         if (lane == 0 && bal != 0) {
             const uint32_t c = (uint32_t)__popcll(bal);
             atomicAdd(&s_sc[(gr >> a.log2Lsc) - firstSc], c);
@@ -989,6 +990,36 @@ template __global__ void step_kernel<2>(const SmallArgs *, const SmallStep, cons
 // n's small populations, the other blocks are GR tiles (network, tile) from
 // the tile table.  Run step kk = *kbase + k (kbase: device counter advanced by
 // a captured graph, or nullptr).
+// The small populations of network d at run step kk (block-wide).
+__device__ __forceinline__ void batch_small(const BatchNet &d, uint32_t kk, float *s_err, uint8_t *lds)
+{
+    const uint64_t t = d.t0 + kk + 1;
+    const uint32_t par = (uint32_t)(t & 1u);
+    const uint32_t nZ = d.gr.numZones;
+    SmallStep st;
+    st.mf = d.mf + (size_t)kk * d.pitch;
+    st.t = t;
+    if (threadIdx.x < (uint32_t)kMaxZones) {
+        const uint32_t z = threadIdx.x;
+        float e = kk == 0 ? d.errDrive0[z] : 0.0f;
+        if (d.err && z < nZ) {
+            const float r = d.err[(size_t)kk * nZ + z];
+            if (r != 0.0f) e = r * d.maxExtIncVIO;
+        }
+        s_err[z] = e;
+    }
+    st.gIncGRtoGO = d.gIncGRtoGO;
+    st.par = par;
+    st.slotPrev = (uint32_t)((t + kRing - 1) % kRing);
+    st.slotT2 = (uint32_t)((t + kRing - 2) % kRing);
+    st.slotG = (uint32_t)((t + kRing - d.grgoLag) % kRing);
+    st.slotNext = (uint32_t)((t + 1) % kRing);
+    st.popHist = (t % d.tsPerPopHistBinPC) == 0;
+    st.resetMFHist = (t % d.numTSinMFHist) == 0;
+    __syncthreads();
+    small_step(*d.small, st, s_err, lds);
+}
+
 template <int CPT>
 __global__ void __launch_bounds__(kSmallThreads) batch_step_kernel(const BatchNet *__restrict__ nets,
                                                                    const uint2 *__restrict__ tiles, uint32_t nNets,
@@ -1005,28 +1036,7 @@ __global__ void __launch_bounds__(kSmallThreads) batch_step_kernel(const BatchNe
     const uint8_t *mf = d.mf + (size_t)kk * d.pitch;
     const uint32_t nZ = d.gr.numZones;
     if (small) {
-        SmallStep st;
-        st.mf = mf;
-        st.t = t;
-        if (threadIdx.x < (uint32_t)kMaxZones) {
-            const uint32_t z = threadIdx.x;
-            float e = kk == 0 ? d.errDrive0[z] : 0.0f;
-            if (d.err && z < nZ) {
-                const float r = d.err[(size_t)kk * nZ + z];
-                if (r != 0.0f) e = r * d.maxExtIncVIO;
-            }
-            s_err[z] = e;
-        }
-        st.gIncGRtoGO = d.gIncGRtoGO;
-        st.par = par;
-        st.slotPrev = (uint32_t)((t + kRing - 1) % kRing);
-        st.slotT2 = (uint32_t)((t + kRing - 2) % kRing);
-        st.slotG = (uint32_t)((t + kRing - d.grgoLag) % kRing);
-        st.slotNext = (uint32_t)((t + 1) % kRing);
-        st.popHist = (t % d.tsPerPopHistBinPC) == 0;
-        st.resetMFHist = (t % d.numTSinMFHist) == 0;
-        __syncthreads();
-        small_step(*d.small, st, s_err, lds);
+        batch_small(d, kk, s_err, lds);
     } else {
         const GrArgs &g = d.gr;
         const uint32_t slot = (uint32_t)(t % kRing);
@@ -1042,6 +1052,7 @@ __global__ void __launch_bounds__(kSmallThreads) batch_step_kernel(const BatchNe
         gs.goCnt = d.goRing + (size_t)slot * g.numGO;
         gs.plastTimer = d.timer2 + (size_t)parPrev * nZ * g.numIO;
         gs.doPlast = (t % d.tsPerHistBinGR) == 0;
+        gs.pfCnt = nullptr;
         gr_tile<CPT, kSmallThreads>(g, gs, tt.y, reinterpret_cast<uint32_t *>(lds));
     }
 }
@@ -1053,6 +1064,112 @@ template __global__ void batch_step_kernel<2>(const BatchNet *, const uint2 *, u
 __global__ void batch_advance_kernel(uint32_t *kbase, uint32_t steps)
 {
     if (threadIdx.x == 0) *kbase += steps;
+}
+
+// ---------------------------------------------------------------- surrogate sweep
+// The input layer of each network replaced by cluster probabilities: block
+// n < nNets steps network n's small populations exactly as above; block
+// nNets + n is its cluster tile.  prob row kk holds, for every cluster of
+// `cells` consecutive GRs, the probability that one of its cells emits a
+// delayed PF spike at run step kk; rows kk-199 .. kk-1 must be readable.
+// The tile writes what the GR tiles would write into this step's ring
+// slots, in expectation:
+//   PF->PC sum of PC j      = sum over its clusters of cells * W[z][c] * p[c]
+//   PF->BC / PF->SC counts  = nearest integer to sum of cells * p[c]
+//   GR->GO counts           = 0 (Golgi cells only feed the replaced layer)
+// and applies the PF->PC rule of the GR tiles (history bit 39: a spike in
+// the 5-step bin that ends 195 steps before) to the cluster weight with the
+// probability of such a spike, 1 - prod_{j=195..199} (1 - p[kk-j][c]):
+//   W = clamp(W + q * step(z, IO of c), 0, 1)   every tsPerHistBinGR steps.
+// Sums run in cluster order, so a run is deterministic.
+__device__ __forceinline__ void cluster_tile(const BatchNet &d, const SweepNet &sw, const float *prob,
+                                             uint64_t pitch, uint32_t kk, uint8_t *lds)
+{
+    const GrArgs &g = d.gr;
+    const uint64_t t = d.t0 + kk + 1;
+    const uint32_t slot = (uint32_t)(t % kRing), parPrev = (uint32_t)(t & 1u) ^ 1u;
+    const uint32_t nZ = g.numZones, nCl = sw.nCl;
+    const float cells = (float)(1u << sw.log2Cells);
+    const bool doPlast = (t % d.tsPerHistBinGR) == 0;
+    float *s_step = reinterpret_cast<float *>(lds);  // [nZ * numIO]
+    float *s_cnt = s_step + nZ * g.numIO;             // [nCl]
+    float *s_pc = s_cnt + nCl;                        // [nZ][nCl]
+    const uint32_t tid = threadIdx.x, nt = blockDim.x;
+    if (doPlast)
+        for (uint32_t i = tid; i < nZ * g.numIO; i += nt) {
+            const uint32_t z = i / g.numIO;
+            const int32_t tm = d.timer2[(size_t)parPrev * nZ * g.numIO + i];
+            s_step[i] = tm < g.ltdEnd ? g.ltd[z] : (tm > g.ltpStart ? g.ltp[z] : 0.0f);
+        }
+    __syncthreads();
+    const float *p = prob + (int64_t)kk * (int64_t)pitch;
+    for (uint32_t c = tid; c < nCl; c += nt) {
+        const float pc = p[c];
+        const float cnt = pc * cells;
+        s_cnt[c] = cnt;
+        float q = 0.0f;
+        if (doPlast) {
+            float keep = 1.0f;
+            for (int j = 195; j <= 199; j++) keep *= 1.0f - p[(int64_t)c - (int64_t)j * (int64_t)pitch];
+            q = 1.0f - keep;
+        }
+        const uint32_t io = (c << sw.log2Cells) >> g.log2GRperIO;
+        for (uint32_t z = 0; z < nZ; z++) {
+            float w = sw.W[(size_t)z * nCl + c];
+            if (doPlast) {
+                w = w + q * s_step[z * g.numIO + io];
+                w = (w > 0) * w;
+                w = (w > 1) + (w <= 1) * w;
+                sw.W[(size_t)z * nCl + c] = w;
+            }
+            s_pc[(size_t)z * nCl + c] = w * cnt;
+        }
+    }
+    __syncthreads();
+    const uint32_t numBC = d.small->s.numBC, numSC = d.small->s.numSC;
+    const uint32_t perPC = nCl / g.numPC, perBC = nCl / numBC, perSC = nCl / numSC;
+    for (uint32_t task = tid; task < nZ * g.numPC; task += nt) {
+        const uint32_t z = task / g.numPC, pc = task % g.numPC;
+        const float *src = s_pc + (size_t)z * nCl + (size_t)pc * perPC;
+        float acc = 0.0f;
+        for (uint32_t i = 0; i < perPC; i++) acc += src[i];
+        d.pfpcRing[(size_t)slot * nZ * g.numPC + task] = acc;
+    }
+    for (uint32_t b = tid; b < numBC; b += nt) {
+        float acc = 0.0f;
+        for (uint32_t i = 0; i < perBC; i++) acc += s_cnt[b * perBC + i];
+        d.bcRing[(size_t)slot * numBC + b] = (uint32_t)rintf(acc);
+    }
+    for (uint32_t sc = tid; sc < numSC; sc += nt) {
+        float acc = 0.0f;
+        for (uint32_t i = 0; i < perSC; i++) acc += s_cnt[sc * perSC + i];
+        d.scRing[(size_t)slot * numSC + sc] = (uint32_t)rintf(acc);
+    }
+}
+
+__global__ void __launch_bounds__(kSmallThreads) sweep_step_kernel(const BatchNet *__restrict__ nets,
+                                                                   const SweepNet *__restrict__ sw, uint32_t nNets,
+                                                                   const float *prob, uint64_t pitch, uint32_t k)
+{
+    extern __shared__ __align__(16) uint8_t lds[];
+    __shared__ float s_err[kMaxZones];
+    if (blockIdx.x < nNets)
+        batch_small(nets[blockIdx.x], k, s_err, lds);
+    else
+        cluster_tile(nets[blockIdx.x - nNets], sw[blockIdx.x - nNets], prob, pitch, k, lds);
+}
+
+// W[z][c] = mean of the PF->PC weights of cluster c's cells, in cell order
+__global__ void cluster_mean_kernel(const float *w, uint32_t numGR, uint32_t nZ, uint32_t log2Cells, float *W)
+{
+    const uint32_t nCl = numGR >> log2Cells;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nZ * nCl) return;
+    const uint32_t z = i / nCl, c = i % nCl, cells = 1u << log2Cells;
+    const float *src = w + (size_t)z * numGR + (size_t)c * cells;
+    float acc = 0.0f;
+    for (uint32_t j = 0; j < cells; j++) acc += src[j];
+    W[i] = acc / (float)cells;
 }
 
 }  // namespace cbmdev

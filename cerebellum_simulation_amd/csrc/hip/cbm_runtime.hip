@@ -777,6 +777,7 @@ static uint32_t recCells(const cbm_sizes &s, int k)
     case CBM_REC_IO: return nZ * s.numIO;
     case CBM_REC_NC: return nZ * s.numNC;
     case CBM_REC_GO: return s.numGO;
+    case CBM_REC_PF: return s.numGR / 64;
     default: return 0;
     }
 }
@@ -785,7 +786,7 @@ int cbm_set_recorder(cbm_handle *h, uint32_t mask, uint32_t capacity)
 {
     if (!h) return fail(CBM_E_ARG, "null handle");
     if (!h->committed) return fail(CBM_E_STATE, "cbm_commit has not been called");
-    if (mask >> 6) return fail(CBM_E_ARG, "unknown recorder population");
+    if (mask >> 7) return fail(CBM_E_ARG, "unknown recorder population");
     HIP_TRY(hipSetDevice(h->o.device));
     int rc = syncAll(h);
     if (rc) return rc;
@@ -795,11 +796,12 @@ int cbm_set_recorder(cbm_handle *h, uint32_t mask, uint32_t capacity)
     a.rec = nullptr;
     a.recRow = a.recCap = 0;
     for (int k = 0; k < 6; k++) a.recOff[k] = 0xFFFFFFFFu;
+    h->recOffPF = 0xFFFFFFFFu;
     if (mask && capacity) {
         uint32_t row = 0;
-        for (int k = 0; k < 6; k++)
+        for (int k = 0; k < 7; k++)
             if (mask & (1u << k)) {
-                a.recOff[k] = row;
+                (k < 6 ? a.recOff[k] : h->recOffPF) = row;
                 row += recCells(h->s, k);
             }
         HIP_TRY(hipMalloc((void **)&h->rec, (size_t)row * capacity));
@@ -817,8 +819,10 @@ int cbm_read_recorded(cbm_handle *h, int population, uint64_t t0, int n, uint8_t
 {
     if (!h || (n > 0 && !dst) || n < 0) return fail(CBM_E_ARG, "bad argument");
     const SmallArgs &a = h->smallHost;
-    if (population < 0 || population >= 6 || !a.rec || a.recOff[population] == 0xFFFFFFFFu)
+    if (population < 0 || population > CBM_REC_PF || !a.rec)
         return fail(CBM_E_ARG, "population is not recorded");
+    const uint32_t off = population == CBM_REC_PF ? h->recOffPF : a.recOff[population];
+    if (off == 0xFFFFFFFFu) return fail(CBM_E_ARG, "population is not recorded");
     if (n == 0) return CBM_OK;
     if (t0 < h->recFrom || t0 + (uint64_t)n - 1 > h->curTime || h->curTime - t0 >= a.recCap)
         return fail(CBM_E_ARG, "steps outside the recorded window");
@@ -831,8 +835,8 @@ int cbm_read_recorded(cbm_handle *h, int population, uint64_t t0, int n, uint8_t
         const uint64_t t = t0 + (uint64_t)done;
         const uint32_t r = (uint32_t)(t % a.recCap);
         const int run = (int)std::min<uint64_t>((uint64_t)(n - done), (uint64_t)(a.recCap - r));
-        HIP_TRY(hipMemcpy2D(dst + (size_t)done * cells, cells, h->rec + (size_t)r * a.recRow + a.recOff[population],
-                            a.recRow, cells, run, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy2D(dst + (size_t)done * cells, cells, h->rec + (size_t)r * a.recRow + off, a.recRow, cells,
+                            run, hipMemcpyDefault));
         done += run;
     }
     return CBM_OK;
@@ -991,6 +995,9 @@ static int launchStep(cbm_handle *h, const uint8_t *mf, int timing)
     gs.goCnt = h->goRing + (size_t)slot * s.numGO;
     gs.plastTimer = h->timer2 + (size_t)parPrev * s.numZones * s.numIO;
     gs.doPlast = (t % p.tsPerHistBinGR) == 0;
+    gs.pfCnt = (h->rec && h->recOffPF != 0xFFFFFFFFu)
+                   ? h->rec + (size_t)(t % h->smallHost.recCap) * h->smallHost.recRow + h->recOffPF
+                   : nullptr;
     SmallStep st;
     st.mf = mf;
     st.t = t;

@@ -203,7 +203,7 @@ class Network:
               "cbm_profile_split")
         return gr.value, sm.value
 
-    REC = {"PC": 0, "BC": 1, "SC": 2, "IO": 3, "NC": 4, "GO": 5}
+    REC = {"PC": 0, "BC": 1, "SC": 2, "IO": 3, "NC": 4, "GO": 5, "PF": 6}
 
     def set_recorder(self, populations, capacity: int) -> None:
         """Record the spikes of the named populations ("PC", "NC", ...) of
@@ -213,14 +213,24 @@ class Network:
             mask |= 1 << self.REC[p]
         check(_abi.hip_lib().cbm_set_recorder(self._h, mask, capacity), "cbm_set_recorder")
 
-    def read_recorded(self, population: str, t0: int, n: int) -> np.ndarray:
-        """uint8 [n, cells] spikes of steps t0 .. t0+n-1 (zones stacked)."""
+    def recorded_cells(self, population: str) -> int:
         s, z = self.sizes, self.sizes.numZones
-        cells = {"PC": z * s.numPC, "BC": z * s.numBC, "SC": s.numSC, "IO": z * s.numIO, "NC": z * s.numNC,
-                 "GO": s.numGO}[population]
-        out = np.empty((n, cells), np.uint8)
-        check(_abi.hip_lib().cbm_read_recorded(self._h, self.REC[population], t0, n, out.ctypes.data),
-              "cbm_read_recorded")
+        return {"PC": z * s.numPC, "BC": z * s.numBC, "SC": s.numSC, "IO": z * s.numIO, "NC": z * s.numNC,
+                "GO": s.numGO, "PF": s.numGR // 64}[population]
+
+    def read_recorded(self, population: str, t0: int, n: int, out=None):
+        """uint8 [n, cells] spikes of steps t0 .. t0+n-1 (zones stacked); "PF"
+        gives the delayed PF spikes counted per 64-GR chunk.  `out` may be a
+        contiguous uint8 device tensor of that shape, filled in place."""
+        cells = self.recorded_cells(population)
+        if out is None:
+            out = np.empty((n, cells), np.uint8)
+            ptr = out.ctypes.data
+        else:
+            if tuple(out.shape) != (n, cells) or not out.is_contiguous() or out.element_size() != 1:
+                raise ValueError(f"out must be a contiguous byte tensor of shape {(n, cells)}")
+            ptr = out.data_ptr()
+        check(_abi.hip_lib().cbm_read_recorded(self._h, self.REC[population], t0, n, ptr), "cbm_read_recorded")
         return out
 
     @property
@@ -316,6 +326,42 @@ class Batch:
         c = C.c_int()
         check(_abi.hip_lib().cbm_batch_autotune(self._b, ptrs, pitch, steps, C.byref(c)), "cbm_batch_autotune")
         return c.value
+
+    def sweep(self, prob, first: int, n: int, train, err_drive=None, cells_per_cluster: int = 256) -> None:
+        """n steps of every network with its input layer replaced by cluster
+        probabilities (cbm_batch_sweep): prob FP32 [rows, clusters] device
+        tensor whose row `first` + k drives run step k (first >= 199 rows of
+        look-back); train uint8 [>= n, numMF] device tensor shared by all
+        networks; err_drive[i]: None or float32 [n, numZones_i]."""
+        torch = __import__("torch")
+        if first < 199 or first + n > prob.shape[0]:
+            raise ValueError("prob needs 199 rows before `first` and n rows from it")
+        if prob.dtype != torch.float32 or prob.stride(1) != 1 or not prob.is_cuda:
+            raise TypeError("prob must be an FP32 device tensor with contiguous rows")
+        ptr, pitch, rows = _device_rows(train)
+        if rows < n:
+            raise ValueError("MF train shorter than n")
+        err = None
+        if err_drive is not None:
+            keep = [None if e is None else np.ascontiguousarray(e, dtype=np.float32) for e in err_drive]
+            err = (C.c_void_p * len(keep))(*[None if e is None else e.ctypes.data for e in keep])
+        torch.cuda.current_stream().synchronize()
+        check(_abi.hip_lib().cbm_batch_sweep(self._b, prob[first].data_ptr(), prob.stride(0), n, ptr, pitch, err,
+                                             cells_per_cluster), "cbm_batch_sweep")
+        self._cells = cells_per_cluster
+
+    def sweep_weights(self, net: int) -> np.ndarray:
+        """Cluster PF->PC weights of network `net`, [numZones, clusters]."""
+        h = self.nets[net]
+        out = np.empty(h.sizes.numZones * self._clusters(net), np.float32)
+        check(_abi.hip_lib().cbm_batch_sweep_weights(self._b, net, out.ctypes.data), "cbm_batch_sweep_weights")
+        return out.reshape(h.sizes.numZones, -1)
+
+    def _clusters(self, net: int) -> int:
+        return self.nets[net].sizes.numGR // self._cells if getattr(self, "_cells", 0) else 0
+
+    def sweep_reset(self) -> None:
+        check(_abi.hip_lib().cbm_batch_sweep_reset(self._b), "cbm_batch_sweep_reset")
 
     def info(self) -> dict:
         c, blocks, g = C.c_int(), C.c_int(), C.c_int()

@@ -425,7 +425,11 @@ int cbm_step_mode(cbm_handle *h);
 /* Populations the spike recorder can keep (bit masks for cbm_set_recorder, and
  * the population argument of cbm_read_recorded). */
 enum cbm_rec_population {
-    CBM_REC_PC = 0, CBM_REC_BC = 1, CBM_REC_SC = 2, CBM_REC_IO = 3, CBM_REC_NC = 4, CBM_REC_GO = 5
+    CBM_REC_PC = 0, CBM_REC_BC = 1, CBM_REC_SC = 2, CBM_REC_IO = 3, CBM_REC_NC = 4, CBM_REC_GO = 5,
+    /* PF spikes as the PCs, BCs and SCs see them (after each GR's PF delay),
+     * counted per 64-GR chunk: numGR/64 bytes per step, values 0..64.  Only
+     * single-network steps write them (not batched runs). */
+    CBM_REC_PF = 6
 };
 
 /* Spike recorder for trial readouts (the per-ms exportAP* calls of the
@@ -437,7 +441,8 @@ enum cbm_rec_population {
 int cbm_set_recorder(cbm_handle *h, uint32_t mask, uint32_t capacity);
 
 /* Copy the recorded spikes of one population for steps t0 .. t0+n-1 (all
- * still in the ring) into dst, [n][cells] bytes with zones stacked. */
+ * still in the ring) into dst, [n][cells] bytes with zones stacked.  dst may
+ * be host or device memory. */
 int cbm_read_recorded(cbm_handle *h, int population, uint64_t t0, int n, uint8_t *dst);
 
 /* ---------------------------------------------------------------- batches */

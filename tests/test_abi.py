@@ -19,6 +19,7 @@ def _declared(header):
 
 
 @pytest.mark.parametrize("header,libname", [("include/cbm_hip.h", "libcbm_hip.so"),
+                                            ("include/cbm_surrogate.h", "libcbm_hip.so"),
                                             ("include/cbm_state.h", "libcbm_state.so")])
 def test_exports(header, libname):
     names = [n for n in _declared(os.path.join(ROOT, header)) if n.startswith("cbm_")]
@@ -66,3 +67,16 @@ def test_create_without_gpu_fails_loudly():
     st = configs.build_state("t4k")
     with pytest.raises(CbmError, match="no HIP device"):
         Network(st.sizes, st.params)
+
+
+def test_surrogate_struct_layouts(tmp_path):
+    src = tmp_path / "sg.c"
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "cbm_surrogate.h"\n'
+                   'int main(){printf("%zu %zu %zu %zu %zu\\n", sizeof(cbm_sg_desc), sizeof(cbm_sg_batch),'
+                   ' sizeof(cbm_sg_adam), offsetof(cbm_sg_desc, device), offsetof(cbm_sg_batch, targetPitch));'
+                   'return 0;}\n')
+    exe = tmp_path / "sg"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)], check=True)
+    got = list(map(int, subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()))
+    assert got == [C.sizeof(_abi.SgDesc), C.sizeof(_abi.SgBatch), C.sizeof(_abi.SgAdam),
+                   _abi.SgDesc.device.offset, _abi.SgBatch.targetPitch.offset]
