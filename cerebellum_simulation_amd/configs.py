@@ -281,6 +281,39 @@ def mf_rates(num_mf: int, seed: int = 0, *, cs_frac: float = 0.03, ctx_frac: flo
     return rate, np.sort(cs_idx)
 
 
+def order_gr_by_inputs(state, key_mfs) -> np.ndarray:
+    """Relabel the GRs of `state` in place so that, within every SC block of
+    numGR/numSC consecutive GRs, the GRs with the most inputs from `key_mfs`
+    come first (stable otherwise).  Every PF target block (SC, BC, PC, IO) is
+    a union of SC blocks and every GR field is GR-major, so this is an exact
+    relabelling of the network; it makes consecutive GRs alike, which is what
+    a summary per cluster of consecutive GRs (the surrogate's) needs.
+    Returns the permutation (new GR g was old GR perm[g])."""
+    s = state.sizes
+    pm = state.field("P_GR_FROM_MF_TO_GR")
+    nm = state.field("NUMP_GR_FROM_MF_TO_GR")
+    valid = np.arange(pm.shape[1])[None, :] < nm[:, None]
+    hits = (np.isin(pm, np.asarray(key_mfs)) & valid).sum(axis=1)
+    blk = s.numGR // s.numSC
+    perm = np.empty(s.numGR, np.int64)
+    for b in range(s.numSC):
+        lo = b * blk
+        perm[lo:lo + blk] = lo + np.argsort(-hits[lo:lo + blk], kind="stable")
+    from ._abi import FIELDS, field_info
+    for name in FIELDS:
+        info = field_info(FIELDS[name], s)
+        if info["kind"] == 2:
+            continue
+        if name == "PFPC_SYNW":
+            for z in range(state.num_zones):
+                a = state.field(name, z)
+                a[...] = a.reshape(-1)[perm].reshape(a.shape)
+        elif not info["zoned"] and info["rows"] == s.numGR:
+            a = state.field(name)
+            a[...] = a[perm]
+    return perm
+
+
 def us_err_drive(steps: int, num_zones: int = 1, *, t0: int = 0, trial_ms: int = 2000,
                  us_ms: int = 999) -> np.ndarray:
     """float32 [steps, num_zones] relative error drive: 1.0 at trial time

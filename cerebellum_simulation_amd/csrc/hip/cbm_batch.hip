@@ -23,8 +23,9 @@ template <int CPT>
 __global__ void batch_step_kernel(const BatchNet *nets, const uint2 *tiles, uint32_t nNets, const uint32_t *kbase,
                                   uint32_t k);
 __global__ void batch_advance_kernel(uint32_t *kbase, uint32_t steps);
-__global__ void sweep_step_kernel(const BatchNet *nets, const SweepNet *sw, uint32_t nNets, const float *prob,
-                                  uint64_t pitch, uint32_t k);
+__global__ void sweep_step_kernel(const BatchNet *nets, const SweepNet *sw, uint32_t nNets, const SweepRun *run,
+                                  const uint32_t *kbase, uint32_t k);
+__global__ void sweep_q_kernel(const float *prob, uint64_t pitch, uint32_t n, uint32_t nCl, float *Q);
 __global__ void cluster_mean_kernel(const float *w, uint32_t numGR, uint32_t nZ, uint32_t log2Cells, float *W);
 }  // namespace cbmdev
 
@@ -52,7 +53,13 @@ struct cbm_batch {
     SweepNet *dsw = nullptr;
     size_t swLds = 0;
     uint32_t swCells = 0;
+    float *swQ = nullptr;            // per-step bin probabilities of the plasticity rule
+    size_t swQCap = 0;
+    SweepRun hswRun{}, *dswRun = nullptr;  // probability rows of the current sweep run
+    hipGraphExec_t swGraph = nullptr;      // kSweepGraphSteps sweep steps reading the counter kbase
 };
+
+constexpr uint32_t kSweepGraphSteps = 64;
 
 namespace {
 
@@ -240,9 +247,45 @@ int run(cbm_batch *b, const uint8_t *const *mf, const size_t *pitch, int n, cons
     if (evs) HIP_TRY(hipEventRecord(evs[0], b->stream));
     int k = 0;
     if (prob) {
+        uint32_t maxCl = 0;
+        for (const SweepNet &w : b->hsw) maxCl = std::max(maxCl, w.nCl);
+        const size_t need = (size_t)n * probPitch;
+        if (need > b->swQCap) {
+            if (b->swQ) (void)hipFree(b->swQ);
+            b->swQ = nullptr;
+            b->swQCap = 0;
+            HIP_TRY(hipMalloc((void **)&b->swQ, need * 4));
+            b->swQCap = need;
+        }
+        const size_t cnt = (size_t)n * maxCl;
+        hipLaunchKernelGGL(sweep_q_kernel, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, b->stream, prob,
+                           (uint64_t)probPitch, (uint32_t)n, maxCl, b->swQ);
+        HIP_TRY(hipGetLastError());
+        b->hswRun = SweepRun{prob, b->swQ, (uint64_t)probPitch};
+        HIP_TRY(hipMemcpyAsync(b->dswRun, &b->hswRun, sizeof(SweepRun), hipMemcpyHostToDevice, b->stream));
+        const dim3 grid(2 * (uint32_t)N), block(kSmallThreads);
+        if ((uint32_t)n >= kSweepGraphSteps) {
+            if (!b->swGraph) {
+                hipGraph_t g = nullptr;
+                HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+                for (uint32_t j = 0; j < kSweepGraphSteps; j++)
+                    hipLaunchKernelGGL(sweep_step_kernel, grid, block, b->swLds, b->stream, b->dnet, b->dsw,
+                                       (uint32_t)N, b->dswRun, b->kbase, j);
+                hipLaunchKernelGGL(batch_advance_kernel, dim3(1), dim3(64), 0, b->stream, b->kbase, kSweepGraphSteps);
+                const hipError_t e = hipStreamEndCapture(b->stream, &g);
+                if (e != hipSuccess) return fail(CBM_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+                const hipError_t e2 = hipGraphInstantiate(&b->swGraph, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                if (e2 != hipSuccess) return fail(CBM_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e2));
+            }
+            const uint32_t zero = 0;
+            HIP_TRY(hipMemcpyAsync(b->kbase, &zero, 4, hipMemcpyHostToDevice, b->stream));
+            HIP_TRY(hipStreamSynchronize(b->stream));  // the host values above must be copied before they go
+            for (; k + (int)kSweepGraphSteps <= n; k += (int)kSweepGraphSteps) HIP_TRY(hipGraphLaunch(b->swGraph, b->stream));
+        }
         for (; k < n; k++) {
-            hipLaunchKernelGGL(sweep_step_kernel, dim3(2 * (uint32_t)N), dim3(kSmallThreads), b->swLds, b->stream,
-                               b->dnet, b->dsw, (uint32_t)N, prob, (uint64_t)probPitch, (uint32_t)k);
+            hipLaunchKernelGGL(sweep_step_kernel, grid, block, b->swLds, b->stream, b->dnet, b->dsw, (uint32_t)N,
+                               b->dswRun, nullptr, (uint32_t)k);
             HIP_TRY(hipGetLastError());
         }
     }
@@ -291,6 +334,7 @@ int sweepSetup(cbm_batch *b, uint32_t cells)
         const uint32_t nCl = sz.numGR / cells;
         if (sz.numGR % cells || nCl % sz.numPC || nCl % sz.numBC || nCl % sz.numSC || nCl % sz.numIO || nCl == 0)
             return fail(CBM_E_UNSUPPORTED, "clusters must tile every PC, BC, SC and IO block of GRs");
+        if (nCl > 8 * (uint32_t)kSmallThreads) return fail(CBM_E_UNSUPPORTED, "at most 8192 clusters per network");
         lds = std::max(lds, (size_t)4 * (sz.numZones * sz.numIO + (size_t)nCl * (1 + sz.numZones)));
     }
     HIP_TRY(hipSetDevice(b->device));
@@ -312,6 +356,9 @@ int sweepSetup(cbm_batch *b, uint32_t cells)
         HIP_TRY(hipGetLastError());
     }
     if (!b->dsw) HIP_TRY(hipMalloc((void **)&b->dsw, b->nets.size() * sizeof(SweepNet)));
+    if (!b->dswRun) HIP_TRY(hipMalloc((void **)&b->dswRun, sizeof(SweepRun)));
+    if (b->swGraph) (void)hipGraphExecDestroy(b->swGraph);  // captured with the old LDS size
+    b->swGraph = nullptr;
     HIP_TRY(hipMemcpy(b->dsw, b->hsw.data(), b->hsw.size() * sizeof(SweepNet), hipMemcpyHostToDevice));
     if (lds > 64 * 1024)
         HIP_TRY(hipFuncSetAttribute((const void *)sweep_step_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -396,6 +443,9 @@ int cbm_batch_destroy(cbm_batch *b)
     for (SweepNet &w : b->hsw)
         if (w.W) (void)hipFree(w.W);
     if (b->dsw) (void)hipFree(b->dsw);
+    if (b->swQ) (void)hipFree(b->swQ);
+    if (b->dswRun) (void)hipFree(b->dswRun);
+    if (b->swGraph) (void)hipGraphExecDestroy(b->swGraph);
     if (b->done) (void)hipEventDestroy(b->done);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;

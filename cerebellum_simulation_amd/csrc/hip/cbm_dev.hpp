@@ -154,6 +154,14 @@ struct SweepNet {
     uint32_t nCl, log2Cells;
 };
 
+// The probability rows of the current sweep run (device memory): row k of
+// prob drives run step k; Q row k is its plasticity bin probability.
+struct SweepRun {
+    const float *prob;
+    const float *Q;
+    uint64_t pitch;
+};
+
 // Compact GR connectivity: up to 4 MF inputs, 3 GO inputs and 2 GO targets as
 // 11-bit cell indices (kNoCell11 = empty) and the three delays as 6-bit bit
 // indices (kNever6 = never), in two 64-bit halves:

@@ -1067,9 +1067,9 @@ __global__ void batch_advance_kernel(uint32_t *kbase, uint32_t steps)
 }
 
 // ---------------------------------------------------------------- surrogate sweep
-// The input layer of each network replaced by cluster probabilities: block
-// n < nNets steps network n's small populations exactly as above; block
-// nNets + n is its cluster tile.  prob row kk holds, for every cluster of
+// The input layer of a network replaced by cluster probabilities: the
+// cluster tile below takes the place of the GR tiles, and the network's small
+// populations step exactly as above.  prob row kk holds, for every cluster of
 // `cells` consecutive GRs, the probability that one of its cells emits a
 // delayed PF spike at run step kk; rows kk-199 .. kk-1 must be readable.
 // The tile writes what the GR tiles would write into this step's ring
@@ -1081,10 +1081,37 @@ __global__ void batch_advance_kernel(uint32_t *kbase, uint32_t steps)
 // the 5-step bin that ends 195 steps before) to the cluster weight with the
 // probability of such a spike, 1 - prod_{j=195..199} (1 - p[kk-j][c]):
 //   W = clamp(W + q * step(z, IO of c), 0, 1)   every tsPerHistBinGR steps.
-// Sums run in cluster order, so a run is deterministic.
-__device__ __forceinline__ void cluster_tile(const BatchNet &d, const SweepNet &sw, const float *prob,
-                                             uint64_t pitch, uint32_t kk, uint8_t *lds)
+// Sums run in a fixed order, so a run is deterministic.
+// out(i, sum of src[i*len .. i*len+len-1]) for i < nSeg, len a power of two;
+// every sum in the same fixed order (lanes stride over a segment, then an
+// xor butterfly), so results do not depend on timing.
+template <typename Out>
+__device__ __forceinline__ void segment_sums(const float *src, uint32_t nSeg, uint32_t len, Out out)
 {
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6, waves = blockDim.x >> 6;
+    if (len >= 64) {
+        for (uint32_t i = wave; i < nSeg; i += waves) {
+            float a = 0.0f;
+            for (uint32_t e = lane; e < len; e += 64) a += src[(size_t)i * len + e];
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o);
+            if (lane == 0) out(i, a);
+        }
+    } else {
+        const uint32_t per = 64 / len;  // segments per wave
+        for (uint32_t i0 = wave * per; i0 < nSeg; i0 += waves * per) {
+            const uint32_t i = i0 + lane / len;
+            float a = i < nSeg ? src[(size_t)i * len + (lane % len)] : 0.0f;
+            for (uint32_t o = len >> 1; o > 0; o >>= 1) a += __shfl_xor(a, (int)o);
+            if (lane % len == 0 && i < nSeg) out(i, a);
+        }
+    }
+}
+
+__device__ __forceinline__ void cluster_tile(const BatchNet &d, const SweepNet &sw, const float *prob,
+                                          const float *Q, uint64_t pitch, uint32_t kk, uint8_t *lds)
+{
+    constexpr int kPer = 4;  // clusters per thread and group
     const GrArgs &g = d.gr;
     const uint64_t t = d.t0 + kk + 1;
     const uint32_t slot = (uint32_t)(t % kRing), parPrev = (uint32_t)(t & 1u) ^ 1u;
@@ -1102,61 +1129,86 @@ __device__ __forceinline__ void cluster_tile(const BatchNet &d, const SweepNet &
             s_step[i] = tm < g.ltdEnd ? g.ltd[z] : (tm > g.ltpStart ? g.ltp[z] : 0.0f);
         }
     __syncthreads();
-    const float *p = prob + (int64_t)kk * (int64_t)pitch;
-    for (uint32_t c = tid; c < nCl; c += nt) {
-        const float pc = p[c];
-        const float cnt = pc * cells;
-        s_cnt[c] = cnt;
-        float q = 0.0f;
-        if (doPlast) {
-            float keep = 1.0f;
-            for (int j = 195; j <= 199; j++) keep *= 1.0f - p[(int64_t)c - (int64_t)j * (int64_t)pitch];
-            q = 1.0f - keep;
+    // this thread's clusters c = c0 + tid + i*nt, four at a time: every load
+    // of a group before any store
+    const float *p = prob + (size_t)kk * pitch, *q = Q + (size_t)kk * pitch;
+    for (uint32_t c0 = 0; c0 < nCl; c0 += kPer * nt) {
+        float pv[kPer], qv[kPer];
+#pragma unroll
+        for (int i = 0; i < kPer; i++) {
+            const uint32_t c = c0 + tid + i * nt;
+            const bool in = c < nCl;
+            pv[i] = in ? p[c] : 0.0f;
+            qv[i] = in && doPlast ? q[c] : 0.0f;
         }
-        const uint32_t io = (c << sw.log2Cells) >> g.log2GRperIO;
         for (uint32_t z = 0; z < nZ; z++) {
-            float w = sw.W[(size_t)z * nCl + c];
-            if (doPlast) {
-                w = w + q * s_step[z * g.numIO + io];
-                w = (w > 0) * w;
-                w = (w > 1) + (w <= 1) * w;
-                sw.W[(size_t)z * nCl + c] = w;
+            float wv[kPer];
+#pragma unroll
+            for (int i = 0; i < kPer; i++) {
+                const uint32_t c = c0 + tid + i * nt;
+                wv[i] = c < nCl ? sw.W[(size_t)z * nCl + c] : 0.0f;
             }
-            s_pc[(size_t)z * nCl + c] = w * cnt;
+#pragma unroll
+            for (int i = 0; i < kPer; i++) {
+                const uint32_t c = c0 + tid + i * nt;
+                if (c >= nCl) continue;
+                const float cnt = pv[i] * cells;
+                float w = wv[i];
+                if (doPlast) {
+                    const uint32_t io = (c << sw.log2Cells) >> g.log2GRperIO;
+                    w = w + qv[i] * s_step[z * g.numIO + io];
+                    w = (w > 0) * w;
+                    w = (w > 1) + (w <= 1) * w;
+                    sw.W[(size_t)z * nCl + c] = w;
+                }
+                if (z == 0) s_cnt[c] = cnt;
+                s_pc[(size_t)z * nCl + c] = w * cnt;
+            }
         }
     }
     __syncthreads();
     const uint32_t numBC = d.small->s.numBC, numSC = d.small->s.numSC;
-    const uint32_t perPC = nCl / g.numPC, perBC = nCl / numBC, perSC = nCl / numSC;
-    for (uint32_t task = tid; task < nZ * g.numPC; task += nt) {
-        const uint32_t z = task / g.numPC, pc = task % g.numPC;
-        const float *src = s_pc + (size_t)z * nCl + (size_t)pc * perPC;
-        float acc = 0.0f;
-        for (uint32_t i = 0; i < perPC; i++) acc += src[i];
-        d.pfpcRing[(size_t)slot * nZ * g.numPC + task] = acc;
-    }
-    for (uint32_t b = tid; b < numBC; b += nt) {
-        float acc = 0.0f;
-        for (uint32_t i = 0; i < perBC; i++) acc += s_cnt[b * perBC + i];
-        d.bcRing[(size_t)slot * numBC + b] = (uint32_t)rintf(acc);
-    }
-    for (uint32_t sc = tid; sc < numSC; sc += nt) {
-        float acc = 0.0f;
-        for (uint32_t i = 0; i < perSC; i++) acc += s_cnt[sc * perSC + i];
-        d.scRing[(size_t)slot * numSC + sc] = (uint32_t)rintf(acc);
-    }
+    float *pfpc = d.pfpcRing + (size_t)slot * nZ * g.numPC;
+    segment_sums(s_pc, nZ * g.numPC, nCl / g.numPC, [&](uint32_t i, float a) { pfpc[i] = a; });
+    uint32_t *bc = d.bcRing + (size_t)slot * numBC;
+    segment_sums(s_cnt, numBC, nCl / numBC, [&](uint32_t i, float a) { bc[i] = (uint32_t)rintf(a); });
+    uint32_t *sc = d.scRing + (size_t)slot * numSC;
+    segment_sums(s_cnt, numSC, nCl / numSC, [&](uint32_t i, float a) { sc[i] = (uint32_t)rintf(a); });
 }
 
+// Q[k][c] = 1 - prod_{j=195..199} (1 - prob[k-j][c]): the probability of a
+// PF spike in the 5-step history bin the plasticity rule of step k reads,
+// shared by every network of a sweep (they share prob).
+__global__ void sweep_q_kernel(const float *prob, uint64_t pitch, uint32_t n, uint32_t nCl, float *Q)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)n * nCl) return;
+    const uint32_t k = (uint32_t)(i / nCl), c = (uint32_t)(i % nCl);
+    const float *p = prob + (size_t)k * pitch + c;
+    float keep = 1.0f;
+#pragma unroll
+    for (int j = 195; j <= 199; j++) keep *= 1.0f - p[-(int64_t)j * (int64_t)pitch];
+    Q[(size_t)k * pitch + c] = 1.0f - keep;
+}
+
+// One step of every sweep network: block n < nNets steps network n's small
+// populations, block nNets + n is its cluster tile (the two are independent
+// within a step, as the GR tiles and the small step of the exact step are).
+// Run step kk = *kbase + k (kbase: counter advanced by a captured graph, or
+// nullptr); the run's probability rows come from the device block `run`, so
+// one captured graph serves every run.
 __global__ void __launch_bounds__(kSmallThreads) sweep_step_kernel(const BatchNet *__restrict__ nets,
                                                                    const SweepNet *__restrict__ sw, uint32_t nNets,
-                                                                   const float *prob, uint64_t pitch, uint32_t k)
+                                                                   const SweepRun *run, const uint32_t *kbase,
+                                                                   uint32_t k)
 {
     extern __shared__ __align__(16) uint8_t lds[];
     __shared__ float s_err[kMaxZones];
+    const uint32_t kk = (kbase ? *kbase : 0u) + k;
     if (blockIdx.x < nNets)
-        batch_small(nets[blockIdx.x], k, s_err, lds);
+        batch_small(nets[blockIdx.x], kk, s_err, lds);
     else
-        cluster_tile(nets[blockIdx.x - nNets], sw[blockIdx.x - nNets], prob, pitch, k, lds);
+        cluster_tile(nets[blockIdx.x - nNets], sw[blockIdx.x - nNets], run->prob, run->Q, run->pitch, kk, lds);
 }
 
 // W[z][c] = mean of the PF->PC weights of cluster c's cells, in cell order

@@ -63,6 +63,42 @@ class EyelidIntegrator:
         return np.array([self.step(row) for row in ap_nc], f32)
 
 
+class EyelidBank:
+    """Many EyelidIntegrators stepped together (one per row), with the
+    arithmetic of EyelidIntegrator.step applied element-wise, so row m gives
+    exactly what its own integrator would."""
+
+    def __init__(self, count: int, num_in: int, ms_per_ts: int = 1, **kw):
+        one = EyelidIntegrator(num_in, ms_per_ts, **kw)
+        self.gDecay, self.gInc, self.gShift, self.evMax, self.gLeak = (one.gDecay, one.gInc, one.gShift,
+                                                                         one.evMax, one.gLeak)
+        self.vm = np.zeros(count, f32)
+        self.gIn = np.zeros((count, num_in), f32)
+
+    def trace(self, ap: np.ndarray) -> np.ndarray:
+        """ap: [steps, count, num_in] spikes -> eyelid positions [steps, count]."""
+        out = np.empty(ap.shape[:2], f32)
+        zero = f32(0.0)
+        for k in range(ap.shape[0]):
+            self.gIn = self.gIn * self.gDecay
+            self.gIn = self.gIn + ap[k].astype(f32) * self.gInc
+            g = np.zeros(self.vm.shape, f32)
+            for j in range(self.gIn.shape[1]):
+                g = (g + self.gIn[:, j]).astype(f32)
+            g = (g + self.gShift).astype(f32)
+            g = ((g > 0).astype(f32) * g).astype(f32)
+            leak = (self.gLeak * (zero - self.vm)).astype(f32)
+            self.vm = ((self.vm + leak).astype(f32) + (g * (self.evMax - self.vm)).astype(f32)).astype(f32)
+            out[k] = self.vm
+        return out
+
+
+def cr_amplitude(trace: np.ndarray, cs_on: int, cs_off: int, baseline_ms: int = 200) -> float:
+    """Largest closure in [cs_on, cs_off) over the mean of the baseline_ms before cs_on."""
+    base = float(np.mean(trace[cs_on - baseline_ms:cs_on], dtype=np.float64))
+    return float(np.max(trace[cs_on:cs_off].astype(np.float64)) - base)
+
+
 def cr_onset(trace: np.ndarray, cs_on: int, cs_off: int, threshold: float = 5.0, baseline_ms: int = 200):
     """First trial time in [cs_on, cs_off) at which trace - mean(trace over the
     baseline_ms before cs_on) exceeds threshold (eyelid units, 100 = closed);

@@ -277,3 +277,212 @@ def pf_rmse(pred, target_bits, cells: int = 256) -> float:
     y = bf16_bits_to_float(target_bits)
     d = (pred.to(torch.float64) - y.to(torch.float64)) * cells
     return float(torch.sqrt(torch.mean(d * d)))
+
+
+# ---------------------------------------------------------------- sweeps
+def _segments(dt, trial: int):
+    """(first, last+1, LTD/LTP window on) of the delay protocol's trial
+    `trial`, as row offsets of a stream of back-to-back trials."""
+    base = trial * dt.trial_ms
+    return [(base + a, base + b, what == "set") for a, b, what in dt.segments()]
+
+
+def sweep_experiment(net_name: str = "c1", *, seed: int = 0, mf_seed: int = 11, train_trials: int = 1000,
+                     train_steps: int = 600, segments: int = 64, outputs: int = 256, lr: float = 2e-3,
+                     sweep_trials: int = 10, ltd_scales=(0.25, 0.5, 1, 2, 4, 8, 16, 32),
+                     ltp_scales=(0.25, 0.5, 1, 2, 4, 8, 16, 32), judged=(0, 18, 36, 63), cells: int = 256,
+                     hidden: int = 128, log=print) -> dict:
+    """The learned input layer driving a plasticity sweep, judged by the exact step.
+
+    1. The network `net_name` (GRs ordered within SC blocks by their number of
+       CS MF inputs) runs `train_trials` delay trials; its MF rows and cluster
+       PF fractions train the surrogate.
+    2. A fresh MF stream of 1 + sweep_trials trials: the surrogate predicts the
+       PF probabilities of the last sweep_trials (the first gives context).
+    3. The sweep: one network per (LTD scale, LTP scale) pair, from PF->PC
+       weights of a network that has learned the CS, runs the sweep trials
+       with its input layer replaced by the predictions (cbm_batch_sweep); the
+       protocol's LTD/LTP steps and the reset steps are scaled by the pair.
+    4. The judge: the networks `judged` run the same trials with the exact
+       step; PF rates (RMSE per cluster and ms), CR onset and CR amplitude per
+       trial are compared.  The same judged pairs are also swept with each
+       cluster's mean training rate in place of the predictions (the sweep
+       without the trained network).
+    Returns the figures as a dict."""
+    import time
+    torch = _torch()
+    from . import configs
+    from .eyelid import DelayTrials, EyelidBank, cr_amplitude, cr_onset
+    from .network import Batch, Network
+    res = {"network": net_name, "cells_per_cluster": cells}
+    st = configs.build_state(net_name, seed=seed)
+    s = st.sizes
+    rates, cs = configs.mf_rates(s.numMF, seed=mf_seed)
+    configs.order_gr_by_inputs(st, cs)
+    w0 = configs.trained_pfpc_weights(st, cs)
+    n_cl = s.numGR // cells
+    Z = st.num_zones
+    dt = DelayTrials(st.params, s.numNC)
+    TM = dt.trial_ms
+
+    # 1. data and training
+    gen_net = Network.from_state(st)
+    sg = Surrogate(num_in=s.numMF, hidden=hidden, num_out=n_cl, max_rows=max(16384, segments * (49 + outputs)))
+    t = time.time()
+    step = max(1, train_trials // 10)
+    data = InputLayerData.record(gen_net, sg, train_trials, rates, cs, cells_per_cluster=cells, seed=seed + 1,
+                                 progress=lambda i: (i + 1) % step == 0 and log(f"[c4] recorded trial {i + 1}"))
+    torch.cuda.synchronize()
+    res["record_s"] = time.time() - t
+    res["windows"] = data.rows - 1
+    gen_net.close()
+    y = bf16_bits_to_float(data.y)
+    mean = y[:-1].mean(dim=0)
+    m = float(mean.mean())
+    sg.init_params(seed=seed, out_bias=float(np.log(m / (1 - m))))
+    t = time.time()
+    losses = train(sg, data, train_steps, segments=segments, outputs=outputs, lr=lr, lr_end=lr / 10, seed=seed)
+    res["train_s"] = time.time() - t
+    us, fl = sg.last_timing()
+    res["train_step_us"], res["train_step_tflops"] = us, fl / us / 1e6
+    res["loss_first"], res["loss_last"] = float(losses[:10].mean()), float(losses[-10:].mean())
+    log(f"[c4] recorded {res['windows']} windows in {res['record_s']:.1f}s, trained {train_steps} steps in "
+        f"{res['train_s']:.1f}s ({res['train_step_tflops']:.0f} TFLOP/s per step); loss {res['loss_first']:.5f} -> "
+        f"{res['loss_last']:.5f}")
+    del data, y
+
+    # 2. the sweep stream and its predictions
+    K = sweep_trials
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed + 7)
+    mf = mf_train_device(rates, cs, (1 + K) * TM, generator=gen)
+    R0 = TM
+    t = time.time()
+    prob = sg.predict(mf, R0 - 200, 199 + K * TM)
+    torch.cuda.synchronize()
+    res["predict_s"] = time.time() - t
+    us, fl = sg.last_timing()
+    res["forward_tflops"] = fl / us / 1e6
+    train_mf = mf[R0:]
+    flat_prob = torch.zeros_like(prob)
+    flat_prob[:] = mean.unsqueeze(0)
+    err = configs.us_err_drive(K * TM, Z)
+    pairs = [(a, b) for a in ltd_scales for b in ltp_scales]
+    ltd_def, ltp_def = st.params.synLTDStepSizeGRtoPC, st.params.synLTPStepSizeGRtoPC
+
+    def make(pair):
+        n = Network.from_state(st)
+        for z in range(Z):
+            n.write("PFPC_SYNW", w0, z)
+        n.set_recorder(["NC", "PC"], K * TM)
+        return n
+
+    def steps_for(pair, on):
+        a, b = pair
+        return (float(dt.ltd) * a, float(dt.ltp) * b) if on else (ltd_def * a, ltp_def * b)
+
+    spikes = {}
+
+    def readout(nets, tag):
+        nc = np.stack([n.read_recorded("NC", 1, K * TM)[:, :s.numNC] for n in nets], axis=1)
+        pc = np.stack([n.read_recorded("PC", 1, K * TM)[:, :s.numPC] for n in nets], axis=1)
+        spikes[tag] = {"NC": nc.reshape(K, TM, len(nets), -1).sum(axis=(1, 3)).T.tolist(),
+                       "PC": pc.reshape(K, TM, len(nets), -1).sum(axis=(1, 3)).T.tolist()}
+        eye = EyelidBank(len(nets), s.numNC).trace(nc)
+        out = []
+        for i in range(len(nets)):
+            per = []
+            for k in range(K):
+                tr = eye[k * TM:(k + 1) * TM, i]
+                per.append((cr_onset(tr, dt.cs_on, dt.cs_off), cr_amplitude(tr, dt.cs_on, dt.cs_off)))
+            out.append(per)
+        return out
+
+    def run_sweep(pairs_, p_, tag):
+        nets = [make(p) for p in pairs_]
+        b = Batch(nets)
+        torch.cuda.synchronize()
+        t0 = time.time()
+        for k in range(K):
+            for lo, hi, on in _segments(dt, k):
+                for n, p in zip(nets, pairs_):
+                    for z in range(Z):
+                        n.set_grpc_plast_steps(z, *steps_for(p, on))
+                b.sweep(p_, 199 + lo, hi - lo, train_mf[lo:hi], err_drive=[err[lo:hi]] * len(nets),
+                        cells_per_cluster=cells)
+        b.sync()
+        el = time.time() - t0
+        r = readout(nets, tag)
+        b.close()
+        for n in nets:
+            n.close()
+        return r, el
+
+    # 3. the sweep
+    sweep, t_sweep = run_sweep(pairs, prob, "surrogate")
+    res["members"] = len(pairs)
+    res["sweep_s"] = t_sweep
+    res["sweep_network_ms_per_s"] = len(pairs) * K * TM / t_sweep
+    log(f"[c4] sweep of {len(pairs)} parameter sets x {K} trials: {t_sweep:.2f}s, "
+        f"{res['sweep_network_ms_per_s']:.0f} network-ms/s")
+
+    # 4. the judge: exact runs of the judged pairs, and the sweep without the trained network
+    jp = [pairs[j] for j in judged]
+    exact_nets = [make(p) for p in jp]
+    exact_nets[0].set_recorder(["NC", "PC", "PF"], K * TM)
+    torch.cuda.synchronize()
+    t0 = time.time()
+    for k in range(K):
+        for lo, hi, on in _segments(dt, k):
+            for n, p in zip(exact_nets, jp):
+                for z in range(Z):
+                    n.set_grpc_plast_steps(z, *steps_for(p, on))
+                n.run_mf_train(train_mf[lo:hi], err_drive=err[lo:hi])
+    for n in exact_nets:
+        n.sync()
+    t_exact = time.time() - t0
+    res["exact_network_ms_per_s"] = len(jp) * K * TM / t_exact
+    exact = readout(exact_nets, "exact")
+    pf = torch.empty((K * TM, s.numGR // 64), dtype=torch.uint8, device="cuda")
+    exact_nets[0].read_recorded("PF", 1, K * TM, out=pf)
+    torch.cuda.synchronize()
+    yk = torch.zeros((K * TM, n_cl), dtype=torch.int16, device="cuda")
+    sg.targets_from_pf(pf, cells // 64, yk)
+    sg.sync()
+    for n in exact_nets:
+        n.close()
+    res["pf_rmse"] = pf_rmse(prob[199:199 + K * TM], yk, cells)
+    res["pf_rmse_cluster_mean"] = pf_rmse(flat_prob[199:199 + K * TM].contiguous(), yk, cells)
+    flat, _ = run_sweep(jp, flat_prob, "cluster_mean")
+    # the sweep driven by the exact network's own cluster PF fractions: what
+    # summarising each cluster by its mean costs, without the surrogate's error
+    own = torch.zeros_like(prob)
+    own[199:199 + K * TM] = bf16_bits_to_float(yk)
+    own_cr, _ = run_sweep(jp, own, "exact_fractions")
+
+    def compare(got):
+        d_on, d_amp, miss = [], [], 0
+        for g, e in zip(got, exact):
+            for (go, ga), (eo, ea) in zip(g, e):
+                d_amp.append(abs(ga - ea))
+                if (go is None) != (eo is None):
+                    miss += 1
+                elif go is not None:
+                    d_on.append(abs(go - eo))
+        return {"onset_abs_err_ms": float(np.mean(d_on)) if d_on else None, "onset_presence_mismatch": miss,
+                "amplitude_abs_err": float(np.mean(d_amp)), "trials": len(d_amp)}
+
+    res["judged"] = [list(p) for p in jp]
+    res["surrogate_vs_exact"] = compare([sweep[j] for j in judged])
+    res["cluster_mean_vs_exact"] = compare(flat)
+    res["exact_fractions_vs_exact"] = compare(own_cr)
+    res["spikes_per_trial"] = {k: {p: v[p][:2] for p in v} for k, v in spikes.items()}
+    res["exact_fractions_cr"] = [[(o, round(a, 3)) for o, a in e] for e in own_cr]
+    res["exact_cr"] = [[(o, round(a, 3)) for o, a in e] for e in exact]
+    res["surrogate_cr"] = [[(o, round(a, 3)) for o, a in sweep[j]] for j in judged]
+    res["speedup_vs_exact"] = res["sweep_network_ms_per_s"] / res["exact_network_ms_per_s"]
+    log(f"[c4] exact: {res['exact_network_ms_per_s']:.0f} network-ms/s; PF rmse surrogate {res['pf_rmse']:.4f} vs "
+        f"cluster mean {res['pf_rmse_cluster_mean']:.4f}; CR vs exact: surrogate {res['surrogate_vs_exact']}, "
+        f"cluster mean {res['cluster_mean_vs_exact']}, exact fractions {res['exact_fractions_vs_exact']}")
+    sg.close()
+    return res

@@ -290,3 +290,15 @@ def test_training_beats_cluster_and_phase_means(gpu):
           f"trial-phase mean {ph:.4f}", flush=True)
     assert rmse < 0.95 * base and rmse < ph
     sg.close()
+
+
+def test_sweep_experiment_small(gpu):
+    print("[case] surrogate-driven sweep judged by the exact step, 262K GR, 4 parameter sets", flush=True)
+    res = SG.sweep_experiment("b262k", train_trials=10, train_steps=250, segments=32, sweep_trials=2,
+                              ltd_scales=(1, 8), ltp_scales=(1, 8), judged=(0, 3), log=lambda m: print(m, flush=True))
+    print({k: v for k, v in res.items() if k not in ("exact_cr", "surrogate_cr")}, flush=True)
+    print("exact CR", res["exact_cr"], "surrogate CR", res["surrogate_cr"], flush=True)
+    assert res["pf_rmse"] < res["pf_rmse_cluster_mean"]
+    assert res["speedup_vs_exact"] > 1.0
+    # the pre-trained network closes the lid on its first trial in the exact step
+    assert res["exact_cr"][0][0][0] is not None
