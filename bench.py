@@ -23,6 +23,11 @@ batched: the 256-network ragged pack (64 x 16K + 96 x 65K + 64 x 262K + 32 x
 1M GR, each with its own seed, CS subset and GO-GO inhibition on or off)
 stepped by one launch per ms, in network-ms per wall-second, against the same
 networks run one at a time on the same GPU ("plain").
+
+surrogate: config 4 -- the learned input layer trained on 2,000,000 windows of
+the 1M-GR network, the sweep of 64 plasticity parameter sets it drives
+(network-ms/s against the exact step), and the exact step's verdict on PF
+rates and conditioned responses.
 """
 import argparse
 import json
@@ -120,6 +125,38 @@ def batch_bench(which: str, steps: int) -> dict:
             "algorithmic_bytes_per_step": alg}
 
 
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def surrogate_bench(which: str) -> dict:
+    """Config 4: the learned input layer (3-layer causal temporal conv, kernel
+    8, 128 channels, BF16 MFMA with FP32 accumulation) trained on windows the
+    1M-GR simulator generates, driving a sweep of 64 LTD/LTP parameter sets,
+    judged by the exact step (surrogate.sweep_experiment)."""
+    from cerebellum_simulation_amd import surrogate as SG
+    log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+    if which == "full":
+        res = SG.sweep_experiment("c1", train_trials=1000, train_steps=600, sweep_trials=10, log=log)
+        workload = ("1M-GR network (config 1 sizes); 2,000,000 training windows of 50 ms x 1,024 MFs -> "
+                    "4,096 clusters of 256 GRs; sweep of 64 (LTD, LTP) scale pairs x 10 trials")
+    else:
+        res = SG.sweep_experiment("b262k", train_trials=10, train_steps=250, segments=32, sweep_trials=2,
+                                  ltd_scales=(1, 8), ltp_scales=(1, 8), judged=(0, 3), cells=64, log=log)
+        workload = "262K-GR network; 20,000 windows -> 4,096 clusters of 64 GRs; 4 parameter sets x 2 trials"
+    tf = res["train_step_tflops"]
+    keep = ("windows", "record_s", "train_s", "train_step_us", "loss_first", "loss_last", "forward_tflops",
+            "members", "sweep_s", "sweep_network_ms_per_s", "exact_network_ms_per_s", "speedup_vs_exact",
+            "pf_rmse", "pf_rmse_cluster_mean", "judged", "surrogate_vs_exact", "cluster_mean_vs_exact",
+            "exact_fractions_vs_exact", "cells_per_cluster")
+    out = {"workload": workload, **{k: res[k] for k in keep}}
+    out["roofline"] = {"bound": "mfma", "achieved": round(tf, 1), "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                       "frac": round(tf / MFMA_BF16_PEAK_TFLOPS, 4), "kernel": "training step (all kernels)"}
+    for k in ("record_s", "train_s", "train_step_us", "sweep_s", "sweep_network_ms_per_s", "exact_network_ms_per_s",
+              "speedup_vs_exact", "pf_rmse", "pf_rmse_cluster_mean", "forward_tflops"):
+        out[k] = round(out[k], 4)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -135,6 +172,8 @@ def main():
     ap.add_argument("--batch", default="full", choices=["full", "small", "none"],
                     help="ragged batch to measure beside the single network (small: 1/8 of each class)")
     ap.add_argument("--batch-steps", type=int, default=200)
+    ap.add_argument("--surrogate", default="full", choices=["full", "small", "none"],
+                    help="config 4: the learned input layer and the plasticity sweep it drives")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB, 5 times (the known-size kernel that scales "
                          "the FETCH_SIZE / WRITE_SIZE counters)")
@@ -207,6 +246,13 @@ def main():
         del dtr
         batched = batch_bench(args.batch, args.batch_steps)
 
+    surrogate = None
+    if args.surrogate != "none":
+        try:
+            surrogate = surrogate_bench(args.surrogate)
+        except Exception as e:  # the headline line must still be printed
+            surrogate = {"error": f"{type(e).__name__}: {e}"}
+
     cpu = None
     if not args.no_cpu:
         cpu_steps = args.cpu_steps or {"c0": 5000, "c1": 2000, "c2": 400}.get(args.config, 100)
@@ -220,7 +266,7 @@ def main():
         "config": {"workload": WORKLOADS.get(args.config, args.config), "name": args.config,
                    "numGR": s.numGR, "numGO": s.numGO, "numMF": s.numMF, "numPC": s.numPC,
                    "parallelism": "single GPU", "step_mode": "fused" if fused else "two streams"},
-        "roofline": roofline, "cpu_baseline": cpu, "batched": batched,
+        "roofline": roofline, "cpu_baseline": cpu, "batched": batched, "surrogate": surrogate,
     }
     print(json.dumps(out), flush=True)
 

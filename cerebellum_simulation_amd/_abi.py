@@ -207,6 +207,12 @@ def state_lib() -> C.CDLL:
         L.cbm_state_field.argtypes = [P, I, I, C.POINTER(V), C.POINTER(C.c_size_t)]
         L.cbm_state_load_into.argtypes = [P, P]
         L.cbm_sfmt_sample.argtypes = [I, C.c_uint64, I, I, I, I, V]
+        L.cbm_state_write_checkpoint.argtypes = [P, C.c_char_p]
+        L.cbm_state_read_checkpoint.argtypes = [C.c_char_p, C.POINTER(P)]
+        L.cbm_state_checkpoint_to_memory.argtypes = [P, C.POINTER(V), C.POINTER(C.c_size_t)]
+        L.cbm_state_checkpoint_from_memory.argtypes = [V, C.c_size_t, C.POINTER(P), C.POINTER(C.c_size_t)]
+        L.cbm_state_free_memory.argtypes = [V]
+        L.cbm_state_free_memory.restype = None
         _state = L
     return _state
 

@@ -15,6 +15,7 @@
 #include <climits>
 #include <cstring>
 #include <ctime>
+#include <iterator>
 #include <sstream>
 #include <stdexcept>
 #include <string>
@@ -99,6 +100,27 @@ void CBMState::build(const std::string &act, const std::string &con, unsigned in
     stateCheck(cbm_state_create(act.c_str(), con.c_str(), &g, &st), "cbm_state_create");
     cbm_state_sizes(st, &sz);
     cbm_state_params(st, &ap);
+}
+
+CBMState::CBMState(std::fstream &infile) : st(nullptr)
+{
+    const std::streampos start = infile.tellg();
+    const std::string buf((std::istreambuf_iterator<char>(infile)), std::istreambuf_iterator<char>());
+    size_t used = 0;
+    stateCheck(cbm_state_checkpoint_from_memory(buf.data(), buf.size(), &st, &used), "CBMState(fstream&)");
+    infile.clear();
+    infile.seekg(start + (std::streamoff)used);
+    cbm_state_sizes(st, &sz);
+    cbm_state_params(st, &ap);
+}
+
+void CBMState::writeState(std::fstream &outfile)
+{
+    void *data = nullptr;
+    size_t n = 0;
+    stateCheck(cbm_state_checkpoint_to_memory(st, &data, &n), "CBMState::writeState");
+    outfile.write(static_cast<const char *>(data), (std::streamsize)n);
+    cbm_state_free_memory(data);
 }
 
 CBMState::~CBMState()
@@ -267,6 +289,17 @@ void CBMSimCore::writeToState()
             void *p = simState->field(f, (int)z, &b);
             if (b) hipCheck(cbm_read(h, f, (int)z, p, b), "writeToState");
         }
+    }
+}
+
+void CBMSimCore::writeToState(std::fstream &outfile)
+{
+    writeToState();
+    simState->writeState(outfile);
+    for (int f : {CBM_F_INPUT_GOGABA_SYNDEP_GO, CBM_F_GO_GABA_OUT_SYNSCALE_GOGO}) {
+        size_t b = 0;
+        void *p = simState->field(f, 0, &b);
+        hipCheck(cbm_write(h, f, 0, p, b), "writeToState(fstream&)");
     }
 }
 

@@ -34,6 +34,10 @@ public:
 
     // Copy the device state back into the CBMState arrays.
     void writeToState();
+    // ... and save it (cbmsimcore.cpp:75-80).  The reference's step reads the
+    // GO synaptic-depression arrays from the state object that writeState
+    // resets, so the running network is reset with it here too.
+    void writeToState(std::fstream &outfile);
 
     InNetInterface *getInputNet();
     MZoneInterface **getMZoneList();

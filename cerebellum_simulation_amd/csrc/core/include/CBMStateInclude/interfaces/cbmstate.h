@@ -20,7 +20,14 @@ public:
     // generator for the NC->IO release draws.
     CBMState(std::fstream &actPFile, std::fstream &conPFile, unsigned int nZones, int innetCRSeed,
              int *mzoneCRSeed, int *mzoneARSeed);
+    // A saved state (cbmstate.cpp:12-33): read from the stream's position; the
+    // stream is left after the state.
+    CBMState(std::fstream &infile);
     virtual ~CBMState();
+
+    // Save the state (cbmstate.cpp:86-103).  As in the reference, this sets
+    // the GO synaptic-depression arrays to 1 and 0 (innetactivitystate.cpp:247-248).
+    void writeState(std::fstream &outfile);
 
     ct_uint32_t getNumZones();
 

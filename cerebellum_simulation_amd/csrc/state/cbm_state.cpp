@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <fstream>
 #include <map>
 #include <sstream>
 #include <string>
@@ -234,6 +235,12 @@ struct cbm_state {
     cbm_gen_spec gen{};
     uint32_t grPFVel = 1, grAFDelay = 1, msPerStep = 1;
     std::vector<uint8_t> f[CBM_F_COUNT][kMaxZones];
+    // the parameter files as parsed (the checkpoint header writes them back)
+    ConParams con;
+    std::map<std::string, float> act;
+    // checkpoint sections the step does not use, kept byte for byte
+    std::map<std::string, std::vector<uint8_t>> blob[kMaxZones];
+    bool fromCheckpoint = false;  // its header is written back unchanged
 
     template <typename T>
     T *F(int field, int z = 0) { return reinterpret_cast<T *>(f[field][z].data()); }
@@ -322,6 +329,8 @@ int cbm_state_create(const char *actText, const char *conText, const cbm_gen_spe
 
     cbm_state *st = new cbm_state();
     st->gen = *gen;
+    st->con = cp;
+    st->act = am.m;
     deriveAct(am, st->p);
     cbm_sizes &s = st->s;
     // ConnectivityParams derived sizes (connectivityparams.cpp:90-184)
@@ -656,5 +665,633 @@ int cbm_state_load_into(cbm_state *st, cbm_handle *h)
     if (rc) return fail(rc, std::string("cbm_commit failed: ") + cbm_last_error());
     return CBM_OK;
 }
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- checkpoint
+// The CBMState file format: CBMState::writeState / CBMState(fstream&)
+// (CBM_STATE_LIB_DEV/src/interfaces/cbmstate.cpp:12-33,86-103).  A text
+// header -- the zone count, ConnectivityParams::writeParams
+// (connectivityparams.cpp:192-218), ActivityParams::writeParams
+// (activityparams.cpp:243-253) -- then the raw arrays of the four stateRW
+// methods in their order: InNetConnectivityState (innetconnectivitystate.cpp:
+// 530-606), MZoneConnectivityState per zone (mzoneconnectivitystate.cpp:
+// 142-163), InNetActivityState (innetactivitystate.cpp:233-275),
+// MZoneActivityState per zone (mzoneactivitystate.cpp:299-355).  Array sizes
+// follow from the header's connectivity parameters as ConnectivityParams
+// derives them (connectivityparams.cpp:90-184).
+namespace {
+
+struct RefSizes {
+    uint32_t numGL, glToGR, glToGO, glFromGO;
+    uint32_t numMF, mfToGL, mfToGO, mfToGR;
+    uint32_t numGO, goFromGR, goFromGL, goFromMF, goToGL, goToGR;
+    uint32_t numGR, grToGO, grFromGO, grFromMFkey;
+    uint32_t numSC, numPC, numBC, numNC, numIO;
+    uint32_t pcFromGR, pcFromBC, pcToBC, pcFromSC, pcToNC, bcToPC, bcFromPC, scToPC;
+    uint32_t ncFromPC, ncToIO, ncFromMF, ioToPC, ioFromNC, ioIn, ioOut;
+    uint32_t gabaIn, gabaOut, coupIn, coupOut;
+};
+
+RefSizes deriveRef(const ConParams &c)
+{
+    auto m = [&](const char *k) -> uint32_t {
+        auto it = c.m.find(k);
+        return it == c.m.end() ? 0u : it->second;
+    };
+    RefSizes r{};
+    r.numGL = (1u << m("glXP2")) * (1u << m("glYP2"));
+    r.glToGR = m("maxnumpGLfromGLtoGR");
+    r.glToGO = m("maxnumpGLfromGLtoGO");
+    r.glFromGO = m("maxnumpGLfromGOtoGL");
+    r.numMF = 1u << m("numMFP2");
+    r.mfToGL = r.numGL / r.numMF;
+    r.mfToGO = r.mfToGL * r.glToGO;
+    r.mfToGR = r.mfToGL * r.glToGR;
+    r.numGO = (1u << m("goXP2")) * (1u << m("goYP2"));
+    r.goFromGR = m("maxnumpGOfromGRtoGO");
+    r.goFromGL = m("maxnumpGOfromGLtoGO");
+    r.goFromMF = r.goFromGL;
+    r.goToGL = m("maxnumpGOfromGOtoGL");
+    r.goToGR = r.goToGL * r.glToGR;
+    r.numGR = (1u << m("grXP2")) * (1u << m("grYP2"));
+    r.grToGO = m("maxnumpGRfromGRtoGO");
+    r.grFromGO = m("maxnumpGRfromGOtoGR");
+    r.grFromMFkey = m("maxnumpGRfromMFtoGR");
+    r.numSC = 1u << m("numSCP2");
+    r.numPC = 1u << m("numPCP2");
+    r.numBC = r.numPC * 4;
+    r.numNC = 1u << m("numNCP2");
+    r.numIO = 1u << m("numIOP2");
+    r.pcFromGR = r.numGR / r.numPC;
+    r.pcFromBC = 16;
+    r.pcToBC = 16;
+    r.pcFromSC = r.numSC / r.numPC;
+    r.pcToNC = m("numpPCfromPCtoNC");
+    r.bcToPC = 4;
+    r.bcFromPC = 4;
+    r.scToPC = 1;
+    r.ncFromPC = r.numPC / r.numNC * r.pcToNC;
+    r.ncToIO = r.numIO;
+    r.ncFromMF = r.numMF / r.numNC;
+    r.ioToPC = r.numPC / r.numIO;
+    r.ioFromNC = r.numNC;
+    r.ioIn = r.numIO - 1;
+    r.ioOut = r.numIO - 1;
+    r.gabaIn = c.maxGABAIn;
+    r.gabaOut = c.maxGABAOut;
+    r.coupIn = c.maxCoupIn;
+    r.coupOut = c.maxCoupOut;
+    return r;
+}
+
+// One array of the file: its name, its field here (-1: kept as a blob, or
+// synthesised when writing a state built here), element size, rows and
+// columns as the file lays them out, and whether each zone has its own.
+struct Sec {
+    const char *name;
+    int field;
+    uint32_t elem;
+    uint64_t rows, cols;
+    bool zoned;
+};
+
+std::vector<Sec> sections(const RefSizes &r, uint32_t popBins)
+{
+    const uint64_t gl = r.numGL, mf = r.numMF, go = r.numGO, gr = r.numGR;
+    std::vector<Sec> v = {
+        // InNetConnectivityState::stateRW
+        {"haspGLfromMFtoGL", -1, 1, gl, 1, false},
+        {"pGLfromMFtoGL", -1, 4, gl, 1, false},
+        {"numpGLfromGLtoGO", -1, 4, gl, 1, false},
+        {"pGLfromGLtoGO", -1, 4, gl, r.glToGO, false},
+        {"numpGLfromGOtoGL", -1, 4, gl, 1, false},
+        {"pGLfromGOtoGL", -1, 4, gl, r.glFromGO, false},
+        {"numpGLfromGLtoGR", -1, 4, gl, 1, false},
+        {"pGLfromGLtoGR", -1, 4, gl, r.glToGR, false},
+        {"numpMFfromMFtoGL", -1, 4, mf, 1, false},
+        {"pMFfromMFtoGL", -1, 4, mf, r.mfToGL, false},
+        {"numpMFfromMFtoGR", -1, 4, mf, 1, false},
+        {"pMFfromMFtoGR", -1, 4, mf, r.mfToGR, false},
+        {"numpMFfromMFtoGO", CBM_F_NUMP_MF_FROM_MF_TO_GO, 4, mf, 1, false},
+        {"pMFfromMFtoGO", CBM_F_P_MF_FROM_MF_TO_GO, 4, mf, r.mfToGO, false},
+        {"numpGOfromGLtoGO", -1, 4, go, 1, false},
+        {"pGOfromGLtoGO", -1, 4, go, r.goFromGL, false},
+        {"numpGOfromGOtoGL", -1, 4, go, 1, false},
+        {"pGOfromGOtoGL", -1, 4, go, r.goToGL, false},
+        {"numpGOfromMFtoGO", -1, 4, go, 1, false},
+        {"pGOfromMFtoGO", -1, 4, go, r.goFromMF, false},
+        {"numpGOfromGOtoGR", -1, 4, go, 1, false},
+        {"pGOfromGOtoGR", -1, 4, go, r.goToGR, false},
+        {"numpGOfromGRtoGO", -1, 4, go, 1, false},
+        {"pGOfromGRtoGO", -1, 4, go, r.goFromGR, false},
+        {"numpGOGABAInGOGO", -1, 4, go, 1, false},
+        {"pGOGABAInGOGO", -1, 4, go, r.gabaIn, false},
+        {"numpGOGABAOutGOGO", CBM_F_NUMP_GO_GABA_OUT_GOGO, 4, go, 1, false},
+        {"pGOGABAOutGOGO", CBM_F_P_GO_GABA_OUT_GOGO, 4, go, r.gabaOut, false},
+        {"numpGOCoupInGOGO", CBM_F_NUMP_GO_COUP_IN_GOGO, 4, go, 1, false},
+        {"pGOCoupInGOGO", CBM_F_P_GO_COUP_IN_GOGO, 4, go, r.coupIn, false},
+        {"numpGOCoupOutGOGO", -1, 4, go, 1, false},
+        {"pGOCoupOutGOGO", -1, 4, go, r.coupOut, false},
+        {"pGRDelayMaskfromGRtoBSP", CBM_F_P_GR_DELAYMASK_GR_TO_BSP, 4, gr, 1, false},
+        {"numpGRfromGLtoGR", -1, 4, gr, 1, false},
+        // allocated and written with maxnumpGRfromGOtoGR columns (innetconnectivitystate.cpp:517,595)
+        {"pGRfromGLtoGR", -1, 4, gr, r.grFromGO, false},
+        {"numpGRfromGRtoGO", CBM_F_NUMP_GR_FROM_GR_TO_GO, 4, gr, 1, false},
+        {"pGRfromGRtoGO", CBM_F_P_GR_FROM_GR_TO_GO, 4, gr, r.grToGO, false},
+        {"pGRDelayMaskfromGRtoGO", CBM_F_P_GR_DELAYMASK_GR_TO_GO, 4, gr, r.grToGO, false},
+        {"numpGRfromGOtoGR", CBM_F_NUMP_GR_FROM_GO_TO_GR, 4, gr, 1, false},
+        {"pGRfromGOtoGR", CBM_F_P_GR_FROM_GO_TO_GR, 4, gr, r.grFromGO, false},
+        {"numpGRfromMFtoGR", CBM_F_NUMP_GR_FROM_MF_TO_GR, 4, gr, 1, false},
+        // also maxnumpGRfromGOtoGR columns (innetconnectivitystate.cpp:527,605)
+        {"pGRfromMFtoGR", CBM_F_P_GR_FROM_MF_TO_GR, 4, gr, r.grFromGO, false},
+        // MZoneConnectivityState::stateRW, per zone
+        {"pBCfromBCtoPC", CBM_F_P_BC_FROM_BC_TO_PC, 4, r.numBC, r.bcToPC, true},
+        {"pBCfromPCtoBC", -1, 4, r.numBC, r.bcFromPC, true},
+        {"pSCfromSCtoPC", -1, 4, r.numSC, r.scToPC, true},
+        {"pPCfromBCtoPC", -1, 4, r.numPC, r.pcFromBC, true},
+        // written in the place of pPCfromPCtoBC, which the file never holds
+        // (mzoneconnectivitystate.cpp:150)
+        {"pBCfromPCtoBC#2", -1, 4, r.numPC, r.pcToBC, true},
+        {"pPCfromSCtoPC", CBM_F_P_PC_FROM_SC_TO_PC, 4, r.numPC, r.pcFromSC, true},
+        {"pPCfromPCtoNC", -1, 4, r.numPC, r.pcToNC, true},
+        {"pPCfromIOtoPC", -1, 4, r.numPC, 1, true},
+        {"pNCfromPCtoNC", CBM_F_P_NC_FROM_PC_TO_NC, 4, r.numNC, r.ncFromPC, true},
+        {"pNCfromNCtoIO", -1, 4, r.numNC, r.ncToIO, true},
+        {"pNCfromMFtoNC", CBM_F_P_NC_FROM_MF_TO_NC, 4, r.numNC, r.ncFromMF, true},
+        {"pIOfromIOtoPC", -1, 4, r.numIO, r.ioToPC, true},
+        {"pIOfromNCtoIO", CBM_F_P_IO_FROM_NC_TO_IO, 4, r.numIO, r.ioFromNC, true},
+        {"pIOInIOIO", CBM_F_P_IO_IN_IOIO, 4, r.numIO, r.ioIn, true},
+        {"pIOOutIOIO", -1, 4, r.numIO, r.ioOut, true},
+        // InNetActivityState::stateRW
+        {"histMF", CBM_F_HIST_MF, 1, mf, 1, false},
+        {"apBufMF", CBM_F_APBUF_MF, 4, mf, 1, false},
+        {"apGO", CBM_F_AP_GO, 1, go, 1, false},
+        {"apBufGO", CBM_F_APBUF_GO, 4, go, 1, false},
+        {"vGO", CBM_F_VM_GO, 4, go, 1, false},
+        {"vCoupleGO", CBM_F_VCOUPLE_GO, 4, go, 1, false},
+        {"threshCurGO", CBM_F_THRESH_GO, 4, go, 1, false},
+        {"inputMFGO", CBM_F_INPUT_MF_GO, 4, go, 1, false},
+        {"inputGOGO", CBM_F_INPUT_GO_GO, 4, go, 1, false},
+        {"gMFGO", CBM_F_G_MF_GO, 4, go, 1, false},
+        {"gGRGO", CBM_F_G_GR_GO, 4, go, 1, false},
+        {"gGOGO", CBM_F_G_GO_GO, 4, go, 1, false},
+        {"gMGluRGO", CBM_F_G_MGLUR_GO, 4, go, 1, false},
+        {"gMGluRIncGO", CBM_F_G_MGLURINC_GO, 4, go, 1, false},
+        {"mGluRGO", CBM_F_MGLUR_GO, 4, go, 1, false},
+        {"gluGO", CBM_F_GLU_GO, 4, go, 1, false},
+        {"apGR", CBM_F_AP_GR, 1, gr, 1, false},
+        {"apBufGR", CBM_F_APBUF_GR, 4, gr, 1, false},
+        {"gMFGR", -1, 4, gr, r.grFromMFkey, false},
+        {"gMFSumGR", CBM_F_G_MF_SUM_GR, 4, gr, 1, false},
+        {"gGOGR", -1, 4, gr, r.grFromGO, false},
+        {"gGOSumGR", CBM_F_G_GO_SUM_GR, 4, gr, 1, false},
+        {"threshGR", CBM_F_THRESH_GR, 4, gr, 1, false},
+        {"vGR", CBM_F_VM_GR, 4, gr, 1, false},
+        {"gKCaGR", CBM_F_GKCA_GR, 4, gr, 1, false},
+        {"historyGR", CBM_F_HIST_GR, 8, gr, 1, false},
+        {"apSC", CBM_F_AP_SC, 1, r.numSC, 1, false},
+        {"apBufSC", CBM_F_APBUF_SC, 4, r.numSC, 1, false},
+        {"gPFSC", CBM_F_G_PF_SC, 4, r.numSC, 1, false},
+        {"threshSC", CBM_F_THRESH_SC, 4, r.numSC, 1, false},
+        {"vSC", CBM_F_VM_SC, 4, r.numSC, 1, false},
+        {"inputSumPFSC", CBM_F_INPUT_SUM_PF_SC, 4, r.numSC, 1, false},
+        // MZoneActivityState::stateRW, per zone
+        {"apBC", CBM_F_AP_BC, 1, r.numBC, 1, true},
+        {"apBufBC", CBM_F_APBUF_BC, 4, r.numBC, 1, true},
+        {"inputPCBC", CBM_F_INPUT_PC_BC, 4, r.numBC, 1, true},
+        {"gPFBC", CBM_F_G_PF_BC, 4, r.numBC, 1, true},
+        {"gPCBC", CBM_F_G_PC_BC, 4, r.numBC, 1, true},
+        {"threshBC", CBM_F_THRESH_BC, 4, r.numBC, 1, true},
+        {"vBC", CBM_F_VM_BC, 4, r.numBC, 1, true},
+        {"apPC", CBM_F_AP_PC, 1, r.numPC, 1, true},
+        {"apBufPC", CBM_F_APBUF_PC, 4, r.numPC, 1, true},
+        {"inputBCPC", CBM_F_INPUT_BC_PC, 4, r.numPC, 1, true},
+        {"inputSCPC", CBM_F_INPUT_SC_PC, 1, r.numPC, r.pcFromSC, true},
+        {"pfSynWeightPC", CBM_F_PFPC_SYNW, 4, r.numPC, r.pcFromGR, true},
+        {"inputSumPFPC", CBM_F_INPUT_SUM_PF_PC, 4, r.numPC, 1, true},
+        {"gPFPC", CBM_F_G_PF_PC, 4, r.numPC, 1, true},
+        {"gBCPC", CBM_F_G_BC_PC, 4, r.numPC, 1, true},
+        {"gSCPC", CBM_F_G_SC_PC, 4, r.numPC, r.pcFromSC, true},
+        {"vPC", CBM_F_VM_PC, 4, r.numPC, 1, true},
+        {"threshPC", CBM_F_THRESH_PC, 4, r.numPC, 1, true},
+        {"histPCPopAct", CBM_F_HIST_PC_POPACT, 4, popBins, 1, true},
+        {"apIO", CBM_F_AP_IO, 1, r.numIO, 1, true},
+        {"apBufIO", CBM_F_APBUF_IO, 4, r.numIO, 1, true},
+        {"inputNCIO", CBM_F_INPUT_NC_IO, 1, r.numIO, r.ioFromNC, true},
+        {"gNCIO", CBM_F_G_NC_IO, 4, r.numIO, r.ioFromNC, true},
+        {"threshIO", CBM_F_THRESH_IO, 4, r.numIO, 1, true},
+        {"vIO", CBM_F_VM_IO, 4, r.numIO, 1, true},
+        {"vCoupleIO", CBM_F_VCOUPLE_IO, 4, r.numIO, 1, true},
+        {"pfPCPlastTimerIO", CBM_F_PFPC_PLAST_TIMER_IO, 4, r.numIO, 1, true},
+        {"apNC", CBM_F_AP_NC, 1, r.numNC, 1, true},
+        {"apBufNC", CBM_F_APBUF_NC, 4, r.numNC, 1, true},
+        {"inputPCNC", CBM_F_INPUT_PC_NC, 1, r.numNC, r.ncFromPC, true},
+        {"inputMFNC", CBM_F_INPUT_MF_NC, 1, r.numNC, r.ncFromMF, true},
+        {"gPCNC", CBM_F_G_PC_NC, 4, r.numNC, r.ncFromPC, true},
+        {"gPCScaleNC", CBM_F_G_PC_SCALE_NC, 4, r.numNC, r.ncFromPC, true},
+        {"mfSynWeightNC", CBM_F_MF_SYNW_NC, 4, r.numNC, r.ncFromMF, true},
+        {"gmaxNMDAofMFtoNC", CBM_F_GMAX_NMDA_MF_NC, 4, r.numNC, r.ncFromMF, true},
+        {"gmaxAMPAofMFtoNC", CBM_F_GMAX_AMPA_MF_NC, 4, r.numNC, r.ncFromMF, true},
+        {"gMFNMDANC", CBM_F_G_MF_NMDA_NC, 4, r.numNC, r.ncFromMF, true},
+        {"gMFAMPANC", CBM_F_G_MF_AMPA_NC, 4, r.numNC, r.ncFromMF, true},
+        {"threshNC", CBM_F_THRESH_NC, 4, r.numNC, 1, true},
+        {"vNC", CBM_F_VM_NC, 4, r.numNC, 1, true},
+        {"synIOPReleaseNC", CBM_F_SYN_IO_PRELEASE_NC, 4, r.numNC, 1, true},
+    };
+    return v;
+}
+
+// the array groups, in file order: InNet con, MZone con (per zone), InNet act, MZone act (per zone)
+enum { G_INNET_CON, G_MZONE_CON, G_INNET_ACT, G_MZONE_ACT };
+int groupOf(const std::vector<Sec> &all, size_t i)
+{
+    const std::string n = all[i].name;
+    size_t firstMZ = 0, firstIA = 0, firstMA = 0;
+    for (size_t k = 0; k < all.size(); k++) {
+        if (!firstMZ && std::string(all[k].name) == "pBCfromBCtoPC") firstMZ = k;
+        if (!firstIA && std::string(all[k].name) == "histMF") firstIA = k;
+        if (!firstMA && std::string(all[k].name) == "apBC") firstMA = k;
+    }
+    return i < firstMZ ? G_INNET_CON : i < firstIA ? G_MZONE_CON : i < firstMA ? G_INNET_ACT : G_MZONE_ACT;
+}
+
+// Copy rows x fromCols elements into rows x toCols (fill the extra columns;
+// fails if a dropped column holds anything but the fill).
+bool repack(const uint8_t *src, uint64_t rows, uint64_t fromCols, uint8_t *dst, uint64_t toCols, uint32_t elem,
+            uint32_t fill)
+{
+    for (uint64_t rI = 0; rI < rows; rI++)
+        for (uint64_t c = 0; c < std::max(fromCols, toCols); c++) {
+            if (c < fromCols && c < toCols) {
+                std::memcpy(dst + (rI * toCols + c) * elem, src + (rI * fromCols + c) * elem, elem);
+            } else if (c < toCols) {
+                std::memcpy(dst + (rI * toCols + c) * elem, &fill, elem);
+            } else {
+                uint32_t v = 0;
+                std::memcpy(&v, src + (rI * fromCols + c) * elem, std::min<uint32_t>(elem, 4));
+                if (v != fill) return false;
+            }
+        }
+    return true;
+}
+
+bool isIndexList(const char *name) { return name[0] == 'p' && std::strncmp(name, "pfSyn", 5) != 0 && std::strncmp(name, "pfPC", 4) != 0; }
+
+// Content of a section a state built here does not hold: what the
+// reference's constructors leave in it (mzoneconnectivitystate.cpp:190-417:
+// pBCfromPCtoBC, pPCfromBCtoPC and pIOOutIOIO are never filled), the reverse
+// lists of the lists the step uses, or empty lists (count 0, entries
+// UINT_MAX) and zero conductances.
+std::vector<uint8_t> synth(cbm_state *st, const Sec &sc, const RefSizes &r, int z)
+{
+    const uint64_t n = sc.rows * sc.cols;
+    std::vector<uint8_t> out(n * sc.elem, 0);
+    uint32_t *u = reinterpret_cast<uint32_t *>(out.data());
+    const std::string nm = sc.name;
+    if (sc.elem == 4 && isIndexList(sc.name)) std::fill(u, u + n, UINT32_MAX);
+    if (nm == "pSCfromSCtoPC") {
+        for (uint64_t i = 0; i < r.numSC; i++) u[i] = (uint32_t)(i / r.pcFromSC);
+    } else if (nm == "pPCfromIOtoPC") {
+        for (uint64_t i = 0; i < r.numPC; i++) u[i] = (uint32_t)(i / r.ioToPC);
+    } else if (nm == "pNCfromNCtoIO") {
+        for (uint64_t i = 0; i < r.numIO; i++)
+            for (uint64_t j = 0; j < r.numNC; j++) u[j * r.ncToIO + i] = (uint32_t)i;
+    } else if (nm == "pIOfromIOtoPC") {
+        for (uint64_t i = 0; i < r.numIO; i++)
+            for (uint64_t j = 0; j < r.ioToPC; j++) u[i * r.ioToPC + j] = (uint32_t)(i * r.ioToPC + j);
+    } else if (nm == "pPCfromPCtoNC") {
+        // connectPCtoNC (mzoneconnectivitystate.cpp:276-358) appends NC i to
+        // a PC's list: first the regular block, then the extra NCs in NC
+        // order, then the last NC's fill
+        const uint32_t *nc = st->F<uint32_t>(CBM_F_P_NC_FROM_PC_TO_NC, z);
+        const uint64_t kN = r.ncFromPC, reg = kN / 3 ? kN / 3 : kN;
+        std::vector<uint32_t> cnt(r.numPC, 0);
+        auto add = [&](uint64_t i, uint64_t j) {
+            const uint32_t pc = nc[i * kN + j];
+            if (pc < r.numPC && cnt[pc] < r.pcToNC) u[(uint64_t)pc * r.pcToNC + cnt[pc]++] = (uint32_t)i;
+        };
+        for (uint64_t i = 0; i < r.numNC; i++)
+            for (uint64_t j = 0; j < reg && j < kN; j++) add(i, j);
+        for (uint64_t i = 0; i + 1 < r.numNC; i++)
+            for (uint64_t j = reg; j < kN; j++) add(i, j);
+        for (uint64_t j = reg; j < kN; j++) add(r.numNC - 1, j);
+    } else if (nm == "numpGOGABAInGOGO" || nm == "pGOGABAInGOGO" || nm == "numpGOCoupOutGOGO" ||
+               nm == "pGOCoupOutGOGO") {
+        // reverse of the GO-GO out-list (GABA) or in-list (coupling), in source order
+        const bool gaba = nm.find("GABA") != std::string::npos;
+        const uint32_t *num = st->F<uint32_t>(gaba ? CBM_F_NUMP_GO_GABA_OUT_GOGO : CBM_F_NUMP_GO_COUP_IN_GOGO);
+        const uint32_t *lst = st->F<uint32_t>(gaba ? CBM_F_P_GO_GABA_OUT_GOGO : CBM_F_P_GO_COUP_IN_GOGO);
+        const uint64_t w = gaba ? st->s.maxnumpGOGABAOutGOGO : st->s.maxnumpGOCoupInGOGO;
+        const uint64_t cols = gaba ? r.gabaIn : r.coupOut;
+        std::vector<uint32_t> c(r.numGO, 0);
+        std::vector<uint32_t> lists(r.numGO * cols, UINT32_MAX);
+        for (uint64_t i = 0; i < r.numGO; i++)
+            for (uint64_t k = 0; k < num[i] && k < w; k++) {
+                const uint32_t t = lst[i * w + k];
+                if (t < r.numGO && c[t] < cols) lists[(uint64_t)t * cols + c[t]++] = (uint32_t)i;
+            }
+        if (nm[0] == 'n')
+            std::memcpy(out.data(), c.data(), r.numGO * 4);
+        else
+            std::memcpy(out.data(), lists.data(), lists.size() * 4);
+    } else if (sc.elem == 4 && nm.compare(0, 4, "nump") == 0) {
+        std::fill(u, u + n, 0u);
+    }
+    return out;
+}
+
+// Header of a state built here: its parameter files, with the sizes the file
+// layout derives made to match the arrays held here (the glomerulus layer of
+// the reference's generator does not exist here: one glomerulus per MF).
+ConParams headerOf(cbm_state *st)
+{
+    if (st->fromCheckpoint) return st->con;
+    ConParams c = st->con;
+    RefSizes r = deriveRef(c);
+    const cbm_sizes &s = st->s;
+    uint32_t mfP2 = 0;
+    while ((1u << mfP2) < s.numMF) mfP2++;
+    if (r.mfToGO != s.maxnumpMFfromMFtoGO || r.mfToGL == 0) {
+        c.m["glXP2"] = mfP2;
+        c.m["glYP2"] = 0;
+        c.m["maxnumpGLfromGLtoGO"] = s.maxnumpMFfromMFtoGO;
+    }
+    const uint32_t wGR = std::max(s.maxnumpGRfromMFtoGR, s.maxnumpGRfromGOtoGR);
+    if (r.grFromGO != wGR) c.m["maxnumpGRfromGOtoGR"] = wGR;
+    if (r.grFromMFkey != s.maxnumpGRfromMFtoGR) c.m["maxnumpGRfromMFtoGR"] = s.maxnumpGRfromMFtoGR;
+    if (r.grToGO != s.maxnumpGRfromGRtoGO) c.m["maxnumpGRfromGRtoGO"] = s.maxnumpGRfromGRtoGO;
+    return c;
+}
+
+int writeCheckpoint(cbm_state *st, std::ostream &f)
+{
+    const ConParams hc = headerOf(st);
+    const RefSizes r = deriveRef(hc);
+    const cbm_sizes &s = st->s;
+    if (r.numGR != s.numGR || r.numGO != s.numGO || r.numMF != s.numMF || r.numPC != s.numPC || r.numNC != s.numNC ||
+        r.numIO != s.numIO || r.numSC != s.numSC)
+        return fail(CBM_E_STATE, "the parameter header does not describe this state's sizes");
+    // InNetActivityState::stateRW sets these on write as on read (innetactivitystate.cpp:247-248)
+    fillF32(st->f[CBM_F_INPUT_GOGABA_SYNDEP_GO][0], 1.0f);
+    fillF32(st->f[CBM_F_GO_GABA_OUT_SYNSCALE_GOGO][0], 0.0f);
+    f << s.numZones << std::endl;
+    for (const auto &kv : hc.m) f << kv.first << " " << kv.second << std::endl;
+    f << "GOGOLocalMatrix" << std::endl;
+    f << "maxnumpGOInGOtoGO " << hc.maxGABAIn << std::endl;
+    f << "maxnumpGOOutGOtoGO " << hc.maxGABAOut << std::endl;
+    for (uint32_t i = 0; i < hc.maxGABAOut; i++)
+        f << hc.gogoGABA[i][0] << " " << hc.gogoGABA[i][1] << " " << hc.gogoGABA[i][2] << std::endl;
+    f << "GOGOCoupLocalMatrix" << std::endl;
+    f << "maxnumpGOCoupInGOtoGO " << hc.maxCoupIn << std::endl;
+    f << "maxnumpGOCoupOutGOtoGO " << hc.maxCoupOut << std::endl;
+    for (uint32_t i = 0; i < hc.maxCoupOut; i++)
+        f << hc.gogoCoup[i][0] << " " << hc.gogoCoup[i][1] << " " << hc.gogoCoup[i][2] << std::endl;
+    f << "connectivityParamEnd 1" << std::endl;
+    for (const auto &kv : st->act) f << kv.first << " " << kv.second << std::endl;
+    f << "activityParamEnd 1" << std::endl;
+
+    const std::vector<Sec> all = sections(r, s.numPopHistBinsPC);
+    auto emit = [&](size_t i, int z) -> int {
+        const Sec &sc = all[i];
+        const uint64_t bytes = sc.rows * sc.cols * sc.elem;
+        std::vector<uint8_t> buf;
+        if (sc.field >= 0) {
+            size_t e, rr, cc;
+            int zoned, kind;
+            cbm_field_shape(sc.field, &st->s, &e, &rr, &cc, &zoned, &kind);
+            const std::vector<uint8_t> &src = st->f[sc.field][sc.zoned ? z : 0];
+            if (rr * cc * e == bytes) {
+                buf = src;
+            } else {
+                buf.assign(bytes, 0);
+                if (rr != sc.rows || e != sc.elem ||
+                    !repack(src.data(), rr, cc, buf.data(), sc.cols, sc.elem, isIndexList(sc.name) ? UINT32_MAX : 0))
+                    return fail(CBM_E_STATE, std::string("array ") + sc.name + " does not fit the file layout");
+            }
+        } else {
+            const std::string key = sc.name;
+            auto it = st->blob[sc.zoned ? z : 0].find(key == "pBCfromPCtoBC#2" ? "pBCfromPCtoBC" : key);
+            if (it != st->blob[sc.zoned ? z : 0].end() && it->second.size() == bytes)
+                buf = it->second;
+            else if (key == "pBCfromPCtoBC#2")
+                buf = synth(st, all[i - 3], r, z);  // the same array as pBCfromPCtoBC
+            else
+                buf = synth(st, sc, r, z);
+            buf.resize(bytes, 0);
+        }
+        f.write(reinterpret_cast<const char *>(buf.data()), (std::streamsize)bytes);
+        return CBM_OK;
+    };
+    for (int g = G_INNET_CON; g <= G_MZONE_ACT; g++) {
+        const bool perZone = g == G_MZONE_CON || g == G_MZONE_ACT;
+        for (int z = 0; z < (perZone ? (int)s.numZones : 1); z++)
+            for (size_t i = 0; i < all.size(); i++)
+                if (groupOf(all, i) == g) {
+                    int rc = emit(i, z);
+                    if (rc) return rc;
+                }
+    }
+    f.flush();
+    if (!f) return fail(CBM_E_ARG, "checkpoint write failed");
+    return CBM_OK;
+}
+
+int readCheckpoint(std::istream &f, cbm_state **out)
+{
+    *out = nullptr;
+    int nz = 0;
+    f >> nz;
+    if (!f || nz < 1 || nz > kMaxZones) return fail(CBM_E_UNSUPPORTED, "zone count must be 1..8");
+    // ConnectivityParams(fstream&) (connectivityparams.cpp:12-88)
+    ConParams cp;
+    std::string key;
+    bool hasCoup = false, ended = false;
+    while (f >> key) {
+        uint32_t v;
+        if (key == "connectivityParamEnd") {
+            f >> v;
+            ended = true;
+            break;
+        }
+        if (key == "GOGOLocalMatrix" || key == "GOGOCoupLocalMatrix") {
+            std::string k2;
+            uint32_t nin, nout;
+            f >> k2 >> nin >> k2 >> nout;
+            std::vector<std::array<float, 3>> rows(nout);
+            for (auto &rw : rows) f >> rw[0] >> rw[1] >> rw[2];
+            if (key == "GOGOLocalMatrix") {
+                cp.maxGABAIn = nin;
+                cp.maxGABAOut = nout;
+                cp.gogoGABA = rows;
+            } else {
+                hasCoup = true;
+                cp.maxCoupIn = nin;
+                cp.maxCoupOut = nout;
+                cp.gogoCoup = rows;
+            }
+        } else {
+            f >> v;
+            cp.m[key] = v;
+        }
+    }
+    if (!ended) return fail(CBM_E_ARG, "no connectivityParamEnd in the checkpoint header");
+    if (!hasCoup) cp.gogoCoup.assign(1, {1.f, 1.f, 0.f});
+    // ActivityParams(fstream&) (activityparams.cpp:12-42)
+    ActMap am;
+    ended = false;
+    float fv;
+    while (f >> key >> fv) {
+        if (key == "activityParamEnd") {
+            ended = true;
+            break;
+        }
+        am.m[key] = fv;
+    }
+    if (!ended) return fail(CBM_E_ARG, "no activityParamEnd in the checkpoint header");
+    f.seekg(1, std::ios::cur);  // CBMState(fstream&) skips the line end (cbmstate.cpp:19)
+
+    cbm_state *st = new cbm_state();
+    st->con = cp;
+    st->act = am.m;
+    st->fromCheckpoint = true;
+    deriveAct(am, st->p);
+    const RefSizes r = deriveRef(cp);
+    cbm_sizes &s = st->s;
+    s.grX = 1u << cp.get("grXP2");
+    s.grY = 1u << cp.get("grYP2");
+    s.numGR = r.numGR;
+    s.goX = 1u << cp.get("goXP2");
+    s.goY = 1u << cp.get("goYP2");
+    s.numGO = r.numGO;
+    s.numMF = r.numMF;
+    s.numSC = r.numSC;
+    s.numPC = r.numPC;
+    s.numBC = r.numBC;
+    s.numNC = r.numNC;
+    s.numIO = r.numIO;
+    s.numZones = (uint32_t)nz;
+    s.maxnumpGRfromMFtoGR = r.grFromGO;  // the file's MF->GR rows have maxnumpGRfromGOtoGR columns
+    s.maxnumpGRfromGOtoGR = r.grFromGO;
+    s.maxnumpGRfromGRtoGO = r.grToGO;
+    s.maxnumpMFfromMFtoGO = r.mfToGO;
+    s.maxnumpGOGABAOutGOGO = r.gabaOut;
+    s.maxnumpGOCoupInGOGO = r.coupIn;
+    s.numpPCfromGRtoPC = r.pcFromGR;
+    s.numpBCfromGRtoBC = r.numGR / r.numBC;
+    s.numpSCfromGRtoSC = r.numGR / r.numSC;
+    s.numpBCfromBCtoPC = r.bcToPC;
+    s.numpPCfromPCtoBC = r.pcToBC;
+    s.numpPCfromSCtoPC = r.pcFromSC;
+    s.numpNCfromPCtoNC = r.ncFromPC;
+    s.numpNCfromMFtoNC = r.ncFromMF;
+    s.numpIOfromNCtoIO = r.ioFromNC;
+    s.numpIOInIOIO = r.ioIn;
+    s.numPopHistBinsPC = st->p.numPopHistBinsPC;
+    st->grPFVel = std::max(1u, cp.get("grPFVelInGRXPerTStep"));
+    st->grAFDelay = cp.get("grAFDelayInTStep");
+    st->msPerStep = std::max(1u, (uint32_t)st->p.msPerTimeStep);
+    allocFields(st);
+
+    const std::vector<Sec> all = sections(r, s.numPopHistBinsPC);
+    for (int g = G_INNET_CON; g <= G_MZONE_ACT; g++) {
+        const bool perZone = g == G_MZONE_CON || g == G_MZONE_ACT;
+        for (int z = 0; z < (perZone ? nz : 1); z++)
+            for (size_t i = 0; i < all.size(); i++) {
+                if (groupOf(all, i) != g) continue;
+                const Sec &sc = all[i];
+                const uint64_t bytes = sc.rows * sc.cols * sc.elem;
+                std::vector<uint8_t> buf(bytes);
+                f.read(reinterpret_cast<char *>(buf.data()), (std::streamsize)bytes);
+                if (!f) {
+                    delete st;
+                    return fail(CBM_E_ARG, std::string("checkpoint ends inside ") + sc.name);
+                }
+                if (sc.field >= 0) {
+                    std::vector<uint8_t> &dst = st->f[sc.field][sc.zoned ? z : 0];
+                    if (dst.size() != bytes) {
+                        delete st;
+                        return fail(CBM_E_UNSUPPORTED, std::string("array ") + sc.name + " has an unexpected size");
+                    }
+                    dst = buf;
+                } else {
+                    // the second copy of pBCfromPCtoBC overwrites the first, as on a reference read
+                    const std::string k = std::string(sc.name) == "pBCfromPCtoBC#2" ? "pBCfromPCtoBC" : sc.name;
+                    std::vector<uint8_t> &b = st->blob[sc.zoned ? z : 0][k];
+                    if (b.size() == buf.size() || b.empty())
+                        b = buf;
+                    else
+                        std::memcpy(b.data(), buf.data(), std::min(b.size(), buf.size()));
+                }
+            }
+    }
+    // Not in the file.  pPCfromPCtoBC is rebuilt as connectPCtoBC makes it
+    // (mzoneconnectivitystate.cpp:241-255); the reference leaves it
+    // unset after a read.  Everything else starts as a new state does.
+    for (int z = 0; z < nz; z++) {
+        uint32_t *pcbc = st->F<uint32_t>(CBM_F_P_PC_FROM_PC_TO_BC, z);
+        const uint32_t ratio = s.numBC / s.numPC;
+        for (uint32_t i = 0; i < s.numPC; i++)
+            for (uint32_t j = 0; j < s.numpPCfromPCtoBC; j++)
+                pcbc[(size_t)i * s.numpPCfromPCtoBC + j] = (uint32_t)wrap((int)(i * ratio) - 6 + (int)j, (int)s.numBC);
+        float steps[2] = {st->p.synLTDStepSizeGRtoPC, st->p.synLTPStepSizeGRtoPC};
+        std::memcpy(st->f[CBM_F_GRPC_PLAST_STEPS][z].data(), steps, 8);
+        st->F<uint64_t>(CBM_F_ZONE_RNG_SKIP, z)[0] = s.numGR;
+    }
+    fillF32(st->f[CBM_F_INPUT_GOGABA_SYNDEP_GO][0], 1.0f);
+    fillF32(st->f[CBM_F_GO_GABA_OUT_SYNSCALE_GOGO][0], 0.0f);
+    std::memcpy(st->f[CBM_F_GINC_GR_TO_GO][0].data(), &st->p.gIncGRtoGO, 4);
+    *out = st;
+    return CBM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cbm_state_write_checkpoint(cbm_state *st, const char *path)
+{
+    if (!st || !path) return fail(CBM_E_ARG, "null argument");
+    std::ofstream f(path, std::ios::out | std::ios::binary | std::ios::trunc);
+    if (!f) return fail(CBM_E_ARG, std::string("cannot open ") + path);
+    return writeCheckpoint(st, f);
+}
+
+int cbm_state_read_checkpoint(const char *path, cbm_state **out)
+{
+    if (!path || !out) return fail(CBM_E_ARG, "null argument");
+    std::ifstream f(path, std::ios::in | std::ios::binary);
+    if (!f) return fail(CBM_E_ARG, std::string("cannot open ") + path);
+    return readCheckpoint(f, out);
+}
+
+int cbm_state_checkpoint_to_memory(cbm_state *st, void **data, size_t *bytes)
+{
+    if (!st || !data || !bytes) return fail(CBM_E_ARG, "null argument");
+    std::ostringstream o(std::ios::out | std::ios::binary);
+    int rc = writeCheckpoint(st, o);
+    if (rc) return rc;
+    const std::string b = o.str();
+    *data = std::malloc(b.size() ? b.size() : 1);
+    if (!*data) return fail(CBM_E_NOMEM, "out of host memory");
+    std::memcpy(*data, b.data(), b.size());
+    *bytes = b.size();
+    return CBM_OK;
+}
+
+int cbm_state_checkpoint_from_memory(const void *data, size_t bytes, cbm_state **out, size_t *consumed)
+{
+    if (!data || !out) return fail(CBM_E_ARG, "null argument");
+    std::istringstream in(std::string(static_cast<const char *>(data), bytes), std::ios::in | std::ios::binary);
+    int rc = readCheckpoint(in, out);
+    if (rc) return rc;
+    if (consumed) {
+        const std::streamoff pos = in.tellg();
+        *consumed = pos < 0 ? bytes : (size_t)pos;
+    }
+    return CBM_OK;
+}
+
+void cbm_state_free_memory(void *p) { std::free(p); }
 
 }  // extern "C"

@@ -63,6 +63,57 @@ class NetState:
         L.cbm_state_params(h, C.byref(self.params))
         self.num_zones = num_zones
 
+    # ------------------------------------------------------------ checkpoints
+    def write_checkpoint(self, path: str) -> None:
+        """Save in the CBMState file format (CBMState::writeState layout)."""
+        L = _abi.state_lib()
+        rc = L.cbm_state_write_checkpoint(self._h, str(path).encode())
+        if rc:
+            raise _abi.CbmError(f"cbm_state_write_checkpoint failed ({rc}): {L.cbm_state_last_error().decode()}")
+
+    def checkpoint_bytes(self) -> bytes:
+        L = _abi.state_lib()
+        d, n = C.c_void_p(), C.c_size_t()
+        rc = L.cbm_state_checkpoint_to_memory(self._h, C.byref(d), C.byref(n))
+        if rc:
+            raise _abi.CbmError(f"cbm_state_checkpoint_to_memory failed ({rc}): {L.cbm_state_last_error().decode()}")
+        try:
+            return C.string_at(d, n.value)
+        finally:
+            L.cbm_state_free_memory(d)
+
+    @classmethod
+    def read_checkpoint(cls, path: str) -> "NetState":
+        """A state saved in the CBMState file format (CBMState(fstream&))."""
+        L = _abi.state_lib()
+        h = C.c_void_p()
+        rc = L.cbm_state_read_checkpoint(str(path).encode(), C.byref(h))
+        if rc:
+            raise _abi.CbmError(f"cbm_state_read_checkpoint failed ({rc}): {L.cbm_state_last_error().decode()}")
+        return cls._wrap(h)
+
+    @classmethod
+    def from_checkpoint_bytes(cls, data: bytes) -> "NetState":
+        L = _abi.state_lib()
+        h = C.c_void_p()
+        used = C.c_size_t()
+        rc = L.cbm_state_checkpoint_from_memory(data, len(data), C.byref(h), C.byref(used))
+        if rc:
+            raise _abi.CbmError(f"cbm_state_checkpoint_from_memory failed ({rc}): {L.cbm_state_last_error().decode()}")
+        return cls._wrap(h)
+
+    @classmethod
+    def _wrap(cls, h) -> "NetState":
+        L = _abi.state_lib()
+        self = cls.__new__(cls)
+        self._h = h
+        self.sizes = Sizes()
+        self.params = ActParams()
+        L.cbm_state_sizes(h, C.byref(self.sizes))
+        L.cbm_state_params(h, C.byref(self.params))
+        self.num_zones = self.sizes.numZones
+        return self
+
     def field(self, name: str, zone: int = 0) -> np.ndarray:
         """numpy view (no copy) of a field in the CBMState layout."""
         info = field_info(FIELDS[name], self.sizes)

@@ -75,6 +75,39 @@ int cbm_state_load_into(cbm_state *st, cbm_handle *h);
  * calls, 2 = IRandom(lo,hi) and 3 = IRandomX(lo,hi) as int32. */
 int cbm_sfmt_sample(int seed, uint64_t skip, int kind, int lo, int hi, int n, void *out);
 
+/* ---------------------------------------------------------------- checkpoints
+ * The CBMState file format (CBMState::writeState / CBMState(fstream&),
+ * CBM_STATE_LIB_DEV/src/interfaces/cbmstate.cpp:12-33,86-103): the zone count,
+ * the connectivity and activity parameters as text
+ * (connectivityparams.cpp:192-218, activityparams.cpp:243-253), then the raw
+ * arrays of InNetConnectivityState, each zone's MZoneConnectivityState,
+ * InNetActivityState and each zone's MZoneActivityState, in the order and
+ * sizes of their stateRW methods (innetconnectivitystate.cpp:530-606,
+ * mzoneconnectivitystate.cpp:142-163, innetactivitystate.cpp:233-275,
+ * mzoneactivitystate.cpp:299-355).  As there:
+ *  - writing sets the GO synaptic-depression fields INPUT_GOGABA_SYNDEP_GO to 1
+ *    and GO_GABA_OUT_SYNSCALE_GOGO to 0 in the state (innetactivitystate.cpp:247-248),
+ *    and reading sets them the same way;
+ *  - the MZone section holds pBCfromPCtoBC twice and never pPCfromPCtoBC
+ *    (mzoneconnectivitystate.cpp:145,150).  A read rebuilds P_PC_FROM_PC_TO_BC
+ *    as connectPCtoBC makes it (the reference leaves it unset).
+ * Arrays the step does not use (the glomerulus layer, reverse lists,
+ * per-synapse GR conductances) are kept as read and written back unchanged,
+ * so a read followed by a write reproduces the file.  A state built by
+ * cbm_state_create has no glomerulus layer: its header gets one glomerulus per
+ * MF and the column counts of its own lists; the lists it does not hold are
+ * written as the reference's constructors leave them (mzoneconnectivitystate.cpp:
+ * 190-417) or as reverse lists of the ones it holds.  Not in the format: the
+ * zone generators (seeded by CBMSimCore), the step counter and the pipeline
+ * rows. */
+int cbm_state_write_checkpoint(cbm_state *st, const char *path);
+int cbm_state_read_checkpoint(const char *path, cbm_state **out);
+/* The same to and from memory: *data is allocated (free it with
+ * cbm_state_free_memory); *consumed receives the bytes the state took. */
+int cbm_state_checkpoint_to_memory(cbm_state *st, void **data, size_t *bytes);
+int cbm_state_checkpoint_from_memory(const void *data, size_t bytes, cbm_state **out, size_t *consumed);
+void cbm_state_free_memory(void *p);
+
 /* Message of the last failing call on this thread. */
 const char *cbm_state_last_error(void);
 
