@@ -17,8 +17,9 @@ Pins, on the CPU:
 * the C++ CBMState(fstream&) / writeState(fstream&) of the drop-in library
   read and write the same bytes.
 On the GPU: a network checkpointed mid-run and restored into a new handle
-continues bit for bit like the original (with the step counter, pipeline rows
-and zone generators, which the format does not hold, carried over).
+continues bit for bit like the original, once what the format does not hold
+is carried over: the step counter, the pipeline rows and the MZone members
+(zone generator, MF->NC plasticity flags, PC activity counters).
 """
 import ctypes as C
 import os
@@ -296,7 +297,9 @@ def test_restored_network_continues_bit_for_bit(gpu, tmp_path):
                  "PIPE_GRGO_SUM", "GINC_GR_TO_GO"):
         b.write(name, a.read(name))
     for z in range(st.num_zones):
-        for name in ("ZONE_RNG_STATE", "PIPE_PFPC_SUM", "GRPC_PLAST_STEPS"):
+        # MZone members, not CBMState: the zone generator, the MF->NC plasticity
+        # flags and the PC activity counters (mzone.cpp:526-605)
+        for name in ("ZONE_RNG_STATE", "PIPE_PFPC_SUM", "GRPC_PLAST_STEPS", "MFNC_NOLTP_NOLTD", "PC_POPACT_CTRS"):
             b.write(name, a.read(name, z), z)
     a.set_recorder(["PC", "NC", "IO", "GO"], 200)
     b.set_recorder(["PC", "NC", "IO", "GO"], 200)

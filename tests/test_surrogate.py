@@ -294,11 +294,18 @@ def test_training_beats_cluster_and_phase_means(gpu):
 
 def test_sweep_experiment_small(gpu):
     print("[case] surrogate-driven sweep judged by the exact step, 262K GR, 4 parameter sets", flush=True)
+    # 64-cell clusters: an SC block of this network holds 512 GRs, about 59 of
+    # them with a CS input, so 256-cell clusters would mix them with the others
     res = SG.sweep_experiment("b262k", train_trials=10, train_steps=250, segments=32, sweep_trials=2,
-                              ltd_scales=(1, 8), ltp_scales=(1, 8), judged=(0, 3), log=lambda m: print(m, flush=True))
+                              ltd_scales=(1, 8), ltp_scales=(1, 8), judged=(0, 3), cells=64,
+                              log=lambda m: print(m, flush=True))
     print({k: v for k, v in res.items() if k not in ("exact_cr", "surrogate_cr")}, flush=True)
     print("exact CR", res["exact_cr"], "surrogate CR", res["surrogate_cr"], flush=True)
     assert res["pf_rmse"] < res["pf_rmse_cluster_mean"]
-    assert res["speedup_vs_exact"] > 1.0
-    # the pre-trained network closes the lid on its first trial in the exact step
-    assert res["exact_cr"][0][0][0] is not None
+    # the pre-trained network closes the lid on every judged trial in the exact step
+    assert all(o is not None for e in res["exact_cr"] for o, _ in e)
+    # the surrogate-driven sweep does too, within 40 ms of the exact onset; the
+    # sweep driven by each cluster's mean rate (no trained network) never does
+    assert res["surrogate_vs_exact"]["onset_presence_mismatch"] == 0
+    assert res["surrogate_vs_exact"]["onset_abs_err_ms"] <= 40
+    assert res["cluster_mean_vs_exact"]["onset_presence_mismatch"] == res["cluster_mean_vs_exact"]["trials"]
