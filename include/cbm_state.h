@@ -98,8 +98,12 @@ int cbm_sfmt_sample(int seed, uint64_t skip, int kind, int lo, int hi, int n, vo
  * MF and the column counts of its own lists; the lists it does not hold are
  * written as the reference's constructors leave them (mzoneconnectivitystate.cpp:
  * 190-417) or as reverse lists of the ones it holds.  Not in the format: the
- * zone generators (seeded by CBMSimCore), the step counter and the pipeline
- * rows. */
+ * step counter, the pipeline rows, the zone generators (seeded by CBMSimCore),
+ * and the MZoneActivityState members its stateRW skips
+ * (mzoneactivitystate.h:69-86 against mzoneactivitystate.cpp:299-355):
+ * PC_POPACT_CTRS, ERR_DRIVE and MFNC_NOLTP_NOLTD.  The reference leaves those
+ * uninitialised after a read (mzoneactivitystate.cpp:21-29); here they read
+ * as zero. */
 int cbm_state_write_checkpoint(cbm_state *st, const char *path);
 int cbm_state_read_checkpoint(const char *path, cbm_state **out);
 /* The same to and from memory: *data is allocated (free it with

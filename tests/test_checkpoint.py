@@ -18,8 +18,9 @@ Pins, on the CPU:
   read and write the same bytes.
 On the GPU: a network checkpointed mid-run and restored into a new handle
 continues bit for bit like the original, once what the format does not hold
-is carried over: the step counter, the pipeline rows and the MZone members
-(zone generator, MF->NC plasticity flags, PC activity counters).
+is carried over: the step counter, the pipeline rows, the zone generator and
+the MZoneActivityState members stateRW skips (MF->NC plasticity flags, PC
+population-activity counters).
 """
 import ctypes as C
 import os
@@ -297,8 +298,9 @@ def test_restored_network_continues_bit_for_bit(gpu, tmp_path):
                  "PIPE_GRGO_SUM", "GINC_GR_TO_GO"):
         b.write(name, a.read(name))
     for z in range(st.num_zones):
-        # MZone members, not CBMState: the zone generator, the MF->NC plasticity
-        # flags and the PC activity counters (mzone.cpp:526-605)
+        # not in the file: the zone generator, and the MZoneActivityState members
+        # stateRW skips (mzoneactivitystate.cpp:299-355), the MF->NC plasticity
+        # flags and the PC population-activity counters
         for name in ("ZONE_RNG_STATE", "PIPE_PFPC_SUM", "GRPC_PLAST_STEPS", "MFNC_NOLTP_NOLTD", "PC_POPACT_CTRS"):
             b.write(name, a.read(name, z), z)
     a.set_recorder(["PC", "NC", "IO", "GO"], 200)
