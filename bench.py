@@ -157,6 +157,39 @@ def surrogate_bench(which: str) -> dict:
     return out
 
 
+def cartpole_bench(which: str) -> dict:
+    """The cart-pole with two microzones in closed loop (include/cbm_env.h):
+    per step the environment's MF rates, MF spikes from PoissonRegenCells,
+    one network step, the NC spikes of both zones read back and the
+    environment's step, all through the library's native loop."""
+    from cerebellum_simulation_amd import Network, configs
+    from cerebellum_simulation_amd import cartpole as CP
+    name = "cp1m" if which == "full" else "t8k2z"
+    st = CP.build_state(configs.act_params_text(), configs.conparams_text(**configs.CONFIGS[name][0]), seed=1)
+    net = Network.from_state(st)
+    env = CP.Cartpole(net, env_seed=1, mf_seed=1, closed_loop=True, max_num_trials=10**6)
+    env.run(1600, trace=False)  # past the first time-out: the pole is aloft from here on
+    net.sync()
+    steps = 4000
+    t0 = time.perf_counter()
+    done = env.run(steps, trace=False)
+    net.sync()
+    el = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    net.calc_activity(steps)
+    net.sync()
+    step_only = time.perf_counter() - t0
+    s = st.sizes
+    out = {"workload": (f"cart-pole, two microzones, closed loop; {s.numGR:,} GR, {s.numMF:,} MF, {s.numNC} NC "
+                        "per zone (the sizes of CBM_Params/conParams.txt)" if which == "full" else
+                        f"cart-pole, two microzones, closed loop; {s.numGR:,} GR"),
+           "steps": done, "sim_ms_per_s": round(done / el, 1), "us_per_step": round(el / done * 1e6, 2),
+           "network_step_only_us": round(step_only / steps * 1e6, 2)}
+    env.close()
+    net.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -174,6 +207,8 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=200)
     ap.add_argument("--surrogate", default="full", choices=["full", "small", "none"],
                     help="config 4: the learned input layer and the plasticity sweep it drives")
+    ap.add_argument("--cartpole", default="full", choices=["full", "small", "none"],
+                    help="the cart-pole closed loop on two microzones (full: 1M-GR network, 2,048 MF)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB, 5 times (the known-size kernel that scales "
                          "the FETCH_SIZE / WRITE_SIZE counters)")
@@ -253,6 +288,13 @@ def main():
         except Exception as e:  # the headline line must still be printed
             surrogate = {"error": f"{type(e).__name__}: {e}"}
 
+    cartpole = None
+    if args.cartpole != "none":
+        try:
+            cartpole = cartpole_bench(args.cartpole)
+        except Exception as e:  # the headline line must still be printed
+            cartpole = {"error": f"{type(e).__name__}: {e}"}
+
     cpu = None
     if not args.no_cpu:
         cpu_steps = args.cpu_steps or {"c0": 5000, "c1": 2000, "c2": 400}.get(args.config, 100)
@@ -267,6 +309,7 @@ def main():
                    "numGR": s.numGR, "numGO": s.numGO, "numMF": s.numMF, "numPC": s.numPC,
                    "parallelism": "single GPU", "step_mode": "fused" if fused else "two streams"},
         "roofline": roofline, "cpu_baseline": cpu, "batched": batched, "surrogate": surrogate,
+        "cartpole": cartpole,
     }
     print(json.dumps(out), flush=True)
 

@@ -217,6 +217,39 @@ def state_lib() -> C.CDLL:
     return _state
 
 
+class CartpoleOpts(C.Structure):
+    """cbm_cartpole_opts (include/cbm_env.h)."""
+    _fields_ = [("maxNumTrials", C.c_int32), ("closedLoop", C.c_int32), ("maxTrialLen", C.c_int64),
+                ("trackLen", C.c_float), ("verbose", C.c_int32), ("logfile", C.c_char_p),
+                ("simStateDir", C.c_char_p)]
+
+
+_core = None
+
+
+def core_lib() -> C.CDLL:
+    """libcbm_core.so (the C++ API and the environments of include/cbm_env.h)
+    with the cbm_env.h argument types declared."""
+    global _core
+    if _core is None:
+        hip_lib()
+        state_lib()
+        L = _load("libcbm_core.so")
+        P, I, V = C.c_void_p, C.c_int, C.c_void_p
+        L.cbm_env_last_error.restype = C.c_char_p
+        L.cbm_cartpole_create.argtypes = [P, C.POINTER(Sizes), I, I, C.POINTER(CartpoleOpts), C.POINTER(P)]
+        L.cbm_cartpole_destroy.argtypes = [P]
+        L.cbm_cartpole_run.argtypes = [P, I, V, C.POINTER(I)]
+        L.cbm_cartpole_terminated.argtypes = [P]
+        L.cbm_cartpole_get_state.argtypes = [P, V]
+        L.cbm_cartpole_mf_spikes.argtypes = [P, V, V]
+        L.cbm_cartpole_step.argtypes = [P, V, V, V, V]
+        L.cbm_cartpole_mf_groups.argtypes = [P, V]
+        L.cbm_cartpole_mf_rates.argtypes = [P, V, V]
+        _core = L
+    return _core
+
+
 class CbmError(RuntimeError):
     """A cbm_* call returned an error code."""
 
@@ -224,7 +257,10 @@ class CbmError(RuntimeError):
 def check(rc: int, where: str, lib: C.CDLL | None = None) -> None:
     if rc != 0:
         L = lib or hip_lib()
-        getter = L.cbm_state_last_error if hasattr(L, "cbm_state_last_error") and L is _state else L.cbm_last_error
+        if L is _core:
+            getter = L.cbm_env_last_error
+        else:
+            getter = L.cbm_state_last_error if hasattr(L, "cbm_state_last_error") and L is _state else L.cbm_last_error
         msg = getter().decode(errors="replace")
         raise CbmError(f"{where} failed ({rc}): {msg}")
 

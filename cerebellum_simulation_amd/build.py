@@ -70,16 +70,20 @@ def build_core() -> str | None:
     return out
 
 
-def build_driver() -> str:
-    """The C++ driver test program (tests/cpp/cbm_driver.cpp), written against
-    the reference's CBMSimCore / CBMState API and linked to libcbm_core."""
+def build_driver() -> list[str]:
+    """The C++ driver test programs (tests/cpp/*.cpp: the eyelid-style driver
+    and the cart-pole driver), written against the reference's CBMSimCore /
+    CBMState / environment API and linked to libcbm_core."""
     out_dir = os.path.join(ROOT, "tests", "cpp", "build")
     os.makedirs(out_dir, exist_ok=True)
-    out = os.path.join(out_dir, "cbm_driver")
-    _run([CXX, *CXX_FLAGS, f"-I{os.path.join(CSRC, 'core', 'include')}", "-o", out,
-          os.path.join(ROOT, "tests", "cpp", "cbm_driver.cpp"), f"-L{LIB}", "-lcbm_core", "-lcbm_state",
-          "-lcbm_hip", "-Wl,-rpath,$ORIGIN/../../../cerebellum_simulation_amd/lib"])
-    return out
+    outs = []
+    for name in ("cbm_driver", "cartpole_driver"):
+        out = os.path.join(out_dir, name)
+        _run([CXX, *CXX_FLAGS, f"-I{os.path.join(CSRC, 'core', 'include')}", "-o", out,
+              os.path.join(ROOT, "tests", "cpp", name + ".cpp"), f"-L{LIB}", "-lcbm_core", "-lcbm_state",
+              "-lcbm_hip", "-Wl,-rpath,$ORIGIN/../../../cerebellum_simulation_amd/lib"])
+        outs.append(out)
+    return outs
 
 
 def build_oracle() -> None:

@@ -173,6 +173,12 @@ CONFIGS = {
     "c2": (dict(grXP2=12, grYP2=10, goXP2=7, goYP2=5, numMFP2=12, numSCP2=9, numPCP2=5, numNCP2=3,
                 numIOP2=2, numpPCfromPCtoNC=3, grPFVelInGRXPerTStep=294),
            dict(n_mf_per_gr=4, n_go_per_gr=3, n_go_out_per_gr=4, n_mf_per_go=16, grgo_window=(16, 4))),
+    # the cart-pole network: the sizes of the reference's CBM_Params/conParams.txt
+    # (1,048,576 GR; 1,024 GO; 2,048 MF; 32 PC; 128 BC; 512 SC; 4 IO; 8 NC) with
+    # two microzones
+    "cp1m": (dict(grXP2=11, grYP2=9, goXP2=6, goYP2=4, numMFP2=11, numSCP2=9, numPCP2=5, numNCP2=3,
+                  numIOP2=2, numpPCfromPCtoNC=3, grPFVelInGRXPerTStep=147),
+             dict(n_mf_per_gr=4, n_go_per_gr=3, n_go_out_per_gr=2, n_mf_per_go=16, num_zones=2)),
     # test networks
     "t4k": (dict(grXP2=6, grYP2=6, goXP2=3, goYP2=3, numMFP2=6, numSCP2=4, numPCP2=2, numNCP2=1,
                  numIOP2=0, numpPCfromPCtoNC=1, grPFVelInGRXPerTStep=10),

@@ -232,12 +232,31 @@ CBMSimCore::CBMSimCore(CBMState *state, int *mzoneRSeed, int gpuIndStart, int nu
     construct(state, mzoneRSeed, gpuIndStart, numGPUP2);
 }
 
+CBMSimCore::CBMSimCore(cbm_handle *handle, const cbm_sizes &sizes)
+    : simState(nullptr), h(handle), ownsHandle(false), numZones(sizes.numZones)
+{
+    if (!handle) throw std::runtime_error("CBMSimCore: null handle");
+    makeViews(sizes);
+}
+
+void CBMSimCore::makeViews(const cbm_sizes &s)
+{
+    inputNet = new CBMInNetView(h, s);
+    zones = new CBMZoneView *[numZones];
+    zoneList = new MZoneInterface *[numZones];
+    for (unsigned int z = 0; z < numZones; z++) {
+        zones[z] = new CBMZoneView(h, s, (int)z);
+        zoneList[z] = zones[z];
+    }
+}
+
 void CBMSimCore::construct(CBMState *state, const int *mzoneRSeed, int gpuIndStart, int numGPUP2)
 {
     if (numGPUP2 > 0) throw std::runtime_error("CBMSimCore: one GPU per network (numGPUP2 must be -1 or 0)");
     simState = state;
     numZones = state->getNumZones();
     h = nullptr;
+    ownsHandle = true;
     cbm_options o;
     std::memset(&o, 0, sizeof(o));
     o.grgoLag = 2;
@@ -250,13 +269,7 @@ void CBMSimCore::construct(CBMState *state, const int *mzoneRSeed, int gpuIndSta
         *seed = mzoneRSeed[z];
     }
     stateCheck(cbm_state_load_into(state->handle(), h), "cbm_state_load_into");
-    inputNet = new CBMInNetView(h, state->sizes());
-    zones = new CBMZoneView *[numZones];
-    zoneList = new MZoneInterface *[numZones];
-    for (unsigned int z = 0; z < numZones; z++) {
-        zones[z] = new CBMZoneView(h, state->sizes(), (int)z);
-        zoneList[z] = zones[z];
-    }
+    makeViews(state->sizes());
 }
 
 CBMSimCore::~CBMSimCore()
@@ -265,7 +278,7 @@ CBMSimCore::~CBMSimCore()
     for (unsigned int z = 0; z < numZones; z++) delete zones[z];
     delete[] zones;
     delete[] zoneList;
-    if (h) cbm_destroy(h);
+    if (h && ownsHandle) cbm_destroy(h);
 }
 
 void CBMSimCore::calcActivity() { hipCheck(cbm_step(h, 1), "calcActivity"); }
@@ -279,6 +292,7 @@ void CBMSimCore::updateErrDrive(unsigned int zoneN, float errDriveRelative)
 
 void CBMSimCore::writeToState()
 {
+    if (!simState) throw std::runtime_error("CBMSimCore::writeToState: this core has no CBMState");
     const cbm_sizes &s = simState->sizes();
     for (int f = 0; f < CBM_F_COUNT; f++) {
         size_t e, r, c;

@@ -15,7 +15,8 @@
 #include "innetinterface.h"
 #include "mzoneinterface.h"
 
-struct cbm_handle;
+#include <cbm_hip.h>
+
 class CBMInNetView;
 class CBMZoneView;
 
@@ -24,6 +25,10 @@ class CBMSimCore
 public:
     CBMSimCore(CBMState *state, int gpuIndStart = -1, int numGPUP2 = -1);
     CBMSimCore(CBMState *state, int *mzoneRSeed, int gpuIndStart = -1, int numGPUP2 = -1);
+    // Not in the reference: a core over a simulation handle made elsewhere
+    // (e.g. by a Python host); it does not own the handle, and it has no
+    // CBMState, so writeToState throws.
+    CBMSimCore(cbm_handle *handle, const cbm_sizes &sizes);
 
     ~CBMSimCore();
 
@@ -49,8 +54,11 @@ private:
     CBMSimCore();
     void construct(CBMState *state, const int *mzoneRSeed, int gpuIndStart, int numGPUP2);
 
+    void makeViews(const cbm_sizes &s);
+
     CBMState *simState;
     cbm_handle *h;
+    bool ownsHandle;
     unsigned int numZones;
     CBMInNetView *inputNet;
     CBMZoneView **zones;

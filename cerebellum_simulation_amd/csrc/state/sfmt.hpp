@@ -64,17 +64,19 @@ public:
         return (double)(int64_t)(v >> 12) * (1.0 / 4503599627370496.0);
     }
 
+    // The interval length hi - lo + 1 wraps modulo 2^32 (IRandom(0, INT_MAX)
+    // draws from 2^31 values); it is formed in unsigned arithmetic.
     int IRandom(int lo, int hi)
     {
         if (hi <= lo) return hi == lo ? lo : (int)0x80000000u;
-        uint32_t n = (uint32_t)(hi - lo + 1);
+        uint32_t n = (uint32_t)hi - (uint32_t)lo + 1u;
         return (int32_t)(uint32_t)(((uint64_t)BRandom() * n) >> 32) + lo;
     }
 
     int IRandomX(int lo, int hi)
     {
         if (hi <= lo) return hi == lo ? lo : (int)0x80000000u;
-        uint32_t n = (uint32_t)(hi - lo + 1);
+        uint32_t n = (uint32_t)hi - (uint32_t)lo + 1u;
         if (n != lastInterval_) {
             rlimit_ = (uint32_t)((((uint64_t)1) << 32) / n) * n - 1;
             lastInterval_ = n;

@@ -99,7 +99,7 @@ double orc_sfmt_random(orc_sfmt *g)
 int orc_sfmt_irandom(orc_sfmt *g, int min, int max)
 {
     if (max <= min) return max == min ? min : (int)0x80000000u;
-    uint32_t interval = (uint32_t)(max - min + 1);
+    uint32_t interval = (uint32_t)max - (uint32_t)min + 1u; /* wraps modulo 2^32 */
     uint64_t lr = (uint64_t)orc_sfmt_brandom(g) * interval;
     return (int32_t)(uint32_t)(lr >> 32) + min;
 }
@@ -107,7 +107,7 @@ int orc_sfmt_irandom(orc_sfmt *g, int min, int max)
 int orc_sfmt_irandomx(orc_sfmt *g, int min, int max)
 {
     if (max <= min) return max == min ? min : (int)0x80000000u;
-    uint32_t interval = (uint32_t)(max - min + 1);
+    uint32_t interval = (uint32_t)max - (uint32_t)min + 1u; /* wraps modulo 2^32 */
     if (interval != g->last_interval) {
         g->rlimit = (uint32_t)(((uint64_t)1 << 32) / interval) * interval - 1;
         g->last_interval = interval;
@@ -118,3 +118,10 @@ int orc_sfmt_irandomx(orc_sfmt *g, int min, int max)
     } while ((uint32_t)lr > g->rlimit);
     return (int32_t)(uint32_t)(lr >> 32) + min;
 }
+
+void orc_sfmt_random_fill(orc_sfmt *g, double *out, int n)
+{
+    for (int i = 0; i < n; i++) out[i] = orc_sfmt_random(g);
+}
+
+int orc_sfmt_size(void) { return (int)sizeof(orc_sfmt); }

@@ -37,5 +37,9 @@ uint32_t orc_sfmt_brandom(orc_sfmt *g);
 double orc_sfmt_random(orc_sfmt *g);
 int orc_sfmt_irandom(orc_sfmt *g, int min, int max);
 int orc_sfmt_irandomx(orc_sfmt *g, int min, int max);
+/* n consecutive Random() outputs */
+void orc_sfmt_random_fill(orc_sfmt *g, double *out, int n);
+/* sizeof(orc_sfmt), for hosts that allocate it */
+int orc_sfmt_size(void);
 
 #endif

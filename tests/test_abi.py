@@ -20,7 +20,8 @@ def _declared(header):
 
 @pytest.mark.parametrize("header,libname", [("include/cbm_hip.h", "libcbm_hip.so"),
                                             ("include/cbm_surrogate.h", "libcbm_hip.so"),
-                                            ("include/cbm_state.h", "libcbm_state.so")])
+                                            ("include/cbm_state.h", "libcbm_state.so"),
+                                            ("include/cbm_env.h", "libcbm_core.so")])
 def test_exports(header, libname):
     names = [n for n in _declared(os.path.join(ROOT, header)) if n.startswith("cbm_")]
     assert len(names) > 5
@@ -80,3 +81,18 @@ def test_surrogate_struct_layouts(tmp_path):
     got = list(map(int, subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()))
     assert got == [C.sizeof(_abi.SgDesc), C.sizeof(_abi.SgBatch), C.sizeof(_abi.SgAdam),
                    _abi.SgDesc.device.offset, _abi.SgBatch.targetPitch.offset]
+
+
+def test_cartpole_struct_layout(tmp_path):
+    src = tmp_path / "cp.c"
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "cbm_env.h"\n'
+                   'int main(){printf("%zu %zu %zu %zu %d\\n", sizeof(cbm_cartpole_opts),'
+                   ' offsetof(cbm_cartpole_opts, maxTrialLen), offsetof(cbm_cartpole_opts, logfile),'
+                   ' offsetof(cbm_cartpole_opts, simStateDir), (int)CBM_CP_COLS);return 0;}\n')
+    exe = tmp_path / "cp"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)], check=True)
+    got = list(map(int, subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()))
+    from cerebellum_simulation_amd import cartpole
+    assert got == [C.sizeof(_abi.CartpoleOpts), _abi.CartpoleOpts.maxTrialLen.offset,
+                   _abi.CartpoleOpts.logfile.offset, _abi.CartpoleOpts.simStateDir.offset, len(cartpole.COLS)]
+
