@@ -135,6 +135,8 @@ def surrogate_bench(which: str) -> dict:
     judged by the exact step (surrogate.sweep_experiment)."""
     from cerebellum_simulation_amd import surrogate as SG
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+    if which == "kernels":
+        return surrogate_kernels()
     if which == "full":
         res = SG.sweep_experiment("c1", train_trials=1000, train_steps=600, sweep_trials=10, log=log)
         workload = ("1M-GR network (config 1 sizes); 2,000,000 training windows of 50 ms x 1,024 MFs -> "
@@ -155,6 +157,45 @@ def surrogate_bench(which: str) -> dict:
               "speedup_vs_exact", "pf_rmse", "pf_rmse_cluster_mean", "forward_tflops"):
         out[k] = round(out[k], 4)
     return out
+
+
+def surrogate_kernels() -> dict:
+    """The surrogate's kernels alone at the full shape, on synthetic windows
+    (Bernoulli MF rows, random soft targets): 20 training steps of 64 x 256
+    windows, one forward pass over 64 x 256 windows, and a sweep of 16
+    members of the 1M-GR network over 2,000 steps.  For kernel profiles."""
+    import torch
+    from cerebellum_simulation_amd import Batch, Network, configs
+    from cerebellum_simulation_amd import surrogate as SG
+    gen = torch.Generator(device="cuda").manual_seed(0)
+    rows = 40000
+    mf = (torch.rand((rows, 1024), device="cuda", generator=gen) < 0.05).to(torch.uint8)
+    y = (torch.randint(0, 20, (rows, 4096), device="cuda", generator=gen).float() / 256).to(torch.bfloat16)
+    sg = SG.Surrogate(num_in=1024, hidden=128, num_out=4096, max_rows=20000)
+    sg.init_params(0, -4.0)
+    data = SG.InputLayerData(mf, y.view(torch.int16), 256)
+    SG.train(sg, data, 20, segments=64, outputs=256)
+    tr_us, tr_fl = sg.last_timing()
+    sg.predict(mf, 64, 16384, outputs=256)
+    fw_us, fw_fl = sg.last_timing()
+    st = configs.build_state("c1", seed=0)
+    nets = [Network.from_state(st) for _ in range(16)]
+    b = Batch(nets)
+    prob = torch.full((199 + 2000, 4096), 0.004, device="cuda")
+    train = torch.from_numpy(configs.mf_train(st.sizes.numMF, 2000, seed=1)).cuda()
+    t0 = time.perf_counter()
+    b.sweep(prob, 199, 2000, train)
+    b.sync()
+    sw = time.perf_counter() - t0
+    b.close()
+    for n in nets:
+        n.close()
+    sg.close()
+    return {"workload": "surrogate kernels, synthetic windows: training 64 x 256 windows (1,024 MF -> 128 -> 128 "
+                        "-> 4,096), forward 64 x 256 windows, sweep of 16 x 1M-GR members x 2,000 steps",
+            "train_step_us": round(tr_us, 1), "train_tflops": round(tr_fl / tr_us / 1e6, 1),
+            "forward_us": round(fw_us, 1), "forward_tflops": round(fw_fl / fw_us / 1e6, 1),
+            "sweep_step_us": round(sw / 2000 * 1e6, 2)}
 
 
 def cartpole_bench(which: str) -> dict:
@@ -205,7 +246,7 @@ def main():
     ap.add_argument("--batch", default="full", choices=["full", "small", "none"],
                     help="ragged batch to measure beside the single network (small: 1/8 of each class)")
     ap.add_argument("--batch-steps", type=int, default=200)
-    ap.add_argument("--surrogate", default="full", choices=["full", "small", "none"],
+    ap.add_argument("--surrogate", default="full", choices=["full", "small", "kernels", "none"],
                     help="config 4: the learned input layer and the plasticity sweep it drives")
     ap.add_argument("--cartpole", default="full", choices=["full", "small", "none"],
                     help="the cart-pole closed loop on two microzones (full: 1M-GR network, 2,048 MF)")

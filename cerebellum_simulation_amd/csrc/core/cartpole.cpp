@@ -291,7 +291,7 @@ void Cartpole::setMZErr()
         if (x < 0 && hit) forceLeftMZ.deliverError();
         if (x > 0 && hit) forceRightMZ.deliverError();
         // the cart's velocity: both cases go to the left microzone in the
-        // reference (cartpole.cpp:266-271), kept as it is
+        // reference (cartpole.cpp:244-250), kept as it is
         p = std::min(.005f * std::fabs(x_dot), maxErrProb);
         hit = randGen->Random() < p;
         if (x_dot > 0 && x < 0 && hit) forceLeftMZ.deliverError();

@@ -50,8 +50,10 @@ class NetState:
         g.nMFperGR, g.nGOperGR, g.nGOoutPerGR, g.nMFperGO = n_mf_per_gr, n_go_per_gr, n_go_out_per_gr, n_mf_per_go
         g.grgoWindowX, g.grgoWindowY = grgo_window
         for z in range(8):
-            g.mzoneConSeed[z] = mzone_con_seeds[z] if mzone_con_seeds else innet_seed + 101 + z
-            g.mzoneRngSeed[z] = mzone_rng_seeds[z] if mzone_rng_seeds else innet_seed + 1001 + z
+            g.mzoneConSeed[z] = mzone_con_seeds[z] if mzone_con_seeds and z < len(mzone_con_seeds) \
+                else innet_seed + 101 + z
+            g.mzoneRngSeed[z] = mzone_rng_seeds[z] if mzone_rng_seeds and z < len(mzone_rng_seeds) \
+                else innet_seed + 1001 + z
         h = C.c_void_p()
         rc = L.cbm_state_create(act_text.encode(), con_text.encode(), C.byref(g), C.byref(h))
         if rc:
