@@ -52,8 +52,8 @@ def build_hip() -> str:
 
 def build_state() -> str:
     out = os.path.join(LIB, "libcbm_state.so")
-    src = os.path.join(CSRC, "state", "cbm_state.cpp")
-    _run([CXX, *CXX_FLAGS, "-shared", "-o", out, src, f"-L{LIB}", "-lcbm_hip",
+    srcs = [os.path.join(CSRC, "state", f) for f in ("cbm_state.cpp", "innet_gen.cpp")]
+    _run([CXX, *CXX_FLAGS, "-shared", "-o", out, *srcs, f"-L{LIB}", "-lcbm_hip",
           "-Wl,-rpath,$ORIGIN"])
     return out
 

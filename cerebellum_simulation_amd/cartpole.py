@@ -60,16 +60,20 @@ def _con_key(text: str, key: str, default: int) -> int:
 
 def build_state(act_text: str, con_text: str, seed: int, num_zones: int = 2):
     """The network a driver run with `seed` builds: CBMState(actPF, conPF,
-    numMZ, seed, crSeeds, arSeeds) -- with the fan-ins the C++ CBMState takes
-    from a conParams file: 4 MF and 3 GO inputs per GR, min(2,
-    maxnumpGRfromGRtoGO) GO targets, maxnumpGOfromGLtoGO MF inputs per GO
-    (16 when absent) -- and CBMSimCore's zone generators seeded seed, seed + 1, ..."""
+    numMZ, seed, crSeeds, arSeeds) -- the reference's glomerular generator when
+    the conParams text has a glomerulus layer, else the synthetic one with the
+    fan-ins the C++ CBMState takes from the file: 4 MF and 3 GO inputs per GR,
+    min(2, maxnumpGRfromGRtoGO) GO targets, maxnumpGOfromGLtoGO MF inputs per
+    GO (16 when absent) -- and CBMSimCore's zone generators seeded seed,
+    seed + 1, ..."""
     from .network import NetState
     cr, _ar = simthread_seeds(seed, num_zones)
+    glomerular = _con_key(con_text, "glXP2", -1) >= 0
     return NetState(act_text, con_text, num_zones=num_zones, innet_seed=seed, mzone_con_seeds=cr,
                     mzone_rng_seeds=[seed + z for z in range(num_zones)], n_mf_per_gr=4, n_go_per_gr=3,
                     n_go_out_per_gr=min(2, _con_key(con_text, "maxnumpGRfromGRtoGO", 2)),
-                    n_mf_per_go=_con_key(con_text, "maxnumpGOfromGLtoGO", 16))
+                    n_mf_per_go=_con_key(con_text, "maxnumpGOfromGLtoGO", 16),
+                    generator="reference" if glomerular else "uniform")
 
 
 class Cartpole:

@@ -158,6 +158,49 @@ def conparams_text(*, grXP2, grYP2, goXP2, goYP2, numMFP2, numSCP2, numPCP2, num
     return "\n".join(lines) + "\n"
 
 
+# The glomerulus-layer parameters of the reference's CBM_Params/conParams.txt
+# (the file drivers load by default), for its glomerular generator
+# (NetState(..., generator="reference")).
+GLOMERULAR_1M = dict(
+    glXP2=9, glYP2=7, maxnumpGLfromGLtoGR=80, lownumpGLfromGLtoGR=64, maxnumpGLfromGLtoGO=1,
+    maxnumpGLfromGOtoGL=1, numMFP2=11, goXP2=6, goYP2=4, maxnumpGOfromGRtoGO=4096, maxnumpGOfromGLtoGO=16,
+    maxnumpGOfromGOtoGL=48, spanGODecDenOnGLX=12, spanGODecDenOnGLY=12, spanGOAscDenOnGRX=2048,
+    spanGOAscDenOnGRY=48, spanGOAxonOnGLX=12, spanGOAxonOnGLY=12, grXP2=11, grYP2=9, grPFVelInGRXPerTStep=147,
+    grAFDelayInTStep=1, maxnumpGRfromGRtoGO=4, maxnumpGRfromGLtoGR=4, maxnumpGRfromGOtoGR=4, maxnumpGRfromMFtoGR=4,
+    spanGRDenOnGLX=4, spanGRDenOnGLY=4, numSCP2=9, numPCP2=5, numpPCfromPCtoNC=3, numNCP2=3, numIOP2=2)
+# the GO-GO matrices of that file: GABA to the 8 neighbours with probability
+# 0.7 (and a self entry with probability 0), coupling entries all 0
+GOGO_GABA_1M = [(0, 1, 0.7), (0, -1, 0.7), (1, 0, 0.7), (-1, 0, 0.7), (1, 1, 0.7), (-1, 1, 0.7), (-1, -1, 0.7),
+                (1, -1, 0.7), (0, 0, 0)]
+GOGO_COUP_1M = [(0, 1, 0), (0, -1, 0), (1, 0, 0), (-1, 0, 0), (1, 1, 0), (-1, 1, 0), (1, -1, 0), (-1, -1, 0)]
+
+GLOMERULAR = {
+    "g1m": GLOMERULAR_1M,
+    # the same grid ratios at a quarter of the GRs: 262,144 GR, 16,384 glomeruli, 1,024 MF, 256 GO
+    "g262k": {**GLOMERULAR_1M, "glXP2": 8, "glYP2": 6, "numMFP2": 10, "goXP2": 5, "goYP2": 3, "grXP2": 10,
+              "grYP2": 8, "spanGOAscDenOnGRX": 1024, "grPFVelInGRXPerTStep": 100},
+    # a test network: 2,048 GR on 128 glomeruli, 16 MF, 32 GO; 4 PC, 2 NC, 1 IO
+    "g2k": {**GLOMERULAR_1M, "glXP2": 4, "glYP2": 3, "numMFP2": 4, "goXP2": 3, "goYP2": 2, "grXP2": 6,
+            "grYP2": 5, "maxnumpGOfromGRtoGO": 224, "maxnumpGOfromGLtoGO": 2, "maxnumpGOfromGOtoGL": 3,
+            "spanGODecDenOnGLX": 4, "spanGODecDenOnGLY": 4, "spanGOAxonOnGLX": 4, "spanGOAxonOnGLY": 4,
+            "spanGOAscDenOnGRX": 64, "spanGOAscDenOnGRY": 8, "grPFVelInGRXPerTStep": 10, "numSCP2": 4,
+            "numPCP2": 2, "numpPCfromPCtoNC": 1, "numNCP2": 1, "numIOP2": 0},
+}
+
+
+def glomerular_conparams_text(name: str = "g1m", **over) -> str:
+    """conParams text with the glomerulus layer, in the layout of the
+    reference's CBM_Params/conParams.txt."""
+    kv = {**GLOMERULAR[name], **over}
+    gogo = ["GOGOLocalMatrix", f"maxnumpGOInfromGOtoGO {len(GOGO_GABA_1M)}",
+            f"maxnumpGOOutfromGOtoGO {len(GOGO_GABA_1M)}", *[f"{a} {b} {c}" for a, b, c in GOGO_GABA_1M],
+            "GOGOCoupLocalMatrix", f"maxnumpGOCoupInfromGOtoGO {len(GOGO_COUP_1M)}",
+            f"maxnumpGOCoupOutfromGOtoGO {len(GOGO_COUP_1M)}", *[f"{a} {b} {c}" for a, b, c in GOGO_COUP_1M]]
+    lines = [f"{k} {v}" for k, v in kv.items()]
+    at = lines.index(f"spanGOAxonOnGLY {kv['spanGOAxonOnGLY']}") + 1
+    return "\n".join(lines[:at] + gogo + lines[at:] + ["connectivityParamEnd 1"]) + "\n"
+
+
 # name -> (conParams keys, generator keys)
 CONFIGS = {
     # 16,384 GR; 256 GO; 256 MF; 8 PC; 32 BC; 128 SC; 1 IO; 2 NC

@@ -84,7 +84,9 @@ void CBMState::build(const std::string &act, const std::string &con, unsigned in
     if (nZones < 1 || nZones > 8) throw std::runtime_error("CBMState: numZones must be 1..8");
     cbm_gen_spec g;
     std::memset(&g, 0, sizeof(g));
-    g.kind = CBM_GEN_UNIFORM;
+    // a parameter file with a glomerulus layer (CBM_Params/conParams.txt) gets
+    // the reference's own generator; one without it the synthetic one
+    g.kind = conKey(con, "glXP2", UINT32_MAX) != UINT32_MAX ? CBM_GEN_REFERENCE : CBM_GEN_UNIFORM;
     g.numZones = nZones;
     g.innetSeed = innetCRSeed;
     // fan-ins of the benchmark networks; the GR->GO out-degree is capped by the

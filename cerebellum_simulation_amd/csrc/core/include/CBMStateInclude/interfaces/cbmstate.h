@@ -15,6 +15,9 @@ class CBMState
 {
 public:
     // Parameter files and zone count; seeds drawn from time(0) as in cbmstate.cpp:35-58.
+    // A conParams file with a glomerulus layer (glXP2, ... as CBM_Params/conParams.txt
+    // has) is wired by the reference's generator (CBM_GEN_REFERENCE); one without
+    // it by the synthetic generator with 4 MF and 3 GO inputs per GR.
     CBMState(std::fstream &actPFile, std::fstream &conPFile, unsigned int nZones);
     // Explicit seeds (cbmstate.cpp:60-64).  mzoneARSeed seeds each zone's
     // generator for the NC->IO release draws.

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "../../../include/cbm_state.h"
+#include "innet_gen.hpp"
 #include "sfmt.hpp"
 
 namespace {
@@ -241,6 +242,8 @@ struct cbm_state {
     // checkpoint sections the step does not use, kept byte for byte
     std::map<std::string, std::vector<uint8_t>> blob[kMaxZones];
     bool fromCheckpoint = false;  // its header is written back unchanged
+    bool refGen = false;          // built by the glomerular generator: the header is the file's
+    uint64_t droppedConnections = 0;
 
     template <typename T>
     T *F(int field, int z = 0) { return reinterpret_cast<T *>(f[field][z].data()); }
@@ -317,7 +320,8 @@ int cbm_state_create(const char *actText, const char *conText, const cbm_gen_spe
                      cbm_state **out)
 {
     if (!actText || !conText || !gen || !out) return fail(CBM_E_ARG, "null argument");
-    if (gen->kind != CBM_GEN_UNIFORM) return fail(CBM_E_UNSUPPORTED, "unknown generator kind");
+    if (gen->kind != CBM_GEN_UNIFORM && gen->kind != CBM_GEN_REFERENCE)
+        return fail(CBM_E_UNSUPPORTED, "unknown generator kind");
     if (gen->numZones < 1 || gen->numZones > (uint32_t)kMaxZones)
         return fail(CBM_E_ARG, "numZones must be 1..8");
     ActMap am;
@@ -358,9 +362,11 @@ int cbm_state_create(const char *actText, const char *conText, const cbm_gen_spe
     s.numpIOfromNCtoIO = s.numNC;
     s.numpIOInIOIO = s.numIO - 1;
     s.numPopHistBinsPC = st->p.numPopHistBinsPC;
-    s.maxnumpGRfromMFtoGR = gen->nMFperGR;
-    s.maxnumpGRfromGOtoGR = gen->nGOperGR;
-    s.maxnumpGRfromGRtoGO = gen->nGOoutPerGR;
+    const bool refGen = gen->kind == CBM_GEN_REFERENCE;
+    st->refGen = refGen;
+    s.maxnumpGRfromMFtoGR = refGen ? cp.get("maxnumpGRfromMFtoGR") : gen->nMFperGR;
+    s.maxnumpGRfromGOtoGR = refGen ? cp.get("maxnumpGRfromGOtoGR") : gen->nGOperGR;
+    s.maxnumpGRfromGRtoGO = refGen ? cp.get("maxnumpGRfromGRtoGO") : gen->nGOoutPerGR;
     s.maxnumpGOGABAOutGOGO = cp.maxGABAOut;
     s.maxnumpGOCoupInGOGO = cp.maxCoupIn;
     st->grPFVel = cp.get("grPFVelInGRXPerTStep");
@@ -370,16 +376,96 @@ int cbm_state_create(const char *actText, const char *conText, const cbm_gen_spe
         delete st;
         return fail(CBM_E_ARG, "grPFVelInGRXPerTStep and msPerTimeStep must be positive");
     }
-    if (gen->nMFperGR > s.numMF || gen->nGOperGR > s.numGO || gen->nGOoutPerGR > s.numGO ||
-        gen->nMFperGO > s.numMF) {
+    if (!refGen && (gen->nMFperGR > s.numMF || gen->nGOperGR > s.numGO || gen->nGOoutPerGR > s.numGO ||
+                    gen->nMFperGO > s.numMF)) {
         delete st;
         return fail(CBM_E_ARG, "fan-in larger than the source population");
     }
     uint32_t pcnc = cp.get("numpPCfromPCtoNC");
 
-    // --- input net: GR draws, then GO dendrites on MFs, then GO-GO wiring
     cbm::Sfmt rng(gen->innetSeed);
     const uint32_t nGR = s.numGR;
+    if (refGen) {
+        // --- input net by the reference's glomerular generator, then GO-GO wiring
+        static const char *keys[] = {"glXP2", "glYP2", "maxnumpGLfromGLtoGR", "lownumpGLfromGLtoGR",
+                                     "maxnumpGLfromGLtoGO", "maxnumpGLfromGOtoGL", "maxnumpGOfromGRtoGO",
+                                     "maxnumpGOfromGLtoGO", "maxnumpGOfromGOtoGL", "maxnumpGRfromGLtoGR",
+                                     "maxnumpGRfromGOtoGR", "maxnumpGRfromMFtoGR", "maxnumpGRfromGRtoGO"};
+        for (const char *k : keys)
+            if (!cp.m.count(k)) {
+                delete st;
+                return fail(CBM_E_ARG, std::string("the glomerular generator needs conParams key ") + k);
+            }
+        cbm::InnetGenParams q{};
+        q.glX = 1u << cp.get("glXP2");
+        q.glY = 1u << cp.get("glYP2");
+        q.grX = s.grX;
+        q.grY = s.grY;
+        q.goX = s.goX;
+        q.goY = s.goY;
+        q.numMF = s.numMF;
+        q.maxGLtoGR = cp.get("maxnumpGLfromGLtoGR");
+        q.lowGLtoGR = cp.get("lownumpGLfromGLtoGR");
+        q.maxGLtoGO = cp.get("maxnumpGLfromGLtoGO");
+        q.maxGLfromGO = cp.get("maxnumpGLfromGOtoGL");
+        q.maxGRfromGL = cp.get("maxnumpGRfromGLtoGR");
+        q.maxGRfromGO = s.maxnumpGRfromGOtoGR;
+        q.maxGRfromMF = s.maxnumpGRfromMFtoGR;
+        q.maxGRtoGO = s.maxnumpGRfromGRtoGO;
+        q.maxGOfromGR = cp.get("maxnumpGOfromGRtoGO");
+        q.maxGOfromGL = cp.get("maxnumpGOfromGLtoGO");
+        q.maxGOtoGL = cp.get("maxnumpGOfromGOtoGL");
+        q.spanGRDenX = cp.get("spanGRDenOnGLX");
+        q.spanGRDenY = cp.get("spanGRDenOnGLY");
+        q.spanGODecX = cp.get("spanGODecDenOnGLX");
+        q.spanGODecY = cp.get("spanGODecDenOnGLY");
+        q.spanGOAxonX = cp.get("spanGOAxonOnGLX");
+        q.spanGOAxonY = cp.get("spanGOAxonOnGLY");
+        q.spanGOAscX = cp.get("spanGOAscDenOnGRX");
+        q.spanGOAscY = cp.get("spanGOAscDenOnGRY");
+        cbm::InnetGen rg;
+        const std::string why = cbm::generateInnet(q, rng, rg);
+        if (!why.empty()) {
+            delete st;
+            return fail(CBM_E_UNSUPPORTED, "glomerular generator: " + why);
+        }
+        s.maxnumpMFfromMFtoGO = rg.mfToGO.cols;
+        allocFields(st);
+        auto put = [&](int field, const cbm::ConList &l, int numField) {
+            uint32_t *dst = st->F<uint32_t>(field);
+            const size_t w = st->f[field][0].size() / 4 / l.rows;
+            for (uint32_t i = 0; i < l.rows; i++)
+                for (size_t j = 0; j < w; j++) dst[(size_t)i * w + j] = j < l.cols ? l.p[(size_t)i * l.cols + j] : UINT32_MAX;
+            std::memcpy(st->f[numField][0].data(), l.num.data(), (size_t)l.rows * 4);
+        };
+        put(CBM_F_P_GR_FROM_MF_TO_GR, rg.grFromMF, CBM_F_NUMP_GR_FROM_MF_TO_GR);
+        put(CBM_F_P_GR_FROM_GO_TO_GR, rg.grFromGO, CBM_F_NUMP_GR_FROM_GO_TO_GR);
+        put(CBM_F_P_GR_FROM_GR_TO_GO, rg.grToGO, CBM_F_NUMP_GR_FROM_GR_TO_GO);
+        put(CBM_F_P_MF_FROM_MF_TO_GO, rg.mfToGO, CBM_F_NUMP_MF_FROM_MF_TO_GO);
+        // the lists the step does not use, in the checkpoint's layout
+        auto keep = [&](const char *name, const void *p, size_t bytes) {
+            st->blob[0][name].assign(static_cast<const uint8_t *>(p), static_cast<const uint8_t *>(p) + bytes);
+        };
+        auto keepList = [&](const char *numName, const char *listName, const cbm::ConList &l) {
+            keep(numName, l.num.data(), l.num.size() * 4);
+            keep(listName, l.p.data(), l.p.size() * 4);
+        };
+        keep("haspGLfromMFtoGL", rg.hasGLfromMF.data(), rg.hasGLfromMF.size());
+        keep("pGLfromMFtoGL", rg.glFromMF.data(), rg.glFromMF.size() * 4);
+        keepList("numpGLfromGLtoGO", "pGLfromGLtoGO", rg.glToGO);
+        keepList("numpGLfromGOtoGL", "pGLfromGOtoGL", rg.glFromGO);
+        keepList("numpGLfromGLtoGR", "pGLfromGLtoGR", rg.glToGR);
+        keepList("numpMFfromMFtoGL", "pMFfromMFtoGL", rg.mfToGL);
+        keepList("numpMFfromMFtoGR", "pMFfromMFtoGR", rg.mfToGR);
+        keepList("numpGOfromGLtoGO", "pGOfromGLtoGO", rg.goFromGL);
+        keepList("numpGOfromGOtoGL", "pGOfromGOtoGL", rg.goToGL);
+        keepList("numpGOfromMFtoGO", "pGOfromMFtoGO", rg.goFromMF);
+        keepList("numpGOfromGOtoGR", "pGOfromGOtoGR", rg.goToGR);
+        keepList("numpGOfromGRtoGO", "pGOfromGRtoGO", rg.goFromGR);
+        keepList("numpGRfromGLtoGR", "pGRfromGLtoGR", rg.grFromGL);
+        st->droppedConnections = rg.droppedConnections;
+    } else {
+    // --- input net: GR draws, then GO dendrites on MFs, then GO-GO wiring
     std::vector<uint32_t> pMF((size_t)nGR * gen->nMFperGR), pGOin((size_t)nGR * gen->nGOperGR),
         pOut((size_t)nGR * gen->nGOoutPerGR);
     for (uint32_t i = 0; i < nGR; i++) {
@@ -431,27 +517,34 @@ int cbm_state_create(const char *actText, const char *conText, const cbm_gen_spe
             ng[i] = (int32_t)gen->nGOperGR;
             no[i] = (int32_t)gen->nGOoutPerGR;
         }
-        // delays: InNetConnectivityState::assignGRDelays (innetconnectivitystate.cpp:920-962)
-        uint32_t *dB = st->F<uint32_t>(CBM_F_P_GR_DELAYMASK_GR_TO_BSP);
-        uint32_t *dG = st->F<uint32_t>(CBM_F_P_GR_DELAYMASK_GR_TO_GO);
-        const uint32_t kOut = gen->nGOoutPerGR;
-        for (uint32_t i = 0; i < nGR; i++) {
-            int grPosX = (int)(i % s.grX);
-            int distB = std::abs((int)(s.grX / 2 - grPosX));
-            dB[i] = delayMask(st, distB);
-            for (uint32_t j = 0; j < kOut; j++) {
-                int goPosX = (int)((pOut[(size_t)i * kOut + j] % s.goX) * (((float)s.grX) / s.goX));
-                int d = std::abs(goPosX - grPosX);
-                if (d > (int)(s.grX / 2)) d = goPosX < grPosX ? goPosX + (int)s.grX - grPosX : grPosX + (int)s.grX - goPosX;
-                dG[(size_t)i * kOut + j] = delayMask(st, d);
-            }
-        }
         int32_t *nmg = st->F<int32_t>(CBM_F_NUMP_MF_FROM_MF_TO_GO);
         uint32_t *pmg = st->F<uint32_t>(CBM_F_P_MF_FROM_MF_TO_GO);
         fillU32(st->f[CBM_F_P_MF_FROM_MF_TO_GO][0], UINT32_MAX);
         for (uint32_t m = 0; m < s.numMF; m++) {
             nmg[m] = (int32_t)mfOut[m].size();
             for (size_t j = 0; j < mfOut[m].size(); j++) pmg[(size_t)m * maxMFGO + j] = mfOut[m][j];
+        }
+    }
+    }  // synthetic generator
+    {
+        // delays: InNetConnectivityState::assignGRDelays (innetconnectivitystate.cpp:920-962);
+        // slots past a GR's GO-target count keep UINT_MAX, as initializeVals leaves them
+        uint32_t *dB = st->F<uint32_t>(CBM_F_P_GR_DELAYMASK_GR_TO_BSP);
+        uint32_t *dG = st->F<uint32_t>(CBM_F_P_GR_DELAYMASK_GR_TO_GO);
+        const int32_t *no = st->F<int32_t>(CBM_F_NUMP_GR_FROM_GR_TO_GO);
+        const uint32_t *pOutF = st->F<uint32_t>(CBM_F_P_GR_FROM_GR_TO_GO);
+        const uint32_t kOut = s.maxnumpGRfromGRtoGO;
+        fillU32(st->f[CBM_F_P_GR_DELAYMASK_GR_TO_GO][0], UINT32_MAX);
+        for (uint32_t i = 0; i < nGR; i++) {
+            int grPosX = (int)(i % s.grX);
+            int distB = std::abs((int)(s.grX / 2 - grPosX));
+            dB[i] = delayMask(st, distB);
+            for (uint32_t j = 0; j < (uint32_t)no[i] && j < kOut; j++) {
+                int goPosX = (int)((pOutF[(size_t)i * kOut + j] % s.goX) * (((float)s.grX) / s.goX));
+                int d = std::abs(goPosX - grPosX);
+                if (d > (int)(s.grX / 2)) d = goPosX < grPosX ? goPosX + (int)s.grX - grPosX : grPosX + (int)s.grX - goPosX;
+                dG[(size_t)i * kOut + j] = delayMask(st, d);
+            }
         }
         // GO->GO: connectGOGO (innetconnectivitystate.cpp:852-918)
         int32_t *nGab = st->F<int32_t>(CBM_F_NUMP_GO_GABA_OUT_GOGO);
@@ -1006,7 +1099,7 @@ std::vector<uint8_t> synth(cbm_state *st, const Sec &sc, const RefSizes &r, int 
 // the reference's generator does not exist here: one glomerulus per MF).
 ConParams headerOf(cbm_state *st)
 {
-    if (st->fromCheckpoint) return st->con;
+    if (st->fromCheckpoint || st->refGen) return st->con;
     ConParams c = st->con;
     RefSizes r = deriveRef(c);
     const cbm_sizes &s = st->s;

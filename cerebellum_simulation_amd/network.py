@@ -41,10 +41,15 @@ class NetState:
     def __init__(self, act_text: str, con_text: str, *, num_zones: int = 1, innet_seed: int = 0,
                  n_mf_per_gr: int = 4, n_go_per_gr: int = 3, n_go_out_per_gr: int = 2,
                  n_mf_per_go: int = 16, grgo_window: tuple[int, int] = (0, 0),
-                 mzone_con_seeds=None, mzone_rng_seeds=None):
+                 mzone_con_seeds=None, mzone_rng_seeds=None, generator: str = "uniform"):
+        """generator "uniform": the synthetic input net of the n_* fan-ins;
+        "reference": the reference's glomerular generator (CBM_GEN_REFERENCE),
+        sized by the conParams text's glomerulus keys."""
         L = _abi.state_lib()
         g = GenSpec()
-        g.kind = 0
+        if generator not in ("uniform", "reference"):
+            raise ValueError("generator must be 'uniform' or 'reference'")
+        g.kind = 0 if generator == "uniform" else 1
         g.numZones = num_zones
         g.innetSeed = innet_seed
         g.nMFperGR, g.nGOperGR, g.nGOoutPerGR, g.nMFperGO = n_mf_per_gr, n_go_per_gr, n_go_out_per_gr, n_mf_per_go

@@ -34,7 +34,19 @@ enum cbm_gen_kind {
      * Delay masks follow InNetConnectivityState::assignGRDelays
      * (innetconnectivitystate.cpp:920-962); GO-GO wiring follows connectGOGO
      * (innetconnectivitystate.cpp:852-918). */
-    CBM_GEN_UNIFORM = 0
+    CBM_GEN_UNIFORM = 0,
+    /* The reference's own generator (InNetConnectivityState, CBM_STATE_LIB_DEV/
+     * src/state/innetconnectivitystate.cpp:13-44,716-1122): GR dendrites, GO
+     * dendrites and GO axons placed on the glomerulus grid by the spatial
+     * connector, MFs given glomeruli at random, the MF->GR, MF->GO and GO->GR
+     * lists relayed through the glomeruli, GO ascending dendrites onto GRs,
+     * then GO-GO wiring and delays, all from one SFMT stream seeded with
+     * innetSeed in the reference's order.  Sizes and spans come from the
+     * conParams keys (glXP2, glYP2, maxnumpGLfromGLtoGR, lownumpGLfromGLtoGR,
+     * spanGRDenOnGLX, ...; CBM_Params/conParams.txt holds them); the nMFperGR
+     * .. grgoWindow fields are not used.  A checkpoint of such a state holds
+     * the glomerulus layer and every list as the reference's does. */
+    CBM_GEN_REFERENCE = 1
 };
 
 typedef struct cbm_gen_spec {

@@ -154,3 +154,54 @@ class Oracle:
 def canonical_sum_f32(x: np.ndarray) -> float:
     x = np.ascontiguousarray(x, dtype=np.float32)
     return float(lib().orc_canonical_sum_f32(x.ctypes.data, x.size))
+
+
+# ---------------------------------------------------------------- input-net generator
+_INNET_KEYS = ("glXP2", "glYP2", "grXP2", "grYP2", "goXP2", "goYP2", "numMFP2", "maxnumpGLfromGLtoGR",
+               "lownumpGLfromGLtoGR", "maxnumpGLfromGLtoGO", "maxnumpGLfromGOtoGL", "maxnumpGRfromGLtoGR",
+               "maxnumpGRfromGOtoGR", "maxnumpGRfromGRtoGO", "maxnumpGOfromGRtoGO", "maxnumpGOfromGLtoGO",
+               "maxnumpGOfromGOtoGL", "spanGRDenOnGLX", "spanGRDenOnGLY", "spanGODecDenOnGLX", "spanGODecDenOnGLY",
+               "spanGOAxonOnGLX", "spanGOAxonOnGLY", "spanGOAscDenOnGRX", "spanGOAscDenOnGRY")
+INNET_LISTS = ("pGLfromGLtoGO", "pGLfromGOtoGL", "pGLfromGLtoGR", "pMFfromMFtoGL", "pMFfromMFtoGR", "pMFfromMFtoGO",
+               "pGOfromGLtoGO", "pGOfromGOtoGL", "pGOfromMFtoGO", "pGOfromGOtoGR", "pGOfromGRtoGO", "pGRfromGLtoGR",
+               "pGRfromGRtoGO", "pGRfromGOtoGR", "pGRfromMFtoGR")
+
+
+class _InnetParams(C.Structure):
+    _fields_ = [(k, C.c_uint32) for k in _INNET_KEYS]
+
+
+def innet_generate(con: dict, seed: int) -> dict:
+    """The oracle's input-net generator (orc_innet.c) on the conParams keys in
+    `con`: {list name: (counts int32 [rows], slots uint32 [rows, cols])},
+    plus "haspGLfromMFtoGL", "pGLfromMFtoGL" and "dropped"."""
+    L = lib()
+    L.orc_innet_generate.argtypes = [C.POINTER(_InnetParams), C.c_int, C.POINTER(C.c_void_p)]
+    L.orc_innet_free.argtypes = [C.c_void_p]
+    L.orc_innet_shape.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+    L.orc_innet_copy.argtypes = [C.c_void_p, C.c_char_p, C.c_void_p, C.c_void_p]
+    L.orc_innet_dropped.argtypes = [C.c_void_p]
+    L.orc_innet_dropped.restype = C.c_uint64
+    p = _InnetParams(*[int(con[k]) for k in _INNET_KEYS])
+    h = C.c_void_p()
+    rc = L.orc_innet_generate(C.byref(p), int(seed), C.byref(h))
+    try:
+        _check(rc, "orc_innet_generate")
+        out = {}
+        r, c = C.c_uint32(), C.c_uint32()
+        for name in INNET_LISTS:
+            L.orc_innet_shape(h, name.encode(), C.byref(r), C.byref(c))
+            num = np.empty(r.value, np.int32)
+            slots = np.empty((r.value, c.value), np.uint32)
+            L.orc_innet_copy(h, name.encode(), num.ctypes.data, slots.ctypes.data)
+            out[name] = (num, slots)
+        L.orc_innet_shape(h, b"pGLfromMFtoGL", C.byref(r), C.byref(c))
+        has = np.empty(r.value, np.uint8)
+        own = np.empty(r.value, np.uint32)
+        L.orc_innet_copy(h, b"haspGLfromMFtoGL", None, has.ctypes.data)
+        L.orc_innet_copy(h, b"pGLfromMFtoGL", None, own.ctypes.data)
+        out["haspGLfromMFtoGL"], out["pGLfromMFtoGL"] = has, own
+        out["dropped"] = int(L.orc_innet_dropped(h))
+        return out
+    finally:
+        L.orc_innet_free(h)

@@ -255,11 +255,15 @@ def test_closed_loop_run_matches_oracle_environment(gpu):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("closed", [False, True], ids=["open", "closed"])
-def test_cpp_driver_matches_library_loop(gpu, tmp_path, closed):
+@pytest.mark.parametrize("closed,glomerular", [(False, False), (True, False), (True, True)],
+                         ids=["open", "closed", "closed-glomerular"])
+def test_cpp_driver_matches_library_loop(gpu, tmp_path, closed, glomerular):
+    """The glomerular case gives the driver a parameter file with a glomerulus
+    layer: CBMState wires it with the reference's generator, as build_state does."""
     from cerebellum_simulation_amd import Network
-    print(f"[case] cart-pole C++ driver vs library loop, closed={closed}", flush=True)
-    con = configs.conparams_text(**configs.CONFIGS["t8k2z"][0])
+    print(f"[case] cart-pole C++ driver vs library loop, closed={closed} glomerular={glomerular}", flush=True)
+    con = (configs.glomerular_conparams_text("g2k", numNCP2=2, numIOP2=1) if glomerular else
+           configs.conparams_text(**configs.CONFIGS["t8k2z"][0]))
     act = configs.act_params_text()
     (tmp_path / "act.txt").write_text(act)
     (tmp_path / "con.txt").write_text(con)
