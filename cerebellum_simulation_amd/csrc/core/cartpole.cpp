@@ -174,6 +174,11 @@ float Cartpole::inverseLogScale(float scaledVal, float gain)
 
 void Cartpole::step(CBMSimCore *simCore) { advance(simCore, nullptr, nullptr); }
 
+void Cartpole::stepWithSpikes(CBMSimCore *simCore, const ct_uint8_t *apNCLeft, const ct_uint8_t *apNCRight)
+{
+    advance(simCore, apNCLeft, apNCRight);
+}
+
 void Cartpole::stepDetached(const ct_uint8_t *apNCLeft, const ct_uint8_t *apNCRight)
 {
     advance(nullptr, apNCLeft, apNCRight);
@@ -206,12 +211,14 @@ void Cartpole::advance(CBMSimCore *simCore, const ct_uint8_t *ncL, const ct_uint
     forceRightMZ.takeErrors();
 
     if (closedLoop) {
-        if (simCore) {
+        if (ncL && ncR) {
+            forceLeftMZ.update(ncL);
+            forceRightMZ.update(ncR);
+        } else if (simCore) {
             forceLeftMZ.update();
             forceRightMZ.update();
         } else {
-            forceLeftMZ.update(ncL);
-            forceRightMZ.update(ncR);
+            throw std::runtime_error("Cartpole: a closed-loop step needs NC spikes or a core");
         }
     }
 

@@ -83,6 +83,9 @@ int cbm_cartpole_create(cbm_handle *h, const cbm_sizes *s, int envSeed, int mfSe
             if (o->simStateDir) opts.simStateDir = o->simStateDir;
         }
         c->closedLoop = opts.closedLoop;
+        if (h && !opts.simStateDir.empty())
+            return fail(CBM_E_UNSUPPORTED,
+                        "cbm_cartpole_create: state files need a CBMState (use the C++ Cartpole with a CBMSimCore)");
         c->rng.reset(new CRandomSFMT0(envSeed));
         if (h) c->core.reset(new CBMSimCore(h, *s));
         c->env.reset(new Cartpole(c->rng.get(), opts));
@@ -110,29 +113,29 @@ int cbm_cartpole_run(cbm_cartpole *c, int n, float *trace, int *done)
     if (!c || n < 0) return fail(CBM_E_ARG, "cbm_cartpole_run: bad arguments");
     if (!c->core) return fail(CBM_E_ARG, "cbm_cartpole_run: created without a simulation handle");
     return guarded([&] {
+        // The loop of SimThread::run with one host round trip per step: the
+        // MF spikes go in, the step runs and every zone's NC spikes come back
+        // in one cbm_step_io pair, and the MF generator draws the next step's
+        // uniforms while the network steps.
         int k = 0;
         const unsigned nNC = c->sz.numNC, nMF = c->sz.numMF;
-        MZoneInterface **zones = c->core->getMZoneList();
+        cbm_handle *h = c->core->handle();
+        std::vector<uint8_t> nc((size_t)c->sz.numZones * nNC);
         for (; k < n && !c->env->terminated(); k++) {
             float *freq = c->env->getState();
             const ct_uint8_t *ap = c->mfs->calcActivity(freq);
-            c->core->updateMFInput(ap);
-            c->core->calcActivity();
+            int rc = cbm_step_io_begin(h, ap, CBM_REC_NC);
+            if (rc) return fail(rc, std::string("cbm_step_io_begin: ") + cbm_last_error());
+            c->mfs->prepare();
+            rc = cbm_step_io_end(h, nc.data());
+            if (rc) return fail(rc, std::string("cbm_step_io_end: ") + cbm_last_error());
             const unsigned long l0 = c->env->leftMZ().errorsDelivered(), r0 = c->env->rightMZ().errorsDelivered();
-            c->env->step(c->core.get());
+            c->env->stepWithSpikes(c->core.get(), nc.data(), nc.data() + nNC);
             if (trace) {
-                float ncL, ncR;
-                if (c->closedLoop) {
-                    ncL = c->env->leftMZ().lastNCSum();
-                    ncR = c->env->rightMZ().lastNCSum();
-                } else {
-                    const ct_uint8_t *a = zones[0]->exportAPNC();
-                    const ct_uint8_t *b = zones[1]->exportAPNC();
-                    ncL = ncR = 0;
-                    for (unsigned i = 0; i < nNC; i++) {
-                        ncL += a[i];
-                        ncR += b[i];
-                    }
+                float ncL = 0, ncR = 0;
+                for (unsigned i = 0; i < nNC; i++) {
+                    ncL += nc[i];
+                    ncR += nc[nNC + i];
                 }
                 int spikes = 0;
                 for (unsigned i = 0; i < nMF; i++) spikes += ap[i];

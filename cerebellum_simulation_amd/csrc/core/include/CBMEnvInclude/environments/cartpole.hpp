@@ -56,7 +56,9 @@ public:
     float *getState();
     void step(CBMSimCore *simCore);
     // The same step with the NC spikes of zones 0 and 1 given (closed loop)
-    // and the errors left in the microzones' counters instead of the core.
+    // rather than read from the core; errors still go to the core.
+    void stepWithSpikes(CBMSimCore *simCore, const ct_uint8_t *apNCLeft, const ct_uint8_t *apNCRight);
+    // The same with no core: errors are only counted by the microzones.
     void stepDetached(const ct_uint8_t *apNCLeft, const ct_uint8_t *apNCRight);
     bool terminated();
 

@@ -24,6 +24,11 @@ public:
     // stays valid until the next call.  A rate that is NaN never spikes.
     const ct_uint8_t *calcActivity(const float *frequencies);
 
+    // Not in the reference: draw the next step's uniforms now (one per cell,
+    // in cell order, the draws calcActivity would make), so that a driver can
+    // do it while the network steps; the next calcActivity uses them.
+    void prepare();
+
     // The relative thresholds after the last step (not in the reference).
     const float *thresholds() const { return threshs.data(); }
 
@@ -35,6 +40,8 @@ private:
     float msPerTS, sPerTS, threshDecay;
     std::vector<ct_uint8_t> aps;
     std::vector<float> threshs;
+    std::vector<double> draws;
+    bool prepared = false;
 };
 
 #endif

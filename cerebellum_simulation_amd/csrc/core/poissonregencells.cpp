@@ -22,13 +22,22 @@ PoissonRegenCells::PoissonRegenCells(unsigned int numCells, int randSeed, float 
 
 PoissonRegenCells::~PoissonRegenCells() { delete gen; }
 
+void PoissonRegenCells::prepare()
+{
+    draws.resize(nCells);
+    for (unsigned int i = 0; i < nCells; i++) draws[i] = gen->Random();
+    prepared = true;
+}
+
 const ct_uint8_t *PoissonRegenCells::calcActivity(const float *frequencies)
 {
+    if (!prepared) prepare();
+    prepared = false;
     for (unsigned int i = 0; i < nCells; i++) {
         float t = threshs[i];
         t = t + (1 - t) * threshDecay;
         const float p = (frequencies[i] * sPerTS) * t;
-        const bool spike = gen->Random() < (double)p;
+        const bool spike = draws[i] < (double)p;
         aps[i] = spike ? 1 : 0;
         threshs[i] = spike ? 0.0f : t;
     }

@@ -61,6 +61,10 @@ struct cbm_handle {
     bool skipFold = true;       // the ring slot of the last step holds host-written PF->PC sums
     uint8_t *dmf = nullptr;
     uint8_t *pinnedMF = nullptr;
+    // cbm_step_io_begin/_end: pinned read-back of one population's spikes
+    uint8_t *pinnedIO = nullptr;
+    uint32_t ioBytes = 0;
+    bool ioPending = false;
     uint32_t anyCoupling = 0;
     SmallArgs *smallArgs = nullptr;  // device copy of the constant small-kernel arguments
     SmallArgs smallHost{};           // host copy of the same (re-uploaded when the recorder changes)

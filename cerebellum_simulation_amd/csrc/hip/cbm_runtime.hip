@@ -243,6 +243,12 @@ int cbm_create(const cbm_sizes *sizes, const cbm_act_params *params, const cbm_o
     if (e == hipSuccess) e = hipEventRecord(h->evG, h->stream);
     if (e == hipSuccess) e = hipEventRecord(h->evS, h->stream2);
     if (e == hipSuccess) e = hipHostMalloc((void **)&h->pinnedMF, h->s.numMF, hipHostMallocDefault);
+    if (e == hipSuccess) {
+        const cbm_sizes &z = h->s;
+        const size_t io = std::max({(size_t)z.numZones * z.numPC, (size_t)z.numZones * z.numBC, (size_t)z.numSC,
+                                    (size_t)z.numZones * z.numIO, (size_t)z.numZones * z.numNC, (size_t)z.numGO});
+        e = hipHostMalloc((void **)&h->pinnedIO, io, hipHostMallocDefault);
+    }
     if (e != hipSuccess) {
         std::string m = hipGetErrorString(e);
         delete h;
@@ -271,6 +277,7 @@ int cbm_destroy(cbm_handle *h)
     for (void *p : h->allocs) (void)hipFree(p);
     if (h->rec) (void)hipFree(h->rec);
     if (h->pinnedMF) (void)hipHostFree(h->pinnedMF);
+    if (h->pinnedIO) (void)hipHostFree(h->pinnedIO);
     if (h->evG) (void)hipEventDestroy(h->evG);
     if (h->evS) (void)hipEventDestroy(h->evS);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -1054,6 +1061,50 @@ static int launchStep(cbm_handle *h, const uint8_t *mf, int timing)
 static int join(cbm_handle *h)
 {
     HIP_TRY(hipStreamWaitEvent(h->stream, h->evS, 0));
+    return CBM_OK;
+}
+
+int cbm_step_io_begin(cbm_handle *h, const uint8_t *mf, int population)
+{
+    if (!h || !mf) return fail(CBM_E_ARG, "null argument");
+    if (!h->committed) return fail(CBM_E_STATE, "cbm_commit has not been called");
+    if (h->ioPending) return fail(CBM_E_STATE, "cbm_step_io_end has not been called for the last step");
+    int field;
+    switch (population) {
+    case CBM_REC_PC: field = CBM_F_AP_PC; break;
+    case CBM_REC_BC: field = CBM_F_AP_BC; break;
+    case CBM_REC_SC: field = CBM_F_AP_SC; break;
+    case CBM_REC_IO: field = CBM_F_AP_IO; break;
+    case CBM_REC_NC: field = CBM_F_AP_NC; break;
+    case CBM_REC_GO: field = CBM_F_AP_GO; break;
+    default: return fail(CBM_E_ARG, "cbm_step_io_begin: population must be PC, BC, SC, IO, NC or GO");
+    }
+    HIP_TRY(hipSetDevice(h->o.device));
+    int rc = syncAll(h);  // the pinned stages of the last step are free again
+    if (rc) return rc;
+    std::memcpy(h->pinnedMF, mf, h->s.numMF);
+    HIP_TRY(hipMemcpyAsync(h->dmf, h->pinnedMF, h->s.numMF, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipEventRecord(h->evG, h->stream));
+    if ((rc = launchStep(h, h->dmf, kNoTiming))) return rc;
+    if ((rc = join(h))) return rc;
+    // the zones of a population sit one after another: one copy for all of them
+    const uint32_t bytes = recCells(h->s, population);
+    const uint8_t *src = parityField(h, field, 0);
+    if (!src) src = static_cast<const uint8_t *>(h->dfield[field]);
+    HIP_TRY(hipMemcpyAsync(h->pinnedIO, src, bytes, hipMemcpyDeviceToHost, h->stream));
+    h->ioBytes = bytes;
+    h->ioPending = true;
+    return CBM_OK;
+}
+
+int cbm_step_io_end(cbm_handle *h, uint8_t *out)
+{
+    if (!h || !out) return fail(CBM_E_ARG, "null argument");
+    if (!h->ioPending) return fail(CBM_E_STATE, "cbm_step_io_begin has not been called");
+    HIP_TRY(hipSetDevice(h->o.device));
+    h->ioPending = false;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    std::memcpy(out, h->pinnedIO, h->ioBytes);
     return CBM_OK;
 }
 

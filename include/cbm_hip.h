@@ -390,6 +390,20 @@ int cbm_reset_ginc_grgo(cbm_handle *h);
  * after the work is queued; cbm_sync() waits for it. */
 int cbm_step(cbm_handle *h, int n);
 
+/* The per-ms round trip of a closed-loop driver in one call pair: the MF
+ * input from host memory, one step, and the spikes of one population of
+ * every zone back to the host -- what SimThread::run and Microzone::update do
+ * with updateMFInput, calcActivity and exportAPNC per zone
+ * (Matthew_Sim_CBM/src/simthread.cpp:105-112, src/microzone.cpp:31-33), with
+ * one host synchronisation instead of one per call.  _begin queues the copy
+ * in, the step and the copy out (population: CBM_REC_PC, _BC, _SC, _IO, _NC
+ * or _GO, not _PF) and returns; the host may work meanwhile (the next step's
+ * draws, say).  _end waits and writes the spikes, zones stacked
+ * ([numZones][cells] for the microzone populations), to out.  Every _begin
+ * needs its _end before the next _begin. */
+int cbm_step_io_begin(cbm_handle *h, const uint8_t *mf, int population);
+int cbm_step_io_end(cbm_handle *h, uint8_t *out);
+
 /* n steps where step k reads MF spikes from mf_dev + k*pitch (a DEVICE
  * pointer to n rows of numMF bytes, e.g. a pre-generated spike train).
  * err_drive_host, if not NULL, gives [n][numZones] relative error drives
