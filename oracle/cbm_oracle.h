@@ -18,6 +18,18 @@
  *                           the canonical order documented in cbm_oracle.c
  *   precision 64  "replay": the same recurrence with every real in binary64
  *                           (constants are the binary32 values, widened)
+ *
+ * How each part is pinned (against checks its formulas do not restate):
+ *   GR update, MF/GO conductances, plasticity and history, PF/GO sums,
+ *   pipeline lags, NC->IO release draws, MF history  tests/test_oracle.py
+ *   Golgi glutamate/mGluR chain, NC conductances and membrane, IO
+ *   conductances and coupling, MF->NC plasticity      tests/test_oracle_pins.py
+ *   SFMT (orc_sfmt.c)                                  tests/test_sfmt.py (the
+ *                                                       reference's own generator)
+ *   derived constants                                  tests/test_params.py
+ *   input-net generator (orc_innet.c)                  tests/test_innet_gen.py
+ * Parity unpinned: the generator's stream equality with the reference's own
+ * build of InNetConnectivityState, which cannot be compiled here.
  */
 #ifndef CBM_ORACLE_H
 #define CBM_ORACLE_H
