@@ -96,6 +96,8 @@ struct ConvArgs {
     int lead;               // rows of a segment without a full receptive field
     float scale;            // 1 / number of outputs (BCE)
     float *loss;
+    float *part;            // split-K: FP32 partial sums [split][M][N] instead of the epilogue
+    int kPerSplit;          // K steps (of BK) per split
 };
 
 template <int ASRC, int EPI>
@@ -164,12 +166,14 @@ __global__ void __launch_bounds__(kThreads) conv_kernel(const ConvArgs a)
 
     const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
     const int fr = lane & 15, fk = (lane >> 4) * 8;
-    const int nk = a.K / BK;
-    SG_LOAD(0)
+    // this block's K steps: all of them, or its split's
+    const int kb0 = a.part ? (int)blockIdx.z * a.kPerSplit : 0;
+    const int nk = a.part ? min(a.K / BK, kb0 + a.kPerSplit) : a.K / BK;
+    SG_LOAD(kb0)
     SG_STORE(0)
     __syncthreads();
-    for (int kb = 0; kb < nk; kb++) {
-        const int buf = kb & 1;
+    for (int kb = kb0; kb < nk; kb++) {
+        const int buf = (kb - kb0) & 1;
         if (kb + 1 < nk) SG_LOAD(kb + 1)
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 32) {
@@ -191,6 +195,19 @@ __global__ void __launch_bounds__(kThreads) conv_kernel(const ConvArgs a)
 #undef SG_LOAD
 #undef SG_STORE
 
+    if (a.part) {  // split-K: this split's partial sums
+        float *dst = a.part + (size_t)blockIdx.z * a.M * a.N;
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int r = m0 + wm + i * 16 + (lane >> 4) * 4 + v;
+                if (r >= a.M) continue;
+#pragma unroll
+                for (int j = 0; j < 4; j++) dst[(size_t)r * a.N + n0 + wn + j * 16 + (lane & 15)] = acc[i][j][v];
+            }
+        return;
+    }
     // epilogue: lane holds column (lane & 15) and rows 4*(lane >> 4) + v of each 16x16 tile
     float lsum = 0.f;
 #pragma unroll
@@ -355,6 +372,36 @@ __global__ void __launch_bounds__(256) colsum_kernel(const u16 *G, size_t ldg, i
     part[(size_t)blockIdx.y * C + c] = s;
 }
 
+// The masked data gradient from split-K partials: out[r][n] = BF16(sum_s
+// part[s][r][n], in split order), or 0 where the forward activation mask[r][n]
+// is not positive (the EPI_MASK epilogue).
+__global__ void __launch_bounds__(256) mask_reduce_kernel(const float *part, int splits, int M, int N, const u16 *mask,
+                                                          size_t ldm, u16 *out, size_t ldo)
+{
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (size_t)M * N) return;
+    const size_t r = i / (size_t)N, n = i - r * (size_t)N;
+    float x = 0.f;
+    for (int k = 0; k < splits; k++) x += part[(size_t)k * M * N + i];
+    const u16 m = mask[r * ldm + n];
+    out[r * ldo + n] = (m & 0x8000u) || !(m & 0x7FFFu) ? (u16)0 : f2bf(x);
+}
+
+// A hidden layer's activations from split-K partials: out[r][n] =
+// BF16(max(sum_s part[s][r][n] + bias[n], 0)), the sum in split order (the
+// EPI_RELU epilogue).
+__global__ void __launch_bounds__(256) relu_reduce_kernel(const float *part, int splits, int M, int N,
+                                                          const float *bias, u16 *out, size_t ldo)
+{
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (size_t)M * N) return;
+    const size_t r = i / (size_t)N, n = i - r * (size_t)N;
+    float x = 0.f;
+    for (int k = 0; k < splits; k++) x += part[(size_t)k * M * N + i];
+    const float z = x + bias[n];
+    out[r * ldo + n] = f2bf(z > 0.f ? z : 0.f);
+}
+
 // out[i] = sum_s part[s][i], in split order
 __global__ void __launch_bounds__(256) reduce_kernel(const float *part, int splits, size_t n, float *out)
 {
@@ -461,21 +508,51 @@ int check_batch(const cbm_sg *sg, const cbm_sg_batch *b, bool training, int *P, 
 
 dim3 grid_for(size_t n) { return dim3((unsigned)((n + 255) / 256)); }
 
-int conv(cbm_sg *sg, int src, int epi, const ConvArgs &a)
+int conv(cbm_sg *sg, int src, int epi, const ConvArgs &a0)
 {
-    const dim3 grid(a.N / BN, (a.M + BM - 1) / BM);
+    ConvArgs a = a0;
+    a.part = nullptr;
+    a.kPerSplit = 0;
+    int splits = 1;
+    const int tiles = (a.N / BN) * ((a.M + BM - 1) / BM), nk = a.K / BK;
+    if ((epi == EPI_MASK || epi == EPI_RELU) && tiles < 512 && nk >= 16) {
+        // too few tiles to fill the chip over a long K: split K, sum in order
+        splits = std::min(8, std::min(nk / 8, (1024 + tiles - 1) / tiles));
+        while (splits > 1 && (size_t)splits * a.M * a.N > sg->partFloats) splits--;
+        if (splits > 1) {
+            a.kPerSplit = (nk + splits - 1) / splits;
+            splits = (nk + a.kPerSplit - 1) / a.kPerSplit;
+            a.part = sg->part;
+        }
+    }
+    const dim3 grid(a.N / BN, (a.M + BM - 1) / BM, splits);
 #define CONV_CASE(S, E)                                                                         \
     if (src == S && epi == E) {                                                                 \
         hipLaunchKernelGGL((conv_kernel<S, E>), grid, dim3(kThreads), 0, sg->stream, a);        \
         sg->lastFlops += 2.0 * a.M * (double)a.N * a.K;                                         \
         HIP_TRY(hipGetLastError());                                                             \
+        if (a.part) {                                                                           \
+            hipLaunchKernelGGL(relu_reduce_kernel, grid_for((size_t)a.M * a.N), dim3(256), 0,   \
+                               sg->stream, a.part, splits, a.M, a.N, a.bias, a.out, a.ldo);     \
+            HIP_TRY(hipGetLastError());                                                         \
+        }                                                                                       \
         return CBM_OK;                                                                          \
     }
     CONV_CASE(SRC_SPIKE, EPI_RELU)
     CONV_CASE(SRC_BF16, EPI_RELU)
     CONV_CASE(SRC_BF16, EPI_SIGMOID)
     CONV_CASE(SRC_BF16, EPI_BCE)
-    CONV_CASE(SRC_BF16, EPI_MASK)
+    if (src == SRC_BF16 && epi == EPI_MASK) {
+        hipLaunchKernelGGL((conv_kernel<SRC_BF16, EPI_MASK>), grid, dim3(kThreads), 0, sg->stream, a);
+        sg->lastFlops += 2.0 * a.M * (double)a.N * a.K;
+        HIP_TRY(hipGetLastError());
+        if (a.part) {
+            hipLaunchKernelGGL(mask_reduce_kernel, grid_for((size_t)a.M * a.N), dim3(256), 0, sg->stream, a.part,
+                               splits, a.M, a.N, a.mask, a.ldm, a.out, a.ldo);
+            HIP_TRY(hipGetLastError());
+        }
+        return CBM_OK;
+    }
 #undef CONV_CASE
     return fail(CBM_E_ARG, "unsupported convolution variant");
 }
