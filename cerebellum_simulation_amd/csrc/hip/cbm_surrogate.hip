@@ -103,8 +103,11 @@ struct ConvArgs {
 template <int ASRC, int EPI>
 __global__ void __launch_bounds__(kThreads) conv_kernel(const ConvArgs a)
 {
-    __shared__ __attribute__((aligned(16))) u16 sA[2][BM * LDK];
-    __shared__ __attribute__((aligned(16))) u16 sB[2][BN * LDK];
+    // the operand tiles, double-buffered; after the last K step the same bytes
+    // hold the output tile for the epilogue
+    __shared__ __attribute__((aligned(16))) u16 smem[2 * (BM + BN) * LDK];
+    u16(*sA)[BM * LDK] = reinterpret_cast<u16(*)[BM * LDK]>(smem);
+    u16(*sB)[BN * LDK] = reinterpret_cast<u16(*)[BN * LDK]>(smem + 2 * BM * LDK);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
 
@@ -208,43 +211,86 @@ __global__ void __launch_bounds__(kThreads) conv_kernel(const ConvArgs a)
             }
         return;
     }
-    // epilogue: lane holds column (lane & 15) and rows 4*(lane >> 4) + v of each 16x16 tile
+    // Epilogue through LDS: a lane holds column (lane & 15) and rows
+    // 4*(lane >> 4) + v of each 16x16 tile; the tile is staged and then
+    // written out row by row with the lanes on consecutive columns, so every
+    // global access of the epilogue is a contiguous run per wave.
+    constexpr int LDT = BN + 4;  // floats per staged row
+    static_assert(BM * LDT * 4 + BM * 8 <= 2 * (BM + BN) * LDK * 2, "staged tile must fit the operand buffers");
+    float *tile = reinterpret_cast<float *>(smem);
+    int *rowSeg = reinterpret_cast<int *>(tile + BM * LDT);  // segment of each tile row
+    int *rowPos = rowSeg + BM;                               // position in it, -1 past M
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                tile[(wm + i * 16 + (lane >> 4) * 4 + v) * LDT + wn + j * 16 + (lane & 15)] = acc[i][j][v];
+    if (tid < BM) {
+        const int r = m0 + tid;
+        const int sg = r < a.M ? r / a.P : 0;
+        rowSeg[tid] = sg;
+        rowPos[tid] = r < a.M ? r - sg * a.P : -1;
+    }
+    __syncthreads();
+    // a thread takes 4 consecutive columns of every 8th row: a wave covers two
+    // whole tile rows per instruction
+    const int c4 = (tid & 31) * 4, n = n0 + c4;
+    float bn[4] = {0.f, 0.f, 0.f, 0.f};
+    if (EPI != EPI_MASK) *(float4 *)bn = *(const float4 *)&a.bias[n];
     float lsum = 0.f;
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
+    for (int it = 0; it < BM / 8; it++) {
+        const int row = it * 8 + (tid >> 5), pos = rowPos[row];
+        if (pos < 0) continue;
+        const int sg = rowSeg[row], r = m0 + row;
+        float x[4];
+        *(float4 *)x = *(const float4 *)&tile[row * LDT + c4];
+        if (EPI == EPI_RELU) {
+            u16 o[4];
 #pragma unroll
-        for (int v = 0; v < 4; v++) {
-            const int r = m0 + wm + i * 16 + (lane >> 4) * 4 + v;
-            if (r >= a.M) continue;
-            const int s = r / a.P, pos = r - s * a.P;
+            for (int c = 0; c < 4; c++) {
+                const float z = x[c] + bn[c];
+                o[c] = f2bf(z > 0.f ? z : 0.f);
+            }
+            *(uint2 *)&a.out[(size_t)r * a.ldo + n] = *(const uint2 *)o;
+        } else if (EPI == EPI_MASK) {
+            u16 m[4], o[4];
+            *(uint2 *)m = *(const uint2 *)&a.mask[(size_t)r * a.ldm + n];
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const int n = n0 + wn + j * 16 + (lane & 15);
-                const float x = acc[i][j][v];
-                if (EPI == EPI_RELU) {
-                    const float z = x + a.bias[n];
-                    a.out[(size_t)r * a.ldo + n] = f2bf(z > 0.f ? z : 0.f);
-                } else if (EPI == EPI_MASK) {
-                    const u16 m = a.mask[(size_t)r * a.ldm + n];
-                    a.out[(size_t)r * a.ldo + n] = (m & 0x8000u) || !(m & 0x7FFFu) ? (u16)0 : f2bf(x);
-                } else if (EPI == EPI_SIGMOID) {
-                    if (pos >= a.lead) {
-                        const float z = x + a.bias[n];
-                        a.prob[((size_t)s * (a.P - a.lead) + (pos - a.lead)) * a.ldp + n] = 1.f / (1.f + expf(-z));
-                    }
-                } else {  // EPI_BCE
-                    float d = 0.f;
-                    if (pos >= a.lead) {
-                        const float z = x + a.bias[n];
-                        const int64_t yrow = a.segY[s] + pos;
-                        const float y = bf2f(a.Y[(size_t)yrow * a.ldy + n]);
-                        const float p = 1.f / (1.f + expf(-z));
-                        d = (p - y) * a.scale;
-                        lsum += fmaxf(z, 0.f) + log1pf(expf(-fabsf(z))) - y * z;
-                    }
-                    a.out[(size_t)r * a.ldo + n] = f2bf(d);
+            for (int c = 0; c < 4; c++) o[c] = (m[c] & 0x8000u) || !(m[c] & 0x7FFFu) ? (u16)0 : f2bf(x[c]);
+            *(uint2 *)&a.out[(size_t)r * a.ldo + n] = *(const uint2 *)o;
+        } else if (EPI == EPI_SIGMOID) {
+            if (pos >= a.lead) {
+                float pr[4];
+#pragma unroll
+                for (int c = 0; c < 4; c++) pr[c] = 1.f / (1.f + expf(-(x[c] + bn[c])));
+                *(float4 *)&a.prob[((size_t)sg * (a.P - a.lead) + (pos - a.lead)) * a.ldp + n] = *(const float4 *)pr;
+            }
+        } else {  // EPI_BCE
+            u16 o[4] = {0, 0, 0, 0};
+            if (pos >= a.lead) {
+                const int64_t yrow = a.segY[sg] + pos;
+                u16 yv[4];
+                *(uint2 *)yv = *(const uint2 *)&a.Y[(size_t)yrow * a.ldy + n];
+#pragma unroll
+                for (int c = 0; c < 4; c++) {
+                    // one exponential serves both: e = exp(-|z|), u = 1 + e,
+                    // sigmoid(z) = 1/u (z >= 0) or e/u, and the loss term
+                    // softplus(z) - y z = max(z, 0) + log1p(e) - y z with
+                    // log1p(e) = log(u) * e / (u - 1) (Goldberg's correction
+                    // for the rounding of u; e itself when u rounds to 1)
+                    const float z = x[c] + bn[c], y = bf2f(yv[c]);
+                    const float e = expf(-fabsf(z)), u = 1.f + e;
+                    const float r = __builtin_amdgcn_rcpf(u);
+                    const float p = z >= 0.f ? r : e * r;
+                    o[c] = f2bf((p - y) * a.scale);
+                    const float lp = u == 1.f ? e : __logf(u) * e * __builtin_amdgcn_rcpf(u - 1.f);
+                    lsum += fmaxf(z, 0.f) + lp - y * z;
                 }
             }
+            *(uint2 *)&a.out[(size_t)r * a.ldo + n] = *(const uint2 *)o;
         }
     }
     if (EPI == EPI_BCE) {
@@ -857,6 +903,8 @@ int cbm_sg_init_params(cbm_sg *sg, uint64_t seed, float outBias)
 int cbm_sg_forward(cbm_sg *sg, const cbm_sg_batch *b, float *prob, size_t probPitch)
 {
     if (!sg || !prob || probPitch < sg->d.numOut) return fail(CBM_E_ARG, "bad argument");
+    if (((uintptr_t)prob & 15) || (probPitch & 3))
+        return fail(CBM_E_ARG, "prob must be 16-byte aligned with a pitch (elements) that is a multiple of 4");
     int P, R;
     int rc = check_batch(sg, b, false, &P, &R);
     if (rc) return rc;

@@ -80,7 +80,8 @@ int cbm_sg_set_params(cbm_sg *sg, const float *host);
  * Clears the Adam state. */
 int cbm_sg_init_params(cbm_sg *sg, uint64_t seed, float outBias);
 
-/* Inference: probabilities, device FP32 [numSeg * outputs][probPitch]. */
+/* Inference: probabilities, device FP32 [numSeg * outputs][probPitch]; prob
+ * 16-byte aligned and probPitch a multiple of 4. */
 int cbm_sg_forward(cbm_sg *sg, const cbm_sg_batch *b, float *prob, size_t probPitch);
 
 /* Forward + backward without an update.  loss (host, nullable) receives the
