@@ -316,7 +316,9 @@ struct WgradArgs {
 
 // C[o][tap*cin + c] over this block's split of rows; both operands have the
 // reduction index (time) as their row, so each thread loads 4 rows x 8
-// columns and writes them transposed: 8 LDS words of 4 rows each.
+// columns and writes them transposed: 8 LDS words of 4 rows each.  The 16
+// lanes that share an LDS cycle take 16 different row quads of the same 8
+// columns, so their 8-byte writes cover all 32 banks once.
 template <int SSRC>
 __global__ void __launch_bounds__(kThreads) wgrad_kernel(const WgradArgs a)
 {
@@ -327,7 +329,7 @@ __global__ void __launch_bounds__(kThreads) wgrad_kernel(const WgradArgs a)
     const int tap = n0 / a.cin, c0 = n0 - tap * a.cin;
     const int r0 = split * a.rowsPerSplit;
     const int r1 = min(a.R, r0 + a.rowsPerSplit);
-    const int cc = (tid & 15) * 8, rg = (tid >> 4) * 4;  // 8 columns x rows rg..rg+3
+    const int cc = (tid >> 4) * 8, rg = (tid & 15) * 4;  // 8 columns x rows rg..rg+3
 
     uint4 rgv[4], rsv[4];
     auto load = [&](int kb) {
