@@ -240,7 +240,7 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=400)
     ap.add_argument("--cpu-steps", type=int, default=0, help="0 = pick a ~15 s sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--step-mode", type=int, default=0, help="0 pick, 1 fused launch, 2 two streams")
+    ap.add_argument("--step-mode", type=int, default=0, help="0 pick, 1 fused launch, 2 two streams, 3 fused with GR tiles, 4 fused with even GR split")
     ap.add_argument("--cells-per-thread", type=int, default=0)
     ap.add_argument("--block-threads", type=int, default=0)
     ap.add_argument("--batch", default="full", choices=["full", "small", "none"],
@@ -305,10 +305,12 @@ def main():
     n_out = s.maxnumpGRfromGRtoGO
     alg_bytes = s.numGR * bytes_per_gr(n_out)
     achieved = alg_bytes / (k_us * 1e-6) / 1e9
-    fused = net.step_mode == 1
+    fused = net.step_mode != 2
+    launch = net.gr_launch
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "step_kernel" if fused else "gr_step_kernel", "kernel_us": round(k_us, 3),
+                "kernel": ("step_span_kernel" if launch["even_split"] else "step_kernel") if fused
+                else "gr_step_kernel", "gr_launch": launch, "kernel_us": round(k_us, 3),
                 "algorithmic_bytes_per_launch": alg_bytes, "step_us_events": round(step_us, 3),
                 "gr_only_us": round(gr_only, 3), "small_only_us": round(small_only, 3),
                 "gr_only_frac": round(alg_bytes / (gr_only * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)}

@@ -201,11 +201,12 @@ __host__ __device__ inline void unpackGrConn(const uint4 c, uint2 *mi, uint2 *gi
     *bsp = dl(lo, 55);
 }
 
-// Bytes of dynamic LDS a GR tile of nt threads x cpt cells uses.
+// Bytes of dynamic LDS a GR tile or span of nt threads x cpt cells uses.
 __host__ __device__ constexpr uint32_t grTileLds(uint32_t nt, uint32_t cpt, uint32_t numGO, uint32_t numMF,
                                                  uint32_t numZones, uint32_t numIO)
 {
-    return numGO * 4 + 2 * (nt * cpt / 64) * 4 + numZones * numIO * 4 + numMF + numGO;
+    return numGO * 4 + 2 * (nt * cpt / 64 + 1) * 4 + numZones * numIO * 4 + ((numMF >> 5) + 1) * 4 +
+           ((numGO >> 5) + 1) * 4;
 }
 
 __host__ __device__ inline uint32_t ilog2(uint32_t x)

@@ -10,10 +10,16 @@
 #include "cbm_dev.hpp"
 
 namespace cbmdev {
-template <int CPT, int NT>
+// FMT: 1 = compact 16-byte connectivity records, 2 = the five arrays
+template <int CPT, int NT, int FMT>
 __global__ void gr_step_kernel(const GrArgs a, const GrStep st);
-template <int CPT>
+template <int CPT, int FMT>
 __global__ void step_kernel(const SmallArgs *args, const SmallStep st, const GrArgs g, const GrStep gs);
+template <int CPT, int FMT>
+__global__ void step_span_kernel(const SmallArgs *args, const SmallStep st, const GrArgs g, const GrStep gs,
+                                 const uint32_t spans);
+template <int CPT, int FMT>
+__global__ void gr_span_kernel(const GrArgs a, const GrStep st, const uint32_t spans);
 __global__ void small_step_kernel(const SmallArgs *args, const SmallStep st);
 __global__ void pfpc_fold_kernel(const float *chunk, float *out, uint32_t nChunks, uint32_t perPC, uint32_t numPC,
                                  uint32_t tasks);
@@ -43,8 +49,11 @@ struct cbm_handle {
     bool committed = false;
     int cpt = 4, nt = 256;   // GR kernel shape of the two-stream step
     int fusedCpt = 1;        // GR cells per thread of the fused step
-    int mode = 0;            // 1 fused, 2 two streams (decided at commit)
+    int mode = 0;            // 1, 3, 4 fused, 2 two streams (decided at commit)
+    uint32_t spans = 0;      // fused step: GR blocks of the even split (0: power-of-two tiles)
+    int spanCpt = 0;         // GR cells per thread of those blocks
     size_t fusedLds = 0;     // dynamic LDS of the fused step kernel
+    bool fused() const { return mode != 2; }
 
     std::vector<uint8_t> host[CBM_F_COUNT];  // staged fields, zones stacked
     bool written[CBM_F_COUNT] = {};

@@ -23,7 +23,8 @@
 //                     MF->NC plasticity (mzone.cpp:258-605).
 //
 //  step_kernel        both in one launch: block 0 is the small step, the other
-//                     blocks are GR tiles.
+//                     blocks are GR tiles (step_span_kernel: the GR cells split
+//                     evenly over one block per remaining CU).
 
 // Floating-point expressions keep the reference's operand order and types and
 // the file is built with -ffp-contract=off, so binary32 results round where the
@@ -33,113 +34,241 @@
 namespace cbmdev {
 
 // ---------------------------------------------------------------- helpers
-__device__ __forceinline__ float wave_sum_butterfly(float x)
+// Sum of a value over the 64 lanes of a wave, valid in lane 0: the halving
+// tree x[i] += x[i+k] for k = 32, 16, ..., 1 (the pairs and order of an xor
+// butterfly, read at lane 0), with lane i < k reading lane i + k through the
+// cross-lane swaps and row shifts of the VALU instead of LDS permutes.
+__device__ __forceinline__ float wave_sum_lane0(float x)
 {
-    // xor butterfly over 64 lanes; every lane ends with the same value, equal
-    // to the halving tree x[i]+=x[i+k] for k=32,16,...,1 evaluated at lane 0
-#pragma unroll
-    for (int k = 32; k >= 1; k >>= 1) x = x + __shfl_xor(x, k, 64);
+    x = x + __uint_as_float(__builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(x), false, false)[1]);
+    x = x + __uint_as_float(__builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(x), false, false)[1]);
+    x = x + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x108, 0xF, 0xF, true));  // row_shl:8
+    x = x + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x104, 0xF, 0xF, true));  // row_shl:4
+    x = x + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x102, 0xF, 0xF, true));  // row_shl:2
+    x = x + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x101, 0xF, 0xF, true));  // row_shl:1
     return x;
 }
 
 __device__ __forceinline__ uint32_t lo16(uint32_t w) { return w & 0xFFFFu; }
 __device__ __forceinline__ uint32_t hi16(uint32_t w) { return w >> 16; }
 
-// stage a byte vector as 0/1 flags in LDS (n multiple of 4)
-__device__ __forceinline__ void stage_flags(uint8_t *dst, const uint8_t *src, uint32_t n, int tid,
-                                            int nthr)
-{
-    const uint32_t *s32 = reinterpret_cast<const uint32_t *>(src);
-    uint32_t *d32 = reinterpret_cast<uint32_t *>(dst);
-    for (uint32_t i = tid; i < n / 4; i += nthr) {
-        uint32_t w = s32[i];
-        uint32_t o = ((w & 0x000000FFu) != 0) | (((w & 0x0000FF00u) != 0) << 8) |
-                     (((w & 0x00FF0000u) != 0) << 16) | (((w & 0xFF000000u) != 0) << 24);
-        d32[i] = o;
-    }
-}
-
 // ---------------------------------------------------------------- PF->PC fold
 // Second level of the PF->PC sum of one (zone, PC), run by one full wave: lane
 // l adds the 64-GR chunk sums l, l+64, l+128, ... in order, then the lanes are
-// combined by the xor butterfly.
+// combined as by the xor butterfly (the result is valid in lane 0).
 __device__ __forceinline__ float fold_pc(const float *__restrict__ c, uint32_t perPC, int lane)
 {
     float acc = 0.0f;
 #pragma unroll 8
     for (uint32_t q = lane; q < perPC; q += 64) acc = acc + c[q];
-    return wave_sum_butterfly(acc);
+    return wave_sum_lane0(acc);
 }
 
-// ---------------------------------------------------------------- GR tile
-// One tile of NT*CPT consecutive granule cells: thread tid owns cells
-// tileBase + j*NT + tid, j < CPT.  Every global load of the tile's cells is
-// issued before the LDS staging barrier, so all of a thread's requests are in
-// flight together.
-template <int CPT, int NT>
-__device__ __forceinline__ void gr_tile(const GrArgs &a, const GrStep &st, const uint32_t tile,
-                                        uint32_t *smem)
+// ---------------------------------------------------------------- GR span
+// Second level of the previous step's PF->PC sums, one wave per (zone, PC),
+// spread over the waves of the GR blocks (block `rank` of `ranks`): the small
+// populations read them one step later.
+template <int NT>
+__device__ __forceinline__ void gr_fold(const GrArgs &a, const GrStep &st, const uint32_t rank, const uint32_t ranks)
 {
-    constexpr int kSeg = NT * CPT / 64;  // 64-GR segments per tile (PF blocks hold >= 64 GRs)
-    uint32_t *s_goh = smem;                          // [numGO] GR->GO counts of the tile
+    if (!st.foldSrc) return;
+    const int lane = threadIdx.x & 63;
+    const uint32_t nChunks = a.numGR >> 6, waves = ranks * (uint32_t)(NT / 64);
+    for (uint32_t task = rank * (uint32_t)(NT / 64) + (threadIdx.x >> 6); task < a.numZones * a.numPC;
+         task += waves) {  // wave-uniform
+        const uint32_t z = task / a.numPC, pc = task % a.numPC;
+        const float acc = fold_pc(st.foldSrc + (size_t)z * nChunks + (size_t)pc * a.perPC, a.perPC, lane);
+        if (lane == 0) st.foldDst[task] = acc;
+    }
+}
+
+// 1 when bit idx of the LDS bitmap bm is set
+__device__ __forceinline__ uint32_t bit_at(const uint32_t *bm, uint32_t idx) { return (bm[idx >> 5] >> (idx & 31u)) & 1u; }
+
+// `count` consecutive granule cells from `first`, both multiples of 64 and
+// count <= NT*CPT: thread tid owns cells first + j*NT + tid below first +
+// count (whole waves are in or out).  The block is `rank` of `ranks` for the
+// fold tasks.  The MF and GO spikes are staged in LDS as bitmaps (one word per
+// 32 cells, one bank each, so the cells' random reads never conflict); each
+// bitmap has one more word than it needs, so bit numMF / numGO reads 0 and an
+// absent input (kNoCell) is clamped onto it.  Order of the loads: the spike
+// bytes and the plasticity timers first, then every load of the span's cells
+// in straight-line code (the connectivity format is a template argument and
+// the records are unpacked after the barrier), so no load waits for an
+// earlier one, the staging waits only for the spike bytes, and each cell's
+// update starts as soon as its own loads have arrived.  FULL: count ==
+// NT*CPT; COMPACT: one 16-byte connectivity record per GR (a.conn).
+template <int CPT, int NT, bool FULL, bool COMPACT>
+__device__ __forceinline__ void gr_span_impl(const GrArgs &a, const GrStep &st, const uint32_t first,
+                                             const uint32_t count, const uint32_t rank, const uint32_t ranks,
+                                             uint32_t *smem)
+{
+    constexpr int kSeg = NT * CPT / 64 + 1;  // PF blocks a span can touch (blocks hold >= 64 GRs)
+    constexpr int kStage = 2048 / NT;        // spike bytes of each vector per thread held in registers
+    const uint32_t nStep = a.numZones * a.numIO;
+    const uint32_t nMFb = (a.numMF >> 5) + 1, nGOb = (a.numGO >> 5) + 1;  // bitmap words
+    uint32_t *s_goh = smem;                          // [numGO] GR->GO counts of the span
     uint32_t *s_sc = s_goh + a.numGO;                // [kSeg]
     uint32_t *s_bc = s_sc + kSeg;                    // [kSeg]
     float *s_step = reinterpret_cast<float *>(s_bc + kSeg);  // [numZones*numIO]
-    uint8_t *s_mf = reinterpret_cast<uint8_t *>(s_step + a.numZones * a.numIO);  // [numMF]
-    uint8_t *s_go = s_mf + a.numMF;                  // [numGO]
+    uint32_t *s_mfb = reinterpret_cast<uint32_t *>(s_step + nStep);  // [nMFb] MF spike bitmap
+    uint32_t *s_gob = s_mfb + nMFb;                  // [nGOb] GO spike bitmap (previous step)
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
-    const uint32_t tileBase = tile * (uint32_t)(NT * CPT);
+    const uint32_t wbase = (uint32_t)(tid & ~63);
+    // spike bytes, from clamped addresses (a load under a branch makes the
+    // compiler wait for every load in flight before the value is used)
+    uint32_t smf[kStage], sgo[kStage];
+#pragma unroll
+    for (int k = 0; k < kStage; k++) {
+        const uint32_t i = tid + k * NT;
+        smf[k] = st.mf[min(i, a.numMF - 1)];
+        sgo[k] = st.apGOprev[min(i, a.numGO - 1)];
+    }
+    const int32_t tm0 = st.plastTimer[min((uint32_t)tid, nStep - 1)];
+    // MZone::runPFPCPlastCUDA step choice (mzone.cpp:660-674)
+    // (the zone's step sizes picked by compares among register copies: an
+    // indexed read of the argument block would be one more memory load to
+    // wait for, and so would a select of its fields, which the compiler turns
+    // into one)
+    auto reg = [](float x) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x))); };
+    auto stepOf = [&](uint32_t i, int32_t tm) {
+        const uint32_t z = i / a.numIO;
+        float ltd = reg(a.ltd[0]), ltp = reg(a.ltp[0]);
+#pragma unroll
+        for (int zz = 1; zz < kMaxZones; zz++)
+            if (z == (uint32_t)zz) {
+                ltd = reg(a.ltd[zz]);
+                ltp = reg(a.ltp[zz]);
+            }
+        return tm < a.ltdEnd ? ltd : (tm > a.ltpStart ? ltp : 0.0f);
+    };
+    // 64 spike flags of one wave into two bitmap words
+    auto stage64 = [&](uint32_t *bm, uint32_t nWords, uint32_t base, bool on) {
+        const uint64_t m = __ballot(on);
+        if (lane == 0 && (base >> 5) < nWords) bm[base >> 5] = (uint32_t)m;
+        if (lane == 0 && (base >> 5) + 1 < nWords) bm[(base >> 5) + 1] = (uint32_t)(m >> 32);
+    };
+    // what the registers do not hold (large networks only), before the cells'
+    // loads: a loop that loads makes the compiler wait for every load in
+    // flight before it
+    if (nStep > (uint32_t)NT && st.doPlast)
+        for (uint32_t i = tid + NT; i < nStep; i += NT) s_step[i] = stepOf(i, st.plastTimer[i]);
+    for (uint32_t base = kStage * NT + wbase; base < 32 * nMFb; base += NT)  // wave-uniform
+        stage64(s_mfb, nMFb, base, base + lane < a.numMF && st.mf[base + lane] != 0);
+    for (uint32_t base = kStage * NT + wbase; base < 32 * nGOb; base += NT)
+        stage64(s_gob, nGOb, base, base + lane < a.numGO && st.apGOprev[base + lane] != 0);
+    __builtin_amdgcn_sched_barrier(0);  // keep the loads above ahead of the cells' loads
 
     float v[CPT], gk[CPT], th[CPT], gE[CPT], gI[CPT], w0[CPT];
-    uint32_t ab[CPT], gd[CPT], bsp[CPT];
-    uint2 mi[CPT], gi[CPT], go[CPT];
-    uint64_t hist[CPT];
+    uint32_t ab[CPT];
+    uint4 rec[CPT];                       // COMPACT
+    uint2 mi_[CPT], gi_[CPT], go_[CPT];   // !COMPACT
+    uint32_t gd_[CPT], bsp_[CPT];         // !COMPACT
+    uint64_t hist[CPT] = {};
+    auto cellOf = [&](int j) -> uint32_t {
+        const uint32_t o = (uint32_t)(j * NT + tid);
+        return first + (FULL ? o : min(o, count - 1u));
+    };
+    // (unconditional loads: cells past the span read its last cell, and the
+    // update below skips them; a load under a branch lets the compiler move
+    // work on its value into the branch and wait for it there)
 #pragma unroll
     for (int j = 0; j < CPT; j++) {
-        const uint32_t gr = tileBase + j * NT + tid;
+        const uint32_t gr = cellOf(j);
         v[j] = a.v[gr];
         gk[j] = a.gKCa[gr];
         th[j] = a.thr[gr];
         gE[j] = a.gE[gr];
         gI[j] = a.gI[gr];
         ab[j] = a.apBuf[gr];
-        if (a.conn) {
-            unpackGrConn(a.conn[gr], &mi[j], &gi[j], &go[j], &gd[j], &bsp[j]);
+        if constexpr (COMPACT) {
+            rec[j] = a.conn[gr];
         } else {
-            mi[j] = a.mfIn[gr];
-            gi[j] = a.goIn[gr];
-            go[j] = a.goOut[gr];
-            gd[j] = a.goOutDly[gr];
-            bsp[j] = a.bspDly[gr];
+            mi_[j] = a.mfIn[gr];
+            gi_[j] = a.goIn[gr];
+            go_[j] = a.goOut[gr];
+            gd_[j] = a.goOutDly[gr];
+            bsp_[j] = a.bspDly[gr];
         }
         w0[j] = a.w[gr];
-        hist[j] = st.doPlast ? a.hist[gr] : 0;
     }
 
+    // LDS writes of what was loaded first, before any branch or loop that
+    // loads (after one, the compiler waits for every load in flight)
+#pragma unroll
+    for (int k = 0; k < kStage; k++) {
+        const uint32_t base = k * NT + wbase, i = base + lane;
+        stage64(s_mfb, nMFb, base, i < a.numMF && smf[k] != 0);
+        stage64(s_gob, nGOb, base, i < a.numGO && sgo[k] != 0);
+    }
+    if (st.doPlast) {
+#pragma unroll
+        for (int j = 0; j < CPT; j++) hist[j] = a.hist[cellOf(j)];
+    }
+    if (st.doPlast && (uint32_t)tid < nStep) s_step[tid] = stepOf(tid, tm0);
     for (uint32_t i = tid; i < a.numGO; i += NT) s_goh[i] = 0;
     if (tid < 2 * kSeg) s_sc[tid] = 0;  // s_sc and s_bc are contiguous
-    if (st.doPlast) {
-        for (uint32_t i = tid; i < a.numZones * a.numIO; i += NT) {
-            const uint32_t z = i / a.numIO;
-            const int32_t tm = st.plastTimer[i];
-            // MZone::runPFPCPlastCUDA step choice (mzone.cpp:660-674)
-            s_step[i] = tm < a.ltdEnd ? a.ltd[z] : (tm > a.ltpStart ? a.ltp[z] : 0.0f);
-        }
-    }
-    stage_flags(s_mf, st.mf, a.numMF, tid, NT);
-    stage_flags(s_go, st.apGOprev, a.numGO, tid, NT);
     __syncthreads();
+    __builtin_amdgcn_sched_barrier(0);  // no cell update ahead of the barrier: it would wait for the cell's loads
 
-    const uint32_t firstSc = tileBase >> a.log2Lsc, firstBc = tileBase >> a.log2Lbc;
+    const uint32_t firstSc = first >> a.log2Lsc, firstBc = first >> a.log2Lbc;
     const uint32_t nChunks = a.numGR >> 6;
 
 #pragma unroll
     for (int j = 0; j < CPT; j++) {
-        const uint32_t gr = tileBase + j * NT + tid;
+        if (!FULL && j * NT + (tid & ~63) >= count) continue;  // wave-uniform
+        const uint32_t gr = first + j * NT + tid;
         float vv = v[j], g = gk[j], t = th[j], e = gE[j], i = gI[j];
         uint32_t b = ab[j];
+        // connectivity fields: MF and GO inputs clamped onto the zero bit of
+        // the bitmaps when absent, the delay of the spike towards BC/PC/SC
+        // (>= 32: never), and up to four GO targets with their delays
+        uint32_t in[8], dBsp, tg[4], dg[4];
+        if constexpr (COMPACT) {
+            // packGrConn: 11-bit indices (0x7FF absent), 6-bit delays (0x3F never)
+            const uint64_t lo = (uint64_t)rec[j].x | (uint64_t)rec[j].y << 32;
+            const uint64_t hi = (uint64_t)rec[j].z | (uint64_t)rec[j].w << 32;
+            auto f11 = [](uint64_t w, int sh) { return (uint32_t)(w >> sh) & kNoCell11; };
+            auto f6 = [](uint64_t w, int sh) { return (uint32_t)(w >> sh) & kNever6; };
+            in[0] = min(f11(lo, 0), a.numMF);
+            in[1] = min(f11(lo, 11), a.numMF);
+            in[2] = min(f11(lo, 22), a.numMF);
+            in[3] = min(f11(lo, 33), a.numMF);
+            in[4] = min(f11(lo, 44), a.numGO);
+            in[5] = min(f11(hi, 0), a.numGO);
+            in[6] = min(f11(hi, 11), a.numGO);
+            in[7] = a.numGO;
+            dBsp = f6(lo, 55);
+            tg[0] = f11(hi, 22);
+            tg[1] = f11(hi, 33);
+            tg[2] = tg[3] = kNoCell11;
+            dg[0] = f6(hi, 44);
+            dg[1] = f6(hi, 50);
+            dg[2] = dg[3] = kNever6;
+        } else {
+            const uint2 mi = mi_[j], gi = gi_[j], go = go_[j];
+            in[0] = min(lo16(mi.x), a.numMF);
+            in[1] = min(hi16(mi.x), a.numMF);
+            in[2] = min(lo16(mi.y), a.numMF);
+            in[3] = min(hi16(mi.y), a.numMF);
+            in[4] = min(lo16(gi.x), a.numGO);
+            in[5] = min(hi16(gi.x), a.numGO);
+            in[6] = min(lo16(gi.y), a.numGO);
+            in[7] = min(hi16(gi.y), a.numGO);
+            dBsp = bsp_[j] == kNever ? 63u : bsp_[j];
+            tg[0] = lo16(go.x);
+            tg[1] = hi16(go.x);
+            tg[2] = lo16(go.y);
+            tg[3] = hi16(go.y);
+            const uint32_t dl = gd_[j];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const uint32_t x = (dl >> (8 * q)) & 0xFFu;
+                dg[q] = x == kNever ? 63u : x;
+            }
+        }
 
         // calcActivityGRGPU (kernels.cu:92-109)
         vv = vv + (a.gLeak + g * g * g * g) * (a.eLeak - vv) - (e)*vv + (i) * (a.eGO - vv);
@@ -152,18 +281,9 @@ __device__ __forceinline__ void gr_tile(const GrArgs &a, const GrStep &st, const
 
         // updateGRInOPGPU for MF then GO inputs (kernels.cu:185-198)
         int cm = 0, cg = 0;
-        {
-            const uint32_t m0 = lo16(mi[j].x), m1 = hi16(mi[j].x), m2 = lo16(mi[j].y), m3 = hi16(mi[j].y);
-            if (m0 != kNoCell) cm += s_mf[m0];
-            if (m1 != kNoCell) cm += s_mf[m1];
-            if (m2 != kNoCell) cm += s_mf[m2];
-            if (m3 != kNoCell) cm += s_mf[m3];
-            const uint32_t g0 = lo16(gi[j].x), g1 = hi16(gi[j].x), g2 = lo16(gi[j].y), g3 = hi16(gi[j].y);
-            if (g0 != kNoCell) cg += s_go[g0];
-            if (g1 != kNoCell) cg += s_go[g1];
-            if (g2 != kNoCell) cg += s_go[g2];
-            if (g3 != kNoCell) cg += s_go[g3];
-        }
+        cm = (int)(bit_at(s_mfb, in[0]) + bit_at(s_mfb, in[1]) + bit_at(s_mfb, in[2]) + bit_at(s_mfb, in[3]));
+        cg = (int)(bit_at(s_gob, in[4]) + bit_at(s_gob, in[5]) + bit_at(s_gob, in[6]));
+        if constexpr (!COMPACT) cg += (int)bit_at(s_gob, in[7]);
         e = e * a.decMF + a.incMF * (cm);
         i = i * a.decGO + a.incGO * (cg);
 
@@ -183,7 +303,7 @@ __device__ __forceinline__ void gr_tile(const GrArgs &a, const GrStep &st, const
         }
 
         // delayed spike towards BC/PC/SC (kernels.cu:256,269)
-        const uint32_t d = (bsp[j] == kNever) ? 0u : ((b >> bsp[j]) & 1u);
+        const uint32_t d = (uint32_t)((uint64_t)b >> dBsp) & 1u;
         const uint64_t bal = __ballot(d);
 
         // PF->PC: plasticity, then this step's 64-GR chunk sum, per zone
@@ -199,7 +319,7 @@ __device__ __forceinline__ void gr_tile(const GrArgs &a, const GrStep &st, const
                 wz[gr] = wv;
             }
             float x = 0.0f;
-            if (bal != 0) x = wave_sum_butterfly(wv * d);
+            if (bal != 0) x = wave_sum_lane0(wv * d);
             if (lane == 0) st.chunk[(size_t)z * nChunks + (gr >> 6)] = x;
         }
 
@@ -212,14 +332,12 @@ __device__ __forceinline__ void gr_tile(const GrArgs &a, const GrStep &st, const
         }
 
         // GR->GO outputs (kernels.cu:133-144)
+        // (a target is absent when its index is numGO or more; a delay of 32 or
+        // more never sees a spike)
         if (b != 0) {
-            const uint32_t t0 = lo16(go[j].x), t1 = hi16(go[j].x), t2 = lo16(go[j].y), t3 = hi16(go[j].y);
-            const uint32_t dl = gd[j];
-            const uint32_t d0 = dl & 0xFFu, d1 = (dl >> 8) & 0xFFu, d2 = (dl >> 16) & 0xFFu, d3 = dl >> 24;
-            if (t0 != kNoCell && d0 != kNever && ((b >> d0) & 1u)) atomicAdd(&s_goh[t0], 1u);
-            if (t1 != kNoCell && d1 != kNever && ((b >> d1) & 1u)) atomicAdd(&s_goh[t1], 1u);
-            if (t2 != kNoCell && d2 != kNever && ((b >> d2) & 1u)) atomicAdd(&s_goh[t2], 1u);
-            if (t3 != kNoCell && d3 != kNever && ((b >> d3) & 1u)) atomicAdd(&s_goh[t3], 1u);
+#pragma unroll
+            for (int q = 0; q < (COMPACT ? 2 : 4); q++)
+                if (tg[q] < a.numGO && (((uint64_t)b >> dg[q]) & 1u)) atomicAdd(&s_goh[tg[q]], 1u);
         }
     }
     __syncthreads();
@@ -228,38 +346,96 @@ __device__ __forceinline__ void gr_tile(const GrArgs &a, const GrStep &st, const
         const uint32_t c = s_goh[i];
         if (c) atomicAdd(&st.goCnt[i], c);
     }
-    const uint32_t tileGR = NT * CPT;
-    const uint32_t nSc = (tileGR >> a.log2Lsc) ? (tileGR >> a.log2Lsc) : 1;
-    const uint32_t nBc = (tileGR >> a.log2Lbc) ? (tileGR >> a.log2Lbc) : 1;
-    if (tid < (int)nSc && s_sc[tid]) atomicAdd(&st.scCnt[firstSc + tid], s_sc[tid]);
-    if (tid < (int)nBc && s_bc[tid]) atomicAdd(&st.bcCnt[firstBc + tid], s_bc[tid]);
+    {
+        // PF blocks the span touches; blocks at either end may be shared with
+        // the neighbouring span, whose counts add in by the same atomics
+        const uint32_t nSc = ((first + count - 1) >> a.log2Lsc) - firstSc + 1;
+        const uint32_t nBc = ((first + count - 1) >> a.log2Lbc) - firstBc + 1;
+        if ((uint32_t)tid < nSc && s_sc[tid]) atomicAdd(&st.scCnt[firstSc + tid], s_sc[tid]);
+        if ((uint32_t)tid < nBc && s_bc[tid]) atomicAdd(&st.bcCnt[firstBc + tid], s_bc[tid]);
+    }
 
-    // second level of the previous step's PF->PC sums, one wave per (zone,
-    // PC) on the first tiles: the small populations read them one step later
-    if (st.foldSrc) {
-        const uint32_t waves = (a.numGR / (NT * CPT)) * (uint32_t)(NT / 64);
-        for (uint32_t task = tile * (uint32_t)(NT / 64) + (uint32_t)(tid >> 6); task < a.numZones * a.numPC;
-             task += waves) {  // wave-uniform
-            const uint32_t z = task / a.numPC, pc = task % a.numPC;
-            const float acc = fold_pc(st.foldSrc + (size_t)z * nChunks + (size_t)pc * a.perPC, a.perPC, lane);
-            if (lane == 0) st.foldDst[task] = acc;
-        }
+    gr_fold<NT>(a, st, rank, ranks);
+}
+
+// Connectivity format of a kernel instantiation: kFmtAny decides per launch
+// from a.conn (the batch, whose networks may differ); a kernel built for one
+// format holds only that variant, which keeps its scalar registers from
+// spilling.
+constexpr int kFmtAny = 0, kFmtCompact = 1, kFmtArrays = 2;
+
+template <int CPT, int NT, bool FULL, int FMT>
+__device__ __forceinline__ void gr_span(const GrArgs &a, const GrStep &st, const uint32_t first, const uint32_t count,
+                                        const uint32_t rank, const uint32_t ranks, uint32_t *smem)
+{
+    if (!FULL && count == 0) {  // an empty span (more blocks than 64-GR chunks) only folds
+        gr_fold<NT>(a, st, rank, ranks);
+        return;
+    }
+    if constexpr (FMT == kFmtCompact) {
+        gr_span_impl<CPT, NT, FULL, true>(a, st, first, count, rank, ranks, smem);
+    } else if constexpr (FMT == kFmtArrays) {
+        gr_span_impl<CPT, NT, FULL, false>(a, st, first, count, rank, ranks, smem);
+    } else {
+        if (a.conn)
+            gr_span_impl<CPT, NT, FULL, true>(a, st, first, count, rank, ranks, smem);
+        else
+            gr_span_impl<CPT, NT, FULL, false>(a, st, first, count, rank, ranks, smem);
     }
 }
 
+// One tile of NT*CPT consecutive granule cells (tile-aligned).
+template <int CPT, int NT, int FMT>
+__device__ __forceinline__ void gr_tile(const GrArgs &a, const GrStep &st, const uint32_t tile, uint32_t *smem)
+{
+    gr_span<CPT, NT, true, FMT>(a, st, tile * (uint32_t)(NT * CPT), (uint32_t)(NT * CPT), tile,
+                                a.numGR / (uint32_t)(NT * CPT), smem);
+}
+
+// Span r of `spans`: chunks [r*C/spans, (r+1)*C/spans) of the C = numGR/64
+// 64-GR chunks, so that every block gets the same work within one chunk.
+template <int CPT, int NT, int FMT>
+__device__ __forceinline__ void gr_span_of(const GrArgs &a, const GrStep &st, const uint32_t r, const uint32_t spans,
+                                           uint32_t *smem)
+{
+    const uint64_t nC = a.numGR >> 6;
+    const uint32_t c0 = (uint32_t)(r * nC / spans), c1 = (uint32_t)((r + 1) * nC / spans);
+    gr_span<CPT, NT, false, FMT>(a, st, c0 << 6, (c1 - c0) << 6, r, spans, smem);
+}
+
 // GR tiles alone (the two-stream step, and GR-only timing)
-template <int CPT, int NT>
+template <int CPT, int NT, int FMT>
 __global__ void __launch_bounds__(NT) gr_step_kernel(const GrArgs a, const GrStep st)
 {
     extern __shared__ uint32_t smem[];
-    gr_tile<CPT, NT>(a, st, blockIdx.x, smem);
+    gr_tile<CPT, NT, FMT>(a, st, blockIdx.x, smem);
 }
 
-template __global__ void gr_step_kernel<1, 256>(const GrArgs, const GrStep);
-template __global__ void gr_step_kernel<2, 256>(const GrArgs, const GrStep);
-template __global__ void gr_step_kernel<4, 256>(const GrArgs, const GrStep);
-template __global__ void gr_step_kernel<1, 1024>(const GrArgs, const GrStep);
-template __global__ void gr_step_kernel<2, 1024>(const GrArgs, const GrStep);
+#define CBM_GR_STEP(CPT, NT)                                                                    \
+    template __global__ void gr_step_kernel<CPT, NT, kFmtCompact>(const GrArgs, const GrStep); \
+    template __global__ void gr_step_kernel<CPT, NT, kFmtArrays>(const GrArgs, const GrStep);
+CBM_GR_STEP(1, 256)
+CBM_GR_STEP(2, 256)
+CBM_GR_STEP(4, 256)
+CBM_GR_STEP(1, 1024)
+CBM_GR_STEP(2, 1024)
+#undef CBM_GR_STEP
+
+// GR spans alone (GR-only timing of the span-split fused step)
+template <int CPT, int FMT>
+__global__ void __launch_bounds__(1024) gr_span_kernel(const GrArgs a, const GrStep st, const uint32_t spans)
+{
+    extern __shared__ uint32_t smem[];
+    gr_span_of<CPT, 1024, FMT>(a, st, blockIdx.x, spans, smem);
+}
+
+#define CBM_GR_SPAN(CPT)                                                                                       \
+    template __global__ void gr_span_kernel<CPT, kFmtCompact>(const GrArgs, const GrStep, const uint32_t); \
+    template __global__ void gr_span_kernel<CPT, kFmtArrays>(const GrArgs, const GrStep, const uint32_t);
+CBM_GR_SPAN(2)
+CBM_GR_SPAN(3)
+CBM_GR_SPAN(5)
+#undef CBM_GR_SPAN
 
 // ---------------------------------------------------------------- SFMT on device
 // The zone generator of MZone (CRandomSFMT0, sfmt.cpp:210-246,337-355),
@@ -972,18 +1148,45 @@ __global__ void __launch_bounds__(kSmallThreads) small_step_kernel(const SmallAr
 // One launch per step: block 0 steps the small populations, blocks 1.. are GR
 // tiles of 1024 threads (one block per CU: the small step needs up to 128
 // VGPRs per thread).
-template <int CPT>
+template <int CPT, int FMT>
 __global__ void __launch_bounds__(kSmallThreads) step_kernel(const SmallArgs *__restrict__ args, const SmallStep st, const GrArgs g, const GrStep gs)
 {
     extern __shared__ __align__(16) uint8_t lds[];
     if (blockIdx.x == 0)
         small_step(*args, st, st.errDrive, lds);
     else
-        gr_tile<CPT, kSmallThreads>(g, gs, blockIdx.x - 1, reinterpret_cast<uint32_t *>(lds));
+        gr_tile<CPT, kSmallThreads, FMT>(g, gs, blockIdx.x - 1, reinterpret_cast<uint32_t *>(lds));
 }
 
-template __global__ void step_kernel<1>(const SmallArgs *, const SmallStep, const GrArgs, const GrStep);
-template __global__ void step_kernel<2>(const SmallArgs *, const SmallStep, const GrArgs, const GrStep);
+template __global__ void step_kernel<1, kFmtCompact>(const SmallArgs *, const SmallStep, const GrArgs, const GrStep);
+template __global__ void step_kernel<2, kFmtCompact>(const SmallArgs *, const SmallStep, const GrArgs, const GrStep);
+template __global__ void step_kernel<1, kFmtArrays>(const SmallArgs *, const SmallStep, const GrArgs, const GrStep);
+template __global__ void step_kernel<2, kFmtArrays>(const SmallArgs *, const SmallStep, const GrArgs, const GrStep);
+
+// The same with the GR cells split evenly over `spans` blocks (one per
+// remaining CU): all of a step's GR loads are issued in one wave of blocks
+// that runs beside the small step.
+template <int CPT, int FMT>
+__global__ void __launch_bounds__(kSmallThreads) step_span_kernel(const SmallArgs *__restrict__ args, const SmallStep st,
+                                                                  const GrArgs g, const GrStep gs, const uint32_t spans)
+{
+    extern __shared__ __align__(16) uint8_t lds[];
+    if (blockIdx.x == 0)
+        small_step(*args, st, st.errDrive, lds);
+    else
+        gr_span_of<CPT, kSmallThreads, FMT>(g, gs, blockIdx.x - 1, spans, reinterpret_cast<uint32_t *>(lds));
+}
+
+#define CBM_STEP_SPAN(CPT, FMT)                                                                              \
+    template __global__ void step_span_kernel<CPT, FMT>(const SmallArgs *, const SmallStep, const GrArgs, \
+                                                        const GrStep, const uint32_t);
+CBM_STEP_SPAN(2, kFmtCompact)
+CBM_STEP_SPAN(3, kFmtCompact)
+CBM_STEP_SPAN(5, kFmtCompact)
+CBM_STEP_SPAN(2, kFmtArrays)
+CBM_STEP_SPAN(3, kFmtArrays)
+CBM_STEP_SPAN(5, kFmtArrays)
+#undef CBM_STEP_SPAN
 
 // ---------------------------------------------------------------- batch
 // One launch per step for a pack of networks: block n < nNets steps network
@@ -1053,7 +1256,7 @@ __global__ void __launch_bounds__(kSmallThreads) batch_step_kernel(const BatchNe
         gs.plastTimer = d.timer2 + (size_t)parPrev * nZ * g.numIO;
         gs.doPlast = (t % d.tsPerHistBinGR) == 0;
         gs.pfCnt = nullptr;
-        gr_tile<CPT, kSmallThreads>(g, gs, tt.y, reinterpret_cast<uint32_t *>(lds));
+        gr_tile<CPT, kSmallThreads, kFmtAny>(g, gs, tt.y, reinterpret_cast<uint32_t *>(lds));
     }
 }
 

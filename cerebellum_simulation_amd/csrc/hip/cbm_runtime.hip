@@ -231,9 +231,9 @@ int cbm_create(const cbm_sizes *sizes, const cbm_act_params *params, const cbm_o
     h->fusedCpt = (h->o.grCellsPerThread == 1 || sizes->numGR < 2048) ? 1 : 2;
     const bool shapeOk = (h->nt == 256 && (h->cpt == 1 || h->cpt == 2 || h->cpt == 4)) ||
                          (h->nt == 1024 && (h->cpt == 1 || h->cpt == 2));
-    if (!shapeOk || h->o.stepMode > 2) {
+    if (!shapeOk || h->o.stepMode > 4) {
         delete h;
-        return fail(CBM_E_ARG, "GR kernel shape must be 256 threads x 1, 2 or 4 cells or 1024 x 1 or 2; stepMode 0..2");
+        return fail(CBM_E_ARG, "GR kernel shape must be 256 threads x 1, 2 or 4 cells or 1024 x 1 or 2; stepMode 0..4");
     }
     hipError_t e = hipSetDevice(h->o.device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
@@ -311,6 +311,10 @@ int cbmdev::syncAll(cbm_handle *h)
     HIP_TRY(hipStreamSynchronize(h->stream2));
     return CBM_OK;
 }
+
+// GR-kernel instantiations by launch shape and connectivity format (defined below)
+static const void *grSpanKernel(int cpt, bool compact);
+static const void *fusedKernel(const cbm_handle *h);
 
 extern "C" {
 
@@ -682,12 +686,32 @@ int cbm_commit(cbm_handle *h)
                                         (int)h->smallLds));
         // fused step: every block gets max(small, GR tile) LDS (one 1024-thread
         // block per CU either way, the small step's registers decide that)
-        h->fusedLds = std::max<size_t>(h->smallLds, grTileLds(1024, h->fusedCpt, s.numGO, s.numMF, nZ, s.numIO));
         h->mode = h->o.stepMode ? (int)h->o.stepMode : 1;
-        if (h->mode == 1 && h->fusedLds > 64 * 1024) {
-            const void *k = h->fusedCpt == 2 ? (const void *)step_kernel<2> : (const void *)step_kernel<1>;
-            HIP_TRY(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->fusedLds));
+        // GR work of the fused step: split evenly over one block per CU beside
+        // the small step's when every block gets between 1,024 and 5,120 cells
+        // (mode 1) or whenever it fits (mode 4); else tiles of 1024 x fusedCpt
+        // cells (mode 3 always)
+        if (h->mode == 1 || h->mode == 4) {
+            int cus = 0;
+            HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->o.device));
+            const uint32_t spans = cus > 1 ? (uint32_t)cus - 1 : 1u;
+            const uint32_t per = (s.numGR / 64 + spans - 1) / spans * 64;  // most cells of a span
+            const int cpt = per <= 2048 ? 2 : per <= 3072 ? 3 : per <= 5120 ? 5 : 0;
+            if (cpt && (h->mode == 4 || per >= 1024)) {
+                h->spans = spans;
+                h->spanCpt = cpt;
+            } else if (h->mode == 4) {
+                return fail(CBM_E_UNSUPPORTED, "stepMode 4: more than 5,120 GR per block");
+            }
         }
+        h->fusedLds = std::max<size_t>(
+            h->smallLds, grTileLds(1024, h->spans ? h->spanCpt : h->fusedCpt, s.numGO, s.numMF, nZ, s.numIO));
+        if (h->fused() && h->fusedLds > 64 * 1024)
+            HIP_TRY(hipFuncSetAttribute(fusedKernel(h), hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->fusedLds));
+        const size_t grLds = grTileLds(1024, h->spans ? h->spanCpt : h->fusedCpt, s.numGO, s.numMF, nZ, s.numIO);
+        if (h->spans && grLds > 64 * 1024)
+            HIP_TRY(hipFuncSetAttribute(grSpanKernel(h->spanCpt, h->conn != nullptr),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)grLds));
         if ((rc = h->dalloc((void **)&h->smallArgs, sizeof(SmallArgs)))) return rc;
         HIP_TRY(hipMemcpy(h->smallArgs, &a, sizeof(SmallArgs), hipMemcpyHostToDevice));
     }
@@ -897,22 +921,47 @@ int cbm_reset_ginc_grgo(cbm_handle *h)
     return CBM_OK;
 }
 
+// The GR-kernel instantiations by launch shape and connectivity format
+// (compact records or the five arrays).
+static const void *grTileKernel(int nt, int cpt, bool compact)
+{
+#define CBM_K(C, N) (compact ? (const void *)gr_step_kernel<C, N, 1> : (const void *)gr_step_kernel<C, N, 2>)
+    if (nt == 1024) return cpt == 2 ? CBM_K(2, 1024) : CBM_K(1, 1024);
+    return cpt == 4 ? CBM_K(4, 256) : cpt == 2 ? CBM_K(2, 256) : CBM_K(1, 256);
+#undef CBM_K
+}
+
+static const void *grSpanKernel(int cpt, bool compact)
+{
+#define CBM_K(C) (compact ? (const void *)gr_span_kernel<C, 1> : (const void *)gr_span_kernel<C, 2>)
+    return cpt == 2 ? CBM_K(2) : cpt == 3 ? CBM_K(3) : CBM_K(5);
+#undef CBM_K
+}
+
+// the fused step kernel of the handle's launch shape
+static const void *fusedKernel(const cbm_handle *h)
+{
+    const bool compact = h->conn != nullptr;
+    if (h->spans) {
+#define CBM_K(C) (compact ? (const void *)step_span_kernel<C, 1> : (const void *)step_span_kernel<C, 2>)
+        return h->spanCpt == 2 ? CBM_K(2) : h->spanCpt == 3 ? CBM_K(3) : CBM_K(5);
+#undef CBM_K
+    }
+#define CBM_K(C) (compact ? (const void *)step_kernel<C, 1> : (const void *)step_kernel<C, 2>)
+    return h->fusedCpt == 2 ? CBM_K(2) : CBM_K(1);
+#undef CBM_K
+}
+
 // Launch the GR tiles alone (nt x cpt shape) on `st`.
-static void launchGr(cbm_handle *h, const GrArgs &g, const GrStep &gs, int nt, int cpt, hipStream_t st)
+static hipError_t launchGr(cbm_handle *h, const GrArgs &g, const GrStep &gs, int nt, int cpt, hipStream_t st)
 {
     const cbm_sizes &s = h->s;
     const uint32_t blocks = s.numGR / (uint32_t)(nt * cpt);
     const size_t lds = grTileLds(nt, cpt, s.numGO, s.numMF, s.numZones, s.numIO);
-    if (nt == 1024) {
-        if (cpt == 2) hipLaunchKernelGGL((gr_step_kernel<2, 1024>), dim3(blocks), dim3(1024), lds, st, g, gs);
-        else hipLaunchKernelGGL((gr_step_kernel<1, 1024>), dim3(blocks), dim3(1024), lds, st, g, gs);
-    } else if (cpt == 4) {
-        hipLaunchKernelGGL((gr_step_kernel<4, 256>), dim3(blocks), dim3(256), lds, st, g, gs);
-    } else if (cpt == 2) {
-        hipLaunchKernelGGL((gr_step_kernel<2, 256>), dim3(blocks), dim3(256), lds, st, g, gs);
-    } else {
-        hipLaunchKernelGGL((gr_step_kernel<1, 256>), dim3(blocks), dim3(256), lds, st, g, gs);
-    }
+    GrArgs ga = g;
+    GrStep gsa = gs;
+    void *args[] = {&ga, &gsa};
+    return hipLaunchKernel(grTileKernel(nt, cpt, h->conn != nullptr), dim3(blocks), dim3(nt), args, lds, st);
 }
 
 }  // extern "C"
@@ -1021,29 +1070,34 @@ static int launchStep(cbm_handle *h, const uint8_t *mf, int timing)
 
     if (timing == kSplit) {
         HIP_TRY(hipEventRecord(h->ev[0], h->stream));
-        launchGr(h, g, gs, h->mode == 1 ? 1024 : h->nt, h->mode == 1 ? h->fusedCpt : h->cpt, h->stream);
+        if (h->spans) {
+            const size_t lds = grTileLds(1024, h->spanCpt, s.numGO, s.numMF, s.numZones, s.numIO);
+            uint32_t spans = h->spans;
+            void *args[] = {&g, &gs, &spans};
+            HIP_TRY(hipLaunchKernel(grSpanKernel(h->spanCpt, h->conn != nullptr), dim3(spans), dim3(1024), args, lds,
+                                    h->stream));
+        } else {
+            HIP_TRY(launchGr(h, g, gs, h->fused() ? 1024 : h->nt, h->fused() ? h->fusedCpt : h->cpt, h->stream));
+        }
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventRecord(h->ev[1], h->stream));
         HIP_TRY(hipEventRecord(h->ev[2], h->stream));
         hipLaunchKernelGGL(small_step_kernel, dim3(1), dim3(kSmallThreads), h->smallLds, h->stream, h->smallArgs, st);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventRecord(h->ev[3], h->stream));
-    } else if (h->mode == 1) {
+    } else if (h->fused()) {
         const uint32_t tiles = s.numGR / (1024u * (uint32_t)h->fusedCpt);
         if (timing == kTimeKernel) HIP_TRY(hipEventRecord(h->ev[0], h->stream));
-        if (h->fusedCpt == 2)
-            hipLaunchKernelGGL(step_kernel<2>, dim3(1 + tiles), dim3(kSmallThreads), h->fusedLds, h->stream,
-                               h->smallArgs, st, g, gs);
-        else
-            hipLaunchKernelGGL(step_kernel<1>, dim3(1 + tiles), dim3(kSmallThreads), h->fusedLds, h->stream,
-                               h->smallArgs, st, g, gs);
-        HIP_TRY(hipGetLastError());
+        const SmallArgs *sa = h->smallArgs;
+        uint32_t spans = h->spans;
+        void *args[] = {&sa, &st, &g, &gs, &spans};  // the tile kernel takes the first four
+        HIP_TRY(hipLaunchKernel(fusedKernel(h), dim3(1 + (spans ? spans : tiles)), dim3(kSmallThreads), args,
+                                h->fusedLds, h->stream));
         if (timing == kTimeKernel) HIP_TRY(hipEventRecord(h->ev[1], h->stream));
     } else {
         HIP_TRY(hipStreamWaitEvent(h->stream, h->evS, 0));
         if (timing == kTimeKernel) HIP_TRY(hipEventRecord(h->ev[0], h->stream));
-        launchGr(h, g, gs, h->nt, h->cpt, h->stream);
-        HIP_TRY(hipGetLastError());
+        HIP_TRY(launchGr(h, g, gs, h->nt, h->cpt, h->stream));
         if (timing == kTimeKernel) HIP_TRY(hipEventRecord(h->ev[1], h->stream));
         HIP_TRY(hipStreamWaitEvent(h->stream2, h->evG, 0));
         hipLaunchKernelGGL(small_step_kernel, dim3(1), dim3(kSmallThreads), h->smallLds, h->stream2, h->smallArgs, st);
@@ -1239,6 +1293,19 @@ int cbm_step_mode(cbm_handle *h)
     if (!h) return fail(CBM_E_ARG, "null handle");
     if (!h->committed) return fail(CBM_E_STATE, "cbm_commit has not been called");
     return h->mode;
+}
+
+int cbm_gr_launch(cbm_handle *h, uint32_t *blocks, uint32_t *threads, uint32_t *cellsPerThread, uint32_t *evenSplit)
+{
+    if (!h || !blocks || !threads || !cellsPerThread || !evenSplit) return fail(CBM_E_ARG, "null argument");
+    if (!h->committed) return fail(CBM_E_STATE, "cbm_commit has not been called");
+    const int nt = h->fused() ? 1024 : h->nt;
+    const int cpt = h->spans ? h->spanCpt : (h->fused() ? h->fusedCpt : h->cpt);
+    *blocks = h->spans ? h->spans : h->s.numGR / (uint32_t)(nt * cpt);
+    *threads = (uint32_t)nt;
+    *cellsPerThread = (uint32_t)cpt;
+    *evenSplit = h->spans ? 1u : 0u;
+    return CBM_OK;
 }
 
 int cbm_field_info(int field, const cbm_sizes *s, size_t *elem, size_t *rows, size_t *cols, int *zoned,

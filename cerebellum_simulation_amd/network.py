@@ -170,7 +170,9 @@ class Network:
     def __init__(self, sizes: Sizes, params: ActParams, *, grgo_lag: int = 2, device: int = 0,
                  cells_per_thread: int = 0, block_threads: int = 0, step_mode: int = 0, conn_format: int = 0):
         """step_mode: 0 = pick, 1 = one fused launch per step, 2 = GR and
-        small-population kernels on two streams (cbm_options.stepMode)."""
+        small-population kernels on two streams, 3 = fused with power-of-two
+        GR tiles, 4 = fused with the GR cells split evenly over the CUs
+        (cbm_options.stepMode)."""
         L = _abi.hip_lib()
         o = Options()
         o.grgoLag, o.device, o.grCellsPerThread = grgo_lag, device, cells_per_thread
@@ -296,6 +298,14 @@ class Network:
         m = _abi.hip_lib().cbm_step_mode(self._h)
         check(min(m, 0), "cbm_step_mode")
         return m
+
+    @property
+    def gr_launch(self) -> dict:
+        """GR workgroups of the step launch: blocks, threads, cells_per_thread
+        and even_split (cells split evenly over one block per CU)."""
+        v = [C.c_uint32() for _ in range(4)]
+        check(_abi.hip_lib().cbm_gr_launch(self._h, *(C.byref(x) for x in v)), "cbm_gr_launch")
+        return dict(zip(("blocks", "threads", "cells_per_thread", "even_split"), (x.value for x in v)))
 
     def sync(self) -> None:
         check(_abi.hip_lib().cbm_sync(self._h), "cbm_sync")

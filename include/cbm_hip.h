@@ -139,9 +139,13 @@ typedef struct cbm_options {
      * two-stream step; 0 = pick (256 x 1).  The fused step uses 1024-thread
      * blocks with 2 cells per thread, or 1 when grCellsPerThread is 1. */
     uint32_t grCellsPerThread, grBlockThreads;
-    /* How a step is launched: 0 = pick (fused), 1 = one fused launch per step (block
-     * 0 steps the small populations, the other blocks are GR tiles), 2 = the GR
-     * and small-population kernels on two streams, ordered by events. */
+    /* How a step is launched: 0 = pick (1), 1 = one fused launch per step (block
+     * 0 steps the small populations; the GR cells are split evenly over one
+     * block per remaining CU when that gives each block 1,024 to 5,120 cells,
+     * else into tiles of 1024 x 2 cells), 2 = the GR and small-population
+     * kernels on two streams, ordered by events, 3 = fused with GR tiles
+     * always, 4 = fused with the even split whenever it gives each block at
+     * most 5,120 cells.  All give the same bits. */
     uint32_t stepMode;
     /* Device layout of the GR connectivity: 0 = pick (one 16-byte record per
      * GR when numMF and numGO are at most 2047 and the fan-ins are at most 4
@@ -431,9 +435,16 @@ int cbm_profile_steps(cbm_handle *h, const uint8_t *mf_dev, size_t pitch, int n,
 int cbm_profile_split(cbm_handle *h, const uint8_t *mf_dev, size_t pitch, int n,
                       double *gr_kernel_us, double *small_kernel_us);
 
-/* The launch mode the handle uses (1 = fused, 2 = two streams), or a negative
- * CBM_E_* code. */
+/* The launch mode the handle uses (1, 3 or 4 = fused, 2 = two streams; see
+ * cbm_options.stepMode), or a negative CBM_E_* code. */
 int cbm_step_mode(cbm_handle *h);
+
+/* How the step's GR work is launched: *blocks GR workgroups of *threads
+ * threads with *cellsPerThread cells each at most, *evenSplit = 1 when the
+ * cells are split evenly over the blocks (one per CU beside the small step)
+ * rather than into power-of-two tiles. */
+int cbm_gr_launch(cbm_handle *h, uint32_t *blocks, uint32_t *threads, uint32_t *cellsPerThread,
+                  uint32_t *evenSplit);
 
 /* ---------------------------------------------------------------- recorder */
 /* Populations the spike recorder can keep (bit masks for cbm_set_recorder, and

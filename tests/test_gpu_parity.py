@@ -183,12 +183,18 @@ def test_multistep_from_initial_state(gpu, mode):
     assert net.read("APBUF_GO").any()
 
 
-@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("mode", [1, 2, 3])
 def test_1m_gr_steps(gpu, mode):
     import orc as O
     print(f"[case] c1 parity 12 steps mode={mode}", flush=True)
     st = configs.build_state("c1", seed=0)
     net = Network.from_state(st, step_mode=mode)
+    # mode 1 splits the 1M cells evenly over the CUs, mode 3 uses 2,048-cell tiles
+    lay = net.gr_launch
+    print("c1 GR launch:", lay, flush=True)
+    assert lay["even_split"] == (mode == 1)
+    if mode == 3:
+        assert lay["blocks"] * lay["threads"] * lay["cells_per_thread"] == st.sizes.numGR
     o32 = O.Oracle.from_state(st, precision=32)
     tr = configs.mf_train(st.sizes.numMF, 12, seed=3)
     # warm the GR population so the outputs are not all zero
@@ -254,7 +260,8 @@ SHAPES = [dict(step_mode=1), dict(step_mode=1, cells_per_thread=1), dict(step_mo
           dict(step_mode=2, cells_per_thread=2), dict(step_mode=2, cells_per_thread=4),
           dict(step_mode=2, block_threads=1024, cells_per_thread=1),
           dict(step_mode=2, block_threads=1024, cells_per_thread=2), dict(step_mode=1, conn_format=1),
-          dict(step_mode=2, conn_format=1, cells_per_thread=2)]
+          dict(step_mode=2, conn_format=1, cells_per_thread=2), dict(step_mode=3), dict(step_mode=4),
+          dict(step_mode=4, conn_format=1)]
 
 
 def test_launch_shapes_agree(gpu):
@@ -267,9 +274,10 @@ def test_launch_shapes_agree(gpu):
     tr = torch.from_numpy(configs.mf_train(st.sizes.numMF, 120, seed=9)).cuda()
     err = configs.us_err_drive(120, st.num_zones, t0=0, trial_ms=120, us_ms=30)
     ref = None
-    for shape in SHAPES + ["split"]:
-        net = Network.from_state(st, **(shape if shape != "split" else {}))
-        if shape == "split":
+    for shape in SHAPES + ["split", "split4"]:
+        kw = {"split": {}, "split4": dict(step_mode=4)}.get(shape, shape) if isinstance(shape, str) else shape
+        net = Network.from_state(st, **kw)
+        if isinstance(shape, str):
             net.run_mf_train(tr[:60], err_drive=err[:60])
             net.profile_split(tr[60:])
         else:
