@@ -1235,11 +1235,14 @@ static int profileRun(cbm_handle *h, const uint8_t *mf_dev, size_t pitch, int n,
     const int per = timing == kSplit ? 4 : 2;
     std::vector<hipEvent_t> ev((size_t)per * n + 2);
     for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
+    // a fused step is one kernel: it is timed over the whole region of n
+    // back-to-back launches (per-launch events would add their own overhead)
+    const bool region = timing == kTimeKernel && h->fused();
     HIP_TRY(hipStreamWaitEvent(h->stream, h->evS, 0));
     HIP_TRY(hipEventRecord(ev[(size_t)per * n], h->stream));
     for (int k = 0; k < n; k++) {
         for (int e = 0; e < per; e++) h->ev[e] = ev[(size_t)per * k + e];
-        if ((rc = launchStep(h, mf_dev + (size_t)k * pitch, timing))) break;
+        if ((rc = launchStep(h, mf_dev + (size_t)k * pitch, region ? kNoTiming : timing))) break;
     }
     for (auto &e : h->ev) e = nullptr;
     if (!rc) rc = join(h);
@@ -1247,7 +1250,7 @@ static int profileRun(cbm_handle *h, const uint8_t *mf_dev, size_t pitch, int n,
         HIP_TRY(hipEventRecord(ev[(size_t)per * n + 1], h->stream));
         HIP_TRY(hipEventSynchronize(ev[(size_t)per * n + 1]));
         double tot[2] = {0, 0};
-        for (int k = 0; k < n; k++)
+        for (int k = 0; k < n && !region; k++)
             for (int j = 0; j < per / 2; j++) {
                 float ms = 0;
                 HIP_TRY(hipEventElapsedTime(&ms, ev[(size_t)per * k + 2 * j], ev[(size_t)per * k + 2 * j + 1]));
@@ -1255,7 +1258,7 @@ static int profileRun(cbm_handle *h, const uint8_t *mf_dev, size_t pitch, int n,
             }
         float total = 0;
         HIP_TRY(hipEventElapsedTime(&total, ev[(size_t)per * n], ev[(size_t)per * n + 1]));
-        us[0] = tot[0] * 1000.0 / n;
+        us[0] = (region ? (double)total : tot[0]) * 1000.0 / n;
         us[1] = tot[1] * 1000.0 / n;
         if (stepUs) *stepUs = (double)total * 1000.0 / n;
         HIP_TRY(hipMemcpyAsync(h->dmf, mf_dev + (size_t)(n - 1) * pitch, h->s.numMF, hipMemcpyDeviceToDevice,

@@ -423,8 +423,9 @@ void *cbm_stream(cbm_handle *h);
 /* Time the dominant kernel over the next n steps with HIP events on the
  * stream it is launched on: runs them with the MF train as in
  * cbm_run_mf_train and reports the mean duration of the step kernel (fused
- * mode) or of the GR kernel (two-stream mode), and the mean whole-step
- * duration, in microseconds. */
+ * modes: one event pair around the n back-to-back launches, divided by n) or
+ * of the GR kernel (two-stream mode: an event pair around each launch), and
+ * the mean whole-step duration, in microseconds. */
 int cbm_profile_steps(cbm_handle *h, const uint8_t *mf_dev, size_t pitch, int n,
                       double *kernel_us, double *step_us);
 
