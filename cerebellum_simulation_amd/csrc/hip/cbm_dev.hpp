@@ -205,8 +205,8 @@ __host__ __device__ inline void unpackGrConn(const uint4 c, uint2 *mi, uint2 *gi
 __host__ __device__ constexpr uint32_t grTileLds(uint32_t nt, uint32_t cpt, uint32_t numGO, uint32_t numMF,
                                                  uint32_t numZones, uint32_t numIO)
 {
-    return numGO * 4 + 2 * (nt * cpt / 64 + 1) * 4 + numZones * numIO * 4 + ((numMF >> 5) + 1) * 4 +
-           ((numGO >> 5) + 1) * 4;
+    return numGO * 4 + 2 * (nt * cpt / 64 + 1) * 4 + numZones * numIO * 4 +
+           ((numMF >> 5) + 1 > 64 ? (numMF >> 5) + 1 : 64) * 4 + ((numGO >> 5) + 1 > 64 ? (numGO >> 5) + 1 : 64) * 4;
 }
 
 __host__ __device__ inline uint32_t ilog2(uint32_t x)

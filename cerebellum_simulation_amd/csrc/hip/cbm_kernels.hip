@@ -106,7 +106,9 @@ __device__ __forceinline__ void gr_span_impl(const GrArgs &a, const GrStep &st, 
     constexpr int kSeg = NT * CPT / 64 + 1;  // PF blocks a span can touch (blocks hold >= 64 GRs)
     constexpr int kStage = 2048 / NT;        // spike bytes of each vector per thread held in registers
     const uint32_t nStep = a.numZones * a.numIO;
-    const uint32_t nMFb = (a.numMF >> 5) + 1, nGOb = (a.numGO >> 5) + 1;  // bitmap words
+    // bitmap words: one past the cells, and at least 64 so that every 11-bit
+    // index of a compact record (0x7FF when absent) reads a zero bit
+    const uint32_t nMFb = max((a.numMF >> 5) + 1, 64u), nGOb = max((a.numGO >> 5) + 1, 64u);
     uint32_t *s_goh = smem;                          // [numGO] GR->GO counts of the span
     uint32_t *s_sc = s_goh + a.numGO;                // [kSeg]
     uint32_t *s_bc = s_sc + kSeg;                    // [kSeg]
@@ -227,18 +229,18 @@ __device__ __forceinline__ void gr_span_impl(const GrArgs &a, const GrStep &st, 
         // (>= 32: never), and up to four GO targets with their delays
         uint32_t in[8], dBsp, tg[4], dg[4];
         if constexpr (COMPACT) {
-            // packGrConn: 11-bit indices (0x7FF absent), 6-bit delays (0x3F never)
+            // packGrConn: 11-bit indices (0x7FF absent: a zero bit), 6-bit delays (0x3F never)
             const uint64_t lo = (uint64_t)rec[j].x | (uint64_t)rec[j].y << 32;
             const uint64_t hi = (uint64_t)rec[j].z | (uint64_t)rec[j].w << 32;
             auto f11 = [](uint64_t w, int sh) { return (uint32_t)(w >> sh) & kNoCell11; };
             auto f6 = [](uint64_t w, int sh) { return (uint32_t)(w >> sh) & kNever6; };
-            in[0] = min(f11(lo, 0), a.numMF);
-            in[1] = min(f11(lo, 11), a.numMF);
-            in[2] = min(f11(lo, 22), a.numMF);
-            in[3] = min(f11(lo, 33), a.numMF);
-            in[4] = min(f11(lo, 44), a.numGO);
-            in[5] = min(f11(hi, 0), a.numGO);
-            in[6] = min(f11(hi, 11), a.numGO);
+            in[0] = f11(lo, 0);
+            in[1] = f11(lo, 11);
+            in[2] = f11(lo, 22);
+            in[3] = f11(lo, 33);
+            in[4] = f11(lo, 44);
+            in[5] = f11(hi, 0);
+            in[6] = f11(hi, 11);
             in[7] = a.numGO;
             dBsp = f6(lo, 55);
             tg[0] = f11(hi, 22);
@@ -308,7 +310,7 @@ __device__ __forceinline__ void gr_span_impl(const GrArgs &a, const GrStep &st, 
 
         // PF->PC: plasticity, then this step's 64-GR chunk sum, per zone
         const uint32_t io = gr >> a.log2GRperIO;
-        for (uint32_t z = 0; z < a.numZones; z++) {
+        for (uint32_t z = 0; z < a.numZones; z++) {  // (numZones is 1 in most networks)
             float *wz = a.w + (size_t)z * a.numGR;
             float wv = z == 0 ? w0[j] : wz[gr];
             if (st.doPlast) {
