@@ -181,26 +181,6 @@ __host__ __device__ inline uint4 packGrConn(const uint16_t m[4], const uint16_t 
     return make_uint4((uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32));
 }
 
-// the record back in the general format's registers (16-bit indices with
-// kNoCell, 8-bit delays with kNever)
-__host__ __device__ inline void unpackGrConn(const uint4 c, uint2 *mi, uint2 *gi, uint2 *go, uint32_t *gd, uint32_t *bsp)
-{
-    const uint64_t lo = (uint64_t)c.x | (uint64_t)c.y << 32, hi = (uint64_t)c.z | (uint64_t)c.w << 32;
-    auto ix = [](uint64_t w, int sh) -> uint32_t {
-        const uint32_t v = (uint32_t)(w >> sh) & kNoCell11;
-        return v == kNoCell11 ? (uint32_t)kNoCell : v;
-    };
-    auto dl = [](uint64_t w, int sh) -> uint32_t {
-        const uint32_t v = (uint32_t)(w >> sh) & kNever6;
-        return v == kNever6 ? (uint32_t)kNever : v;
-    };
-    *mi = make_uint2(ix(lo, 0) | ix(lo, 11) << 16, ix(lo, 22) | ix(lo, 33) << 16);
-    *gi = make_uint2(ix(lo, 44) | ix(hi, 0) << 16, ix(hi, 11) | (uint32_t)kNoCell << 16);
-    *go = make_uint2(ix(hi, 22) | ix(hi, 33) << 16, (uint32_t)kNoCell | (uint32_t)kNoCell << 16);
-    *gd = dl(hi, 44) | dl(hi, 50) << 8 | (uint32_t)kNever << 16 | (uint32_t)kNever << 24;
-    *bsp = dl(lo, 55);
-}
-
 // Bytes of dynamic LDS a GR tile or span of nt threads x cpt cells uses.
 __host__ __device__ constexpr uint32_t grTileLds(uint32_t nt, uint32_t cpt, uint32_t numGO, uint32_t numMF,
                                                  uint32_t numZones, uint32_t numIO)
