@@ -82,6 +82,33 @@ __device__ __forceinline__ void gr_fold(const GrArgs &a, const GrStep &st, const
     }
 }
 
+// Streaming accesses of the per-GR arrays: non-temporal, so the tiles' tens
+// of MB pass through the L2 without evicting what the small-population block
+// and the staging reads keep there (the small step's round trips then hit the
+// cache instead of waiting behind the tiles' HBM traffic).
+template <typename T>
+__device__ __forceinline__ T ntl(const T *p)
+{
+    return __builtin_nontemporal_load(p);
+}
+__device__ __forceinline__ uint2 ntl(const uint2 *p)
+{
+    typedef uint32_t v2u __attribute__((ext_vector_type(2)));
+    const v2u r = __builtin_nontemporal_load(reinterpret_cast<const v2u *>(p));
+    return make_uint2(r.x, r.y);
+}
+__device__ __forceinline__ uint4 ntl(const uint4 *p)
+{
+    typedef uint32_t v4u __attribute__((ext_vector_type(4)));
+    const v4u r = __builtin_nontemporal_load(reinterpret_cast<const v4u *>(p));
+    return make_uint4(r.x, r.y, r.z, r.w);
+}
+template <typename T>
+__device__ __forceinline__ void nts(T *p, T v)
+{
+    __builtin_nontemporal_store(v, p);
+}
+
 // 1 when bit idx of the LDS bitmap bm is set
 __device__ __forceinline__ uint32_t bit_at(const uint32_t *bm, uint32_t idx) { return (bm[idx >> 5] >> (idx & 31u)) & 1u; }
 
@@ -179,22 +206,22 @@ __device__ __forceinline__ void gr_span_impl(const GrArgs &a, const GrStep &st, 
 #pragma unroll
     for (int j = 0; j < CPT; j++) {
         const uint32_t gr = cellOf(j);
-        v[j] = a.v[gr];
-        gk[j] = a.gKCa[gr];
-        th[j] = a.thr[gr];
-        gE[j] = a.gE[gr];
-        gI[j] = a.gI[gr];
-        ab[j] = a.apBuf[gr];
+        v[j] = ntl(&a.v[gr]);
+        gk[j] = ntl(&a.gKCa[gr]);
+        th[j] = ntl(&a.thr[gr]);
+        gE[j] = ntl(&a.gE[gr]);
+        gI[j] = ntl(&a.gI[gr]);
+        ab[j] = ntl(&a.apBuf[gr]);
         if constexpr (COMPACT) {
-            rec[j] = a.conn[gr];
+            rec[j] = ntl(&a.conn[gr]);
         } else {
-            mi_[j] = a.mfIn[gr];
-            gi_[j] = a.goIn[gr];
-            go_[j] = a.goOut[gr];
-            gd_[j] = a.goOutDly[gr];
-            bsp_[j] = a.bspDly[gr];
+            mi_[j] = ntl(&a.mfIn[gr]);
+            gi_[j] = ntl(&a.goIn[gr]);
+            go_[j] = ntl(&a.goOut[gr]);
+            gd_[j] = ntl(&a.goOutDly[gr]);
+            bsp_[j] = ntl(&a.bspDly[gr]);
         }
-        w0[j] = a.w[gr];
+        w0[j] = ntl(&a.w[gr]);
     }
 
     // LDS writes of what was loaded first, before any branch or loop that
@@ -207,7 +234,7 @@ __device__ __forceinline__ void gr_span_impl(const GrArgs &a, const GrStep &st, 
     }
     if (st.doPlast) {
 #pragma unroll
-        for (int j = 0; j < CPT; j++) hist[j] = a.hist[cellOf(j)];
+        for (int j = 0; j < CPT; j++) hist[j] = ntl(&a.hist[cellOf(j)]);
     }
     if (st.doPlast && (uint32_t)tid < nStep) s_step[tid] = stepOf(tid, tm0);
     for (uint32_t i = tid; i < a.numGO; i += NT) s_goh[i] = 0;
@@ -289,19 +316,19 @@ __device__ __forceinline__ void gr_span_impl(const GrArgs &a, const GrStep &st, 
         e = e * a.decMF + a.incMF * (cm);
         i = i * a.decGO + a.incGO * (cg);
 
-        a.v[gr] = vv;
-        a.thr[gr] = thN;
-        a.gKCa[gr] = gkN;
-        a.apBuf[gr] = b;
-        a.gE[gr] = e;
-        a.gI[gr] = i;
+        nts(&a.v[gr], vv);
+        nts(&a.thr[gr], thN);
+        nts(&a.gKCa[gr], gkN);
+        nts(&a.apBuf[gr], b);
+        nts(&a.gE[gr], e);
+        nts(&a.gI[gr], i);
 
         uint32_t bit39 = 0;
         if (st.doPlast) {
             const uint64_t h = hist[j];
             bit39 = (uint32_t)(h >> 39) & 1u;
             // updateGRHistory (kernels.cu:244-245): mask 0x1F is hard-coded there
-            a.hist[gr] = (h << 1) | (uint64_t)(((b & 0x0000001Fu) > 0) * 0x00000001u);
+            nts(&a.hist[gr], (h << 1) | (uint64_t)(((b & 0x0000001Fu) > 0) * 0x00000001u));
         }
 
         // delayed spike towards BC/PC/SC (kernels.cu:256,269)
@@ -312,13 +339,13 @@ __device__ __forceinline__ void gr_span_impl(const GrArgs &a, const GrStep &st, 
         const uint32_t io = gr >> a.log2GRperIO;
         for (uint32_t z = 0; z < a.numZones; z++) {  // (numZones is 1 in most networks)
             float *wz = a.w + (size_t)z * a.numGR;
-            float wv = z == 0 ? w0[j] : wz[gr];
+            float wv = z == 0 ? w0[j] : ntl(&wz[gr]);
             if (st.doPlast) {
                 const float plastStep = s_step[z * a.numIO + io];
                 wv = wv + (bit39 > 0) * plastStep;
                 wv = (wv > 0) * wv;
                 wv = (wv > 1) + (wv <= 1) * wv;
-                wz[gr] = wv;
+                nts(&wz[gr], wv);
             }
             float x = 0.0f;
             if (bal != 0) x = wave_sum_lane0(wv * d);
