@@ -82,9 +82,9 @@ __device__ __forceinline__ void gr_fold(const GrArgs &a, const GrStep &st, const
     }
 }
 
-// Streaming accesses of the per-GR arrays: non-temporal, so the tiles' tens
-// of MB pass through the L2 without evicting what the small-population block
-// and the staging reads keep there (the small step's round trips then hit the
+// Streaming accesses of the per-GR arrays, non-temporal: the tiles' tens of
+// MB pass through the L2 without evicting what the small-population block and
+// the staging reads keep there (the small step's round trips then hit the
 // cache instead of waiting behind the tiles' HBM traffic).
 template <typename T>
 __device__ __forceinline__ T ntl(const T *p)
@@ -107,6 +107,23 @@ template <typename T>
 __device__ __forceinline__ void nts(T *p, T v)
 {
     __builtin_nontemporal_store(v, p);
+}
+// NT: non-temporal or ordinary.  The compact-record build streams
+// non-temporally (networks of up to ~2M GR, whose GR pass is short enough for
+// the small-population block's latency to matter); the five-array build, used
+// for the largest networks, keeps the ordinary policy, under which the MALL
+// keeps more of their streams (4M GR: 70.9 against 75.1 us per step).
+template <bool NT, typename T>
+__device__ __forceinline__ T gld(const T *p)
+{
+    if constexpr (NT) return ntl(p);
+    else return *p;
+}
+template <bool NT, typename T>
+__device__ __forceinline__ void gst(T *p, T v)
+{
+    if constexpr (NT) nts(p, v);
+    else *p = v;
 }
 
 // 1 when bit idx of the LDS bitmap bm is set
@@ -206,22 +223,22 @@ __device__ __forceinline__ void gr_span_impl(const GrArgs &a, const GrStep &st, 
 #pragma unroll
     for (int j = 0; j < CPT; j++) {
         const uint32_t gr = cellOf(j);
-        v[j] = ntl(&a.v[gr]);
-        gk[j] = ntl(&a.gKCa[gr]);
-        th[j] = ntl(&a.thr[gr]);
-        gE[j] = ntl(&a.gE[gr]);
-        gI[j] = ntl(&a.gI[gr]);
-        ab[j] = ntl(&a.apBuf[gr]);
+        v[j] = gld<COMPACT>(&a.v[gr]);
+        gk[j] = gld<COMPACT>(&a.gKCa[gr]);
+        th[j] = gld<COMPACT>(&a.thr[gr]);
+        gE[j] = gld<COMPACT>(&a.gE[gr]);
+        gI[j] = gld<COMPACT>(&a.gI[gr]);
+        ab[j] = gld<COMPACT>(&a.apBuf[gr]);
         if constexpr (COMPACT) {
-            rec[j] = ntl(&a.conn[gr]);
+            rec[j] = gld<COMPACT>(&a.conn[gr]);
         } else {
-            mi_[j] = ntl(&a.mfIn[gr]);
-            gi_[j] = ntl(&a.goIn[gr]);
-            go_[j] = ntl(&a.goOut[gr]);
-            gd_[j] = ntl(&a.goOutDly[gr]);
-            bsp_[j] = ntl(&a.bspDly[gr]);
+            mi_[j] = gld<COMPACT>(&a.mfIn[gr]);
+            gi_[j] = gld<COMPACT>(&a.goIn[gr]);
+            go_[j] = gld<COMPACT>(&a.goOut[gr]);
+            gd_[j] = gld<COMPACT>(&a.goOutDly[gr]);
+            bsp_[j] = gld<COMPACT>(&a.bspDly[gr]);
         }
-        w0[j] = ntl(&a.w[gr]);
+        w0[j] = gld<COMPACT>(&a.w[gr]);
     }
 
     // LDS writes of what was loaded first, before any branch or loop that
@@ -234,7 +251,7 @@ __device__ __forceinline__ void gr_span_impl(const GrArgs &a, const GrStep &st, 
     }
     if (st.doPlast) {
 #pragma unroll
-        for (int j = 0; j < CPT; j++) hist[j] = ntl(&a.hist[cellOf(j)]);
+        for (int j = 0; j < CPT; j++) hist[j] = gld<COMPACT>(&a.hist[cellOf(j)]);
     }
     if (st.doPlast && (uint32_t)tid < nStep) s_step[tid] = stepOf(tid, tm0);
     for (uint32_t i = tid; i < a.numGO; i += NT) s_goh[i] = 0;
@@ -316,19 +333,19 @@ __device__ __forceinline__ void gr_span_impl(const GrArgs &a, const GrStep &st, 
         e = e * a.decMF + a.incMF * (cm);
         i = i * a.decGO + a.incGO * (cg);
 
-        nts(&a.v[gr], vv);
-        nts(&a.thr[gr], thN);
-        nts(&a.gKCa[gr], gkN);
-        nts(&a.apBuf[gr], b);
-        nts(&a.gE[gr], e);
-        nts(&a.gI[gr], i);
+        gst<COMPACT>(&a.v[gr], vv);
+        gst<COMPACT>(&a.thr[gr], thN);
+        gst<COMPACT>(&a.gKCa[gr], gkN);
+        gst<COMPACT>(&a.apBuf[gr], b);
+        gst<COMPACT>(&a.gE[gr], e);
+        gst<COMPACT>(&a.gI[gr], i);
 
         uint32_t bit39 = 0;
         if (st.doPlast) {
             const uint64_t h = hist[j];
             bit39 = (uint32_t)(h >> 39) & 1u;
             // updateGRHistory (kernels.cu:244-245): mask 0x1F is hard-coded there
-            nts(&a.hist[gr], (h << 1) | (uint64_t)(((b & 0x0000001Fu) > 0) * 0x00000001u));
+            gst<COMPACT>(&a.hist[gr], (h << 1) | (uint64_t)(((b & 0x0000001Fu) > 0) * 0x00000001u));
         }
 
         // delayed spike towards BC/PC/SC (kernels.cu:256,269)
@@ -339,13 +356,13 @@ __device__ __forceinline__ void gr_span_impl(const GrArgs &a, const GrStep &st, 
         const uint32_t io = gr >> a.log2GRperIO;
         for (uint32_t z = 0; z < a.numZones; z++) {  // (numZones is 1 in most networks)
             float *wz = a.w + (size_t)z * a.numGR;
-            float wv = z == 0 ? w0[j] : ntl(&wz[gr]);
+            float wv = z == 0 ? w0[j] : gld<COMPACT>(&wz[gr]);
             if (st.doPlast) {
                 const float plastStep = s_step[z * a.numIO + io];
                 wv = wv + (bit39 > 0) * plastStep;
                 wv = (wv > 0) * wv;
                 wv = (wv > 1) + (wv <= 1) * wv;
-                nts(&wz[gr], wv);
+                gst<COMPACT>(&wz[gr], wv);
             }
             float x = 0.0f;
             if (bal != 0) x = wave_sum_lane0(wv * d);
