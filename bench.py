@@ -240,7 +240,7 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=400)
     ap.add_argument("--cpu-steps", type=int, default=0, help="0 = pick a ~15 s sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--step-mode", type=int, default=0, help="0 pick, 1 fused launch (GR tiles), 2 two streams, 3 = 1, 4 fused with an even GR split over the CUs")
+    ap.add_argument("--step-mode", type=int, default=0, help="0 pick, 1 fused launch (even GR split when it fits, else tiles), 2 two streams, 3 fused with GR tiles, 4 fused with the even split")
     ap.add_argument("--cells-per-thread", type=int, default=0)
     ap.add_argument("--block-threads", type=int, default=0)
     ap.add_argument("--batch", default="full", choices=["full", "small", "none"],

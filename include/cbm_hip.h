@@ -140,11 +140,12 @@ typedef struct cbm_options {
      * blocks with 2 cells per thread, or 1 when grCellsPerThread is 1. */
     uint32_t grCellsPerThread, grBlockThreads;
     /* How a step is launched: 0 = pick (1), 1 = one fused launch per step (block
-     * 0 steps the small populations, the other blocks are GR tiles of 1024 x 2
-     * cells), 2 = the GR and small-population kernels on two streams, ordered
-     * by events, 3 = the same as 1, 4 = fused with the GR cells split evenly
-     * over one block per remaining CU (at most 5,120 cells per block, else an
-     * error).  All give the same bits. */
+     * 0 steps the small populations; the GR cells are split evenly over one
+     * block per remaining CU when that gives each block 1,024 to 5,120 cells,
+     * else into tiles of 1024 x 2 cells), 2 = the GR and small-population
+     * kernels on two streams, ordered by events, 3 = fused with GR tiles
+     * always, 4 = fused with the even split whenever it gives each block at
+     * most 5,120 cells (else an error).  All give the same bits. */
     uint32_t stepMode;
     /* Device layout of the GR connectivity: 0 = pick (one 16-byte record per
      * GR when numMF and numGO are at most 2047 and the fan-ins are at most 4
