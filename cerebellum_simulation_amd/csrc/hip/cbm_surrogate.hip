@@ -416,6 +416,7 @@ __global__ void __launch_bounds__(256) colsum_kernel(const u16 *G, size_t ldg, i
     if (c >= C) return;
     const int r0 = blockIdx.y * rowsPerChunk, r1 = min(R, r0 + rowsPerChunk);
     float s = 0.f;
+#pragma unroll 8
     for (int r = r0; r < r1; r++) s += bf2f(G[(size_t)r * ldg + c]);
     part[(size_t)blockIdx.y * C + c] = s;
 }
@@ -456,6 +457,8 @@ __global__ void __launch_bounds__(256) reduce_kernel(const float *part, int spli
     const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     float s = 0.f;
+    // in split order; unrolled so that eight loads are in flight at a time
+#pragma unroll 8
     for (int k = 0; k < splits; k++) s += part[(size_t)k * n + i];
     out[i] = s;
 }
