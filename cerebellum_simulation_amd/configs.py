@@ -251,13 +251,14 @@ BATCH_MIX = [("b16k", 64), ("b65k", 96), ("b262k", 64), ("b1m", 32)]
 
 
 def build_state(name: str, seed: int = 0, act_overrides: dict | None = None, *, gogo_inhibition: bool = True,
-                **gen_overrides):
+                gogo_coup=None, **gen_overrides):
     """NetState of a named configuration or batch size class (GR inputs
-    drawn with `seed`); gogo_inhibition=False leaves the GO-GO GABA matrix empty."""
+    drawn with `seed`); gogo_inhibition=False leaves the GO-GO GABA matrix
+    empty; gogo_coup replaces the GO-GO coupling matrix ([(dx, dy, p)])."""
     from .network import NetState
     con, gen = CONFIGS[name] if name in CONFIGS else BATCH_CLASSES[name]
     gen = {**gen, **gen_overrides}
-    con_text = conparams_text(**con, gogo_gaba=None if gogo_inhibition else [])
+    con_text = conparams_text(**con, gogo_gaba=None if gogo_inhibition else [], gogo_coup=gogo_coup)
     return NetState(act_params_text(act_overrides), con_text, innet_seed=seed, **gen)
 
 

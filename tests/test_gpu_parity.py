@@ -183,6 +183,32 @@ def test_multistep_from_initial_state(gpu, mode):
     assert net.read("APBUF_GO").any()
 
 
+@pytest.mark.parametrize("mode", [1, 2])
+def test_go_coupling_steps(gpu, mode):
+    """GO-GO electrical coupling on (ratio 0.05, four neighbours with
+    probability 1): the sequential threshold update of
+    CBM_CORE_LIB_MGPU_DEV/src/innetmodules/innet.cpp:969-975 against the
+    binary32 oracle."""
+    import orc as O
+    print(f"[case] GO coupling c0 300 steps mode={mode}", flush=True)
+    coup = [(1, 0, 1.0), (0, 1, 1.0), (-1, 0, 1.0), (0, -1, 1.0)]
+    st = configs.build_state("c0", seed=0, act_overrides={"coupleRiRjRatioGO": "0.05"}, gogo_coup=coup)
+    net = Network.from_state(st, step_mode=mode)
+    o32 = O.Oracle.from_state(st, precision=32)
+    assert (net.read("NUMP_GO_COUP_IN_GOGO") > 0).all()
+    steps = 300
+    tr = configs.mf_train(st.sizes.numMF, steps, seed=5, t0=200, trial_ms=600, cs_on=100, cs_off=300)
+    import torch
+    dtr = torch.from_numpy(tr).cuda()
+    for k0 in range(0, steps, 100):
+        net.run_mf_train(dtr[k0:k0 + 100])
+        o32.run_mf_train(tr[k0:k0 + 100])
+        bad = compare(net, o32, 1)
+        print(f"step {k0 + 100}: {bad}", flush=True)
+        assert_parity(bad, where=f"coupled c0 step {k0 + 100}")
+    assert net.read("APBUF_GO").any()
+
+
 @pytest.mark.parametrize("mode", [1, 2, 3])
 def test_1m_gr_steps(gpu, mode):
     import orc as O
