@@ -266,7 +266,8 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the network step runs "
-              "on a single MI355X; see DESIGN.md for how it would shard)", file=sys.stderr)
+              "on a single MI355X; a pack of independent networks would spread over GPUs by GR count, "
+              "one process per GPU)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
