@@ -147,6 +147,8 @@ def hip_lib() -> C.CDLL:
         L.cbm_step.argtypes = [P, I]
         L.cbm_run_mf_train.argtypes = [P, V, S, I, V]
         L.cbm_sync.argtypes = [P]
+        L.cbm_step_io_begin.argtypes = [P, V, I]
+        L.cbm_step_io_end.argtypes = [P, V]
         L.cbm_stream.argtypes = [P]
         L.cbm_stream.restype = P
         L.cbm_profile_steps.argtypes = [P, V, S, I, C.POINTER(C.c_double), C.POINTER(C.c_double)]

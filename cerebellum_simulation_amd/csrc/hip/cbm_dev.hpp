@@ -124,6 +124,10 @@ struct SmallStep {
     uint32_t slotPrev, slotT2, slotG, slotNext;  // ring slots of t-1, t-2, t-grgoLag, t+1
     uint32_t popHist;          // t % tsPerPopHistBinPC == 0: PC population histogram + MF->NC weights
     uint32_t resetMFHist;      // t % numTSinMFHist == 0
+    // this step's spikes of population ioPop (CBM_REC_* order) are stored
+    // into ioOut, pinned host memory (nullptr: none)
+    uint8_t *ioOut;
+    uint32_t ioPop;
 };
 
 // One network of a batch: its constant arguments, the buffers the per-step

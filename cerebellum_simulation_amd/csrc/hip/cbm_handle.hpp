@@ -70,8 +70,11 @@ struct cbm_handle {
     bool skipFold = true;       // the ring slot of the last step holds host-written PF->PC sums
     uint8_t *dmf = nullptr;
     uint8_t *pinnedMF = nullptr;
-    // cbm_step_io_begin/_end: pinned read-back of one population's spikes
+    // cbm_step_io_begin/_end: the small step stores one population's spikes
+    // into pinnedIO (coherent pinned memory); ioOut is set for that one launch
     uint8_t *pinnedIO = nullptr;
+    uint8_t *ioOut = nullptr;
+    uint32_t ioPop = 0;
     uint32_t ioBytes = 0;
     bool ioPending = false;
     uint32_t anyCoupling = 0;

@@ -1098,14 +1098,23 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
         for (uint32_t k = 0; k < bins; k++) F<uint32_t>(a, CBM_F_HIST_PC_POPACT)[(size_t)z * bins + k] = hist[k];
         for (uint32_t k = 0; k < 2; k++) F<uint8_t>(a, CBM_F_MFNC_NOLTP_NOLTD)[z * 2 + k] = (uint8_t)flags[k];
     }
-    // spike recorder (row t mod recCap)
-    if (a.rec) {
-        uint8_t *row = a.rec + (size_t)(st.t % a.recCap) * a.recRow;
+    // spike recorder (row t mod recCap), and one population's spikes for the
+    // host's step-and-read call straight into its pinned buffer
+    {
         const uint8_t *src[6] = {s_apPC, s_apBC, s_apSC, s_apIO, s_apNC, s_apGO};
         const uint32_t cnt[6] = {nZ * s.numPC, nZ * s.numBC, s.numSC, nZ * s.numIO, nZ * s.numNC, s.numGO};
-        for (int k = 0; k < 6; k++) {
-            if (a.recOff[k] == 0xFFFFFFFFu) continue;
-            for (uint32_t i = tid; i < cnt[k]; i += nThr) gp(row)[a.recOff[k] + i] = src[k][i];
+        if (a.rec) {
+            uint8_t *row = a.rec + (size_t)(st.t % a.recCap) * a.recRow;
+            for (int k = 0; k < 6; k++) {
+                if (a.recOff[k] == 0xFFFFFFFFu) continue;
+                for (uint32_t i = tid; i < cnt[k]; i += nThr) gp(row)[a.recOff[k] + i] = src[k][i];
+            }
+        }
+        if (st.ioOut) {
+#pragma unroll
+            for (int k = 0; k < 6; k++)
+                if (st.ioPop == (uint32_t)k)
+                    for (uint32_t i = tid; i < cnt[k]; i += nThr) st.ioOut[i] = src[k][i];
         }
     }
     // the ring slot of step t+1 is the slot PC/BC/SC read above (t-2): clear it now
@@ -1250,6 +1259,8 @@ __device__ __forceinline__ void batch_small(const BatchNet &d, uint32_t kk, floa
     SmallStep st;
     st.mf = d.mf + (size_t)kk * d.pitch;
     st.t = t;
+    st.ioOut = nullptr;
+    st.ioPop = 0;
     if (threadIdx.x < (uint32_t)kMaxZones) {
         const uint32_t z = threadIdx.x;
         float e = kk == 0 ? d.errDrive0[z] : 0.0f;

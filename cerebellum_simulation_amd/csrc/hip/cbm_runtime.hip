@@ -247,7 +247,7 @@ int cbm_create(const cbm_sizes *sizes, const cbm_act_params *params, const cbm_o
         const cbm_sizes &z = h->s;
         const size_t io = std::max({(size_t)z.numZones * z.numPC, (size_t)z.numZones * z.numBC, (size_t)z.numSC,
                                     (size_t)z.numZones * z.numIO, (size_t)z.numZones * z.numNC, (size_t)z.numGO});
-        e = hipHostMalloc((void **)&h->pinnedIO, io, hipHostMallocDefault);
+        e = hipHostMalloc((void **)&h->pinnedIO, io, hipHostMallocCoherent);
     }
     if (e != hipSuccess) {
         std::string m = hipGetErrorString(e);
@@ -1067,6 +1067,8 @@ static int launchStep(cbm_handle *h, const uint8_t *mf, int timing)
     st.slotNext = (uint32_t)((t + 1) % kRing);
     st.popHist = (t % p.tsPerPopHistBinPC) == 0;
     st.resetMFHist = (t % p.numTSinMFHist) == 0;
+    st.ioOut = h->ioOut;
+    st.ioPop = h->ioPop;
     h->skipFold = false;
 
     if (timing == kSplit) {
@@ -1124,30 +1126,23 @@ int cbm_step_io_begin(cbm_handle *h, const uint8_t *mf, int population)
     if (!h || !mf) return fail(CBM_E_ARG, "null argument");
     if (!h->committed) return fail(CBM_E_STATE, "cbm_commit has not been called");
     if (h->ioPending) return fail(CBM_E_STATE, "cbm_step_io_end has not been called for the last step");
-    int field;
-    switch (population) {
-    case CBM_REC_PC: field = CBM_F_AP_PC; break;
-    case CBM_REC_BC: field = CBM_F_AP_BC; break;
-    case CBM_REC_SC: field = CBM_F_AP_SC; break;
-    case CBM_REC_IO: field = CBM_F_AP_IO; break;
-    case CBM_REC_NC: field = CBM_F_AP_NC; break;
-    case CBM_REC_GO: field = CBM_F_AP_GO; break;
-    default: return fail(CBM_E_ARG, "cbm_step_io_begin: population must be PC, BC, SC, IO, NC or GO");
-    }
+    if (population < CBM_REC_PC || population > CBM_REC_GO)
+        return fail(CBM_E_ARG, "cbm_step_io_begin: population must be PC, BC, SC, IO, NC or GO");
     HIP_TRY(hipSetDevice(h->o.device));
     int rc = syncAll(h);  // the pinned stages of the last step are free again
     if (rc) return rc;
     std::memcpy(h->pinnedMF, mf, h->s.numMF);
     HIP_TRY(hipMemcpyAsync(h->dmf, h->pinnedMF, h->s.numMF, hipMemcpyHostToDevice, h->stream));
     HIP_TRY(hipEventRecord(h->evG, h->stream));
-    if ((rc = launchStep(h, h->dmf, kNoTiming))) return rc;
+    // the step's small-population kernel stores the population's spikes (the
+    // zones one after another) into pinnedIO: no copy after the kernel
+    h->ioOut = h->pinnedIO;
+    h->ioPop = (uint32_t)population;
+    rc = launchStep(h, h->dmf, kNoTiming);
+    h->ioOut = nullptr;
+    if (rc) return rc;
     if ((rc = join(h))) return rc;
-    // the zones of a population sit one after another: one copy for all of them
-    const uint32_t bytes = recCells(h->s, population);
-    const uint8_t *src = parityField(h, field, 0);
-    if (!src) src = static_cast<const uint8_t *>(h->dfield[field]);
-    HIP_TRY(hipMemcpyAsync(h->pinnedIO, src, bytes, hipMemcpyDeviceToHost, h->stream));
-    h->ioBytes = bytes;
+    h->ioBytes = recCells(h->s, population);
     h->ioPending = true;
     return CBM_OK;
 }

@@ -311,6 +311,28 @@ class Network:
     def sync(self) -> None:
         check(_abi.hip_lib().cbm_sync(self._h), "cbm_sync")
 
+    STEP_IO_POPULATIONS = ("PC", "BC", "SC", "IO", "NC", "GO")
+
+    def step_io(self, mf: np.ndarray, population: str = "NC") -> np.ndarray:
+        """One step on the MF bytes `mf` (host, numMF) and this step's spikes
+        of `population` (zones one after another), through
+        cbm_step_io_begin/_end: the call pattern of a closed loop, replacing
+        updateMFInput, calcActivity and exportAPNC per zone
+        (CBM_CORE_LIB_MGPU_DEV/src/interface/cbmsimcore.cpp:312, :139;
+        Matthew_Sim_CBM/src/microzone.cpp:31-33)."""
+        L = _abi.hip_lib()
+        mf = np.ascontiguousarray(mf, dtype=np.uint8)
+        if mf.size < self.sizes.numMF:
+            raise ValueError(f"step_io: {mf.size} MF bytes, the network has {self.sizes.numMF}")
+        k = self.STEP_IO_POPULATIONS.index(population)
+        s = self.sizes
+        n = {0: s.numZones * s.numPC, 1: s.numZones * s.numBC, 2: s.numSC, 3: s.numZones * s.numIO,
+             4: s.numZones * s.numNC, 5: s.numGO}[k]
+        out = np.empty(n, dtype=np.uint8)
+        check(L.cbm_step_io_begin(self._h, mf.ctypes.data, k), "cbm_step_io_begin")
+        check(L.cbm_step_io_end(self._h, out.ctypes.data), "cbm_step_io_end")
+        return out
+
     def device_bytes(self) -> int:
         b = C.c_size_t()
         check(_abi.hip_lib().cbm_device_bytes(self._h, C.byref(b)), "cbm_device_bytes")

@@ -400,8 +400,9 @@ int cbm_step(cbm_handle *h, int n);
  * with updateMFInput, calcActivity and exportAPNC per zone
  * (Matthew_Sim_CBM/src/simthread.cpp:105-112, src/microzone.cpp:31-33), with
  * one host synchronisation instead of one per call.  _begin queues the copy
- * in, the step and the copy out (population: CBM_REC_PC, _BC, _SC, _IO, _NC
- * or _GO, not _PF) and returns; the host may work meanwhile (the next step's
+ * in and the step, whose small-population kernel stores the population's
+ * spikes into pinned host memory (population: CBM_REC_PC, _BC, _SC, _IO, _NC
+ * or _GO, not _PF), and returns; the host may work meanwhile (the next step's
  * draws, say).  _end waits and writes the spikes, zones stacked
  * ([numZones][cells] for the microzone populations), to out.  Every _begin
  * needs its _end before the next _begin. */

@@ -209,6 +209,36 @@ def test_go_coupling_steps(gpu, mode):
     assert net.read("APBUF_GO").any()
 
 
+@pytest.mark.parametrize("mode", [1, 2])
+def test_step_io_populations(gpu, mode):
+    """cbm_step_io_begin/_end: the spikes the small-population kernel stores
+    straight into pinned host memory equal the state fields after the step,
+    for every population, on a two-zone network; the step itself equals the
+    oracle's."""
+    import orc as O
+    print(f"[case] step_io t8k2z mode={mode}", flush=True)
+    st = configs.build_state("t8k2z", seed=0)
+    net = Network.from_state(st, step_mode=mode)
+    o32 = O.Oracle.from_state(st, precision=32)
+    s = st.sizes
+    steps = 60
+    tr = configs.mf_train(s.numMF, steps, seed=3, t0=0, trial_ms=600, cs_on=0, cs_off=60)
+    fields = {"PC": "AP_PC", "BC": "AP_BC", "SC": "AP_SC", "IO": "AP_IO", "NC": "AP_NC", "GO": "AP_GO"}
+    zoned = {"PC", "BC", "IO", "NC"}
+    seen = {p: 0 for p in fields}
+    for k in range(steps):
+        pop = Network.STEP_IO_POPULATIONS[k % 6]
+        out = net.step_io(tr[k], pop)
+        o32.update_mf_input(tr[k])
+        o32.calc_activity(1)
+        zs = range(s.numZones) if pop in zoned else [0]
+        want = np.concatenate([net.read(fields[pop], z).astype(np.uint8).ravel() for z in zs])
+        assert np.array_equal(out, want), f"step {k} {pop}"
+        seen[pop] += int(out.sum())
+    assert_parity(compare(net, o32, s.numZones), where=f"t8k2z after {steps} step_io steps")
+    assert sum(seen.values()) > 0, seen  # the read-back saw spikes (NC spikes within the first 60 ms)
+
+
 @pytest.mark.parametrize("mode", [1, 2, 3])
 def test_1m_gr_steps(gpu, mode):
     import orc as O
