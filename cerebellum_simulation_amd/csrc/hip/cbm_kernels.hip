@@ -126,6 +126,17 @@ __device__ __forceinline__ void gst(T *p, T v)
     else *p = v;
 }
 
+// Element i of a per-GR array as its base plus a 32-bit byte offset: the
+// addressing form with a scalar base and one vector offset, so a cell's loads
+// and stores share one offset register instead of a 64-bit address each
+// (byte offsets stay below 2^32 for every supported network size).
+template <typename T>
+__device__ __forceinline__ T *el(T *base, uint32_t i)
+{
+    typedef typename std::conditional<std::is_const<T>::value, const char, char>::type B;
+    return reinterpret_cast<T *>(reinterpret_cast<B *>(base) + (uint32_t)(i * (uint32_t)sizeof(T)));
+}
+
 // 1 when bit idx of the LDS bitmap bm is set
 __device__ __forceinline__ uint32_t bit_at(const uint32_t *bm, uint32_t idx) { return (bm[idx >> 5] >> (idx & 31u)) & 1u; }
 
@@ -144,7 +155,7 @@ __device__ __forceinline__ uint32_t bit_at(const uint32_t *bm, uint32_t idx) { r
 // earlier one, the staging waits only for the spike bytes, and each cell's
 // update starts as soon as its own loads have arrived.  FULL: count ==
 // NT*CPT; COMPACT: one 16-byte connectivity record per GR (a.conn).
-template <int CPT, int NT, bool FULL, bool COMPACT>
+template <int CPT, int NT, bool FULL, bool COMPACT, bool PLAST>
 __device__ __forceinline__ void gr_span_impl(const GrArgs &a, const GrStep &st, const uint32_t first,
                                              const uint32_t count, const uint32_t rank, const uint32_t ranks,
                                              uint32_t *smem)
@@ -201,7 +212,7 @@ __device__ __forceinline__ void gr_span_impl(const GrArgs &a, const GrStep &st, 
     // what the registers do not hold (large networks only), before the cells'
     // loads: a loop that loads makes the compiler wait for every load in
     // flight before it
-    if (nStep > (uint32_t)NT && st.doPlast)
+    if (PLAST && nStep > (uint32_t)NT)
         for (uint32_t i = tid + NT; i < nStep; i += NT) s_step[i] = stepOf(i, st.plastTimer[i]);
     for (uint32_t base = kStage * NT + wbase; base < 32 * nMFb; base += NT)  // wave-uniform
         stage64(s_mfb, nMFb, base, base + lane < a.numMF && st.mf[base + lane] != 0);
@@ -214,34 +225,40 @@ __device__ __forceinline__ void gr_span_impl(const GrArgs &a, const GrStep &st, 
     uint4 rec[CPT];                       // COMPACT
     uint2 mi_[CPT], gi_[CPT], go_[CPT];   // !COMPACT
     uint32_t gd_[CPT], bsp_[CPT];         // !COMPACT
-    uint64_t hist[CPT] = {};
+    uint64_t hist[CPT] = {};              // PLAST
     auto cellOf = [&](int j) -> uint32_t {
         const uint32_t o = (uint32_t)(j * NT + tid);
         return first + (FULL ? o : min(o, count - 1u));
     };
     // (unconditional loads: cells past the span read its last cell, and the
     // update below skips them; a load under a branch lets the compiler move
-    // work on its value into the branch and wait for it there)
+    // work on its value into the branch and wait for it there).  Cell j's
+    // loads, history words included, are all issued before cell j+1's: loads
+    // return in issue order, so cell j's update then waits only for its own
+    // loads while the later cells' are still in flight.
 #pragma unroll
     for (int j = 0; j < CPT; j++) {
         const uint32_t gr = cellOf(j);
-        v[j] = gld<COMPACT>(&a.v[gr]);
-        gk[j] = gld<COMPACT>(&a.gKCa[gr]);
-        th[j] = gld<COMPACT>(&a.thr[gr]);
-        gE[j] = gld<COMPACT>(&a.gE[gr]);
-        gI[j] = gld<COMPACT>(&a.gI[gr]);
-        ab[j] = gld<COMPACT>(&a.apBuf[gr]);
+        __builtin_amdgcn_sched_barrier(0);
+        v[j] = gld<COMPACT>(el(a.v, gr));
+        gk[j] = gld<COMPACT>(el(a.gKCa, gr));
+        th[j] = gld<COMPACT>(el(a.thr, gr));
+        gE[j] = gld<COMPACT>(el(a.gE, gr));
+        gI[j] = gld<COMPACT>(el(a.gI, gr));
+        ab[j] = gld<COMPACT>(el(a.apBuf, gr));
         if constexpr (COMPACT) {
-            rec[j] = gld<COMPACT>(&a.conn[gr]);
+            rec[j] = gld<COMPACT>(el(a.conn, gr));
         } else {
-            mi_[j] = gld<COMPACT>(&a.mfIn[gr]);
-            gi_[j] = gld<COMPACT>(&a.goIn[gr]);
-            go_[j] = gld<COMPACT>(&a.goOut[gr]);
-            gd_[j] = gld<COMPACT>(&a.goOutDly[gr]);
-            bsp_[j] = gld<COMPACT>(&a.bspDly[gr]);
+            mi_[j] = gld<COMPACT>(el(a.mfIn, gr));
+            gi_[j] = gld<COMPACT>(el(a.goIn, gr));
+            go_[j] = gld<COMPACT>(el(a.goOut, gr));
+            gd_[j] = gld<COMPACT>(el(a.goOutDly, gr));
+            bsp_[j] = gld<COMPACT>(el(a.bspDly, gr));
         }
-        w0[j] = gld<COMPACT>(&a.w[gr]);
+        w0[j] = gld<COMPACT>(el(a.w, gr));
+        if constexpr (PLAST) hist[j] = gld<COMPACT>(el(a.hist, gr));
     }
+    __builtin_amdgcn_sched_barrier(0);
 
     // LDS writes of what was loaded first, before any branch or loop that
     // loads (after one, the compiler waits for every load in flight)
@@ -251,11 +268,7 @@ __device__ __forceinline__ void gr_span_impl(const GrArgs &a, const GrStep &st, 
         stage64(s_mfb, nMFb, base, i < a.numMF && smf[k] != 0);
         stage64(s_gob, nGOb, base, i < a.numGO && sgo[k] != 0);
     }
-    if (st.doPlast) {
-#pragma unroll
-        for (int j = 0; j < CPT; j++) hist[j] = gld<COMPACT>(&a.hist[cellOf(j)]);
-    }
-    if (st.doPlast && (uint32_t)tid < nStep) s_step[tid] = stepOf(tid, tm0);
+    if (PLAST && (uint32_t)tid < nStep) s_step[tid] = stepOf(tid, tm0);
     for (uint32_t i = tid; i < a.numGO; i += NT) s_goh[i] = 0;
     if (tid < 2 * kSeg) s_sc[tid] = 0;  // s_sc and s_bc are contiguous
     __syncthreads();
@@ -335,19 +348,19 @@ __device__ __forceinline__ void gr_span_impl(const GrArgs &a, const GrStep &st, 
         e = e * a.decMF + a.incMF * (cm);
         i = i * a.decGO + a.incGO * (cg);
 
-        gst<COMPACT>(&a.v[gr], vv);
-        gst<COMPACT>(&a.thr[gr], thN);
-        gst<COMPACT>(&a.gKCa[gr], gkN);
-        gst<COMPACT>(&a.apBuf[gr], b);
-        gst<COMPACT>(&a.gE[gr], e);
-        gst<COMPACT>(&a.gI[gr], i);
+        gst<COMPACT>(el(a.v, gr), vv);
+        gst<COMPACT>(el(a.thr, gr), thN);
+        gst<COMPACT>(el(a.gKCa, gr), gkN);
+        gst<COMPACT>(el(a.apBuf, gr), b);
+        gst<COMPACT>(el(a.gE, gr), e);
+        gst<COMPACT>(el(a.gI, gr), i);
 
         uint32_t bit39 = 0;
-        if (st.doPlast) {
+        if constexpr (PLAST) {
             const uint64_t h = hist[j];
             bit39 = (uint32_t)(h >> 39) & 1u;
             // updateGRHistory (kernels.cu:244-245): mask 0x1F is hard-coded there
-            gst<COMPACT>(&a.hist[gr], (h << 1) | (uint64_t)(((b & 0x0000001Fu) > 0) * 0x00000001u));
+            gst<COMPACT>(el(a.hist, gr), (h << 1) | (uint64_t)(((b & 0x0000001Fu) > 0) * 0x00000001u));
         }
 
         // delayed spike towards BC/PC/SC (kernels.cu:256,269)
@@ -358,13 +371,13 @@ __device__ __forceinline__ void gr_span_impl(const GrArgs &a, const GrStep &st, 
         const uint32_t io = gr >> a.log2GRperIO;
         for (uint32_t z = 0; z < a.numZones; z++) {  // (numZones is 1 in most networks)
             float *wz = a.w + (size_t)z * a.numGR;
-            float wv = z == 0 ? w0[j] : gld<COMPACT>(&wz[gr]);
-            if (st.doPlast) {
+            float wv = z == 0 ? w0[j] : gld<COMPACT>(el(wz, gr));
+            if constexpr (PLAST) {
                 const float plastStep = s_step[z * a.numIO + io];
                 wv = wv + (bit39 > 0) * plastStep;
                 wv = (wv > 0) * wv;
                 wv = (wv > 1) + (wv <= 1) * wv;
-                gst<COMPACT>(&wz[gr], wv);
+                gst<COMPACT>(el(wz, gr), wv);
             }
             float x = 0.0f;
             if (bal != 0) x = wave_sum_lane0(wv * d);
@@ -420,15 +433,24 @@ __device__ __forceinline__ void gr_span(const GrArgs &a, const GrStep &st, const
         gr_fold<NT>(a, st, rank, ranks);
         return;
     }
-    if constexpr (FMT == kFmtCompact) {
-        gr_span_impl<CPT, NT, FULL, true>(a, st, first, count, rank, ranks, smem);
-    } else if constexpr (FMT == kFmtArrays) {
-        gr_span_impl<CPT, NT, FULL, false>(a, st, first, count, rank, ranks, smem);
-    } else {
-        if (a.conn)
-            gr_span_impl<CPT, NT, FULL, true>(a, st, first, count, rank, ranks, smem);
+    // the plasticity step (every tsPerHistBinGR-th) is a variant of its own, so
+    // that its history words load in order with each cell's other arrays
+    constexpr bool kCompact = FMT == kFmtCompact;
+    if constexpr (FMT != kFmtAny) {
+        if (st.doPlast)
+            gr_span_impl<CPT, NT, FULL, kCompact, true>(a, st, first, count, rank, ranks, smem);
         else
-            gr_span_impl<CPT, NT, FULL, false>(a, st, first, count, rank, ranks, smem);
+            gr_span_impl<CPT, NT, FULL, kCompact, false>(a, st, first, count, rank, ranks, smem);
+    } else if (a.conn) {
+        if (st.doPlast)
+            gr_span_impl<CPT, NT, FULL, true, true>(a, st, first, count, rank, ranks, smem);
+        else
+            gr_span_impl<CPT, NT, FULL, true, false>(a, st, first, count, rank, ranks, smem);
+    } else {
+        if (st.doPlast)
+            gr_span_impl<CPT, NT, FULL, false, true>(a, st, first, count, rank, ranks, smem);
+        else
+            gr_span_impl<CPT, NT, FULL, false, false>(a, st, first, count, rank, ranks, smem);
     }
 }
 
@@ -487,9 +509,10 @@ CBM_GR_SPAN(5)
 
 // ---------------------------------------------------------------- SFMT on device
 // The zone generator of MZone (CRandomSFMT0, sfmt.cpp:210-246,337-355),
-// regenerated by four lanes of one wave: lane k owns 32-bit lane k of every
-// 128-bit state word, and the two byte shifts of the recursion take the
-// neighbouring lane's word with a shuffle.
+// regenerated by the four lanes of one quad: lane k owns 32-bit lane k of
+// every 128-bit state word, and the two byte shifts of the recursion take the
+// neighbouring lane's word by a quad permutation of the VALU (no LDS round
+// trip on the recursion's dependent chain).
 __device__ void sfmt_regen4(uint32_t *st, int k)
 {
     const uint32_t msk = k == 0 ? 0xeffff7fbu : k == 1 ? 0xffffffefu : k == 2 ? 0xdfdfbfffu : 0x7fffdbfdu;
@@ -509,8 +532,9 @@ __device__ void sfmt_regen4(uint32_t *st, int k)
         }
 #pragma unroll
         for (int j = 0; j < 8; j++) {
-            const uint32_t aPrev = __shfl(av[j], k == 0 ? 0 : k - 1, 4);
-            const uint32_t r1Next = __shfl(r1, k == 3 ? 3 : k + 1, 4);
+            // quad_perm [0,0,1,2]: lane k - 1; quad_perm [1,2,3,3]: lane k + 1
+            const uint32_t aPrev = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)av[j], 0x90, 0xF, 0xF, false);
+            const uint32_t r1Next = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)r1, 0xF9, 0xF, 0xF, false);
             const uint32_t sa = (av[j] << 24) | (k == 0 ? 0u : (aPrev >> 8));
             const uint32_t sc = (r1 >> 24) | (k == 3 ? 0u : (r1Next << 8));
             const uint32_t r = av[j] ^ sa ^ ((bv[j] >> 7) & msk) ^ sc ^ (r2 << 14);
