@@ -24,6 +24,11 @@
  *   pipeline lags, NC->IO release draws, MF history  tests/test_oracle.py
  *   Golgi glutamate/mGluR chain, NC conductances and membrane, IO
  *   conductances and coupling, MF->NC plasticity      tests/test_oracle_pins.py
+ *   PC, BC, SC, GO and IO membranes, thresholds and
+ *   spikes, SC->PC saturation, MF->GO and GO->GO
+ *   counts, GO depression and coupling, GO->GR input,
+ *   IO timer and error drive, microzone routing        tests/test_oracle_pins_small.py
+ *   (each pin is also run against a planted mistake, build/liborc_plant.so)
  *   SFMT (orc_sfmt.c)                                  tests/test_sfmt.py (the
  *                                                       reference's own generator)
  *   derived constants                                  tests/test_params.py
@@ -65,6 +70,10 @@ int orc_run_mf_train(orc_net *n, const uint8_t *mf, size_t pitch, int nsteps,
                      const float *err_drive);
 /* Number of OpenMP threads the GR loops use (1 unless built with OpenMP). */
 int orc_threads(void);
+/* Planted mistake k of the build with ORC_PLANTS (build/liborc_plant.so; 0 =
+ * none): tests show with it that a pin fails on a plausible mistake.  Other
+ * builds accept only 0. */
+int orc_set_plant(int k);
 
 /* Canonical PF->PC block sum used by the mimic (exposed for tests):
  * x has len elements, len a multiple of 64. */

@@ -24,8 +24,10 @@ SKIP_ON_LOAD = {"ZONE_RNG_STATE", "CUR_TIME", "SUM_GR_INPUT_GO", "SUM_GO_INPUT_G
                 "PIPE_PFBC_SUM", "PIPE_PFSC_SUM", "PIPE_GRGO_SUM", "MF_INPUT"}
 
 
-def lib(openmp: bool = False) -> C.CDLL:
-    name = "liborc_omp.so" if openmp else "liborc.so"
+def lib(openmp: bool = False, plant: bool = False) -> C.CDLL:
+    """The oracle library: the single-threaded checker, the OpenMP build, or
+    the checker with planted mistakes (orc_set_plant)."""
+    name = "liborc_plant.so" if plant else ("liborc_omp.so" if openmp else "liborc.so")
     if name not in _libs:
         path = os.path.join(HERE, "build", name)
         if not os.path.exists(path):
@@ -46,6 +48,7 @@ def lib(openmp: bool = False) -> C.CDLL:
         L.orc_step.argtypes = [P, I]
         L.orc_run_mf_train.argtypes = [P, V, S, I, V]
         L.orc_threads.restype = I
+        L.orc_set_plant.argtypes = [I]
         L.orc_canonical_sum_f32.argtypes = [V, S]
         L.orc_canonical_sum_f32.restype = C.c_float
         _libs[name] = L
@@ -60,8 +63,13 @@ def _check(rc: int, what: str) -> None:
 class Oracle:
     """The reference step restated on the CPU; precision 32 (mimic) or 64 (replay)."""
 
-    def __init__(self, sizes, params, *, precision: int = 32, grgo_lag: int = 2, openmp: bool = False):
-        self.L = lib(openmp)
+    def __init__(self, sizes, params, *, precision: int = 32, grgo_lag: int = 2, openmp: bool = False,
+                 plant: int = 0):
+        """plant=k > 0: the build with planted mistakes, with mistake k in place
+        (a process-wide setting of that library: tests only)."""
+        self.L = lib(openmp, plant=plant > 0)
+        if plant:
+            _check(self.L.orc_set_plant(plant), "set_plant")
         o = Options()
         o.grgoLag = grgo_lag
         h = C.c_void_p()

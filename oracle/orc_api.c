@@ -123,6 +123,20 @@ int orc_run_mf_train(orc_net *n, const uint8_t *mf, size_t pitch, int nsteps, co
     return CBM_OK;
 }
 
+#ifdef ORC_PLANTS
+int orc_plant = 0;
+#endif
+
+int orc_set_plant(int k)
+{
+#ifdef ORC_PLANTS
+    orc_plant = k;
+    return CBM_OK;
+#else
+    return k ? CBM_E_UNSUPPORTED : CBM_OK;
+#endif
+}
+
 int orc_threads(void)
 {
 #ifdef _OPENMP

@@ -126,6 +126,7 @@ def test_one_step_from_identical_state(gpu, cfg, warm, mode):
     assert_parity(compare(net, o32, z, rep), where="after transfer")
     o64 = O.Oracle.from_state(st, precision=64)
     transfer(o32, o64, z)
+    th_prev = o64.read_f64("THRESH_GR").copy()
     for m in (o32, o64, net):
         m.update_mf_input(tr[warm])
         m.calc_activity(1)
@@ -148,12 +149,22 @@ def test_one_step_from_identical_state(gpu, cfg, warm, mode):
         a = net.read("PIPE_PFPC_SUM", zz).astype(np.float64)[0]
         r = o64.read_f64("PIPE_PFPC_SUM", zz)[0]
         assert np.allclose(a, r, rtol=1e-5, atol=1e-6), (a, r)
-    # spikes of the step: bit-exact apart from cells within 1e-5 of threshold in binary64
+    # spikes of the step: bit-exact apart from cells within 1e-5 of threshold
+    # in binary64 -- each differing cell's membrane lies that close to the
+    # threshold its spike test read (the decayed one, kernels.cu:98-101)
     sp_gpu = net.read("APBUF_GR") & 1
     sp_64 = o64.read("APBUF_GR") & 1
-    diff = int((sp_gpu != sp_64).sum())
-    print(f"GR spikes {int(sp_gpu.sum())}, differing from binary64 replay: {diff}", flush=True)
-    assert diff <= max(2, sp_gpu.size // 100000)
+    differ = np.flatnonzero(sp_gpu != sp_64)
+    p = st.params
+    rest, dec = float(np.float32(p.threshRestGR)), float(np.float32(p.threshDecGR))
+    th_test = th_prev + (rest - th_prev) * dec
+    gap = np.abs(o64.read_f64("VM_GR") - th_test)[differ]
+    near = gap <= 1e-5 * np.maximum(1.0, np.abs(th_test[differ]))
+    print(f"GR spikes {int(sp_gpu.sum())}, differing from binary64 replay: {differ.size} "
+          f"(all within 1e-5 of threshold: {bool(near.all())}; largest gap {gap.max() if gap.size else 0:.3g} mV)",
+          flush=True)
+    assert near.all(), (differ[~near][:8], gap[~near][:8])
+    assert differ.size <= max(2, sp_gpu.size // 100000)
 
 
 @pytest.mark.parametrize("mode", [1, 2])
