@@ -2,7 +2,8 @@
 // (CBMState, CBMSimCore, InNetInterface, MZoneInterface), the way
 // ECManagementDelay uses it (Eyelid_Sim_CBM_Dev/src/ectrial/ecmanagementdelay.cpp:201-273):
 // per ms updateMFInput, an error drive at one step, a plasticity-step change,
-// calcActivity, and export reads.
+// a GR->GO increment set and reset (InNetInterface::setGIncGRtoGO), calcActivity,
+// and export reads.
 //
 //   cbm_driver <actParams> <conParams> <steps> <seed> <out.bin> [<trials.dat> [<state.st>]]
 //
@@ -79,6 +80,8 @@ int main(int argc, char **argv)
             for (unsigned i = 0; i < numMF; i++) mf[i] = ((i * 7u + (unsigned)t * 13u) % 97u) < 3u;
             if (t == steps / 2) sim.updateErrDrive(0, 1.0f);
             if (t == steps / 3) sim.getMZoneList()[0]->setGRPCPlastSteps(-0.0009f, 0.0005f);
+            if (t == steps / 4) sim.getInputNet()->setGIncGRtoGO(0.0003f);
+            if (t == 3 * steps / 4) sim.getInputNet()->resetGIncGRtoGO();
             sim.updateMFInput(mf.data());
             sim.calcActivity();
             const float pos = eyelid.calcStep(sim.getMZoneList()[0]->exportAPNC());

@@ -69,6 +69,10 @@ def test_driver_matches_python_mirror(gpu, tmp_path):
             net.update_err_drive(0, 1.0)
         if t == steps // 3:
             net.set_grpc_plast_steps(0, -0.0009, 0.0005)
+        if t == steps // 4:
+            net.set_ginc_gr_to_go(0.0003)
+        if t == 3 * steps // 4:
+            net.reset_ginc_gr_to_go()
         net.update_mf_input(mf)
         net.calc_activity(1)
         lid.append(eye.step(net.read("AP_NC", 0)))

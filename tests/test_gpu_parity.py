@@ -394,3 +394,41 @@ def test_api_errors(gpu):
     s2.numGR = 1000
     with pytest.raises(CbmError, match="1024"):
         Network(s2, st.params)
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_ginc_grgo_knob_mid_run(gpu, mode):
+    """InNet::setGIncGRtoGO / resetGIncGRtoGO (innet.cpp:669-677) through
+    cbm_set_ginc_grgo / cbm_reset_ginc_grgo mid-run: the GPU equals the
+    binary32 oracle, which takes the same knob, in every field after each
+    phase; the knob changes the Golgi cells' GR conductance."""
+    import orc as O
+    print(f"[case] gIncGRtoGO knob c0 mode={mode}", flush=True)
+    st = configs.build_state("c0", seed=0)
+    net = Network.from_state(st, step_mode=mode)
+    o32 = O.Oracle.from_state(st, precision=32)
+    ref = O.Oracle.from_state(st, precision=32)  # the same run without the knob
+    steps = 240
+    tr = configs.mf_train(st.sizes.numMF, steps, seed=9, t0=400, trial_ms=600, cs_on=100, cs_off=500)
+    import torch
+    dtr = torch.from_numpy(tr).cuda()
+    inc = float(np.float32(3.0 * st.params.gIncGRtoGO))
+    for k0, k1, knob in ((0, 80, None), (80, 160, inc), (160, 240, "reset")):
+        if knob == "reset":
+            net.reset_ginc_gr_to_go()
+            o32.set_ginc_gr_to_go(st.params.gIncGRtoGO)
+        elif knob is not None:
+            net.set_ginc_gr_to_go(knob)
+            o32.set_ginc_gr_to_go(knob)
+        net.run_mf_train(dtr[k0:k1])
+        o32.run_mf_train(tr[k0:k1])
+        ref.run_mf_train(tr[k0:k1])
+        bad = compare(net, o32, 1)
+        print(f"steps {k0}-{k1} knob={knob}: {bad}", flush=True)
+        assert_parity(bad, where=f"gIncGRtoGO knob after step {k1}")
+        assert np.array_equal(net.read("GINC_GR_TO_GO"), o32.read("GINC_GR_TO_GO"))
+        if knob is None:
+            assert np.array_equal(net.read("G_GR_GO"), ref.read("G_GR_GO"))
+        else:
+            assert not np.array_equal(net.read("G_GR_GO"), ref.read("G_GR_GO"))
+    assert net.read("GINC_GR_TO_GO")[0] == np.float32(st.params.gIncGRtoGO)
