@@ -87,6 +87,11 @@ FIELDS: dict[str, int] = {k[len("CBM_F_"):]: v for k, v in _parse_enum(_TEXT, "c
 FIELD_COUNT = len(FIELDS)
 FIELD_NAMES = {v: k for k, v in FIELDS.items()}
 KIND_CON, KIND_STATE, KIND_RUNTIME = 0, 1, 2
+# CBM_E_* status codes and the no-allocation batch run's staging limit
+ERRORS: dict[str, int] = {m.group(1): int(m.group(2)) for m in
+                          re.finditer(r"#define (CBM_E_\w+) \((-?\d+)\)", open(HEADER).read())}
+NOALLOC_ERR_FLOATS = 1 << int(re.search(r"#define CBM_NOALLOC_ERR_FLOATS \(1u << (\d+)\)",
+                                        open(HEADER).read()).group(1))
 
 _hip = None
 _state = None
@@ -163,6 +168,7 @@ def hip_lib() -> C.CDLL:
         L.cbm_batch_create.argtypes = [PP, I, C.POINTER(BatchOptions), C.POINTER(P)]
         L.cbm_batch_destroy.argtypes = [P]
         L.cbm_batch_run_mf_train.argtypes = [P, PP, SP, I, PP]
+        L.cbm_batch_run_mf_train_noalloc.argtypes = [P, PP, SP, I, PP]
         L.cbm_batch_sync.argtypes = [P]
         L.cbm_batch_profile.argtypes = [P, PP, SP, I, C.POINTER(C.c_double)]
         L.cbm_batch_autotune.argtypes = [P, PP, SP, I, C.POINTER(I)]

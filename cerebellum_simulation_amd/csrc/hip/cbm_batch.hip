@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -60,6 +61,39 @@ struct cbm_batch {
 };
 
 constexpr uint32_t kSweepGraphSteps = 64;
+
+// Staging of the error-drive rows of cbm_batch_run_mf_train_noalloc (1 MiB,
+// CBM_NOALLOC_ERR_FLOATS floats): a file-scope array that every GPU the
+// library runs on holds until the process ends.  Host code reaches it only
+// through its device address, resolved once per device with that device
+// current (the array's name is a host-side stand-in, not a device address).
+static __device__ __align__(256) float scratch[CBM_NOALLOC_ERR_FLOATS];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratchMu;
+float *g_scratch[kMaxDevices];  // device address of `scratch` on each device (nullptr: not resolved)
+
+// The calling thread's current device must be `device`.
+int resolveScratch(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return fail(CBM_E_UNSUPPORTED, "device ordinal above 63");
+    std::lock_guard<std::mutex> lock(g_scratchMu);
+    if (g_scratch[device]) return CBM_OK;
+    float *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        return fail(CBM_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch[device] = ptr;
+    return CBM_OK;
+}
+
+float *scratchOf(int device)
+{
+    std::lock_guard<std::mutex> lock(g_scratchMu);
+    return device >= 0 && device < kMaxDevices ? g_scratch[device] : nullptr;
+}
+}  // namespace
 
 namespace {
 
@@ -199,8 +233,10 @@ int restoreSnapshot(cbm_batch *b, Snapshot *sn)
 }
 
 // Run n batched steps; time the launches when evs is given (one pair).
+// noAlloc: stage the error drive in the device's `scratch` array and launch
+// step by step (no buffer growth, no graph capture).
 int run(cbm_batch *b, const uint8_t *const *mf, const size_t *pitch, int n, const float *const *err,
-        hipEvent_t *evs, const float *prob = nullptr, size_t probPitch = 0)
+        hipEvent_t *evs, const float *prob = nullptr, size_t probPitch = 0, bool noAlloc = false)
 {
     if (!mf || !pitch || n < 0) return fail(CBM_E_ARG, "bad argument");
     HIP_TRY(hipSetDevice(b->device));
@@ -216,12 +252,20 @@ int run(cbm_batch *b, const uint8_t *const *mf, const size_t *pitch, int n, cons
     size_t errFloats = 0;
     for (size_t i = 0; i < N; i++)
         if (err && err[i]) errFloats += (size_t)n * b->nets[i]->s.numZones;
-    if (errFloats > b->derrCap) {
+    float *errStage = b->derr;
+    if (noAlloc) {
+        if (errFloats > CBM_NOALLOC_ERR_FLOATS)
+            return fail(CBM_E_ARG, "cbm_batch_run_mf_train_noalloc: n x (numZones of the networks with an error "
+                                   "drive) exceeds CBM_NOALLOC_ERR_FLOATS");
+        errStage = scratchOf(b->device);
+        if (!errStage) return fail(CBM_E_STATE, "scratch array not resolved on this device");
+    } else if (errFloats > b->derrCap) {
         if (b->derr) (void)hipFree(b->derr);
         b->derr = nullptr;
         b->derrCap = 0;
         HIP_TRY(hipMalloc((void **)&b->derr, errFloats * 4));
         b->derrCap = errFloats;
+        errStage = b->derr;
     }
     size_t off = 0;
     for (size_t i = 0; i < N; i++) {
@@ -233,8 +277,8 @@ int run(cbm_batch *b, const uint8_t *const *mf, const size_t *pitch, int n, cons
         d.err = nullptr;
         if (err && err[i]) {
             const size_t cnt = (size_t)n * h->s.numZones;
-            HIP_TRY(hipMemcpy(b->derr + off, err[i], cnt * 4, hipMemcpyHostToDevice));
-            d.err = b->derr + off;
+            HIP_TRY(hipMemcpy(errStage + off, err[i], cnt * 4, hipMemcpyHostToDevice));
+            d.err = errStage + off;
             off += cnt;
         }
         d.t0 = h->curTime;
@@ -289,7 +333,7 @@ int run(cbm_batch *b, const uint8_t *const *mf, const size_t *pitch, int n, cons
             HIP_TRY(hipGetLastError());
         }
     }
-    if (b->graphSteps && (uint32_t)n >= b->graphSteps && !prob) {
+    if (b->graphSteps && (uint32_t)n >= b->graphSteps && !prob && !noAlloc) {
         if ((rc = captureGraph(b))) return rc;
         const uint32_t zero = 0;
         HIP_TRY(hipMemcpyAsync(b->kbase, &zero, 4, hipMemcpyHostToDevice, b->stream));
@@ -423,6 +467,7 @@ int cbm_batch_create(cbm_handle *const *nets, int count, const cbm_batch_options
     }
     int rc = buildTiles(b);
     if (rc) return bail(rc);
+    if ((rc = resolveScratch(b->device))) return bail(rc);  // for cbm_batch_run_mf_train_noalloc
     for (cbm_handle *h : b->nets) h->batch = b;
     *out = b;
     return CBM_OK;
@@ -457,6 +502,13 @@ int cbm_batch_run_mf_train(cbm_batch *b, const uint8_t *const *mf_dev, const siz
 {
     if (!b) return fail(CBM_E_ARG, "null batch");
     return run(b, mf_dev, pitch, n, err_host, nullptr);
+}
+
+int cbm_batch_run_mf_train_noalloc(cbm_batch *b, const uint8_t *const *mf_dev, const size_t *pitch, int n,
+                                   const float *const *err_host)
+{
+    if (!b) return fail(CBM_E_ARG, "null batch");
+    return run(b, mf_dev, pitch, n, err_host, nullptr, nullptr, 0, true);
 }
 
 int cbm_batch_sync(cbm_batch *b)

@@ -505,6 +505,22 @@ int cbm_batch_destroy(cbm_batch *b);
 int cbm_batch_run_mf_train(cbm_batch *b, const uint8_t *const *mf_dev, const size_t *pitch, int n,
                            const float *const *err_host);
 
+/* cbm_batch_run_mf_train without allocating: no counterpart in the reference
+ * (which runs one network per process and has no batch).  It is for callers
+ * that cannot allocate device memory -- code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc: it calls no hipMalloc,
+ * hipFree or other allocating routine (cbm_batch_run_mf_train may grow its
+ * error-drive staging buffer and instantiate graphs).  The error-drive rows
+ * are staged in a fixed device array of CBM_NOALLOC_ERR_FLOATS floats, one
+ * per GPU, held until the process ends; a run whose n x (sum of numZones over
+ * the networks given an error drive) exceeds it fails with CBM_E_ARG.  Steps
+ * are launched one by one (no graph).  Every call on one GPU shares that
+ * array: calls that overlap on one GPU (other threads, or another batch's run
+ * still in flight on another stream) race. */
+#define CBM_NOALLOC_ERR_FLOATS (1u << 18)
+int cbm_batch_run_mf_train_noalloc(cbm_batch *b, const uint8_t *const *mf_dev, const size_t *pitch, int n,
+                                   const float *const *err_host);
+
 /* Wait for the batch's queued work. */
 int cbm_batch_sync(cbm_batch *b);
 
