@@ -71,13 +71,14 @@ def build_core() -> str | None:
 
 
 def build_driver() -> list[str]:
-    """The C++ driver test programs (tests/cpp/*.cpp: the eyelid-style driver
-    and the cart-pole driver), written against the reference's CBMSimCore /
-    CBMState / environment API and linked to libcbm_core."""
+    """The C++ driver test programs (tests/cpp/*.cpp: the eyelid-style driver,
+    the cart-pole driver and the ECMFPopulation file writer), written against
+    the reference's CBMSimCore / CBMState / environment / tools API and linked
+    to libcbm_core."""
     out_dir = os.path.join(ROOT, "tests", "cpp", "build")
     os.makedirs(out_dir, exist_ok=True)
     outs = []
-    for name in ("cbm_driver", "cartpole_driver"):
+    for name in ("cbm_driver", "cartpole_driver", "ecmfpop_driver", "views_driver"):
         out = os.path.join(out_dir, name)
         _run([CXX, *CXX_FLAGS, f"-I{os.path.join(CSRC, 'core', 'include')}", "-o", out,
               os.path.join(ROOT, "tests", "cpp", name + ".cpp"), f"-L{LIB}", "-lcbm_core", "-lcbm_state",

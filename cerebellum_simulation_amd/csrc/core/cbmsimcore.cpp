@@ -127,6 +127,7 @@ void CBMState::writeState(std::fstream &outfile)
 
 CBMState::~CBMState()
 {
+    deleteViews();
     if (st) cbm_state_destroy(st);
 }
 

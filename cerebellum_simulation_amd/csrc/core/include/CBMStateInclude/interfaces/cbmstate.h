@@ -5,11 +5,17 @@
 #define CBM_CBMSTATE_H
 
 #include <fstream>
+#include <map>
 #include <string>
 
 #include <CXXToolsInclude/stdDefinitions/pstdint.h>
 
 #include <cbm_state.h>
+
+#include "iactivityparams.h"
+#include "iconnectivityparams.h"
+#include "iinnetconstate.h"
+#include "imzoneactstate.h"
 
 class CBMState
 {
@@ -34,6 +40,14 @@ public:
 
     ct_uint32_t getNumZones();
 
+    // Views of the parameters and state (cbmstate.h:47-53).  They stay valid
+    // while the CBMState lives; they read and write its arrays, which
+    // CBMSimCore::writeToState refreshes from a running simulation.
+    IConnectivityParams *getConnectivityParams();
+    IActivityParams *getActivityParams();
+    IInNetConState *getInnetConState();
+    IMZoneActState *getMZoneActState(unsigned int zoneN);
+
     // The underlying state handle and its arrays (CBMState layout).
     cbm_state *handle() { return st; }
     const cbm_sizes &sizes() const { return sz; }
@@ -47,6 +61,12 @@ private:
     cbm_state *st;
     cbm_sizes sz;
     cbm_act_params ap;
+    IConnectivityParams *conView = nullptr;
+    IActivityParams *actView = nullptr;
+    IInNetConState *innetView = nullptr;
+    IMZoneActState *zoneView[8] = {};
+    void deleteViews();
+    friend class CBMStateActView;
 };
 
 #endif

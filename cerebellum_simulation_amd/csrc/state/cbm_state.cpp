@@ -10,6 +10,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <iterator>
 #include <map>
 #include <sstream>
 #include <string>
@@ -305,6 +306,35 @@ uint32_t delayMask(const cbm_state *st, int dist)
 extern "C" {
 
 const char *cbm_state_last_error(void) { return g_err.c_str(); }
+
+int cbm_state_param_at(const cbm_state *st, int which, size_t i, const char **name, double *value)
+{
+    if (!st || !name || !value || (which != 0 && which != 1)) return fail(CBM_E_ARG, "bad argument");
+    if (which == 0) {
+        if (i >= st->act.size()) return CBM_E_ARG;
+        auto it = std::next(st->act.begin(), (std::ptrdiff_t)i);
+        *name = it->first.c_str();
+        *value = it->second;
+    } else {
+        if (i >= st->con.m.size()) return CBM_E_ARG;
+        auto it = std::next(st->con.m.begin(), (std::ptrdiff_t)i);
+        *name = it->first.c_str();
+        *value = it->second;
+    }
+    return CBM_OK;
+}
+
+int cbm_state_set_act_param(cbm_state *st, const char *name, float value)
+{
+    if (!st || !name) return fail(CBM_E_ARG, "bad argument");
+    auto it = st->act.find(name);
+    if (it == st->act.end()) return fail(CBM_E_ARG, std::string("no activity parameter ") + name);
+    it->second = value;
+    ActMap am;
+    am.m = st->act;
+    deriveAct(am, st->p);
+    return CBM_OK;
+}
 
 int cbm_act_params_from_text(const char *text, cbm_act_params *out)
 {

@@ -73,6 +73,21 @@ int cbm_state_destroy(cbm_state *st);
 int cbm_state_sizes(const cbm_state *st, cbm_sizes *out);
 int cbm_state_params(const cbm_state *st, cbm_act_params *out);
 
+/* Entry i, in key order, of a parameter file as parsed: which = 0 the actParams
+ * file (ActivityParams::paramMap; ActivityParams::getParamCopy / getParam,
+ * CBM_STATE_LIB_DEV/src/params/activityparams.cpp:404-421), 1 the conParams
+ * file (ConnectivityParams::paramMap; ConnectivityParams::getParamCopy,
+ * connectivityparams.cpp:372-390).  *name stays valid until the state is
+ * destroyed or an activity parameter is set.  CBM_E_ARG past the last entry. */
+int cbm_state_param_at(const cbm_state *st, int which, size_t i, const char **name, double *value);
+
+/* ActivityParams::setParam (activityparams.cpp:423-434): change an activity
+ * parameter the file gave and re-derive the step's constants
+ * (cbm_state_params).  CBM_E_ARG for a name the file did not give.  A
+ * simulation handle loaded from the state before keeps the constants it was
+ * given (the reference's modules read the parameter object every step). */
+int cbm_state_set_act_param(cbm_state *st, const char *name, float value);
+
 /* Pointer to the state's storage of a field (CBMState layout) and its size in
  * bytes.  The pointer stays valid until cbm_state_destroy. */
 int cbm_state_field(cbm_state *st, int field, int zone, void **ptr, size_t *bytes);
