@@ -7,17 +7,25 @@ row of a synthetic MF spike train that is resident in HBM before the timed
 region (CS on a seeded 3% of MFs at 100 Hz in 500-1000 ms of every 2,000 ms
 trial, the US as error drive at 999 ms).  Prints one JSON line.
 
+The timed region also carries the outputs a driver reads each ms: the
+device spike recorder stores every step's PC, NC and IO spikes, and the
+recorded rows are copied to the host before the region ends.
+
 roofline: the step's dominant kernel -- the fused step kernel (one launch
 per step: the GR tiles plus one block for the small populations), or the GR
 kernel when the two-stream step is used -- with the GR algorithmic bytes per
 launch (numGR x 104 B at fan-ins (4,3,2), 120 B at (4,3,4); the compulsory
 per-GR bytes with the reference's data types) divided by its mean duration,
-timed with HIP events on the stream it runs on, against MI355X's 8 TB/s.  The
+timed with HIP events on the network's own stream around the launches of the
+timed region, against MI355X's 8 TB/s.  traffic is the HBM bytes per launch
+from the FETCH_SIZE / WRITE_SIZE counters (profiles/traffic_<config>.json),
+and traffic_frac the bandwidth those bytes give over the same duration.  The
 GR and small-population kernels launched one after the other are timed too
 (gr_only_us, small_only_us).
 
-cpu_baseline: the CPU oracle (a C restatement of the same step, one thread),
-timed on this host over a bounded sample of the same workload.
+cpu_baseline: the CPU oracle (a C restatement of the same step), timed on
+this host over a bounded sample of the same workload with OpenMP over the
+host's cores (value) and with one thread (single_thread).
 
 batched: the 256-network ragged pack (64 x 16K + 96 x 65K + 64 x 262K + 32 x
 1M GR, each with its own seed, CS subset and GO-GO inhibition on or off)
@@ -57,21 +65,59 @@ def bytes_per_gr(n_out: int) -> int:
     return 40 + 8 + 28 + 8 * n_out + 4 + 4 + 4
 
 
-def cpu_baseline(cfg: str, sample_steps: int):
+def host_cpu() -> dict:
+    """CPU model, logical CPUs, the ones this process may run on, and
+    physical cores (distinct physical id / core id pairs of /proc/cpuinfo)."""
+    model, cores, phys, core = platform.processor() or platform.machine(), set(), None, None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "model name":
+                    model = v
+                elif k == "physical id":
+                    phys = v
+                elif k == "core id":
+                    core = v
+                elif not k and phys is not None:
+                    cores.add((phys, core))
+        if phys is not None:
+            cores.add((phys, core))
+    except OSError:
+        pass
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return {"model": model, "logical_cpus": os.cpu_count(), "usable_cpus": usable,
+            "physical_cores": len(cores) or None}
+
+
+def cpu_baseline(cfg: str, sample_steps: int, omp_steps: int):
+    """The binary32 oracle on the host: OpenMP over the host's cores and one
+    thread, each over a bounded sample of the workload's steps."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import orc
     from cerebellum_simulation_amd import configs
 
     st = configs.build_state(cfg, seed=0)
-    ref = orc.Oracle.from_state(st, precision=32)
-    tr = configs.mf_train(st.sizes.numMF, sample_steps, seed=11)
-    t0 = time.perf_counter()
-    ref.run_mf_train(tr)
-    dt = time.perf_counter() - t0
-    return {"value": round(sample_steps / dt, 3), "unit": "sim-ms/s", "cores": 1, "kind": "port",
-            "sample": f"{sample_steps} steps of the {cfg} network from its initial state, oracle binary32 "
-                      f"build (oracle/build/liborc.so), one thread, {dt:.2f} s; host {platform.processor() or platform.machine()}"
-                      f" with {os.cpu_count()} logical CPUs"}
+    host = host_cpu()
+    out = {}
+    for openmp, steps in ((True, omp_steps), (False, sample_steps)):
+        ref = orc.Oracle.from_state(st, precision=32, openmp=openmp)
+        tr = configs.mf_train(st.sizes.numMF, steps, seed=11)
+        t0 = time.perf_counter()
+        ref.run_mf_train(tr)
+        dt = time.perf_counter() - t0
+        threads = ref.L.orc_threads() if openmp else 1
+        out[openmp] = (steps / dt, dt, steps, threads)
+        ref.close()
+    v, dt, n, thr = out[True]
+    v1, dt1, n1, _ = out[False]
+    return {"value": round(v, 3), "unit": "sim-ms/s", "cores": thr, "kind": "port",
+            "sample": f"{n} steps of the {cfg} network from its initial state, oracle binary32 build with OpenMP "
+                      f"(oracle/build/liborc_omp.so, {thr} threads), {dt:.2f} s",
+            "single_thread": {"value": round(v1, 3), "unit": "sim-ms/s", "cores": 1,
+                              "sample": f"{n1} steps, oracle/build/liborc.so, {dt1:.2f} s"},
+            "host": host}
 
 
 def batch_bench(which: str, steps: int) -> dict:
@@ -79,7 +125,7 @@ def batch_bench(which: str, steps: int) -> dict:
     and of the same networks stepped one at a time."""
     import torch
 
-    from cerebellum_simulation_amd import Batch, Network, configs
+    from cerebellum_simulation_amd import Batch, Network, configs, stream_ceiling
 
     mix = configs.BATCH_MIX if which == "full" else [(c, max(1, n // 8)) for c, n in configs.BATCH_MIX]
     plan = configs.batch_plan(mix)
@@ -94,7 +140,8 @@ def batch_bench(which: str, steps: int) -> dict:
     gr_total = sum(n.sizes.numGR for n in nets)
     print(f"[bench] batch of {len(nets)} networks ({gr_total} GR) built in {time.perf_counter() - t_build:.1f} s",
           file=sys.stderr, flush=True)
-    b = Batch(nets)
+    graph_steps = 20
+    b = Batch(nets, graph_steps=graph_steps)
     b.run_mf_train([t[:warm] for t in trains])
     cpt = b.autotune([t[warm:warm + 8] for t in trains], steps=8)
     k0 = warm + 8
@@ -117,10 +164,14 @@ def batch_bench(which: str, steps: int) -> dict:
     alg = sum(n.sizes.numGR * bytes_per_gr(n.sizes.maxnumpGRfromGRtoGO) for n in nets)
     step_us = el / steps * 1e6
     b.close()
+    ceil_us, ceil_bytes = stream_ceiling(gr_total, launches=20)
     return {"networks": len(nets), "gr_total": gr_total, "mix": [list(m) for m in mix],
             "value": round(batched, 1), "unit": "network-ms/s", "plain": round(plain, 1),
             "speedup_vs_plain": round(batched / plain, 2), "steps": steps, "plain_steps": plain_steps,
-            "step_us": round(step_us, 2), "cells_per_thread": cpt,
+            "step_us": round(step_us, 2), "cells_per_thread": cpt, "graph_steps": graph_steps,
+            "stream_ceiling": {"us": round(ceil_us, 2), "bytes": ceil_bytes,
+                               "GB/s": round(ceil_bytes / (ceil_us * 1e-6) / 1e9, 1),
+                               "step_vs_ceiling": round(ceil_us / step_us, 4)},
             "roofline_frac": round(alg / (step_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
             "algorithmic_bytes_per_step": alg}
 
@@ -238,7 +289,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--config", default="c1")
     ap.add_argument("--profile-steps", type=int, default=400)
-    ap.add_argument("--cpu-steps", type=int, default=0, help="0 = pick a ~15 s sample")
+    ap.add_argument("--cpu-steps", type=int, default=0, help="one-thread CPU sample in steps (0 = pick ~10 s)")
+    ap.add_argument("--cpu-omp-steps", type=int, default=0, help="OpenMP CPU sample in steps (0 = pick)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--step-mode", type=int, default=0, help="0 pick, 1 fused launch (even GR split when it fits, else tiles), 2 two streams, 3 fused with GR tiles, 4 fused with the even split")
     ap.add_argument("--cells-per-thread", type=int, default=0)
@@ -273,7 +325,7 @@ def main():
     import numpy as np
     import torch
 
-    from cerebellum_simulation_amd import Network, configs
+    from cerebellum_simulation_amd import Network, configs, stream_ceiling
 
     t_build = time.perf_counter()
     st = configs.build_state(args.config, seed=0)
@@ -288,20 +340,32 @@ def main():
     print(f"[bench] {args.config}: built in {time.perf_counter() - t_build:.1f} s, "
           f"{net.device_bytes() / 2**20:.0f} MiB on device", file=sys.stderr, flush=True)
 
+    rec = ("PC", "NC", "IO")
+    net.set_recorder(rec, args.steps)
     net.run_mf_train(dtr[: args.warmup], err_drive=err[: args.warmup])
     net.sync()
     torch.cuda.synchronize()
+    # HIP events on the network's own stream, around the timed launches
+    stream = torch.cuda.ExternalStream(net.stream_ptr())
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t_first = int(net.read("CUR_TIME")[0]) + 1
     t0 = time.perf_counter()
+    ev0.record(stream)
     net.run_mf_train(dtr[args.warmup:n], err_drive=err[args.warmup:n])
+    ev1.record(stream)
+    out_rows = {p: net.read_recorded(p, t_first, args.steps) for p in rec}  # the outputs, on the host
     net.sync()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    k_us = ev0.elapsed_time(ev1) * 1000.0 / args.steps
     sim_ms_per_s = args.steps / elapsed
-    print(f"[bench] timed {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
+    spikes = {p: int(v.sum()) for p, v in out_rows.items()}
+    print(f"[bench] timed {args.steps} steps in {elapsed:.3f} s ({k_us:.3f} us per step on the GPU); "
+          f"recorded spikes {spikes}", file=sys.stderr, flush=True)
 
-    # dominant-kernel duration with HIP events on the stream it runs on
+    # the same launches alone, and the two halves one after the other
     half = args.profile_steps // 2
-    k_us, step_us = net.profile_steps(dtr[n:n + half])
+    k_us_alone, step_us = net.profile_steps(dtr[n:n + half])
     gr_only, small_only = net.profile_split(dtr[n + half:n + args.profile_steps])
     n_out = s.maxnumpGRfromGRtoGO
     alg_bytes = s.numGR * bytes_per_gr(n_out)
@@ -312,13 +376,23 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "kernel": ("step_span_kernel" if launch["even_split"] else "step_kernel") if fused
                 else "gr_step_kernel", "gr_launch": launch, "kernel_us": round(k_us, 3),
-                "algorithmic_bytes_per_launch": alg_bytes, "step_us_events": round(step_us, 3),
+                "kernel_us_note": "HIP event pair on the network's stream around the timed region's launches, "
+                                  "divided by the steps",
+                "algorithmic_bytes_per_launch": alg_bytes, "kernel_us_untimed_run": round(k_us_alone, 3),
                 "gr_only_us": round(gr_only, 3), "small_only_us": round(small_only, 3),
                 "gr_only_frac": round(alg_bytes / (gr_only * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)}
+    ceil_us, ceil_bytes = stream_ceiling(s.numGR)
+    roofline["stream_ceiling"] = {"us": round(ceil_us, 3), "bytes": ceil_bytes,
+                                  "GB/s": round(ceil_bytes / (ceil_us * 1e-6) / 1e9, 1),
+                                  "kernel_vs_ceiling": round(ceil_us / k_us, 4)}
     traffic_file = os.path.join(ROOT, "profiles", f"traffic_{args.config}.json")
     if os.path.exists(traffic_file):
         with open(traffic_file) as f:
-            roofline["traffic"] = json.load(f).get("hbm_bytes_per_launch")
+            tr_bytes = json.load(f).get("hbm_bytes_per_launch")
+        roofline["traffic"] = tr_bytes
+        if tr_bytes:
+            roofline["traffic_achieved"] = round(tr_bytes / (k_us * 1e-6) / 1e9, 1)
+            roofline["traffic_frac"] = round(tr_bytes / (k_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)
 
     batched = None
     if args.batch != "none":
@@ -341,14 +415,16 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu_steps = args.cpu_steps or {"c0": 5000, "c1": 2000, "c2": 400}.get(args.config, 100)
-        cpu = cpu_baseline(args.config, cpu_steps)
+        cpu_steps = args.cpu_steps or {"c0": 5000, "c1": 1200, "c2": 300}.get(args.config, 100)
+        omp_steps = args.cpu_omp_steps or {"c0": 5000, "c1": 4000, "c2": 1000}.get(args.config, 200)
+        cpu = cpu_baseline(args.config, cpu_steps, omp_steps)
 
     out = {
         "metric": METRIC, "value": round(sim_ms_per_s, 2), "unit": "sim-ms/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1000.0 * elapsed / args.steps, 5),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-        "data": "synthetic: seeded Bernoulli MF trains, seeded synthetic connectivity",
+        "data": "synthetic: seeded Bernoulli MF trains, seeded synthetic connectivity; PC, NC and IO spikes of "
+                "every step recorded and copied to the host in the timed region",
         "config": {"workload": WORKLOADS.get(args.config, args.config), "name": args.config,
                    "numGR": s.numGR, "numGO": s.numGO, "numMF": s.numMF, "numPC": s.numPC,
                    "parallelism": "single GPU", "step_mode": "fused" if fused else "two streams"},

@@ -154,6 +154,7 @@ def hip_lib() -> C.CDLL:
         L.cbm_sync.argtypes = [P]
         L.cbm_step_io_begin.argtypes = [P, V, I]
         L.cbm_step_io_end.argtypes = [P, V]
+        L.cbm_stream_ceiling.argtypes = [I, C.c_uint64, I, I, I, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]
         L.cbm_stream.argtypes = [P]
         L.cbm_stream.restype = P
         L.cbm_profile_steps.argtypes = [P, V, S, I, C.POINTER(C.c_double), C.POINTER(C.c_double)]

@@ -20,7 +20,8 @@
 #include "cbm_handle.hpp"
 
 namespace cbmdev {
-template <int CPT>
+// FMT: 1 = compact connectivity records, 2 = the five arrays, 0 = decided per tile
+template <int CPT, int FMT>
 __global__ void batch_step_kernel(const BatchNet *nets, const uint2 *tiles, uint32_t nNets, const uint32_t *kbase,
                                   uint32_t k);
 __global__ void batch_advance_kernel(uint32_t *kbase, uint32_t steps);
@@ -38,6 +39,7 @@ struct cbm_batch {
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;       // end of the last run, waited on by the networks' streams
     int cpt = 2;                     // GR cells per thread of the batched tiles
+    int fmt = 0;                     // connectivity format shared by the pack (0: mixed)
     uint32_t graphSteps = 0;         // steps per captured graph (0: plain launches)
     std::vector<BatchNet> hnet;
     BatchNet *dnet = nullptr;
@@ -97,6 +99,15 @@ float *scratchOf(int device)
 
 namespace {
 
+const void *batchKernel(int cpt, int fmt)
+{
+#define CBM_K(C) (fmt == 1   ? (const void *)batch_step_kernel<C, 1> \
+                  : fmt == 2 ? (const void *)batch_step_kernel<C, 2> \
+                             : (const void *)batch_step_kernel<C, 0>)
+    return cpt == 2 ? CBM_K(2) : CBM_K(1);
+#undef CBM_K
+}
+
 int buildTiles(cbm_batch *b)
 {
     std::vector<int> order(b->nets.size());
@@ -118,8 +129,14 @@ int buildTiles(cbm_batch *b)
         lds = std::max(lds, std::max<size_t>(h->smallLds, grTileLds(1024, b->cpt, h->s.numGO, h->s.numMF,
                                                                          h->s.numZones, h->s.numIO)));
     b->lds = lds;
-    const void *k = b->cpt == 2 ? (const void *)batch_step_kernel<2> : (const void *)batch_step_kernel<1>;
-    if (lds > 64 * 1024) HIP_TRY(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    bool allCompact = true, allArrays = true;
+    for (cbm_handle *h : b->nets) {
+        allCompact = allCompact && h->conn != nullptr;
+        allArrays = allArrays && h->conn == nullptr;
+    }
+    b->fmt = allCompact ? 1 : allArrays ? 2 : 0;
+    if (lds > 64 * 1024)
+        HIP_TRY(hipFuncSetAttribute(batchKernel(b->cpt, b->fmt), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     if (b->graph) (void)hipGraphExecDestroy(b->graph);
     b->graph = nullptr;
     return CBM_OK;
@@ -127,14 +144,12 @@ int buildTiles(cbm_batch *b)
 
 int launch(cbm_batch *b, const uint32_t *kbase, uint32_t k)
 {
-    const dim3 grid((uint32_t)b->nets.size() + b->ntiles), block(kSmallThreads);
-    if (b->cpt == 2)
-        hipLaunchKernelGGL(batch_step_kernel<2>, grid, block, b->lds, b->stream, b->dnet, b->dtiles,
-                           (uint32_t)b->nets.size(), kbase, k);
-    else
-        hipLaunchKernelGGL(batch_step_kernel<1>, grid, block, b->lds, b->stream, b->dnet, b->dtiles,
-                           (uint32_t)b->nets.size(), kbase, k);
-    HIP_TRY(hipGetLastError());
+    const BatchNet *nets = b->dnet;
+    const uint2 *tiles = b->dtiles;
+    uint32_t nNets = (uint32_t)b->nets.size();
+    void *args[] = {&nets, &tiles, &nNets, &kbase, &k};
+    HIP_TRY(hipLaunchKernel(batchKernel(b->cpt, b->fmt), dim3(nNets + b->ntiles), dim3(kSmallThreads), args, b->lds,
+                            b->stream));
     return CBM_OK;
 }
 

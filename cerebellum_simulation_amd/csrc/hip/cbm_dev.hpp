@@ -130,6 +130,16 @@ struct SmallStep {
     uint32_t ioPop;
 };
 
+// Arguments of the streaming-ceiling kernel: nRead 4-byte arrays (the first
+// nWrite written back) and one 16-byte record array, `cells` long each.
+constexpr int kMaxStreamWords = 8;
+struct StreamArgs {
+    uint32_t *word[kMaxStreamWords];
+    const uint4 *rec;
+    uint64_t cells;
+    int nRead, nWrite;
+};
+
 // One network of a batch: its constant arguments, the buffers the per-step
 // arguments point into, and the run the batch is executing.  Kernels derive
 // step t = t0 + k + 1 of run step k from these.

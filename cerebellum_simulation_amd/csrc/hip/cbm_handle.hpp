@@ -21,6 +21,7 @@ __global__ void step_span_kernel(const SmallArgs *args, const SmallStep st, cons
 template <int CPT, int FMT>
 __global__ void gr_span_kernel(const GrArgs a, const GrStep st, const uint32_t spans);
 __global__ void small_step_kernel(const SmallArgs *args, const SmallStep st);
+__global__ void stream_ceiling_kernel(StreamArgs a);
 __global__ void pfpc_fold_kernel(const float *chunk, float *out, uint32_t nChunks, uint32_t perPC, uint32_t numPC,
                                  uint32_t tasks);
 
@@ -83,6 +84,7 @@ struct cbm_handle {
     uint8_t *rec = nullptr;          // spike recorder ring (owned; freed on change and destroy)
     uint64_t recFrom = 0;            // first step written to the current recorder
     uint32_t recOffPF = 0xFFFFFFFFu; // offset of the PF chunk counts in a recorder row
+    std::vector<uint8_t> recStage;   // host copy of recorder rows being read
     size_t smallLds = 0;             // dynamic LDS of the small kernel
 
     uint64_t curTime = 0;

@@ -1269,15 +1269,51 @@ CBM_STEP_SPAN(3, kFmtArrays)
 CBM_STEP_SPAN(5, kFmtArrays)
 #undef CBM_STEP_SPAN
 
+// ---------------------------------------------------------------- streaming ceiling
+// A per-cell pass with the GR step's streams and no arithmetic to speak of:
+// every cell reads nRead 4-byte values and a 16-byte record, and writes the
+// first nWrite values back (the 4-byte streams non-temporal).  Block b covers cells
+// [b * 4096, b * 4096 + 4096), four per thread, 1024 cells apart.
+__global__ void __launch_bounds__(1024) stream_ceiling_kernel(StreamArgs a)
+{
+    constexpr int kCpt = 4;
+    uint32_t x[kCpt][kMaxStreamWords];
+    uint4 r[kCpt];
+    const uint64_t first = (uint64_t)blockIdx.x * (1024u * kCpt) + threadIdx.x;
+#pragma unroll
+    for (int j = 0; j < kCpt; j++) {
+        const uint64_t g = first + (uint64_t)j * 1024u;
+        const bool in = g < a.cells;
+#pragma unroll
+        for (int w = 0; w < kMaxStreamWords; w++)
+            x[j][w] = in && w < a.nRead ? __builtin_nontemporal_load(a.word[w] + g) : 0u;
+        r[j] = in ? a.rec[g] : make_uint4(0, 0, 0, 0);
+    }
+#pragma unroll
+    for (int j = 0; j < kCpt; j++) {
+        const uint64_t g = first + (uint64_t)j * 1024u;
+        if (g >= a.cells) continue;
+        const uint32_t k = r[j].x ^ r[j].y ^ r[j].z ^ r[j].w;
+#pragma unroll
+        for (int w = 0; w < kMaxStreamWords; w++)
+            if (w < a.nWrite) __builtin_nontemporal_store(x[j][w] + x[j][(w + 1) % kMaxStreamWords] + k, a.word[w] + g);
+    }
+}
+
 // ---------------------------------------------------------------- batch
-// One launch per step for a pack of networks: block n < nNets steps network
-// n's small populations, the other blocks are GR tiles (network, tile) from
-// the tile table.  Run step kk = *kbase + k (kbase: device counter advanced by
-// a captured graph, or nullptr).
+// Block-uniform values derived in vector arithmetic (the 64-bit remainders
+// below, pointers read from a network's record), moved to scalar registers so
+// that they do not occupy vector registers across the step.
+__device__ __forceinline__ uint32_t sreg(uint32_t x) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)x); }
+__device__ __forceinline__ uint64_t sreg64(uint64_t x)
+{
+    return ((uint64_t)sreg((uint32_t)(x >> 32)) << 32) | sreg((uint32_t)x);
+}
+
 // The small populations of network d at run step kk (block-wide).
 __device__ __forceinline__ void batch_small(const BatchNet &d, uint32_t kk, float *s_err, uint8_t *lds)
 {
-    const uint64_t t = d.t0 + kk + 1;
+    const uint64_t t = sreg64(d.t0 + kk + 1);
     const uint32_t par = (uint32_t)(t & 1u);
     const uint32_t nZ = d.gr.numZones;
     SmallStep st;
@@ -1296,17 +1332,26 @@ __device__ __forceinline__ void batch_small(const BatchNet &d, uint32_t kk, floa
     }
     st.gIncGRtoGO = d.gIncGRtoGO;
     st.par = par;
-    st.slotPrev = (uint32_t)((t + kRing - 1) % kRing);
-    st.slotT2 = (uint32_t)((t + kRing - 2) % kRing);
-    st.slotG = (uint32_t)((t + kRing - d.grgoLag) % kRing);
-    st.slotNext = (uint32_t)((t + 1) % kRing);
-    st.popHist = (t % d.tsPerPopHistBinPC) == 0;
-    st.resetMFHist = (t % d.numTSinMFHist) == 0;
+    st.slotPrev = sreg((uint32_t)((t + kRing - 1) % kRing));
+    st.slotT2 = sreg((uint32_t)((t + kRing - 2) % kRing));
+    st.slotG = sreg((uint32_t)((t + kRing - d.grgoLag) % kRing));
+    st.slotNext = sreg((uint32_t)((t + 1) % kRing));
+    st.popHist = sreg((t % d.tsPerPopHistBinPC) == 0);
+    st.resetMFHist = sreg((t % d.numTSinMFHist) == 0);
     __syncthreads();
-    small_step(*d.small, st, s_err, lds);
+    // d.small points into device global memory: taken through a scalar
+    // register as a global pointer, the argument block is read with global
+    // and scalar loads as a kernel argument is, not with flat loads whose
+    // results held 22 vector registers in scratch memory
+    const SmallArgs *sa = (const SmallArgs *)(const G<SmallArgs> *)sreg64((uint64_t)d.small);
+    small_step(*sa, st, s_err, lds);
 }
 
-template <int CPT>
+// One launch per step for a pack of networks: block n < nNets steps network
+// n's small populations, the other blocks are GR tiles (network, tile) from
+// the tile table.  Run step kk = *kbase + k (kbase: device counter advanced by
+// a captured graph, or nullptr).
+template <int CPT, int FMT>
 __global__ void __launch_bounds__(kSmallThreads) batch_step_kernel(const BatchNet *__restrict__ nets,
                                                                    const uint2 *__restrict__ tiles, uint32_t nNets,
                                                                    const uint32_t *kbase, uint32_t k)
@@ -1339,12 +1384,21 @@ __global__ void __launch_bounds__(kSmallThreads) batch_step_kernel(const BatchNe
         gs.plastTimer = d.timer2 + (size_t)parPrev * nZ * g.numIO;
         gs.doPlast = (t % d.tsPerHistBinGR) == 0;
         gs.pfCnt = nullptr;
-        gr_tile<CPT, kSmallThreads, kFmtAny>(g, gs, tt.y, reinterpret_cast<uint32_t *>(lds));
+        gr_tile<CPT, kSmallThreads, FMT>(g, gs, tt.y, reinterpret_cast<uint32_t *>(lds));
     }
 }
 
-template __global__ void batch_step_kernel<1>(const BatchNet *, const uint2 *, uint32_t, const uint32_t *, uint32_t);
-template __global__ void batch_step_kernel<2>(const BatchNet *, const uint2 *, uint32_t, const uint32_t *, uint32_t);
+// FMT: every network of the pack with compact records, every one with the
+// five arrays, or a mix (decided per tile)
+#define CBM_BATCH(CPT, FMT) \
+    template __global__ void batch_step_kernel<CPT, FMT>(const BatchNet *, const uint2 *, uint32_t, const uint32_t *, uint32_t);
+CBM_BATCH(1, kFmtCompact)
+CBM_BATCH(2, kFmtCompact)
+CBM_BATCH(1, kFmtArrays)
+CBM_BATCH(2, kFmtArrays)
+CBM_BATCH(1, kFmtAny)
+CBM_BATCH(2, kFmtAny)
+#undef CBM_BATCH
 
 // advances the run-step counter of a captured batch graph
 __global__ void batch_advance_kernel(uint32_t *kbase, uint32_t steps)

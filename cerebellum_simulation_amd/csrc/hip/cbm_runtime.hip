@@ -862,13 +862,26 @@ int cbm_read_recorded(cbm_handle *h, int population, uint64_t t0, int n, uint8_t
     int rc = syncAll(h);
     if (rc) return rc;
     const uint32_t cells = recCells(h->s, population);
+    // a device destination takes one strided copy per ring segment; a host
+    // one takes the segment's whole rows in one copy and the columns are
+    // picked on the host (a strided copy into host memory moves row by row)
+    hipPointerAttribute_t attr;
+    const bool toDevice = hipPointerGetAttributes(&attr, dst) == hipSuccess && attr.type == hipMemoryTypeDevice;
+    if (!toDevice) (void)hipGetLastError();
     int done = 0;
     while (done < n) {
         const uint64_t t = t0 + (uint64_t)done;
         const uint32_t r = (uint32_t)(t % a.recCap);
         const int run = (int)std::min<uint64_t>((uint64_t)(n - done), (uint64_t)(a.recCap - r));
-        HIP_TRY(hipMemcpy2D(dst + (size_t)done * cells, cells, h->rec + (size_t)r * a.recRow + off, a.recRow, cells,
-                            run, hipMemcpyDefault));
+        const uint8_t *src = h->rec + (size_t)r * a.recRow;
+        if (toDevice) {
+            HIP_TRY(hipMemcpy2D(dst + (size_t)done * cells, cells, src + off, a.recRow, cells, run, hipMemcpyDefault));
+        } else {
+            h->recStage.resize((size_t)run * a.recRow);
+            HIP_TRY(hipMemcpy(h->recStage.data(), src, h->recStage.size(), hipMemcpyDeviceToHost));
+            for (int i = 0; i < run; i++)
+                std::memcpy(dst + (size_t)(done + i) * cells, h->recStage.data() + (size_t)i * a.recRow + off, cells);
+        }
         done += run;
     }
     return CBM_OK;
@@ -1217,6 +1230,68 @@ int cbm_sync(cbm_handle *h)
 }
 
 void *cbm_stream(cbm_handle *h) { return h ? (void *)h->stream : nullptr; }
+
+int cbm_stream_ceiling(int device, uint64_t cells, int read_words, int write_words, int launches,
+                       double *us_per_launch, uint64_t *bytes_per_launch)
+{
+    if (!us_per_launch || cells == 0 || launches <= 0 || read_words < 1 || read_words > kMaxStreamWords ||
+        write_words < 0 || write_words > read_words)
+        return fail(CBM_E_ARG, "bad argument");
+    if (cells > (uint64_t)UINT32_MAX * 4096u) return fail(CBM_E_ARG, "too many cells");
+    HIP_TRY(hipSetDevice(device));
+    StreamArgs a{};
+    a.cells = cells;
+    a.nRead = read_words;
+    a.nWrite = write_words;
+    std::vector<void *> bufs;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto done = [&](int rc) {
+        for (void *p : bufs) (void)hipFree(p);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (st) (void)hipStreamDestroy(st);
+        return rc;
+    };
+    hipError_t e = hipSuccess;
+    for (int w = 0; w < read_words && e == hipSuccess; w++) {
+        void *p = nullptr;
+        e = hipMalloc(&p, cells * 4);
+        if (e == hipSuccess) {
+            bufs.push_back(p);
+            a.word[w] = (uint32_t *)p;
+            e = hipMemset(p, 0, cells * 4);
+        }
+    }
+    if (e == hipSuccess) {
+        void *p = nullptr;
+        e = hipMalloc(&p, cells * 16);
+        if (e == hipSuccess) {
+            bufs.push_back(p);
+            a.rec = (const uint4 *)p;
+            e = hipMemset(p, 0, cells * 16);
+        }
+    }
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e != hipSuccess) return done(fail(CBM_E_HIP, std::string("stream ceiling setup: ") + hipGetErrorString(e)));
+    const dim3 grid((uint32_t)((cells + 4095) / 4096)), block(1024);
+    for (int i = 0; i < 10; i++) hipLaunchKernelGGL(stream_ceiling_kernel, grid, block, 0, st, a);
+    e = hipEventRecord(e0, st);
+    for (int i = 0; i < launches && e == hipSuccess; i++) {
+        hipLaunchKernelGGL(stream_ceiling_kernel, grid, block, 0, st, a);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventRecord(e1, st);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e != hipSuccess) return done(fail(CBM_E_HIP, std::string("stream ceiling: ") + hipGetErrorString(e)));
+    *us_per_launch = ms * 1000.0 / launches;
+    if (bytes_per_launch) *bytes_per_launch = cells * (uint64_t)(4 * read_words + 16 + 4 * write_words);
+    return done(CBM_OK);
+}
 
 // n steps with event pairs around the timed launches (one pair per step, or
 // two in split mode), read back after the whole run so that timing inserts no

@@ -311,6 +311,14 @@ class Network:
     def sync(self) -> None:
         check(_abi.hip_lib().cbm_sync(self._h), "cbm_sync")
 
+    def stream_ptr(self) -> int:
+        """The hipStream_t the network's step kernels run on (for events or
+        torch.cuda.ExternalStream)."""
+        p = _abi.hip_lib().cbm_stream(self._h)
+        if not p:
+            raise RuntimeError("cbm_stream: no stream (the network is not committed)")
+        return p
+
     STEP_IO_POPULATIONS = ("PC", "BC", "SC", "IO", "NC", "GO")
 
     def step_io(self, mf: np.ndarray, population: str = "NC") -> np.ndarray:
@@ -361,6 +369,17 @@ class Network:
             self.close()
         except Exception:
             pass
+
+
+def stream_ceiling(cells: int, read_words: int = 7, write_words: int = 6, launches: int = 200,
+                   device: int = 0) -> tuple[float, int]:
+    """(mean launch duration in us, bytes per launch) of a hand-written pass
+    that only streams the GR step's per-cell arrays over `cells` cells: the
+    bandwidth a GR pass of that size can reach on this GPU."""
+    us, nbytes = C.c_double(), C.c_uint64()
+    check(_abi.hip_lib().cbm_stream_ceiling(device, cells, read_words, write_words, launches, C.byref(us),
+                                            C.byref(nbytes)), "cbm_stream_ceiling")
+    return us.value, nbytes.value
 
 
 def _device_rows(train):

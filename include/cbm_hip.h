@@ -437,6 +437,19 @@ int cbm_profile_steps(cbm_handle *h, const uint8_t *mf_dev, size_t pitch, int n,
 int cbm_profile_split(cbm_handle *h, const uint8_t *mf_dev, size_t pitch, int n,
                       double *gr_kernel_us, double *small_kernel_us);
 
+/* Streaming ceiling of a per-cell pass on `device` (a measurement aid; no
+ * counterpart in the reference): `cells` cells, each reading `read_words`
+ * 4-byte values (one per array) and one 16-byte record and writing the first
+ * `write_words` of the values back in place -- the GR step's per-cell streams
+ * (read_words 7, write_words 6: five state values, the spike register, the
+ * PF->PC weight; the connectivity record).  Non-temporal loads and stores,
+ * 1024-thread blocks of 4,096 contiguous cells.  Allocates the arrays,
+ * launches `launches` times after 10 untimed launches and reports the mean
+ * launch duration in microseconds (HIP events) and the bytes one launch
+ * moves.  read_words 1..8, write_words <= read_words. */
+int cbm_stream_ceiling(int device, uint64_t cells, int read_words, int write_words, int launches,
+                       double *us_per_launch, uint64_t *bytes_per_launch);
+
 /* The launch mode the handle uses (1, 3 or 4 = fused, 2 = two streams; see
  * cbm_options.stepMode), or a negative CBM_E_* code. */
 int cbm_step_mode(cbm_handle *h);

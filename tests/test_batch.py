@@ -108,3 +108,26 @@ def test_batch_errors(gpu):
     from cerebellum_simulation_amd import hip_lib
     assert hip_lib().cbm_destroy(n1._h) != 0  # still in a batch
     b.close()
+
+
+@pytest.mark.parametrize("pack", ["arrays", "mixed"])
+def test_batch_connectivity_formats(gpu, pack):
+    """Packs whose networks all keep the five connectivity arrays, or mix
+    them with compact records, take their own kernel builds; each network
+    still ends as its own run leaves it."""
+    print(f"[case] batch pack format {pack}", flush=True)
+    plan = _plan()
+    fmts = [1 if pack == "arrays" or i % 2 else 0 for i in range(len(plan))]
+    states = [configs.build_state(cls, seed=seed, gogo_inhibition=inh) for cls, seed, inh in plan]
+    solo = [Network.from_state(st, conn_format=f) for st, f in zip(states, fmts)]
+    grouped = [Network.from_state(st, conn_format=f) for st, f in zip(states, fmts)]
+    steps = 60
+    trs, errs = _trains(states, steps, seed0=3)
+    for net, tr, e in zip(solo, trs, errs):
+        net.run_mf_train(tr, err_drive=e)
+    b = Batch(grouped)
+    b.run_mf_train(trs, err_drive=errs)
+    b.sync()
+    for i, (x, y) in enumerate(zip(solo, grouped)):
+        _assert_same(_snapshot(x), _snapshot(y), f"net {i} {plan[i]} format {fmts[i]}")
+    b.close()
