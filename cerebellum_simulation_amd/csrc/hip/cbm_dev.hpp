@@ -21,10 +21,24 @@ constexpr uint32_t kSmallLdsMax = 152 * 1024;  // dynamic LDS cap of the small k
 // Per-GR arrays are structure-of-arrays of numGR elements; connectivity is
 // packed four 16-bit cell indices per GR (kNoCell = empty slot) and one 8-bit
 // delay (bit index of the spike register) per output slot.
+// The GR history (CBM_F_HIST_GR, one uint64_t per GR in the reference's
+// layout: bit 0 the latest 5-step bin, shifted left at every update) lives on
+// the device as a ring of 64 bit-planes, plane-major: plane q holds one bit
+// per GR, word (q * numGR / 64 + GR / 64), bit GR % 64.  Update c (at step
+// t = c * tsPerHistBinGR) reads bit 39 of the old word from plane (c - 40)
+// mod 64 and writes its bits into plane c mod 64, the plane of the bit the
+// shift drops, so an update moves two bits per GR instead of the 16 bytes of
+// reading and writing the word.  At time T the word's bit k is in plane
+// (T / tsPerHistBinGR - k) mod 64.
+__host__ __device__ inline uint32_t histPlane(uint64_t update, int back)
+{
+    return (uint32_t)((update - (uint64_t)back) & 63u);
+}
+
 struct GrArgs {
     float *v, *thr, *gKCa, *gE, *gI;
     uint32_t *apBuf;
-    uint64_t *hist;
+    uint64_t *hist;            // history bit-planes [64][numGR / 64]
     float *w;                  // [numZones][numGR] PF->PC weights
     const uint2 *mfIn;         // 4 x u16 MF inputs
     const uint2 *goIn;         // 4 x u16 GO inputs
@@ -52,6 +66,10 @@ struct GrStep {
     uint32_t *bcCnt, *scCnt, *goCnt;  // this step's ring slot (zeroed beforehand)
     const int32_t *plastTimer;        // [numZones][numIO] as of the previous step
     uint32_t doPlast;          // t % tsPerHistBinGR == 0: plasticity + history this step
+    // GR history planes of this update (doPlast): bit 39 of each GR's history
+    // word is read from histOld and this update's bits go to histNew
+    const uint64_t *histOld;
+    uint64_t *histNew;
     uint8_t *pfCnt;            // [numGR/64] delayed PF spikes per 64-GR chunk (nullptr: not recorded)
 };
 

@@ -225,7 +225,7 @@ __device__ __forceinline__ void gr_span_impl(const GrArgs &a, const GrStep &st, 
     uint4 rec[CPT];                       // COMPACT
     uint2 mi_[CPT], gi_[CPT], go_[CPT];   // !COMPACT
     uint32_t gd_[CPT], bsp_[CPT];         // !COMPACT
-    uint64_t hist[CPT] = {};              // PLAST
+    uint64_t hist[CPT] = {};              // PLAST: the wave's 64 history words' bit 39, one plane word
     auto cellOf = [&](int j) -> uint32_t {
         const uint32_t o = (uint32_t)(j * NT + tid);
         return first + (FULL ? o : min(o, count - 1u));
@@ -256,7 +256,7 @@ __device__ __forceinline__ void gr_span_impl(const GrArgs &a, const GrStep &st, 
             bsp_[j] = gld<COMPACT>(el(a.bspDly, gr));
         }
         w0[j] = gld<COMPACT>(el(a.w, gr));
-        if constexpr (PLAST) hist[j] = gld<COMPACT>(el(a.hist, gr));
+        if constexpr (PLAST) hist[j] = st.histOld[gr >> 6];  // one address per wave
     }
     __builtin_amdgcn_sched_barrier(0);
 
@@ -357,10 +357,11 @@ __device__ __forceinline__ void gr_span_impl(const GrArgs &a, const GrStep &st, 
 
         uint32_t bit39 = 0;
         if constexpr (PLAST) {
-            const uint64_t h = hist[j];
-            bit39 = (uint32_t)(h >> 39) & 1u;
-            // updateGRHistory (kernels.cu:244-245): mask 0x1F is hard-coded there
-            gst<COMPACT>(el(a.hist, gr), (h << 1) | (uint64_t)(((b & 0x0000001Fu) > 0) * 0x00000001u));
+            bit39 = (uint32_t)(hist[j] >> lane) & 1u;
+            // updateGRHistory (kernels.cu:244-245): mask 0x1F is hard-coded
+            // there; the new bit of every GR of the wave in one plane word
+            const uint64_t nb = __ballot((b & 0x0000001Fu) > 0);
+            if (lane == 0) gst<COMPACT>(st.histNew + (gr >> 6), nb);
         }
 
         // delayed spike towards BC/PC/SC (kernels.cu:256,269)
@@ -1383,6 +1384,9 @@ __global__ void __launch_bounds__(kSmallThreads) batch_step_kernel(const BatchNe
         gs.goCnt = d.goRing + (size_t)slot * g.numGO;
         gs.plastTimer = d.timer2 + (size_t)parPrev * nZ * g.numIO;
         gs.doPlast = (t % d.tsPerHistBinGR) == 0;
+        const uint64_t upd = t / d.tsPerHistBinGR;
+        gs.histOld = g.hist + (size_t)histPlane(upd, 40) * nChunks;
+        gs.histNew = g.hist + (size_t)histPlane(upd, 0) * nChunks;
         gs.pfCnt = nullptr;
         gr_tile<CPT, kSmallThreads, FMT>(g, gs, tt.y, reinterpret_cast<uint32_t *>(lds));
     }

@@ -344,6 +344,62 @@ static int foldLastStep(cbm_handle *h)
     return CBM_OK;
 }
 
+// ---- GR history: the reference's words <-> the device's bit-planes
+// a[i] bit j <-> a[j] bit i (the masked-swap transpose of a 64 x 64 bit block)
+static void transpose64(uint64_t a[64])
+{
+    uint64_t m = 0x00000000FFFFFFFFull;
+    for (int j = 32; j != 0; j >>= 1, m ^= (m << j)) {
+        for (int k = 0; k < 64; k = ((k | j) + 1) & ~j) {
+            const uint64_t t = ((a[k] >> j) ^ a[k | j]) & m;
+            a[k] ^= t << j;
+            a[k | j] ^= t;
+        }
+    }
+}
+
+// the history update index at time T (its words' bit k is in plane histPlane(c, k))
+static uint64_t histUpdate(const cbm_handle *h) { return h->curTime / h->p.tsPerHistBinGR; }
+
+// words[numGR] (bit k of GR g's word) -> planes[64][numGR / 64]
+static void histToPlanes(const uint64_t *words, uint32_t numGR, uint64_t c, uint64_t *planes)
+{
+    const uint32_t nChunks = numGR / 64;
+    uint64_t blk[64];
+    for (uint32_t q = 0; q < nChunks; q++) {
+        std::memcpy(blk, words + (size_t)q * 64, sizeof(blk));
+        transpose64(blk);  // blk[k] bit l = bit k of GR 64q + l
+        for (int k = 0; k < 64; k++) planes[(size_t)histPlane(c, k) * nChunks + q] = blk[k];
+    }
+}
+
+static void histFromPlanes(const uint64_t *planes, uint32_t numGR, uint64_t c, uint64_t *words)
+{
+    const uint32_t nChunks = numGR / 64;
+    uint64_t blk[64];
+    for (uint32_t q = 0; q < nChunks; q++) {
+        for (int k = 0; k < 64; k++) blk[k] = planes[(size_t)histPlane(c, k) * nChunks + q];
+        transpose64(blk);
+        std::memcpy(words + (size_t)q * 64, blk, sizeof(blk));
+    }
+}
+
+static int writeHist(cbm_handle *h, const void *src)
+{
+    std::vector<uint64_t> planes(h->s.numGR);
+    histToPlanes((const uint64_t *)src, h->s.numGR, histUpdate(h), planes.data());
+    HIP_TRY(hipMemcpy(h->dfield[CBM_F_HIST_GR], planes.data(), planes.size() * 8, hipMemcpyHostToDevice));
+    return CBM_OK;
+}
+
+static int readHist(cbm_handle *h, void *dst)
+{
+    std::vector<uint64_t> planes(h->s.numGR);
+    HIP_TRY(hipMemcpy(planes.data(), h->dfield[CBM_F_HIST_GR], planes.size() * 8, hipMemcpyDeviceToHost));
+    histFromPlanes(planes.data(), h->s.numGR, histUpdate(h), (uint64_t *)dst);
+    return CBM_OK;
+}
+
 int cbm_write(cbm_handle *h, int field, int zone, const void *src, size_t bytes)
 {
     if (!h || (!src && bytes)) return fail(CBM_E_ARG, "null argument");
@@ -387,11 +443,14 @@ int cbm_write(cbm_handle *h, int field, int zone, const void *src, size_t bytes)
         if ((rc = cbm_read(h, CBM_F_PIPE_PFSC_SUM, 0, sc.data(), sc.size()))) return rc;
         if ((rc = cbm_read(h, CBM_F_PIPE_GRGO_SUM, 0, go.data(), go.size()))) return rc;
         std::vector<uint8_t> apgo(h->s.numGO), tim(h->host[CBM_F_PFPC_PLAST_TIMER_IO].size());
+        std::vector<uint64_t> hist(h->s.numGR);  // its planes are placed by the time
         const size_t tz = h->zoneBytes(CBM_F_PFPC_PLAST_TIMER_IO);
+        if ((rc = readHist(h, hist.data()))) return rc;
         if ((rc = cbm_read(h, CBM_F_AP_GO, 0, apgo.data(), apgo.size()))) return rc;
         for (uint32_t z = 0; z < h->s.numZones; z++)
             if ((rc = cbm_read(h, CBM_F_PFPC_PLAST_TIMER_IO, (int)z, tim.data() + z * tz, tz))) return rc;
         std::memcpy(&h->curTime, src, 8);
+        if ((rc = writeHist(h, hist.data()))) return rc;
         if ((rc = cbm_write(h, CBM_F_AP_GO, 0, apgo.data(), apgo.size()))) return rc;
         for (uint32_t z = 0; z < h->s.numZones; z++)
             if ((rc = cbm_write(h, CBM_F_PFPC_PLAST_TIMER_IO, (int)z, tim.data() + z * tz, tz))) return rc;
@@ -408,6 +467,7 @@ int cbm_write(cbm_handle *h, int field, int zone, const void *src, size_t bytes)
         std::memcpy(&h->ltp[zone], (const uint8_t *)src + 4, 4);
         return CBM_OK;
     case CBM_F_MF_INPUT: HIP_TRY(hipMemcpy(h->dmf, src, bytes, hipMemcpyHostToDevice)); return CBM_OK;
+    case CBM_F_HIST_GR: return writeHist(h, src);
     case CBM_F_PC_POPACT_CTRS:
         if (((const uint32_t *)src)[1] >= h->s.numPopHistBinsPC)
             return fail(CBM_E_ARG, "PC_POPACT_CTRS[1] must be below numPopHistBinsPC");
@@ -628,7 +688,11 @@ int cbm_commit(cbm_handle *h)
         if (!deviceField(f)) continue;
         const size_t b = h->host[f].size();
         if ((rc = h->dalloc(&h->dfield[f], b))) return rc;
-        if (b) HIP_TRY(hipMemcpy(h->dfield[f], h->host[f].data(), b, hipMemcpyHostToDevice));
+        if (f == CBM_F_HIST_GR) {
+            if ((rc = writeHist(h, h->host[f].data()))) return rc;
+        } else if (b) {
+            HIP_TRY(hipMemcpy(h->dfield[f], h->host[f].data(), b, hipMemcpyHostToDevice));
+        }
     }
     const uint32_t par = (uint32_t)(h->curTime & 1u);
     if ((rc = h->dalloc((void **)&h->apGO2, 2 * (size_t)s.numGO))) return rc;
@@ -760,6 +824,7 @@ int cbm_read(cbm_handle *h, int field, int zone, void *dst, size_t bytes)
         std::memcpy((uint8_t *)dst + 4, &h->ltp[zone], 4);
         return CBM_OK;
     case CBM_F_MF_INPUT: HIP_TRY(hipMemcpy(dst, h->dmf, bytes, hipMemcpyDeviceToHost)); return CBM_OK;
+    case CBM_F_HIST_GR: return readHist(h, dst);
     case CBM_F_ZONE_RNG_STATE:
         HIP_TRY(hipMemcpy(dst, h->rngState + (size_t)zone * 353, bytes, hipMemcpyDeviceToHost));
         return CBM_OK;
@@ -1065,6 +1130,8 @@ static int launchStep(cbm_handle *h, const uint8_t *mf, int timing)
     gs.goCnt = h->goRing + (size_t)slot * s.numGO;
     gs.plastTimer = h->timer2 + (size_t)parPrev * s.numZones * s.numIO;
     gs.doPlast = (t % p.tsPerHistBinGR) == 0;
+    gs.histOld = g.hist + (size_t)histPlane(t / p.tsPerHistBinGR, 40) * (s.numGR / 64);
+    gs.histNew = g.hist + (size_t)histPlane(t / p.tsPerHistBinGR, 0) * (s.numGR / 64);
     gs.pfCnt = (h->rec && h->recOffPF != 0xFFFFFFFFu)
                    ? h->rec + (size_t)(t % h->smallHost.recCap) * h->smallHost.recRow + h->recOffPF
                    : nullptr;
