@@ -204,7 +204,8 @@ void listState(cbm_handle *h, std::vector<std::pair<void *, size_t>> *out)
     out->push_back({h->bcRing, (size_t)kRing * s.numBC * 4});
     out->push_back({h->scRing, (size_t)kRing * s.numSC * 4});
     out->push_back({h->goRing, (size_t)kRing * s.numGO * 4});
-    out->push_back({h->rngState, nZ * 353 * 4});
+    out->push_back({h->rngState, 2 * nZ * kRngZone * 4});
+    out->push_back({h->rngNext, 2 * nZ * kRngNext * 4});
     out->push_back({h->dmf, s.numMF});
 }
 

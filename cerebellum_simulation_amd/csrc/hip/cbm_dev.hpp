@@ -14,6 +14,12 @@ constexpr int kMaxIO = 64;
 constexpr uint16_t kNoCell = 0xFFFFu;   // unused connectivity slot
 constexpr uint8_t kNever = 0xFFu;       // delay mask 0: never delivers
 constexpr int kRing = 3;                // input-sum ring slots (t, t-1, t-2)
+// Zone generator state on the device, per zone: the 352 words of the current
+// SFMT block, the index of the next unused word and the block's generation
+// (blocks regenerated since the host last set the state); and the prebuilt
+// next block, 352 words and the generation + 1 it holds (a tag that matches
+// nothing after a host write).  Both are double-buffered by step parity.
+constexpr int kRngZone = 354, kRngNext = 353;
 constexpr int kSmallThreads = 1024;     // threads of the small-population kernel
 constexpr uint32_t kSmallLdsMax = 152 * 1024;  // dynamic LDS cap of the small kernel
 
@@ -83,7 +89,8 @@ struct SmallLds {
     uint32_t synPC, synN, synA, synP, synIO;
     uint32_t apPC, apBC, apSC, apIO, apNC;  // u8 spikes of this step (zones stacked)
     uint32_t vIO, rel;             // f32 [nZ*numIO], [nZ*numNC]
-    uint32_t rng;                  // u32 [nZ][353]: generator words + index
+    uint32_t rng;                  // u32 [nZ][kRngZone]: generator words, index, generation
+    uint32_t rngNext;              // u32 [nZ][kRngNext]: the prebuilt next block and its tag
     uint32_t pop;                  // u32 [nZ][popStride]: 3 counters, bins, 2 flags
     uint32_t popStride;
     uint32_t idx;                  // u16 blob (16-byte copy units below)
@@ -108,7 +115,8 @@ struct SmallArgs {
     const float *chunk2;       // [2][numZones][numGR/64] chunk sums by step parity
     float *pfpcRing;           // [kRing][numZones][numPC]
     uint32_t *bcRing, *scRing, *goRing;  // [kRing][n]
-    uint32_t *rngState;        // [numZones][353]
+    uint32_t *rngState;        // [2][numZones][kRngZone], by step parity
+    uint32_t *rngNext;         // [2][numZones][kRngNext], by step parity
     uint8_t *apGO2;            // [2][numGO] GO spikes by step parity
     int32_t *timer2;           // [2][numZones][numIO] PF->PC plasticity timers by step parity
     const uint4 *idxBlob;      // u16 index lists of the microzone outputs, staged into LDS

@@ -65,7 +65,7 @@ struct cbm_handle {
     uint32_t *goOutDly = nullptr;
     uint8_t *bspDly = nullptr;
     float *chunk2 = nullptr, *pfpcRing = nullptr;  // chunk sums [2][nZ][numGR/64] by step parity
-    uint32_t *bcRing = nullptr, *scRing = nullptr, *goRing = nullptr, *rngState = nullptr;
+    uint32_t *bcRing = nullptr, *scRing = nullptr, *goRing = nullptr, *rngState = nullptr, *rngNext = nullptr;
     uint8_t *apGO2 = nullptr;   // [2][numGO]
     int32_t *timer2 = nullptr;  // [2][nZ][numIO]
     bool skipFold = true;       // the ring slot of the last step holds host-written PF->PC sums

@@ -659,8 +659,11 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
     // InNet::updateMFtoGOOut (innet.cpp:912-931): each spiking MF adds one to
     // every GO it targets; the counts are integers, so the order is free
     {
-        const uint32_t nMFw = s.numMF / 4, nRng = nZ * 353, nPop = nZ * L.popStride;
-        const uint32_t n = max(max(max(nMFw, L.idx16), max(nRng, nPop)), s.numMF);
+        const uint32_t nMFw = s.numMF / 4, nRng = nZ * kRngZone, nNext = nZ * kRngNext, nPop = nZ * L.popStride;
+        const uint32_t n = max(max(max(nMFw, L.idx16), max(max(nRng, nNext), nPop)), s.numMF);
+        const G<uint32_t> *rngIn = gp(a.rngState) + (size_t)parPrev * nRng;
+        const G<uint32_t> *nextIn = gp(a.rngNext) + (size_t)parPrev * nNext;
+        uint32_t *s_next = at<uint32_t>(lds, L.rngNext);
         const G<uint32_t> *mfw = (const G<uint32_t> *)st.mf;
         uint32_t *mfl = at<uint32_t>(lds, L.mf);
         u32x4 *bd = at<u32x4>(lds, L.idx);
@@ -672,7 +675,8 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
             const uint32_t im = min(i, s.numMF - 1), ip = min(i, nPop - 1);
             const uint32_t w = mfw[min(i, nMFw - 1)];
             const u32x4 bl = ((const G<u32x4> *)a.idxBlob)[min(i, L.idx16 - 1)];
-            const uint32_t r = gp(a.rngState)[min(i, nRng - 1)];
+            const uint32_t r = rngIn[min(i, nRng - 1)];
+            const uint32_t rn = nextIn[min(i, nNext - 1)];
             const uint32_t pz = ip / L.popStride, pk = ip % L.popStride;
             const G<uint32_t> *pw = pk < 3 ? F<uint32_t>(a, CBM_F_PC_POPACT_CTRS) + pz * 3 + pk
                                            : F<uint32_t>(a, CBM_F_HIST_PC_POPACT) + (size_t)pz * bins +
@@ -686,6 +690,7 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
             if (i < nMFw) mfl[i] = w;
             if (i < L.idx16) bd[i] = bl;
             if (i < nRng) s_rng[i] = r;
+            if (i < nNext) s_next[i] = rn;
             if (i < nPop) s_pop[i] = pv;
             if (i < s.numMF) {
                 const uint8_t ap = m > 0;
@@ -1152,21 +1157,30 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
     // MZone::updateNCOut draws (mzone.cpp:505-511): one Random() per (IO, NC)
     // in row order from the zone's stream.  Draw k of a run uses generator
     // words ix+2k, ix+2k+1, so up to 64 draws that fit before the next
-    // regeneration are taken at once, one per lane.
+    // regeneration are taken at once, one per lane.  A regeneration takes the
+    // block prebuilt by an earlier launch when its tag says it follows the
+    // current one, and computes it here otherwise.
     if (wave == 0) {
         auto inNC = F<uint8_t>(a, CBM_F_INPUT_NC_IO);
         const uint32_t n = s.numIO * kNCIO;
+        const uint32_t *s_next = at<uint32_t>(lds, L.rngNext);
         for (uint32_t z = 0; z < nZ; z++) {
-            uint32_t *g = s_rng + (size_t)z * 353;
-            uint32_t ix = g[352];
+            uint32_t *g = s_rng + (size_t)z * kRngZone;
+            const uint32_t *nx = s_next + (size_t)z * kRngNext;
+            uint32_t ix = g[352], gen = g[353];
             const uint16_t *pn = s_idx + L.iNCIO + (size_t)z * n;
             const float *rel = s_rel + (size_t)z * s.numNC;
 #pragma unroll 1
             for (uint32_t done = 0; done < n;) {
                 if (ix >= 351) {
                     __threadfence_block();
-                    if (lane < 4) sfmt_regen4(g, lane);
+                    if (nx[352] == gen + 1u) {  // wave-uniform
+                        for (uint32_t w = lane; w < 352; w += 64) g[w] = nx[w];
+                    } else if (lane < 4) {
+                        sfmt_regen4(g, lane);
+                    }
                     __threadfence_block();
+                    gen += 1u;
                     ix = 0;
                 }
                 uint32_t k = (352u - ix) >> 1;
@@ -1181,10 +1195,13 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
                 ix += 2 * k;
                 done += k;
             }
-            if (lane == 0) g[352] = ix;
+            if (lane == 0) {
+                g[352] = ix;
+                g[353] = gen;
+            }
         }
         __threadfence_block();
-        for (uint32_t i = lane; i < nZ * 353; i += 64) gp(a.rngState)[i] = s_rng[i];
+        for (uint32_t i = lane; i < nZ * kRngZone; i += 64) gp(a.rngState)[(size_t)par * nZ * kRngZone + i] = s_rng[i];
     }
     // GO->GO synaptic depression scale (innet.cpp:952-954), after the GABA
     // gather above has used the previous scale
@@ -1219,6 +1236,32 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
     }
 }
 
+// The zone generators' next blocks, built ahead of use by the last block of
+// a fused launch (one wave; lanes 4z..4z+3 build zone z as sfmt_regen4 does):
+// from the state the previous step left, block gen + 1 into this step's
+// slot, or a copy of the previous slot when that already holds it.  The
+// small step takes a prebuilt block only from a slot an earlier launch wrote.
+__device__ void sfmt_prebuild(const SmallArgs &a, const SmallStep &st, uint32_t *lds)
+{
+    const uint32_t nZ = a.s.numZones, lane = threadIdx.x, z = lane >> 2, k = lane & 3u;
+    if (threadIdx.x >= 64 || z >= nZ) return;  // whole quads
+    const uint32_t par = st.par, parPrev = par ^ 1u;
+    const G<uint32_t> *cur = gp(a.rngState) + ((size_t)parPrev * nZ + z) * kRngZone;
+    const G<uint32_t> *nin = gp(a.rngNext) + ((size_t)parPrev * nZ + z) * kRngNext;
+    G<uint32_t> *nout = gp(a.rngNext) + ((size_t)par * nZ + z) * kRngNext;
+    const uint32_t gen = cur[353], tag = nin[352];
+    if (tag == gen + 1u) {
+        for (uint32_t w = k; w < 352; w += 4) nout[w] = nin[w];
+        if (k == 0) nout[352] = tag;
+        return;
+    }
+    uint32_t *blk = lds + (size_t)z * 352;  // lane k touches only words 4i + k
+    for (uint32_t w = k; w < 352; w += 4) blk[w] = cur[w];
+    sfmt_regen4(blk, (int)k);
+    for (uint32_t w = k; w < 352; w += 4) nout[w] = blk[w];
+    if (k == 0) nout[352] = gen + 1u;
+}
+
 // the small populations alone (the two-stream step, and per-kernel timing)
 __global__ void __launch_bounds__(kSmallThreads) small_step_kernel(const SmallArgs *__restrict__ args,
                                                                    const SmallStep st)
@@ -1229,13 +1272,15 @@ __global__ void __launch_bounds__(kSmallThreads) small_step_kernel(const SmallAr
 
 // One launch per step: block 0 steps the small populations, blocks 1.. are GR
 // tiles of 1024 threads (one block per CU: the small step needs up to 128
-// VGPRs per thread).
+// VGPRs per thread), and the last block prebuilds the generators' next blocks.
 template <int CPT, int FMT>
 __global__ void __launch_bounds__(kSmallThreads) step_kernel(const SmallArgs *__restrict__ args, const SmallStep st, const GrArgs g, const GrStep gs)
 {
     extern __shared__ __align__(16) uint8_t lds[];
     if (blockIdx.x == 0)
         small_step(*args, st, st.errDrive, lds);
+    else if (blockIdx.x == gridDim.x - 1)
+        sfmt_prebuild(*args, st, reinterpret_cast<uint32_t *>(lds));
     else
         gr_tile<CPT, kSmallThreads, FMT>(g, gs, blockIdx.x - 1, reinterpret_cast<uint32_t *>(lds));
 }
@@ -1245,9 +1290,9 @@ template __global__ void step_kernel<2, kFmtCompact>(const SmallArgs *, const Sm
 template __global__ void step_kernel<1, kFmtArrays>(const SmallArgs *, const SmallStep, const GrArgs, const GrStep);
 template __global__ void step_kernel<2, kFmtArrays>(const SmallArgs *, const SmallStep, const GrArgs, const GrStep);
 
-// The same with the GR cells split evenly over `spans` blocks (one per
-// remaining CU): all of a step's GR loads are issued in one wave of blocks
-// that runs beside the small step.
+// The same with the GR cells split evenly over `spans` blocks (one per CU
+// but the small step's and the generator block's): all of a step's GR loads
+// are issued in one wave of blocks that runs beside the small step.
 template <int CPT, int FMT>
 __global__ void __launch_bounds__(kSmallThreads) step_span_kernel(const SmallArgs *__restrict__ args, const SmallStep st,
                                                                   const GrArgs g, const GrStep gs, const uint32_t spans)
@@ -1255,6 +1300,8 @@ __global__ void __launch_bounds__(kSmallThreads) step_span_kernel(const SmallArg
     extern __shared__ __align__(16) uint8_t lds[];
     if (blockIdx.x == 0)
         small_step(*args, st, st.errDrive, lds);
+    else if (blockIdx.x > spans)
+        sfmt_prebuild(*args, st, reinterpret_cast<uint32_t *>(lds));
     else
         gr_span_of<CPT, kSmallThreads, FMT>(g, gs, blockIdx.x - 1, spans, reinterpret_cast<uint32_t *>(lds));
 }

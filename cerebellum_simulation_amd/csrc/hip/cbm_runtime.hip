@@ -129,7 +129,8 @@ SmallLds smallLayout(const cbm_sizes &s, uint32_t slotsBCPC, uint32_t slotsPCBC,
     L.apNC = take(nZ * s.numNC);
     L.vIO = take(4 * nZ * s.numIO);
     L.rel = take(4 * nZ * s.numNC);
-    L.rng = take(4 * nZ * 353);
+    L.rng = take(4 * nZ * kRngZone);
+    L.rngNext = take(4 * nZ * kRngNext);
     L.popStride = 3 + s.numPopHistBinsPC + 2;
     L.pop = take(4 * nZ * L.popStride);
     uint32_t e = 0;  // u16 elements
@@ -444,13 +445,18 @@ int cbm_write(cbm_handle *h, int field, int zone, const void *src, size_t bytes)
         if ((rc = cbm_read(h, CBM_F_PIPE_GRGO_SUM, 0, go.data(), go.size()))) return rc;
         std::vector<uint8_t> apgo(h->s.numGO), tim(h->host[CBM_F_PFPC_PLAST_TIMER_IO].size());
         std::vector<uint64_t> hist(h->s.numGR);  // its planes are placed by the time
+        std::vector<uint32_t> rng((size_t)h->s.numZones * 353);  // in the parity slot of the time
         const size_t tz = h->zoneBytes(CBM_F_PFPC_PLAST_TIMER_IO);
         if ((rc = readHist(h, hist.data()))) return rc;
+        for (uint32_t z = 0; z < h->s.numZones; z++)
+            if ((rc = cbm_read(h, CBM_F_ZONE_RNG_STATE, (int)z, &rng[(size_t)z * 353], 353 * 4))) return rc;
         if ((rc = cbm_read(h, CBM_F_AP_GO, 0, apgo.data(), apgo.size()))) return rc;
         for (uint32_t z = 0; z < h->s.numZones; z++)
             if ((rc = cbm_read(h, CBM_F_PFPC_PLAST_TIMER_IO, (int)z, tim.data() + z * tz, tz))) return rc;
         std::memcpy(&h->curTime, src, 8);
         if ((rc = writeHist(h, hist.data()))) return rc;
+        for (uint32_t z = 0; z < h->s.numZones; z++)
+            if ((rc = cbm_write(h, CBM_F_ZONE_RNG_STATE, (int)z, &rng[(size_t)z * 353], 353 * 4))) return rc;
         if ((rc = cbm_write(h, CBM_F_AP_GO, 0, apgo.data(), apgo.size()))) return rc;
         for (uint32_t z = 0; z < h->s.numZones; z++)
             if ((rc = cbm_write(h, CBM_F_PFPC_PLAST_TIMER_IO, (int)z, tim.data() + z * tz, tz))) return rc;
@@ -472,10 +478,20 @@ int cbm_write(cbm_handle *h, int field, int zone, const void *src, size_t bytes)
         if (((const uint32_t *)src)[1] >= h->s.numPopHistBinsPC)
             return fail(CBM_E_ARG, "PC_POPACT_CTRS[1] must be below numPopHistBinsPC");
         break;
-    case CBM_F_ZONE_RNG_STATE:
+    case CBM_F_ZONE_RNG_STATE: {
         if (((const uint32_t *)src)[352] > 352) return fail(CBM_E_ARG, "generator index above 352");
-        HIP_TRY(hipMemcpy(h->rngState + (size_t)zone * 353, src, bytes, hipMemcpyHostToDevice));
+        // into this step's parity slot with generation 0; both prebuilt blocks
+        // of the zone lose their tag
+        const size_t nZ = h->s.numZones, par = h->curTime & 1u;
+        uint32_t st[kRngZone];
+        std::memcpy(st, src, 353 * 4);
+        st[353] = 0;
+        HIP_TRY(hipMemcpy(h->rngState + (par * nZ + (size_t)zone) * kRngZone, st, sizeof(st), hipMemcpyHostToDevice));
+        const uint32_t none = 0;
+        for (size_t q = 0; q < 2; q++)
+            HIP_TRY(hipMemcpy(h->rngNext + (q * nZ + (size_t)zone) * kRngNext + 352, &none, 4, hipMemcpyHostToDevice));
         return CBM_OK;
+    }
     case CBM_F_PIPE_PFPC_SUM: {
         const size_t row = (size_t)h->s.numPC * 4;
         if ((rc = foldLastStep(h))) return rc;  // other zones' row 0 keeps its value
@@ -709,14 +725,15 @@ int cbm_commit(cbm_handle *h)
     if ((rc = h->dalloc((void **)&h->dmf, s.numMF))) return rc;
     HIP_TRY(hipMemcpy(h->dmf, h->host[CBM_F_MF_INPUT].data(), s.numMF, hipMemcpyHostToDevice));
     {
-        std::vector<uint32_t> rs((size_t)nZ * 353);
+        // this step's parity slot holds the state; generation 0, no prebuilt block
+        std::vector<uint32_t> rs(2 * (size_t)nZ * kRngZone, 0u), rn(2 * (size_t)nZ * kRngNext, 0u);
         for (uint32_t z = 0; z < nZ; z++) {
             cbm::Sfmt g(h->H<int32_t>(CBM_F_ZONE_RNG_SEED, z)[0]);
             const uint64_t skip = h->H<uint64_t>(CBM_F_ZONE_RNG_SKIP, z)[0];
             for (uint64_t k = 0; k < skip; k++) (void)g.Random();
-            g.save(&rs[(size_t)z * 353]);
+            g.save(&rs[((size_t)par * nZ + z) * kRngZone]);
         }
-        if ((rc = h->upload(&h->rngState, rs))) return rc;
+        if ((rc = h->upload(&h->rngState, rs)) || (rc = h->upload(&h->rngNext, rn))) return rc;
     }
     {
         SmallArgs a;
@@ -727,6 +744,7 @@ int cbm_commit(cbm_handle *h)
         a.scRing = h->scRing;
         a.goRing = h->goRing;
         a.rngState = h->rngState;
+        a.rngNext = h->rngNext;
         a.apGO2 = h->apGO2;
         a.timer2 = h->timer2;
         a.idxBlob = reinterpret_cast<const uint4 *>(dBlob);
@@ -759,7 +777,8 @@ int cbm_commit(cbm_handle *h)
         if (h->mode == 1 || h->mode == 4) {
             int cus = 0;
             HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->o.device));
-            const uint32_t spans = cus > 1 ? (uint32_t)cus - 1 : 1u;
+            // one CU each for the small step and the generator block
+            const uint32_t spans = cus > 2 ? (uint32_t)cus - 2 : 1u;
             const uint32_t per = (s.numGR / 64 + spans - 1) / spans * 64;  // most cells of a span
             const int cpt = per <= 2048 ? 2 : per <= 3072 ? 3 : per <= 5120 ? 5 : 0;
             if (cpt && (h->mode == 4 || per >= 1024)) {
@@ -826,7 +845,8 @@ int cbm_read(cbm_handle *h, int field, int zone, void *dst, size_t bytes)
     case CBM_F_MF_INPUT: HIP_TRY(hipMemcpy(dst, h->dmf, bytes, hipMemcpyDeviceToHost)); return CBM_OK;
     case CBM_F_HIST_GR: return readHist(h, dst);
     case CBM_F_ZONE_RNG_STATE:
-        HIP_TRY(hipMemcpy(dst, h->rngState + (size_t)zone * 353, bytes, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(dst, h->rngState + ((size_t)(h->curTime & 1u) * h->s.numZones + (size_t)zone) * kRngZone,
+                          bytes, hipMemcpyDeviceToHost));
         return CBM_OK;
     case CBM_F_AP_GR: {
         std::vector<uint32_t> ab(h->s.numGR);
@@ -1174,7 +1194,7 @@ static int launchStep(cbm_handle *h, const uint8_t *mf, int timing)
         const SmallArgs *sa = h->smallArgs;
         uint32_t spans = h->spans;
         void *args[] = {&sa, &st, &g, &gs, &spans};  // the tile kernel takes the first four
-        HIP_TRY(hipLaunchKernel(fusedKernel(h), dim3(1 + (spans ? spans : tiles)), dim3(kSmallThreads), args,
+        HIP_TRY(hipLaunchKernel(fusedKernel(h), dim3(2 + (spans ? spans : tiles)), dim3(kSmallThreads), args,
                                 h->fusedLds, h->stream));
         if (timing == kTimeKernel) HIP_TRY(hipEventRecord(h->ev[1], h->stream));
     } else {

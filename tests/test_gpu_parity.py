@@ -432,3 +432,63 @@ def test_ginc_grgo_knob_mid_run(gpu, mode):
         else:
             assert not np.array_equal(net.read("G_GR_GO"), ref.read("G_GR_GO"))
     assert net.read("GINC_GR_TO_GO")[0] == np.float32(st.params.gIncGRtoGO)
+
+
+@pytest.mark.parametrize("cfg,mode", [("t8k2z", 1), ("t8k2z", 2), ("b262k", 1)])
+def test_generator_prebuilt_blocks(gpu, cfg, mode):
+    """The fused launches' last block builds each zone's next SFMT block ahead
+    of use, and the small step swaps it in when its tag shows that it follows
+    the current block; the two-stream launch has no such block and
+    regenerates in place.  Over several regenerations per zone, a host write
+    of ZONE_RNG_STATE (which voids the prebuilt blocks) and a CUR_TIME change
+    by an odd amount (which moves the state to the other parity slot), the GPU
+    equals the binary32 oracle in every field.  (t8k2z: two zones, GR tiles;
+    b262k: the even GR split.)"""
+    import torch
+    import orc as O
+    print(f"[case] generator prebuilt blocks {cfg} mode={mode}", flush=True)
+    st = configs.build_state(cfg, seed=0)
+    s, nz = st.sizes, st.num_zones
+    net = Network.from_state(st, step_mode=mode)
+    o32 = O.Oracle.from_state(st, precision=32)
+    steps, chunk = 480, 40
+    tr = configs.mf_train(s.numMF, steps, seed=7, trial_ms=240, cs_on=40, cs_off=200)
+    err = configs.us_err_drive(steps, nz, t0=0, trial_ms=240, us_ms=199)
+    dtr = torch.from_numpy(tr).cuda()
+    draws = s.numIO * s.numpIOfromNCtoIO  # Random() calls per zone and step
+
+    def advance(ix, n_steps):
+        """(index, regenerations) after n_steps of draws: two words per draw,
+        a new block when fewer than two words are left (sfmt.cpp:337-355)."""
+        regen = 0
+        for _ in range(n_steps):
+            for _ in range(draws):
+                if ix >= 351:
+                    ix, regen = 0, regen + 1
+                ix += 2
+        return ix, regen
+
+    regens = [0] * nz
+    ixs = [int(net.read("ZONE_RNG_STATE", z)[352]) for z in range(nz)]
+    for k0 in range(0, steps, chunk):
+        if k0 == 160:
+            donor = o32.read("ZONE_RNG_STATE", nz - 1).copy()
+            donor[352] = 17  # a mid-block index
+            for m in (net, o32):
+                m.write("ZONE_RNG_STATE", donor, 0)
+            ixs[0] = 17
+        if k0 == 320:
+            t = net.read("CUR_TIME").copy()
+            t[0] += 1001
+            for m in (net, o32):
+                m.write("CUR_TIME", t)
+        net.run_mf_train(dtr[k0:k0 + chunk], err_drive=err[k0:k0 + chunk])
+        o32.run_mf_train(tr[k0:k0 + chunk], err_drive=err[k0:k0 + chunk])
+        for z in range(nz):
+            ixs[z], r = advance(ixs[z], chunk)
+            regens[z] += r
+            assert int(net.read("ZONE_RNG_STATE", z)[352]) == ixs[z]
+        bad = compare(net, o32, nz)
+        print(f"steps {k0}-{k0 + chunk}: regenerations so far {regens}, {bad}", flush=True)
+        assert_parity(bad, where=f"{cfg} mode {mode} after step {k0 + chunk}")
+    assert min(regens) >= 15, regens  # (t8k2z: about 21 per zone, b262k: about 87)
