@@ -38,8 +38,17 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef uint16_t u16;
 
-constexpr int BM = 128, BN = 128, BK = 64, LDK = BK + 8;  // LDS row stride: 144 B
+#ifndef SG_BK
+#define SG_BK 32
+#endif
+// 128 x 128 output tile per 256-thread block, K steps of BK through
+// double-buffered LDS (row stride BK + 8 elements)
+constexpr int BM = 128, BN = 128, BK = SG_BK, LDK = BK + 8;
 constexpr int kThreads = 256;
+static_assert(BK == 32 || BK == 64, "K step");
+// operand loader of conv_kernel: 16-byte vectors, kVec per tile row; a pass of
+// the block covers kRowStep rows, kPass passes cover a 128-row tile
+constexpr int kVec = BK / 8, kRowStep = kThreads / kVec, kPass = BM / kRowStep;
 
 enum { SRC_BF16 = 0, SRC_SPIKE = 1 };
 enum { EPI_RELU = 0, EPI_SIGMOID = 1, EPI_BCE = 2, EPI_MASK = 3 };
@@ -111,14 +120,16 @@ __global__ void __launch_bounds__(kThreads) conv_kernel(const ConvArgs a)
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
 
-    // loader: 4 rows (lr + 32*i) x 8 consecutive k per thread, for A and B.
-    // Per row: its position in the segment and the source row of position 0.
-    const int lr = tid >> 3, lk = (tid & 7) * 8;
-    int64_t ab0, ab1, ab2, ab3;
-    int ap0, ap1, ap2, ap3;
+    // loader: kPass rows (lr + kRowStep*i) x 8 consecutive k per thread, for
+    // A and B.  Per row: its position in the segment and the source row of
+    // position 0.
+    const int lr = tid / kVec, lk = (tid % kVec) * 8;
+    // (scalars, not arrays: staging arrays were left in scratch memory)
+    int64_t ab0 = 0, ab1 = 0, ab2 = 0, ab3 = 0;
+    int ap0 = 0, ap1 = 0, ap2 = 0, ap3 = 0;
 #define SG_ROW(i, AB, AP)                                                       \
-    {                                                                           \
-        const int r = m0 + lr + 32 * i;                                         \
+    if (i < kPass) {                                                            \
+        const int r = m0 + lr + kRowStep * i;                                   \
         const int s = r < a.M ? r / a.P : 0;                                    \
         AP = r < a.M ? r - s * a.P : -(1 << 30);                                \
         AB = r < a.M ? (a.seg ? a.seg[s] : (int64_t)s * a.P) : 0;               \
@@ -127,8 +138,8 @@ __global__ void __launch_bounds__(kThreads) conv_kernel(const ConvArgs a)
 #undef SG_ROW
 
     uint4 ra0, ra1, ra2, ra3, rb0, rb1, rb2, rb3;
-#define SG_LOADA(AB, AP, RA)                                                                            \
-    {                                                                                                   \
+#define SG_LOADA(i, AB, AP, RA)                                                                         \
+    if (i < kPass) {                                                                                    \
         const int sp = AP + tap * a.shift;                                                              \
         const bool ok = sp >= 0 && sp < a.P;                                                            \
         const int64_t row = AB + (ok ? sp : 0);                                                         \
@@ -138,27 +149,25 @@ __global__ void __launch_bounds__(kThreads) conv_kernel(const ConvArgs a)
             RA = ok ? spikes_to_bf16(*(const uint2 *)((const uint8_t *)a.A + (size_t)row * a.lda + c))  \
                     : make_uint4(0, 0, 0, 0);                                                           \
     }
-#define SG_LOAD(kb)                                                                   \
-    {                                                                                 \
-        const int k0 = (kb) * BK;                                                     \
-        const int tap = k0 / a.cin, c = k0 - tap * a.cin + lk;                        \
-        SG_LOADA(ab0, ap0, ra0) SG_LOADA(ab1, ap1, ra1) SG_LOADA(ab2, ap2, ra2) SG_LOADA(ab3, ap3, ra3) \
-        const u16 *bp = a.Bt + (size_t)(n0 + lr) * a.K + k0 + lk;                     \
-        rb0 = *(const uint4 *)bp;                                                     \
-        rb1 = *(const uint4 *)(bp + (size_t)32 * a.K);                                \
-        rb2 = *(const uint4 *)(bp + (size_t)64 * a.K);                                \
-        rb3 = *(const uint4 *)(bp + (size_t)96 * a.K);                                \
+#define SG_LOADB(i, RB) \
+    if (i < kPass) RB = *(const uint4 *)(bp + (size_t)(kRowStep * i) * a.K);
+#define SG_LOAD(kb)                                                                                     \
+    {                                                                                                   \
+        const int k0 = (kb) * BK;                                                                       \
+        const int tap = k0 / a.cin, c = k0 - tap * a.cin + lk;                                          \
+        SG_LOADA(0, ab0, ap0, ra0) SG_LOADA(1, ab1, ap1, ra1) SG_LOADA(2, ab2, ap2, ra2) SG_LOADA(3, ab3, ap3, ra3) \
+        const u16 *bp = a.Bt + (size_t)(n0 + lr) * a.K + k0 + lk;                                       \
+        SG_LOADB(0, rb0) SG_LOADB(1, rb1) SG_LOADB(2, rb2) SG_LOADB(3, rb3)                             \
     }
-#define SG_STORE(buf)                                                 \
-    {                                                                 \
-        *(uint4 *)&sA[buf][(lr + 0) * LDK + lk] = ra0;                \
-        *(uint4 *)&sA[buf][(lr + 32) * LDK + lk] = ra1;               \
-        *(uint4 *)&sA[buf][(lr + 64) * LDK + lk] = ra2;               \
-        *(uint4 *)&sA[buf][(lr + 96) * LDK + lk] = ra3;               \
-        *(uint4 *)&sB[buf][(lr + 0) * LDK + lk] = rb0;                \
-        *(uint4 *)&sB[buf][(lr + 32) * LDK + lk] = rb1;               \
-        *(uint4 *)&sB[buf][(lr + 64) * LDK + lk] = rb2;               \
-        *(uint4 *)&sB[buf][(lr + 96) * LDK + lk] = rb3;               \
+#define SG_STORE1(i, RA, RB)                                               \
+    if (i < kPass) {                                                       \
+        *(uint4 *)&sA[buf_][(lr + kRowStep * i) * LDK + lk] = RA;          \
+        *(uint4 *)&sB[buf_][(lr + kRowStep * i) * LDK + lk] = RB;          \
+    }
+#define SG_STORE(buf)                                                                          \
+    {                                                                                          \
+        const int buf_ = (buf);                                                                \
+        SG_STORE1(0, ra0, rb0) SG_STORE1(1, ra1, rb1) SG_STORE1(2, ra2, rb2) SG_STORE1(3, ra3, rb3) \
     }
 
     f32x4 acc[4][4];
@@ -177,7 +186,10 @@ __global__ void __launch_bounds__(kThreads) conv_kernel(const ConvArgs a)
     __syncthreads();
     for (int kb = kb0; kb < nk; kb++) {
         const int buf = (kb - kb0) & 1;
+        // the next K step's loads stay ahead of this step's MFMAs (left to
+        // itself the scheduler moves them down to their LDS stores)
         if (kb + 1 < nk) SG_LOAD(kb + 1)
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 32) {
             bf16x8 af[4], bfr[4];
@@ -195,7 +207,9 @@ __global__ void __launch_bounds__(kThreads) conv_kernel(const ConvArgs a)
         __syncthreads();
     }
 #undef SG_LOADA
+#undef SG_LOADB
 #undef SG_LOAD
+#undef SG_STORE1
 #undef SG_STORE
 
     if (a.part) {  // split-K: this split's partial sums
@@ -214,84 +228,95 @@ __global__ void __launch_bounds__(kThreads) conv_kernel(const ConvArgs a)
     // Epilogue through LDS: a lane holds column (lane & 15) and rows
     // 4*(lane >> 4) + v of each 16x16 tile; the tile is staged and then
     // written out row by row with the lanes on consecutive columns, so every
-    // global access of the epilogue is a contiguous run per wave.
+    // global access of the epilogue is a contiguous run per wave.  When the
+    // operand buffers cannot hold the whole 128-row tile (BK = 32), it goes
+    // through in two halves of 64 rows (the waves with wm == 64 * half stage
+    // theirs).
     constexpr int LDT = BN + 4;  // floats per staged row
-    static_assert(BM * LDT * 4 + BM * 8 <= 2 * (BM + BN) * LDK * 2, "staged tile must fit the operand buffers");
+    constexpr int HM = BM * LDT * 4 + BM * 8 <= 2 * (BM + BN) * LDK * 2 ? BM : BM / 2;
+    static_assert(HM * LDT * 4 + BM * 8 <= 2 * (BM + BN) * LDK * 2, "staged tile must fit the operand buffers");
     float *tile = reinterpret_cast<float *>(smem);
-    int *rowSeg = reinterpret_cast<int *>(tile + BM * LDT);  // segment of each tile row
+    int *rowSeg = reinterpret_cast<int *>(tile + HM * LDT);  // segment of each tile row
     int *rowPos = rowSeg + BM;                               // position in it, -1 past M
-#pragma unroll
-    for (int i = 0; i < 4; i++)
-#pragma unroll
-        for (int j = 0; j < 4; j++)
-#pragma unroll
-            for (int v = 0; v < 4; v++)
-                tile[(wm + i * 16 + (lane >> 4) * 4 + v) * LDT + wn + j * 16 + (lane & 15)] = acc[i][j][v];
     if (tid < BM) {
         const int r = m0 + tid;
         const int sg = r < a.M ? r / a.P : 0;
         rowSeg[tid] = sg;
         rowPos[tid] = r < a.M ? r - sg * a.P : -1;
     }
-    __syncthreads();
     // a thread takes 4 consecutive columns of every 8th row: a wave covers two
     // whole tile rows per instruction
     const int c4 = (tid & 31) * 4, n = n0 + c4;
     float bn[4] = {0.f, 0.f, 0.f, 0.f};
     if (EPI != EPI_MASK) *(float4 *)bn = *(const float4 *)&a.bias[n];
     float lsum = 0.f;
+#pragma unroll 1
+    for (int half = 0; half < BM / HM; half++) {
+        if (HM == BM || wm == half * HM) {
 #pragma unroll
-    for (int it = 0; it < BM / 8; it++) {
-        const int row = it * 8 + (tid >> 5), pos = rowPos[row];
-        if (pos < 0) continue;
-        const int sg = rowSeg[row], r = m0 + row;
-        float x[4];
-        *(float4 *)x = *(const float4 *)&tile[row * LDT + c4];
-        if (EPI == EPI_RELU) {
-            u16 o[4];
+            for (int i = 0; i < 4; i++)
 #pragma unroll
-            for (int c = 0; c < 4; c++) {
-                const float z = x[c] + bn[c];
-                o[c] = f2bf(z > 0.f ? z : 0.f);
-            }
-            *(uint2 *)&a.out[(size_t)r * a.ldo + n] = *(const uint2 *)o;
-        } else if (EPI == EPI_MASK) {
-            u16 m[4], o[4];
-            *(uint2 *)m = *(const uint2 *)&a.mask[(size_t)r * a.ldm + n];
+                for (int j = 0; j < 4; j++)
 #pragma unroll
-            for (int c = 0; c < 4; c++) o[c] = (m[c] & 0x8000u) || !(m[c] & 0x7FFFu) ? (u16)0 : f2bf(x[c]);
-            *(uint2 *)&a.out[(size_t)r * a.ldo + n] = *(const uint2 *)o;
-        } else if (EPI == EPI_SIGMOID) {
-            if (pos >= a.lead) {
-                float pr[4];
+                    for (int v = 0; v < 4; v++)
+                        tile[(wm - half * HM + i * 16 + (lane >> 4) * 4 + v) * LDT + wn + j * 16 + (lane & 15)] =
+                            acc[i][j][v];
+        }
+        __syncthreads();
 #pragma unroll
-                for (int c = 0; c < 4; c++) pr[c] = 1.f / (1.f + expf(-(x[c] + bn[c])));
-                *(float4 *)&a.prob[((size_t)sg * (a.P - a.lead) + (pos - a.lead)) * a.ldp + n] = *(const float4 *)pr;
-            }
-        } else {  // EPI_BCE
-            u16 o[4] = {0, 0, 0, 0};
-            if (pos >= a.lead) {
-                const int64_t yrow = a.segY[sg] + pos;
-                u16 yv[4];
-                *(uint2 *)yv = *(const uint2 *)&a.Y[(size_t)yrow * a.ldy + n];
+        for (int it = 0; it < HM / 8; it++) {
+            const int hrow = it * 8 + (tid >> 5), row = half * HM + hrow, pos = rowPos[row];
+            if (pos < 0) continue;
+            const int sg = rowSeg[row], r = m0 + row;
+            float x[4];
+            *(float4 *)x = *(const float4 *)&tile[hrow * LDT + c4];
+            if (EPI == EPI_RELU) {
+                u16 o[4];
 #pragma unroll
                 for (int c = 0; c < 4; c++) {
-                    // one exponential serves both: e = exp(-|z|), u = 1 + e,
-                    // sigmoid(z) = 1/u (z >= 0) or e/u, and the loss term
-                    // softplus(z) - y z = max(z, 0) + log1p(e) - y z with
-                    // log1p(e) = log(u) * e / (u - 1) (Goldberg's correction
-                    // for the rounding of u; e itself when u rounds to 1)
-                    const float z = x[c] + bn[c], y = bf2f(yv[c]);
-                    const float e = expf(-fabsf(z)), u = 1.f + e;
-                    const float r = __builtin_amdgcn_rcpf(u);
-                    const float p = z >= 0.f ? r : e * r;
-                    o[c] = f2bf((p - y) * a.scale);
-                    const float lp = u == 1.f ? e : __logf(u) * e * __builtin_amdgcn_rcpf(u - 1.f);
-                    lsum += fmaxf(z, 0.f) + lp - y * z;
+                    const float z = x[c] + bn[c];
+                    o[c] = f2bf(z > 0.f ? z : 0.f);
                 }
+                *(uint2 *)&a.out[(size_t)r * a.ldo + n] = *(const uint2 *)o;
+            } else if (EPI == EPI_MASK) {
+                u16 m[4], o[4];
+                *(uint2 *)m = *(const uint2 *)&a.mask[(size_t)r * a.ldm + n];
+#pragma unroll
+                for (int c = 0; c < 4; c++) o[c] = (m[c] & 0x8000u) || !(m[c] & 0x7FFFu) ? (u16)0 : f2bf(x[c]);
+                *(uint2 *)&a.out[(size_t)r * a.ldo + n] = *(const uint2 *)o;
+            } else if (EPI == EPI_SIGMOID) {
+                if (pos >= a.lead) {
+                    float pr[4];
+#pragma unroll
+                    for (int c = 0; c < 4; c++) pr[c] = 1.f / (1.f + expf(-(x[c] + bn[c])));
+                    *(float4 *)&a.prob[((size_t)sg * (a.P - a.lead) + (pos - a.lead)) * a.ldp + n] = *(const float4 *)pr;
+                }
+            } else {  // EPI_BCE
+                u16 o[4] = {0, 0, 0, 0};
+                if (pos >= a.lead) {
+                    const int64_t yrow = a.segY[sg] + pos;
+                    u16 yv[4];
+                    *(uint2 *)yv = *(const uint2 *)&a.Y[(size_t)yrow * a.ldy + n];
+#pragma unroll
+                    for (int c = 0; c < 4; c++) {
+                        // one exponential serves both: e = exp(-|z|), u = 1 + e,
+                        // sigmoid(z) = 1/u (z >= 0) or e/u, and the loss term
+                        // softplus(z) - y z = max(z, 0) + log1p(e) - y z with
+                        // log1p(e) = log(u) * e / (u - 1) (Goldberg's correction
+                        // for the rounding of u; e itself when u rounds to 1)
+                        const float z = x[c] + bn[c], y = bf2f(yv[c]);
+                        const float e = expf(-fabsf(z)), u = 1.f + e;
+                        const float rr = __builtin_amdgcn_rcpf(u);
+                        const float pp = z >= 0.f ? rr : e * rr;
+                        o[c] = f2bf((pp - y) * a.scale);
+                        const float lp = u == 1.f ? e : __logf(u) * e * __builtin_amdgcn_rcpf(u - 1.f);
+                        lsum += fmaxf(z, 0.f) + lp - y * z;
+                    }
+                }
+                *(uint2 *)&a.out[(size_t)r * a.ldo + n] = *(const uint2 *)o;
             }
-            *(uint2 *)&a.out[(size_t)r * a.ldo + n] = *(const uint2 *)o;
         }
+        __syncthreads();
     }
     if (EPI == EPI_BCE) {
 #pragma unroll
@@ -329,41 +354,53 @@ __global__ void __launch_bounds__(kThreads) wgrad_kernel(const WgradArgs a)
     const int tap = n0 / a.cin, c0 = n0 - tap * a.cin;
     const int r0 = split * a.rowsPerSplit;
     const int r1 = min(a.R, r0 + a.rowsPerSplit);
-    const int cc = (tid >> 4) * 8, rg = (tid & 15) * 4;  // 8 columns x rows rg..rg+3
+    constexpr int kQ = BK / 16;  // rows per thread of each 8-column group
+    const int cc = (tid >> 4) * 8, rg = (tid & 15) * kQ;  // 8 columns x rows rg..rg+kQ-1
 
-    uint4 rgv[4], rsv[4];
-    auto load = [&](int kb) {
-        const int rb = r0 + kb * BK + rg;
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int r = rb + q;
-            const bool in = r < r1;
-            rgv[q] = in ? *(const uint4 *)(a.G + (size_t)r * a.ldg + m0 + cc) : make_uint4(0, 0, 0, 0);
-            const int s = in ? r / a.P : 0;
-            const int sp = in ? r - s * a.P - tap * a.dil : -1;
-            const bool ok = sp >= 0;
-            const int64_t row = ok ? (a.seg ? a.seg[s] : (int64_t)s * a.P) + sp : 0;
-            if (SSRC == SRC_BF16)
-                rsv[q] = ok ? *(const uint4 *)((const u16 *)a.S + (size_t)row * a.lds + c0 + cc) : make_uint4(0, 0, 0, 0);
-            else
-                rsv[q] = ok ? spikes_to_bf16(*(const uint2 *)((const uint8_t *)a.S + (size_t)row * a.lds + c0 + cc))
-                            : make_uint4(0, 0, 0, 0);
-        }
-    };
-    auto store = [&](int buf) {
-        const u16 *g[4] = {(const u16 *)&rgv[0], (const u16 *)&rgv[1], (const u16 *)&rgv[2], (const u16 *)&rgv[3]};
-        const u16 *sv[4] = {(const u16 *)&rsv[0], (const u16 *)&rsv[1], (const u16 *)&rsv[2], (const u16 *)&rsv[3]};
-#pragma unroll
-        for (int e = 0; e < 8; e++) {
-            uint2 wg, ws;
-            wg.x = (uint32_t)g[0][e] | ((uint32_t)g[1][e] << 16);
-            wg.y = (uint32_t)g[2][e] | ((uint32_t)g[3][e] << 16);
-            ws.x = (uint32_t)sv[0][e] | ((uint32_t)sv[1][e] << 16);
-            ws.y = (uint32_t)sv[2][e] | ((uint32_t)sv[3][e] << 16);
-            *(uint2 *)&sG[buf][(cc + e) * LDK + rg] = wg;
-            *(uint2 *)&sS[buf][(cc + e) * LDK + rg] = ws;
-        }
-    };
+    uint4 rgv[kQ], rsv[kQ];
+#define SG_WLOAD(kb)                                                                                          \
+    {                                                                                                         \
+        const int rb = r0 + (kb) * BK + rg;                                                                   \
+        _Pragma("unroll") for (int q = 0; q < kQ; q++)                                                        \
+        {                                                                                                     \
+            const int r = rb + q;                                                                             \
+            const bool in = r < r1;                                                                           \
+            rgv[q] = in ? *(const uint4 *)(a.G + (size_t)r * a.ldg + m0 + cc) : make_uint4(0, 0, 0, 0);       \
+            const int s = in ? r / a.P : 0;                                                                   \
+            const int sp = in ? r - s * a.P - tap * a.dil : -1;                                               \
+            const bool ok = sp >= 0;                                                                          \
+            const int64_t row = ok ? (a.seg ? a.seg[s] : (int64_t)s * a.P) + sp : 0;                          \
+            if (SSRC == SRC_BF16)                                                                             \
+                rsv[q] = ok ? *(const uint4 *)((const u16 *)a.S + (size_t)row * a.lds + c0 + cc) : make_uint4(0, 0, 0, 0); \
+            else                                                                                              \
+                rsv[q] = ok ? spikes_to_bf16(*(const uint2 *)((const uint8_t *)a.S + (size_t)row * a.lds + c0 + cc)) \
+                            : make_uint4(0, 0, 0, 0);                                                         \
+        }                                                                                                     \
+    }
+    // written transposed: column cc + e gets this thread's kQ rows in one
+    // 2 * kQ-byte word; the 16 lanes of a column group cover 32 * kQ bytes
+    // of one LDS row
+#define SG_WSTORE(buf)                                                                       \
+    {                                                                                        \
+        _Pragma("unroll") for (int e = 0; e < 8; e++)                                        \
+        {                                                                                    \
+            uint32_t wg[kQ / 2], ws[kQ / 2];                                                 \
+            _Pragma("unroll") for (int q = 0; q < kQ; q += 2)                                \
+            {                                                                                \
+                const u16 *g0 = (const u16 *)&rgv[q], *g1 = (const u16 *)&rgv[q + 1];       \
+                const u16 *s0 = (const u16 *)&rsv[q], *s1 = (const u16 *)&rsv[q + 1];       \
+                wg[q / 2] = (uint32_t)g0[e] | ((uint32_t)g1[e] << 16);                       \
+                ws[q / 2] = (uint32_t)s0[e] | ((uint32_t)s1[e] << 16);                       \
+            }                                                                                \
+            if constexpr (kQ == 4) {                                                         \
+                *(uint2 *)&sG[buf][(cc + e) * LDK + rg] = make_uint2(wg[0], wg[1]);          \
+                *(uint2 *)&sS[buf][(cc + e) * LDK + rg] = make_uint2(ws[0], ws[1]);          \
+            } else {                                                                         \
+                *(uint32_t *)&sG[buf][(cc + e) * LDK + rg] = wg[0];                          \
+                *(uint32_t *)&sS[buf][(cc + e) * LDK + rg] = ws[0];                          \
+            }                                                                                \
+        }                                                                                    \
+    }
 
     f32x4 acc[4][4];
 #pragma unroll
@@ -374,13 +411,14 @@ __global__ void __launch_bounds__(kThreads) wgrad_kernel(const WgradArgs a)
     const int fr = lane & 15, fk = (lane >> 4) * 8;
     const int nk = r1 > r0 ? (r1 - r0 + BK - 1) / BK : 0;
     if (nk > 0) {
-        load(0);
-        store(0);
+        SG_WLOAD(0)
+        SG_WSTORE(0)
     }
     __syncthreads();
     for (int kb = 0; kb < nk; kb++) {
         const int buf = kb & 1;
-        if (kb + 1 < nk) load(kb + 1);
+        if (kb + 1 < nk) SG_WLOAD(kb + 1)
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 32) {
             bf16x8 af[4], bfr[4];
@@ -394,9 +432,11 @@ __global__ void __launch_bounds__(kThreads) wgrad_kernel(const WgradArgs a)
                 for (int j = 0; j < 4; j++)
                     acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
         }
-        if (kb + 1 < nk) store(buf ^ 1);
+        if (kb + 1 < nk) SG_WSTORE(buf ^ 1)
         __syncthreads();
     }
+#undef SG_WLOAD
+#undef SG_WSTORE
     float *out = a.part + (size_t)split * a.cout * a.Nw;
 #pragma unroll
     for (int i = 0; i < 4; i++)
