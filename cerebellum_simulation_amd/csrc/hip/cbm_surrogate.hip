@@ -46,6 +46,22 @@ typedef uint16_t u16;
 constexpr int BM = 128, BN = 128, BK = SG_BK, LDK = BK + 8;
 constexpr int kThreads = 256;
 static_assert(BK == 32 || BK == 64, "K step");
+// conv_kernel's operand tiles: unpadded rows of BK elements (64 B at BK = 32)
+// whose 16-byte chunks are XOR-swizzled by row, so that neither the loader's
+// stores (16 lanes: 4 rows x 4 chunks) nor the MFMA fragment reads (16 lanes:
+// 16 consecutive rows, one chunk) meet in a bank, for 32 or 64 LDS banks:
+// within an aligned group of 16 rows, the 4 rows of equal r mod 4 get 4
+// distinct chunk permutations, and so do the 4 rows of equal parity within 8.
+// The pattern repeats every 16 rows, so a wave's fragments 16 rows apart
+// share their swizzle.
+__device__ __forceinline__ int swz(int r, int c)
+{
+    if constexpr (BK == 32)
+        return c ^ ((((r & 15) >> 1) ^ ((r & 15) >> 3)) & 3);
+    else
+        return c;
+}
+constexpr int LDC = BK == 32 ? BK : LDK;  // conv_kernel's operand row stride
 // operand loader of conv_kernel: 16-byte vectors, kVec per tile row; a pass of
 // the block covers kRowStep rows, kPass passes cover a 128-row tile
 constexpr int kVec = BK / 8, kRowStep = kThreads / kVec, kPass = BM / kRowStep;
@@ -107,111 +123,18 @@ struct ConvArgs {
     float *loss;
     float *part;            // split-K: FP32 partial sums [split][M][N] instead of the epilogue
     int kPerSplit;          // K steps (of BK) per split
+    const u16 *zeros;       // 16 zero bytes: the source of rows outside their segment (conv_dma_kernel)
 };
 
-template <int ASRC, int EPI>
-__global__ void __launch_bounds__(kThreads) conv_kernel(const ConvArgs a)
+// The epilogue of a 128 x 128 tile held as each wave's 4 x 4 MFMA
+// accumulators (wave rows wm.., columns wn..): split-K partial sums, or the
+// fused epilogue EPI through the block's shared buffer (SMEM_BYTES), which
+// the caller has finished reading.
+template <int EPI, int SMEM_BYTES>
+__device__ __forceinline__ void conv_epilogue(const ConvArgs &a, const f32x4 (&acc)[4][4], u16 *smem, int m0, int n0,
+                                              int wm, int wn)
 {
-    // the operand tiles, double-buffered; after the last K step the same bytes
-    // hold the output tile for the epilogue
-    __shared__ __attribute__((aligned(16))) u16 smem[2 * (BM + BN) * LDK];
-    u16(*sA)[BM * LDK] = reinterpret_cast<u16(*)[BM * LDK]>(smem);
-    u16(*sB)[BN * LDK] = reinterpret_cast<u16(*)[BN * LDK]>(smem + 2 * BM * LDK);
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
-
-    // loader: kPass rows (lr + kRowStep*i) x 8 consecutive k per thread, for
-    // A and B.  Per row: its position in the segment and the source row of
-    // position 0.
-    const int lr = tid / kVec, lk = (tid % kVec) * 8;
-    // (scalars, not arrays: staging arrays were left in scratch memory)
-    int64_t ab0 = 0, ab1 = 0, ab2 = 0, ab3 = 0;
-    int ap0 = 0, ap1 = 0, ap2 = 0, ap3 = 0;
-#define SG_ROW(i, AB, AP)                                                       \
-    if (i < kPass) {                                                            \
-        const int r = m0 + lr + kRowStep * i;                                   \
-        const int s = r < a.M ? r / a.P : 0;                                    \
-        AP = r < a.M ? r - s * a.P : -(1 << 30);                                \
-        AB = r < a.M ? (a.seg ? a.seg[s] : (int64_t)s * a.P) : 0;               \
-    }
-    SG_ROW(0, ab0, ap0) SG_ROW(1, ab1, ap1) SG_ROW(2, ab2, ap2) SG_ROW(3, ab3, ap3)
-#undef SG_ROW
-
-    uint4 ra0, ra1, ra2, ra3, rb0, rb1, rb2, rb3;
-#define SG_LOADA(i, AB, AP, RA)                                                                         \
-    if (i < kPass) {                                                                                    \
-        const int sp = AP + tap * a.shift;                                                              \
-        const bool ok = sp >= 0 && sp < a.P;                                                            \
-        const int64_t row = AB + (ok ? sp : 0);                                                         \
-        if (ASRC == SRC_BF16)                                                                           \
-            RA = ok ? *(const uint4 *)((const u16 *)a.A + (size_t)row * a.lda + c) : make_uint4(0, 0, 0, 0); \
-        else                                                                                            \
-            RA = ok ? spikes_to_bf16(*(const uint2 *)((const uint8_t *)a.A + (size_t)row * a.lda + c))  \
-                    : make_uint4(0, 0, 0, 0);                                                           \
-    }
-#define SG_LOADB(i, RB) \
-    if (i < kPass) RB = *(const uint4 *)(bp + (size_t)(kRowStep * i) * a.K);
-#define SG_LOAD(kb)                                                                                     \
-    {                                                                                                   \
-        const int k0 = (kb) * BK;                                                                       \
-        const int tap = k0 / a.cin, c = k0 - tap * a.cin + lk;                                          \
-        SG_LOADA(0, ab0, ap0, ra0) SG_LOADA(1, ab1, ap1, ra1) SG_LOADA(2, ab2, ap2, ra2) SG_LOADA(3, ab3, ap3, ra3) \
-        const u16 *bp = a.Bt + (size_t)(n0 + lr) * a.K + k0 + lk;                                       \
-        SG_LOADB(0, rb0) SG_LOADB(1, rb1) SG_LOADB(2, rb2) SG_LOADB(3, rb3)                             \
-    }
-#define SG_STORE1(i, RA, RB)                                               \
-    if (i < kPass) {                                                       \
-        *(uint4 *)&sA[buf_][(lr + kRowStep * i) * LDK + lk] = RA;          \
-        *(uint4 *)&sB[buf_][(lr + kRowStep * i) * LDK + lk] = RB;          \
-    }
-#define SG_STORE(buf)                                                                          \
-    {                                                                                          \
-        const int buf_ = (buf);                                                                \
-        SG_STORE1(0, ra0, rb0) SG_STORE1(1, ra1, rb1) SG_STORE1(2, ra2, rb2) SG_STORE1(3, ra3, rb3) \
-    }
-
-    f32x4 acc[4][4];
-#pragma unroll
-    for (int i = 0; i < 4; i++)
-#pragma unroll
-        for (int j = 0; j < 4; j++) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-
-    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
-    const int fr = lane & 15, fk = (lane >> 4) * 8;
-    // this block's K steps: all of them, or its split's
-    const int kb0 = a.part ? (int)blockIdx.z * a.kPerSplit : 0;
-    const int nk = a.part ? min(a.K / BK, kb0 + a.kPerSplit) : a.K / BK;
-    SG_LOAD(kb0)
-    SG_STORE(0)
-    __syncthreads();
-    for (int kb = kb0; kb < nk; kb++) {
-        const int buf = (kb - kb0) & 1;
-        // the next K step's loads stay ahead of this step's MFMAs (left to
-        // itself the scheduler moves them down to their LDS stores)
-        if (kb + 1 < nk) SG_LOAD(kb + 1)
-        __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-        for (int kk = 0; kk < BK; kk += 32) {
-            bf16x8 af[4], bfr[4];
-#pragma unroll
-            for (int i = 0; i < 4; i++) af[i] = *(const bf16x8 *)&sA[buf][(wm + i * 16 + fr) * LDK + kk + fk];
-#pragma unroll
-            for (int j = 0; j < 4; j++) bfr[j] = *(const bf16x8 *)&sB[buf][(wn + j * 16 + fr) * LDK + kk + fk];
-#pragma unroll
-            for (int i = 0; i < 4; i++)
-#pragma unroll
-                for (int j = 0; j < 4; j++)
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
-        }
-        if (kb + 1 < nk) SG_STORE(buf ^ 1)
-        __syncthreads();
-    }
-#undef SG_LOADA
-#undef SG_LOADB
-#undef SG_LOAD
-#undef SG_STORE1
-#undef SG_STORE
-
+    const int tid = threadIdx.x, lane = tid & 63;
     if (a.part) {  // split-K: this split's partial sums
         float *dst = a.part + (size_t)blockIdx.z * a.M * a.N;
 #pragma unroll
@@ -225,7 +148,7 @@ __global__ void __launch_bounds__(kThreads) conv_kernel(const ConvArgs a)
             }
         return;
     }
-    // Epilogue through LDS: a lane holds column (lane & 15) and rows
+    // Through LDS: a lane holds column (lane & 15) and rows
     // 4*(lane >> 4) + v of each 16x16 tile; the tile is staged and then
     // written out row by row with the lanes on consecutive columns, so every
     // global access of the epilogue is a contiguous run per wave.  When the
@@ -233,8 +156,8 @@ __global__ void __launch_bounds__(kThreads) conv_kernel(const ConvArgs a)
     // through in two halves of 64 rows (the waves with wm == 64 * half stage
     // theirs).
     constexpr int LDT = BN + 4;  // floats per staged row
-    constexpr int HM = BM * LDT * 4 + BM * 8 <= 2 * (BM + BN) * LDK * 2 ? BM : BM / 2;
-    static_assert(HM * LDT * 4 + BM * 8 <= 2 * (BM + BN) * LDK * 2, "staged tile must fit the operand buffers");
+    constexpr int HM = BM * LDT * 4 + BM * 8 <= SMEM_BYTES ? BM : BM / 2;
+    static_assert(HM * LDT * 4 + BM * 8 <= SMEM_BYTES, "staged tile must fit the shared buffer");
     float *tile = reinterpret_cast<float *>(smem);
     int *rowSeg = reinterpret_cast<int *>(tile + HM * LDT);  // segment of each tile row
     int *rowPos = rowSeg + BM;                               // position in it, -1 past M
@@ -323,6 +246,236 @@ __global__ void __launch_bounds__(kThreads) conv_kernel(const ConvArgs a)
         for (int o = 32; o > 0; o >>= 1) lsum += __shfl_xor(lsum, o);
         if (lane == 0 && lsum != 0.f) atomicAdd(a.loss, lsum * a.scale);
     }
+}
+
+template <int ASRC, int EPI>
+__global__ void __launch_bounds__(kThreads) conv_kernel(const ConvArgs a)
+{
+    // the operand tiles, double-buffered; after the last K step the same bytes
+    // hold the output tile for the epilogue (128 x 128 floats, or its halves)
+    constexpr int kOperands = 2 * (BM + BN) * LDC;
+    constexpr int kStaged = ((BM * (BN + 4) * 4 + BM * 8 <= kOperands * 2 ? BM : BM / 2) * (BN + 4) * 4 + BM * 8) / 2;
+    __shared__ __attribute__((aligned(16))) u16 smem[kOperands > kStaged ? kOperands : kStaged];
+    u16(*sA)[BM * LDC] = reinterpret_cast<u16(*)[BM * LDC]>(smem);
+    u16(*sB)[BN * LDC] = reinterpret_cast<u16(*)[BN * LDC]>(smem + 2 * BM * LDC);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+
+    // loader: kPass rows (lr + kRowStep*i) x 8 consecutive k per thread, for
+    // A and B.  Per row: its position in the segment and the source row of
+    // position 0.
+    const int lr = tid / kVec, lk = (tid % kVec) * 8;
+    // (scalars, not arrays: staging arrays were left in scratch memory)
+    int64_t ab0 = 0, ab1 = 0, ab2 = 0, ab3 = 0;
+    int ap0 = 0, ap1 = 0, ap2 = 0, ap3 = 0;
+#define SG_ROW(i, AB, AP)                                                       \
+    if (i < kPass) {                                                            \
+        const int r = m0 + lr + kRowStep * i;                                   \
+        const int s = r < a.M ? r / a.P : 0;                                    \
+        AP = r < a.M ? r - s * a.P : -(1 << 30);                                \
+        AB = r < a.M ? (a.seg ? a.seg[s] : (int64_t)s * a.P) : 0;               \
+    }
+    SG_ROW(0, ab0, ap0) SG_ROW(1, ab1, ap1) SG_ROW(2, ab2, ap2) SG_ROW(3, ab3, ap3)
+#undef SG_ROW
+
+    uint4 ra0, ra1, ra2, ra3, rb0, rb1, rb2, rb3;
+#define SG_LOADA(i, AB, AP, RA)                                                                         \
+    if (i < kPass) {                                                                                    \
+        const int sp = AP + tap * a.shift;                                                              \
+        const bool ok = sp >= 0 && sp < a.P;                                                            \
+        const int64_t row = AB + (ok ? sp : 0);                                                         \
+        if (ASRC == SRC_BF16)                                                                           \
+            RA = ok ? *(const uint4 *)((const u16 *)a.A + (size_t)row * a.lda + c) : make_uint4(0, 0, 0, 0); \
+        else                                                                                            \
+            RA = ok ? spikes_to_bf16(*(const uint2 *)((const uint8_t *)a.A + (size_t)row * a.lda + c))  \
+                    : make_uint4(0, 0, 0, 0);                                                           \
+    }
+#define SG_LOADB(i, RB) \
+    if (i < kPass) RB = *(const uint4 *)(bp + (size_t)(kRowStep * i) * a.K);
+#define SG_LOAD(kb)                                                                                     \
+    {                                                                                                   \
+        const int k0 = (kb) * BK;                                                                       \
+        const int tap = k0 / a.cin, c = k0 - tap * a.cin + lk;                                          \
+        SG_LOADA(0, ab0, ap0, ra0) SG_LOADA(1, ab1, ap1, ra1) SG_LOADA(2, ab2, ap2, ra2) SG_LOADA(3, ab3, ap3, ra3) \
+        const u16 *bp = a.Bt + (size_t)(n0 + lr) * a.K + k0 + lk;                                       \
+        SG_LOADB(0, rb0) SG_LOADB(1, rb1) SG_LOADB(2, rb2) SG_LOADB(3, rb3)                             \
+    }
+#define SG_STORE1(i, RA, RB)                                                              \
+    if (i < kPass) {                                                                      \
+        const int row = lr + kRowStep * i, at = row * LDC + swz(row, lk >> 3) * 8;        \
+        *(uint4 *)&sA[buf_][at] = RA;                                                     \
+        *(uint4 *)&sB[buf_][at] = RB;                                                     \
+    }
+#define SG_STORE(buf)                                                                          \
+    {                                                                                          \
+        const int buf_ = (buf);                                                                \
+        SG_STORE1(0, ra0, rb0) SG_STORE1(1, ra1, rb1) SG_STORE1(2, ra2, rb2) SG_STORE1(3, ra3, rb3) \
+    }
+
+    f32x4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+    const int fr = lane & 15, fk = (lane >> 4) * 8;
+    // this block's K steps: all of them, or its split's
+    const int kb0 = a.part ? (int)blockIdx.z * a.kPerSplit : 0;
+    const int nk = a.part ? min(a.K / BK, kb0 + a.kPerSplit) : a.K / BK;
+    SG_LOAD(kb0)
+    SG_STORE(0)
+    __syncthreads();
+    for (int kb = kb0; kb < nk; kb++) {
+        const int buf = (kb - kb0) & 1;
+        // the next K step's loads stay ahead of this step's MFMAs (left to
+        // itself the scheduler moves them down to their LDS stores)
+        if (kb + 1 < nk) SG_LOAD(kb + 1)
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 32) {
+            bf16x8 af[4], bfr[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int row = wm + i * 16 + fr;
+                af[i] = *(const bf16x8 *)&sA[buf][row * LDC + swz(row, (kk + fk) >> 3) * 8];
+            }
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int row = wn + j * 16 + fr;
+                bfr[j] = *(const bf16x8 *)&sB[buf][row * LDC + swz(row, (kk + fk) >> 3) * 8];
+            }
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+        }
+        if (kb + 1 < nk) SG_STORE(buf ^ 1)
+        __syncthreads();
+    }
+#undef SG_LOADA
+#undef SG_LOADB
+#undef SG_LOAD
+#undef SG_STORE1
+#undef SG_STORE
+
+    conv_epilogue<EPI, (int)sizeof(smem)>(a, acc, smem, m0, n0, wm, wn);
+}
+
+// conv_kernel for BF16 sources with the operand tiles streamed straight from
+// global memory into LDS (16-byte global_load_lds per lane, no staging
+// registers) through a ring of kStages stages, two K steps ahead of the MFMAs.
+// Wave w's DMA instruction q fills rows (2w + q) * 16 .. + 15 of the stage's
+// A tile (and then of its B tile); lane l writes bytes 16 l .. 16 l + 15 of
+// that 1 KB, i.e. row (2w + q) * 16 + (l >> 2) at physical chunk l & 3, so it
+// loads the logical chunk swz(row, l & 3) (the swizzle is an XOR, its own
+// inverse).  Rows outside their segment load 16 zero bytes.  Each wave waits
+// for its own DMA with an explicit vmcnt and the block meets at a bare
+// s_barrier (a __syncthreads release fence would wait for every DMA in
+// flight); compiler barriers keep LDS accesses on their side of it.
+constexpr int kStages = 3;
+template <int EPI>
+__global__ void __launch_bounds__(kThreads) conv_dma_kernel(const ConvArgs a)
+{
+    static_assert(BK == 32 && BM == 128 && BN == 128 && kThreads == 256, "DMA map of conv_dma_kernel");
+    typedef __attribute__((address_space(3))) void lds_t;
+    typedef __attribute__((address_space(1))) void gbl_t;
+    constexpr int kStageU16 = (BM + BN) * BK;  // 16 KB: A rows 0..127, then B rows 0..127
+    constexpr int kOps = kStages * kStageU16;
+    constexpr int kStaged = ((BM / 2) * (BN + 4) * 4 + BM * 8) / 2;
+    __shared__ __attribute__((aligned(16))) u16 smem[kOps > kStaged ? kOps : kStaged];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+    const int lrow = lane >> 2, lch = swz(lrow, lane & 3);
+
+    // the lane's two A rows (q = 0, 1): position in the segment, source row of position 0
+    int64_t ab0 = 0, ab1 = 0;
+    int ap0 = 0, ap1 = 0;
+#define SG_ROW(q, AB, AP)                                                   \
+    {                                                                       \
+        const int r = m0 + (2 * wave + q) * 16 + lrow;                      \
+        const int s = r < a.M ? r / a.P : 0;                                \
+        AP = r < a.M ? r - s * a.P : -(1 << 30);                            \
+        AB = r < a.M ? (a.seg ? a.seg[s] : (int64_t)s * a.P) : 0;           \
+    }
+    SG_ROW(0, ab0, ap0) SG_ROW(1, ab1, ap1)
+#undef SG_ROW
+    const u16 *bq0 = a.Bt + (size_t)(n0 + (2 * wave + 0) * 16 + lrow) * a.K + lch * 8;
+    const u16 *bq1 = a.Bt + (size_t)(n0 + (2 * wave + 1) * 16 + lrow) * a.K + lch * 8;
+    const u16 *A = (const u16 *)a.A;
+#define SG_DMA_A(q, AB, AP)                                                                              \
+    {                                                                                                    \
+        const int sp = AP + tap * a.shift;                                                               \
+        const bool ok = sp >= 0 && sp < a.P;                                                             \
+        const u16 *src = ok ? A + (size_t)(AB + sp) * a.lda + c : a.zeros;                               \
+        __builtin_amdgcn_global_load_lds((gbl_t *)src, (lds_t *)(stg + (2 * wave + q) * 16 * BK), 16, 0, 0); \
+    }
+#define SG_ISSUE(kb, st)                                                                                 \
+    {                                                                                                    \
+        const int k0 = (kb) * BK;                                                                        \
+        const int tap = k0 / a.cin, c = k0 - tap * a.cin + lch * 8;                                      \
+        u16 *stg = smem + (st) * kStageU16;                                                              \
+        SG_DMA_A(0, ab0, ap0) SG_DMA_A(1, ab1, ap1)                                                      \
+        __builtin_amdgcn_global_load_lds((gbl_t *)(bq0 + k0), (lds_t *)(stg + BM * BK + (2 * wave + 0) * 16 * BK), 16, 0, 0); \
+        __builtin_amdgcn_global_load_lds((gbl_t *)(bq1 + k0), (lds_t *)(stg + BM * BK + (2 * wave + 1) * 16 * BK), 16, 0, 0); \
+    }
+    // s_waitcnt with vmcnt = n (n < 16), expcnt and lgkmcnt not waited for
+#define SG_WAIT_VM(n) __builtin_amdgcn_s_waitcnt(((n) & 0xF) | (7 << 4) | (0xF << 8))
+#define SG_BARRIER()                    \
+    {                                   \
+        asm volatile("" ::: "memory");  \
+        __builtin_amdgcn_s_barrier();   \
+        asm volatile("" ::: "memory");  \
+    }
+
+    f32x4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+    const int fr = lane & 15, fk = (lane >> 4) * 8;
+    const int kb0 = a.part ? (int)blockIdx.z * a.kPerSplit : 0;
+    const int nk = a.part ? min(a.K / BK, kb0 + a.kPerSplit) : a.K / BK;
+    const int cnt = nk - kb0;  // >= 1
+    SG_ISSUE(kb0, 0)
+    if (cnt > 1) SG_ISSUE(kb0 + 1, 1)
+    for (int it = 0; it < cnt; it++) {
+        const int st = it % kStages;
+        // this step's stage has landed (each wave's own four DMAs), the next
+        // step's may still be in flight
+        if (it + 1 < cnt)
+            SG_WAIT_VM(4);
+        else
+            SG_WAIT_VM(0);
+        SG_BARRIER()
+        // every wave is done with the stage read in the step before: refill it
+        if (it + 2 < cnt) SG_ISSUE(kb0 + it + 2, (it + 2) % kStages)
+        const u16 *sA = smem + st * kStageU16, *sB = sA + BM * BK;
+        bf16x8 af[4], bfr[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int row = wm + i * 16 + fr;
+            af[i] = *(const bf16x8 *)&sA[row * BK + swz(row, fk >> 3) * 8];
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int row = wn + j * 16 + fr;
+            bfr[j] = *(const bf16x8 *)&sB[row * BK + swz(row, fk >> 3) * 8];
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+    }
+#undef SG_DMA_A
+#undef SG_ISSUE
+#undef SG_WAIT_VM
+#undef SG_BARRIER
+    __syncthreads();  // every wave is done with the last stage (no DMA is in flight)
+    conv_epilogue<EPI, (int)sizeof(smem)>(a, acc, smem, m0, n0, wm, wn);
 }
 
 struct WgradArgs {
@@ -578,6 +731,7 @@ struct cbm_sg {
     float *part = nullptr;
     size_t partFloats = 0;
     float *loss = nullptr;
+    u16 *zeros = nullptr;  // 64 zero bytes
     int lastRows = 0;
 };
 
@@ -599,6 +753,16 @@ int check_batch(const cbm_sg *sg, const cbm_sg_batch *b, bool training, int *P, 
 
 dim3 grid_for(size_t n) { return dim3((unsigned)((n + 255) / 256)); }
 
+// conv_dma_kernel for the data gradient (EPI_MASK: N = 128 columns, split
+// over K, few blocks per CU, where the deeper pipeline pays: 154 -> 135 us per
+// pass at the full shape); the wide forward passes ran 3-7 % slower with it
+// than with register staging and four blocks per CU
+template <int E>
+constexpr bool conv_dma(int src)
+{
+    return src == SRC_BF16 && E == EPI_MASK;
+}
+
 int conv(cbm_sg *sg, int src, int epi, const ConvArgs &a0)
 {
     ConvArgs a = a0;
@@ -617,9 +781,16 @@ int conv(cbm_sg *sg, int src, int epi, const ConvArgs &a0)
         }
     }
     const dim3 grid(a.N / BN, (a.M + BM - 1) / BM, splits);
+    a.zeros = sg->zeros;
+    // (spike bytes always go through registers: they are widened to BF16 on the way)
+#define CONV_LAUNCH(S, E)                                                                         \
+    if (conv_dma<E>(S))                                                                           \
+        hipLaunchKernelGGL((conv_dma_kernel<E>), grid, dim3(kThreads), 0, sg->stream, a);         \
+    else                                                                                          \
+        hipLaunchKernelGGL((conv_kernel<S, E>), grid, dim3(kThreads), 0, sg->stream, a);
 #define CONV_CASE(S, E)                                                                         \
     if (src == S && epi == E) {                                                                 \
-        hipLaunchKernelGGL((conv_kernel<S, E>), grid, dim3(kThreads), 0, sg->stream, a);        \
+        CONV_LAUNCH(S, E)                                                                       \
         sg->lastFlops += 2.0 * a.M * (double)a.N * a.K;                                         \
         HIP_TRY(hipGetLastError());                                                             \
         if (a.part) {                                                                           \
@@ -634,7 +805,7 @@ int conv(cbm_sg *sg, int src, int epi, const ConvArgs &a0)
     CONV_CASE(SRC_BF16, EPI_SIGMOID)
     CONV_CASE(SRC_BF16, EPI_BCE)
     if (src == SRC_BF16 && epi == EPI_MASK) {
-        hipLaunchKernelGGL((conv_kernel<SRC_BF16, EPI_MASK>), grid, dim3(kThreads), 0, sg->stream, a);
+        CONV_LAUNCH(SRC_BF16, EPI_MASK)
         sg->lastFlops += 2.0 * a.M * (double)a.N * a.K;
         HIP_TRY(hipGetLastError());
         if (a.part) {
@@ -645,6 +816,7 @@ int conv(cbm_sg *sg, int src, int epi, const ConvArgs &a0)
         return CBM_OK;
     }
 #undef CONV_CASE
+#undef CONV_LAUNCH
     return fail(CBM_E_ARG, "unsupported convolution variant");
 }
 
@@ -862,6 +1034,8 @@ int cbm_sg_create(const cbm_sg_desc *desc, cbm_sg **out)
     rc = rc ? rc : alloc((void **)&sg->dz3, R * d.numOut * 2);
     rc = rc ? rc : alloc((void **)&sg->part, part * 4);
     rc = rc ? rc : alloc((void **)&sg->loss, 4);
+    rc = rc ? rc : alloc((void **)&sg->zeros, 64);
+    if (!rc && hipMemset(sg->zeros, 0, 64) != hipSuccess) rc = fail(CBM_E_HIP, "hipMemset failed");
     if (!rc && hipStreamCreateWithFlags(&sg->stream, hipStreamNonBlocking) != hipSuccess)
         rc = fail(CBM_E_HIP, "hipStreamCreate failed");
     if (!rc && (hipEventCreate(&sg->ev0) != hipSuccess || hipEventCreate(&sg->ev1) != hipSuccess))
@@ -887,7 +1061,7 @@ int cbm_sg_destroy(cbm_sg *sg)
     (void)hipSetDevice(sg->d.device);
     if (sg->stream) (void)hipStreamSynchronize(sg->stream);
     void *ptrs[] = {sg->theta, sg->mom, sg->vel, sg->grad, sg->wf[0], sg->wf[1], sg->wf[2], sg->wt[0], sg->wt[1],
-                    sg->wt[2], sg->h1, sg->h2, sg->dz1, sg->dz2, sg->dz3, sg->part, sg->loss};
+                    sg->wt[2], sg->h1, sg->h2, sg->dz1, sg->dz2, sg->dz3, sg->part, sg->loss, sg->zeros};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (sg->ev0) (void)hipEventDestroy(sg->ev0);
