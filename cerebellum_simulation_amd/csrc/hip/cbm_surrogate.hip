@@ -173,8 +173,22 @@ __device__ __forceinline__ void conv_epilogue(const ConvArgs &a, const f32x4 (&a
     float bn[4] = {0.f, 0.f, 0.f, 0.f};
     if (EPI != EPI_MASK) *(float4 *)bn = *(const float4 *)&a.bias[n];
     float lsum = 0.f;
-#pragma unroll 1
+#pragma unroll
     for (int half = 0; half < BM / HM; half++) {
+        // BCE: the target words of this part's rows, requested before any of
+        // them is used (one round trip for the part instead of one per row;
+        // rows without a target load nothing)
+        uint2 ypre[EPI == EPI_BCE ? HM / 8 : 1];
+        if constexpr (EPI == EPI_BCE) {
+#pragma unroll
+            for (int it = 0; it < HM / 8; it++) {
+                const int r = m0 + half * HM + it * 8 + (tid >> 5);
+                const int sg = r < a.M ? r / a.P : 0, pos = r < a.M ? r - sg * a.P : -1;
+                const bool use = pos >= a.lead;
+                const int64_t yrow = use ? a.segY[sg] + pos : 0;
+                ypre[it] = use ? *(const uint2 *)&a.Y[(size_t)yrow * a.ldy + n] : make_uint2(0u, 0u);
+            }
+        }
         if (HM == BM || wm == half * HM) {
 #pragma unroll
             for (int i = 0; i < 4; i++)
@@ -211,15 +225,14 @@ __device__ __forceinline__ void conv_epilogue(const ConvArgs &a, const f32x4 (&a
                 if (pos >= a.lead) {
                     float pr[4];
 #pragma unroll
-                    for (int c = 0; c < 4; c++) pr[c] = 1.f / (1.f + expf(-(x[c] + bn[c])));
+                    for (int c = 0; c < 4; c++) pr[c] = __builtin_amdgcn_rcpf(1.f + __expf(-(x[c] + bn[c])));
                     *(float4 *)&a.prob[((size_t)sg * (a.P - a.lead) + (pos - a.lead)) * a.ldp + n] = *(const float4 *)pr;
                 }
             } else {  // EPI_BCE
                 u16 o[4] = {0, 0, 0, 0};
                 if (pos >= a.lead) {
-                    const int64_t yrow = a.segY[sg] + pos;
                     u16 yv[4];
-                    *(uint2 *)yv = *(const uint2 *)&a.Y[(size_t)yrow * a.ldy + n];
+                    *(uint2 *)yv = ypre[it];  // row half * HM + it * 8 + (tid >> 5)
 #pragma unroll
                     for (int c = 0; c < 4; c++) {
                         // one exponential serves both: e = exp(-|z|), u = 1 + e,
