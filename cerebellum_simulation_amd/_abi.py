@@ -160,6 +160,7 @@ def hip_lib() -> C.CDLL:
         L.cbm_profile_steps.argtypes = [P, V, S, I, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.cbm_profile_split.argtypes = [P, V, S, I, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.cbm_step_mode.argtypes = [P]
+        L.cbm_resident.argtypes = [P]
         U32P = C.POINTER(C.c_uint32)
         L.cbm_gr_launch.argtypes = [P, U32P, U32P, U32P, U32P]
         L.cbm_set_recorder.argtypes = [P, C.c_uint32, C.c_uint32]

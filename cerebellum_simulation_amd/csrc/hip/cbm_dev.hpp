@@ -187,6 +187,31 @@ struct BatchNet {
     uint32_t grgoLag, tsPerHistBinGR, tsPerPopHistBinPC, numTSinMFHist;
 };
 
+// A resident run: n steps of one network in one cooperative launch (one block
+// per CU), the GR cells' state held in the registers of their blocks from the
+// first step to the last, a grid barrier between steps.  Step t = t0 + k + 1
+// of run step k takes MF row k; its per-step values are derived on the
+// device as launchStep derives them on the host.
+constexpr int kBarGroups = 8;      // barrier counters (one per group of blocks)
+constexpr int kBarStride = 64;     // words between counters (256 bytes)
+struct ResRun {
+    GrArgs g;
+    const uint8_t *mf;          // MF row k at mf + k * pitch (device memory)
+    uint64_t pitch;
+    const float *err;           // [n][numZones] relative error drive (device), or nullptr
+    uint64_t t0;                // curTime before the run
+    float errDrive0[kMaxZones]; // drive pending from the host for the first step
+    float maxExtIncVIO, gIncGRtoGO;
+    uint32_t n;                 // steps
+    uint32_t spans;             // GR blocks (1 .. spans); block spans + 1 prebuilds generator blocks
+    uint32_t skipFold0;         // the host wrote the PF->PC sums of step t0
+    uint32_t recOffPF;          // offset of the PF chunk counts in a recorder row (UINT32_MAX: none)
+    uint32_t *bar;              // [kBarGroups * kBarStride] barrier counters (monotonic)
+    uint32_t barBase;           // their sum before this launch
+    uint32_t *status;           // set to 1 when a barrier wait timed out
+    uint32_t recLds;            // byte offset in dynamic LDS of the GR blocks' records [CPT][1024]
+};
+
 // Per network of a surrogate sweep: cluster weights [numZones][nCl] and the
 // cluster size (2^log2Cells consecutive GRs).
 struct SweepNet {

@@ -21,6 +21,8 @@ __global__ void step_span_kernel(const SmallArgs *args, const SmallStep st, cons
 template <int CPT, int FMT>
 __global__ void gr_span_kernel(const GrArgs a, const GrStep st, const uint32_t spans);
 __global__ void small_step_kernel(const SmallArgs *args, const SmallStep st);
+template <int CPT>
+__global__ void step_resident_kernel(const SmallArgs *args, const ResRun r);
 __global__ void stream_ceiling_kernel(StreamArgs a);
 __global__ void pfpc_fold_kernel(const float *chunk, float *out, uint32_t nChunks, uint32_t perPC, uint32_t numPC,
                                  uint32_t tasks);
@@ -55,6 +57,18 @@ struct cbm_handle {
     int spanCpt = 0;         // GR cells per thread of those blocks
     size_t fusedLds = 0;     // dynamic LDS of the fused step kernel
     bool fused() const { return mode != 2; }
+    // resident runs (cbm_run_mf_train of two or more steps): one cooperative
+    // launch of spans + 2 workgroups, GR state in registers, grid barriers
+    bool resident = false;
+    size_t residentLds = 0;           // its dynamic LDS (the small step's, or a span's + its records)
+    uint32_t residentRecLds = 0;      // byte offset of the GR records in that LDS
+    uint32_t *resBar = nullptr;       // barrier counters [kBarGroups * kBarStride], monotonic
+    uint32_t resBarBase = 0;          // their sum after the last resident launch
+    uint32_t *resStatus = nullptr;    // device: 1 when a barrier wait timed out
+    uint32_t *resStatusHost = nullptr;  // pinned copy, read at the next synchronisation
+    bool resCheck = false;            // a resident run's status is still to be read
+    float *resErr = nullptr;          // device copy of a run's error drive
+    size_t resErrCap = 0;             // its capacity in floats
 
     std::vector<uint8_t> host[CBM_F_COUNT];  // staged fields, zones stacked
     bool written[CBM_F_COUNT] = {};

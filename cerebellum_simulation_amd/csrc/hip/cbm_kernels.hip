@@ -34,6 +34,23 @@
 namespace cbmdev {
 
 // ---------------------------------------------------------------- helpers
+// Device pointers read from an argument block in memory are generic; casting
+// them to the global address space where they are used gives
+// global_load/global_store/global_atomic, which do not share the LDS wait
+// counter the way flat accesses do.
+template <typename T>
+using G = __attribute__((address_space(1))) T;
+template <typename T>
+__device__ __forceinline__ G<T> *ga(T *p)
+{
+    return (G<T> *)p;
+}
+// atomicAdd (relaxed, agent scope) through a global pointer
+__device__ __forceinline__ void gadd(uint32_t *p, uint32_t v)
+{
+    __hip_atomic_fetch_add(ga(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Sum of a value over the 64 lanes of a wave, valid in lane 0: the halving
 // tree x[i] += x[i+k] for k = 32, 16, ..., 1 (the pairs and order of an xor
 // butterfly, read at lane 0), with lane i < k reading lane i + k through the
@@ -52,15 +69,66 @@ __device__ __forceinline__ float wave_sum_lane0(float x)
 __device__ __forceinline__ uint32_t lo16(uint32_t w) { return w & 0xFFFFu; }
 __device__ __forceinline__ uint32_t hi16(uint32_t w) { return w >> 16; }
 
+// Block-uniform values derived in vector arithmetic (64-bit remainders,
+// pointers read from a record in memory), moved to scalar registers so that
+// they do not occupy vector registers across a step.
+__device__ __forceinline__ uint32_t sreg(uint32_t x) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)x); }
+__device__ __forceinline__ uint64_t sreg64(uint64_t x)
+{
+    return ((uint64_t)sreg((uint32_t)(x >> 32)) << 32) | sreg((uint32_t)x);
+}
+template <typename T>
+__device__ __forceinline__ T *sptr(T *p)
+{
+    return reinterpret_cast<T *>(sreg64(reinterpret_cast<uint64_t>(p)));
+}
+
+// ---------------------------------------------------------------- cross-block data
+// Arrays that one block writes and another reads within one launch of the
+// resident run (X = true): the GO spikes, plasticity timers, chunk sums, ring
+// slots and generator blocks.  gfx950 keeps one L2 per XCD and these L2s are
+// not coherent with each other inside a kernel, so such arrays are accessed
+// with relaxed agent-scope atomics: loads and stores (sc1) that go through to
+// the device's point of coherence.  A grid barrier (grid_sync) orders them
+// between steps.  Data a block alone touches during the run keeps plain
+// accesses and stays in its XCD's caches.  X = false: plain accesses.
+template <bool X, typename T>
+__device__ __forceinline__ T xld(const T *p)
+{
+    if constexpr (X) return __hip_atomic_load(ga(const_cast<T *>(p)), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else return *ga(p);
+}
+template <bool X, typename T>
+__device__ __forceinline__ void xst(T *p, T v)
+{
+    if constexpr (X) __hip_atomic_store(ga(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else *ga(p) = v;
+}
+
 // ---------------------------------------------------------------- PF->PC fold
 // Second level of the PF->PC sum of one (zone, PC), run by one full wave: lane
 // l adds the 64-GR chunk sums l, l+64, l+128, ... in order, then the lanes are
 // combined as by the xor butterfly (the result is valid in lane 0).
+template <bool X = false>
 __device__ __forceinline__ float fold_pc(const float *__restrict__ c, uint32_t perPC, int lane)
 {
     float acc = 0.0f;
+    if constexpr (X) {
+        // eight loads in flight at clamped indices, then the adds in the same
+        // order under the same bounds (an atomic load under a branch is not
+        // issued ahead of it)
+        for (uint32_t q0 = (uint32_t)lane; q0 < perPC; q0 += 8 * 64) {
+            float x[8];
+#pragma unroll
+            for (int j = 0; j < 8; j++) x[j] = xld<X>(c + min(q0 + (uint32_t)j * 64u, perPC - 1u));
+#pragma unroll
+            for (int j = 0; j < 8; j++)
+                if (q0 + (uint32_t)j * 64u < perPC) acc = acc + x[j];
+        }
+    } else {
 #pragma unroll 8
-    for (uint32_t q = lane; q < perPC; q += 64) acc = acc + c[q];
+        for (uint32_t q = lane; q < perPC; q += 64) acc = acc + c[q];
+    }
     return wave_sum_lane0(acc);
 }
 
@@ -68,19 +136,33 @@ __device__ __forceinline__ float fold_pc(const float *__restrict__ c, uint32_t p
 // Second level of the previous step's PF->PC sums, one wave per (zone, PC),
 // spread over the waves of the GR blocks (block `rank` of `ranks`): the small
 // populations read them one step later.
-template <int NT>
+template <int NT, bool X = false>
 __device__ __forceinline__ void gr_fold(const GrArgs &a, const GrStep &st, const uint32_t rank, const uint32_t ranks)
 {
     if (!st.foldSrc) return;
-    const int lane = threadIdx.x & 63;
+    uint32_t tid = threadIdx.x;
+    if constexpr (X) asm volatile("" : "+v"(tid));  // as in gr_span_impl
+    const int lane = tid & 63;
     const uint32_t nChunks = a.numGR >> 6, waves = ranks * (uint32_t)(NT / 64);
-    for (uint32_t task = rank * (uint32_t)(NT / 64) + (threadIdx.x >> 6); task < a.numZones * a.numPC;
+    for (uint32_t task = rank * (uint32_t)(NT / 64) + (tid >> 6); task < a.numZones * a.numPC;
          task += waves) {  // wave-uniform
         const uint32_t z = task / a.numPC, pc = task % a.numPC;
-        const float acc = fold_pc(st.foldSrc + (size_t)z * nChunks + (size_t)pc * a.perPC, a.perPC, lane);
-        if (lane == 0) st.foldDst[task] = acc;
+        const float acc = fold_pc<X>(st.foldSrc + (size_t)z * nChunks + (size_t)pc * a.perPC, a.perPC, lane);
+        if (lane == 0) xst<X>(st.foldDst + task, acc);
     }
 }
+
+// The per-cell state of a block's GR cells held in registers across the steps
+// of a resident run (cell j of thread tid: first + j * 1024 + tid): membrane,
+// threshold, K-Ca conductance, MF and GO conductances, spike register and
+// zone-0 PF->PC weight; the cells' compact connectivity records are held in
+// LDS (rec[j * 1024 + tid]).
+template <int CPT>
+struct GrRegs {
+    float v[CPT], th[CPT], gk[CPT], gE[CPT], gI[CPT], w0[CPT];
+    uint32_t ab[CPT];
+    const uint4 *rec;
+};
 
 // Streaming accesses of the per-GR arrays, non-temporal: the tiles' tens of
 // MB pass through the L2 without evicting what the small-population block and
@@ -89,24 +171,24 @@ __device__ __forceinline__ void gr_fold(const GrArgs &a, const GrStep &st, const
 template <typename T>
 __device__ __forceinline__ T ntl(const T *p)
 {
-    return __builtin_nontemporal_load(p);
+    return __builtin_nontemporal_load(ga(p));
 }
 __device__ __forceinline__ uint2 ntl(const uint2 *p)
 {
     typedef uint32_t v2u __attribute__((ext_vector_type(2)));
-    const v2u r = __builtin_nontemporal_load(reinterpret_cast<const v2u *>(p));
+    const v2u r = __builtin_nontemporal_load(ga(reinterpret_cast<const v2u *>(p)));
     return make_uint2(r.x, r.y);
 }
 __device__ __forceinline__ uint4 ntl(const uint4 *p)
 {
     typedef uint32_t v4u __attribute__((ext_vector_type(4)));
-    const v4u r = __builtin_nontemporal_load(reinterpret_cast<const v4u *>(p));
+    const v4u r = __builtin_nontemporal_load(ga(reinterpret_cast<const v4u *>(p)));
     return make_uint4(r.x, r.y, r.z, r.w);
 }
 template <typename T>
 __device__ __forceinline__ void nts(T *p, T v)
 {
-    __builtin_nontemporal_store(v, p);
+    __builtin_nontemporal_store(v, ga(p));
 }
 // NT: non-temporal or ordinary.  The compact-record build streams
 // non-temporally (networks of up to ~2M GR, whose GR pass is short enough for
@@ -117,12 +199,14 @@ template <bool NT, typename T>
 __device__ __forceinline__ T gld(const T *p)
 {
     if constexpr (NT) return ntl(p);
-    else return *p;
+    else if constexpr (std::is_scalar<T>::value) return *ga(p);
+    else return *p;  // (the vector types have no constructor from a global-space object)
 }
 template <bool NT, typename T>
 __device__ __forceinline__ void gst(T *p, T v)
 {
     if constexpr (NT) nts(p, v);
+    else if constexpr (std::is_scalar<T>::value) *ga(p) = v;
     else *p = v;
 }
 
@@ -155,10 +239,10 @@ __device__ __forceinline__ uint32_t bit_at(const uint32_t *bm, uint32_t idx) { r
 // earlier one, the staging waits only for the spike bytes, and each cell's
 // update starts as soon as its own loads have arrived.  FULL: count ==
 // NT*CPT; COMPACT: one 16-byte connectivity record per GR (a.conn).
-template <int CPT, int NT, bool FULL, bool COMPACT, bool PLAST>
+template <int CPT, int NT, bool FULL, bool COMPACT, bool PLAST, bool RES = false>
 __device__ __forceinline__ void gr_span_impl(const GrArgs &a, const GrStep &st, const uint32_t first,
                                              const uint32_t count, const uint32_t rank, const uint32_t ranks,
-                                             uint32_t *smem)
+                                             uint32_t *smem, GrRegs<CPT> *R = nullptr)
 {
     constexpr int kSeg = NT * CPT / 64 + 1;  // PF blocks a span can touch (blocks hold >= 64 GRs)
     constexpr int kStage = 2048 / NT;        // spike bytes of each vector per thread held in registers
@@ -173,19 +257,27 @@ __device__ __forceinline__ void gr_span_impl(const GrArgs &a, const GrStep &st, 
     uint32_t *s_mfb = reinterpret_cast<uint32_t *>(s_step + nStep);  // [nMFb] MF spike bitmap
     uint32_t *s_gob = s_mfb + nMFb;                  // [nGOb] GO spike bitmap (previous step)
 
-    const int tid = threadIdx.x;
+    int tid = threadIdx.x;
+    // resident: the thread index through an opaque copy, so that the values
+    // derived from it (LDS addresses, cell offsets) are computed again in each
+    // step instead of being held in registers across the run's step loop
+    if constexpr (RES) asm volatile("" : "+v"(tid));
     const int lane = tid & 63;
     const uint32_t wbase = (uint32_t)(tid & ~63);
+    // the plasticity step: a variant of its own (PLAST), or in a resident run
+    // a uniform branch of one variant (two inlined variants would hold more
+    // registers than the cells' state leaves)
+    const bool plast = RES ? (PLAST && st.doPlast != 0) : PLAST;
     // spike bytes, from clamped addresses (a load under a branch makes the
     // compiler wait for every load in flight before the value is used)
     uint32_t smf[kStage], sgo[kStage];
 #pragma unroll
     for (int k = 0; k < kStage; k++) {
         const uint32_t i = tid + k * NT;
-        smf[k] = st.mf[min(i, a.numMF - 1)];
-        sgo[k] = st.apGOprev[min(i, a.numGO - 1)];
+        smf[k] = *ga(st.mf + min(i, a.numMF - 1));
+        sgo[k] = xld<RES>(st.apGOprev + min(i, a.numGO - 1));
     }
-    const int32_t tm0 = st.plastTimer[min((uint32_t)tid, nStep - 1)];
+    const int32_t tm0 = xld<RES>(st.plastTimer + min((uint32_t)tid, nStep - 1));
     // MZone::runPFPCPlastCUDA step choice (mzone.cpp:660-674)
     // (the zone's step sizes picked by compares among register copies: an
     // indexed read of the argument block would be one more memory load to
@@ -212,12 +304,12 @@ __device__ __forceinline__ void gr_span_impl(const GrArgs &a, const GrStep &st, 
     // what the registers do not hold (large networks only), before the cells'
     // loads: a loop that loads makes the compiler wait for every load in
     // flight before it
-    if (PLAST && nStep > (uint32_t)NT)
-        for (uint32_t i = tid + NT; i < nStep; i += NT) s_step[i] = stepOf(i, st.plastTimer[i]);
+    if (plast && nStep > (uint32_t)NT)
+        for (uint32_t i = tid + NT; i < nStep; i += NT) s_step[i] = stepOf(i, xld<RES>(st.plastTimer + i));
     for (uint32_t base = kStage * NT + wbase; base < 32 * nMFb; base += NT)  // wave-uniform
-        stage64(s_mfb, nMFb, base, base + lane < a.numMF && st.mf[base + lane] != 0);
+        stage64(s_mfb, nMFb, base, base + lane < a.numMF && *ga(st.mf + min(base + lane, a.numMF - 1)) != 0);
     for (uint32_t base = kStage * NT + wbase; base < 32 * nGOb; base += NT)
-        stage64(s_gob, nGOb, base, base + lane < a.numGO && st.apGOprev[base + lane] != 0);
+        stage64(s_gob, nGOb, base, base + lane < a.numGO && xld<RES>(st.apGOprev + min(base + lane, a.numGO - 1)) != 0);
     __builtin_amdgcn_sched_barrier(0);  // keep the loads above ahead of the cells' loads
 
     float v[CPT], gk[CPT], th[CPT], gE[CPT], gI[CPT], w0[CPT];
@@ -240,6 +332,10 @@ __device__ __forceinline__ void gr_span_impl(const GrArgs &a, const GrStep &st, 
     for (int j = 0; j < CPT; j++) {
         const uint32_t gr = cellOf(j);
         __builtin_amdgcn_sched_barrier(0);
+        if constexpr (RES) {  // the block's cells are in registers (R)
+            if (plast) hist[j] = *ga(st.histOld + (gr >> 6));
+            continue;
+        }
         v[j] = gld<COMPACT>(el(a.v, gr));
         gk[j] = gld<COMPACT>(el(a.gKCa, gr));
         th[j] = gld<COMPACT>(el(a.thr, gr));
@@ -256,7 +352,7 @@ __device__ __forceinline__ void gr_span_impl(const GrArgs &a, const GrStep &st, 
             bsp_[j] = gld<COMPACT>(el(a.bspDly, gr));
         }
         w0[j] = gld<COMPACT>(el(a.w, gr));
-        if constexpr (PLAST) hist[j] = st.histOld[gr >> 6];  // one address per wave
+        if constexpr (PLAST) hist[j] = *ga(st.histOld + (gr >> 6));  // one address per wave
     }
     __builtin_amdgcn_sched_barrier(0);
 
@@ -268,7 +364,7 @@ __device__ __forceinline__ void gr_span_impl(const GrArgs &a, const GrStep &st, 
         stage64(s_mfb, nMFb, base, i < a.numMF && smf[k] != 0);
         stage64(s_gob, nGOb, base, i < a.numGO && sgo[k] != 0);
     }
-    if (PLAST && (uint32_t)tid < nStep) s_step[tid] = stepOf(tid, tm0);
+    if (plast && (uint32_t)tid < nStep) s_step[tid] = stepOf(tid, tm0);
     for (uint32_t i = tid; i < a.numGO; i += NT) s_goh[i] = 0;
     if (tid < 2 * kSeg) s_sc[tid] = 0;  // s_sc and s_bc are contiguous
     __syncthreads();
@@ -280,17 +376,27 @@ __device__ __forceinline__ void gr_span_impl(const GrArgs &a, const GrStep &st, 
 #pragma unroll
     for (int j = 0; j < CPT; j++) {
         if (!FULL && j * NT + (tid & ~63) >= count) continue;  // wave-uniform
+        // resident: one cell at a time (interleaving the cells' updates would
+        // hold more temporaries than the registers the cells' state leaves)
+        if constexpr (RES) __builtin_amdgcn_sched_barrier(0);
         const uint32_t gr = first + j * NT + tid;
-        float vv = v[j], g = gk[j], t = th[j], e = gE[j], i = gI[j];
-        uint32_t b = ab[j];
+        float vv, g, t, e, i;
+        uint32_t b;
+        uint4 rj;
+        if constexpr (RES) {
+            vv = R->v[j], g = R->gk[j], t = R->th[j], e = R->gE[j], i = R->gI[j], b = R->ab[j], rj = R->rec[j * NT + tid];
+        } else {
+            vv = v[j], g = gk[j], t = th[j], e = gE[j], i = gI[j], b = ab[j];
+            if constexpr (COMPACT) rj = rec[j];
+        }
         // connectivity fields: MF and GO inputs clamped onto the zero bit of
         // the bitmaps when absent, the delay of the spike towards BC/PC/SC
         // (>= 32: never), and up to four GO targets with their delays
         uint32_t in[8], dBsp, tg[4], dg[4];
         if constexpr (COMPACT) {
             // packGrConn: 11-bit indices (0x7FF absent: a zero bit), 6-bit delays (0x3F never)
-            const uint64_t lo = (uint64_t)rec[j].x | (uint64_t)rec[j].y << 32;
-            const uint64_t hi = (uint64_t)rec[j].z | (uint64_t)rec[j].w << 32;
+            const uint64_t lo = (uint64_t)rj.x | (uint64_t)rj.y << 32;
+            const uint64_t hi = (uint64_t)rj.z | (uint64_t)rj.w << 32;
             auto f11 = [](uint64_t w, int sh) { return (uint32_t)(w >> sh) & kNoCell11; };
             auto f6 = [](uint64_t w, int sh) { return (uint32_t)(w >> sh) & kNever6; };
             in[0] = f11(lo, 0);
@@ -348,15 +454,19 @@ __device__ __forceinline__ void gr_span_impl(const GrArgs &a, const GrStep &st, 
         e = e * a.decMF + a.incMF * (cm);
         i = i * a.decGO + a.incGO * (cg);
 
-        gst<COMPACT>(el(a.v, gr), vv);
-        gst<COMPACT>(el(a.thr, gr), thN);
-        gst<COMPACT>(el(a.gKCa, gr), gkN);
-        gst<COMPACT>(el(a.apBuf, gr), b);
-        gst<COMPACT>(el(a.gE, gr), e);
-        gst<COMPACT>(el(a.gI, gr), i);
+        if constexpr (RES) {
+            R->v[j] = vv, R->th[j] = thN, R->gk[j] = gkN, R->ab[j] = b, R->gE[j] = e, R->gI[j] = i;
+        } else {
+            gst<COMPACT>(el(a.v, gr), vv);
+            gst<COMPACT>(el(a.thr, gr), thN);
+            gst<COMPACT>(el(a.gKCa, gr), gkN);
+            gst<COMPACT>(el(a.apBuf, gr), b);
+            gst<COMPACT>(el(a.gE, gr), e);
+            gst<COMPACT>(el(a.gI, gr), i);
+        }
 
         uint32_t bit39 = 0;
-        if constexpr (PLAST) {
+        if (plast) {
             bit39 = (uint32_t)(hist[j] >> lane) & 1u;
             // updateGRHistory (kernels.cu:244-245): mask 0x1F is hard-coded
             // there; the new bit of every GR of the wave in one plane word
@@ -372,21 +482,22 @@ __device__ __forceinline__ void gr_span_impl(const GrArgs &a, const GrStep &st, 
         const uint32_t io = gr >> a.log2GRperIO;
         for (uint32_t z = 0; z < a.numZones; z++) {  // (numZones is 1 in most networks)
             float *wz = a.w + (size_t)z * a.numGR;
-            float wv = z == 0 ? w0[j] : gld<COMPACT>(el(wz, gr));
-            if constexpr (PLAST) {
+            float wv = z == 0 ? (RES ? R->w0[j] : w0[j]) : gld<COMPACT>(el(wz, gr));
+            if (plast) {
                 const float plastStep = s_step[z * a.numIO + io];
                 wv = wv + (bit39 > 0) * plastStep;
                 wv = (wv > 0) * wv;
                 wv = (wv > 1) + (wv <= 1) * wv;
-                gst<COMPACT>(el(wz, gr), wv);
+                if (RES && z == 0) R->w0[j] = wv;
+                else gst<COMPACT>(el(wz, gr), wv);
             }
             float x = 0.0f;
             if (bal != 0) x = wave_sum_lane0(wv * d);
-            if (lane == 0) st.chunk[(size_t)z * nChunks + (gr >> 6)] = x;
+            if (lane == 0) xst<RES>(st.chunk + (size_t)z * nChunks + (gr >> 6), x);
         }
 
         // PF->BC and PF->SC spike counts of this wave's 64 GRs
-        if (st.pfCnt && lane == 0) st.pfCnt[gr >> 6] = (uint8_t)__popcll(bal);
+        if (st.pfCnt && lane == 0) *ga(st.pfCnt + (gr >> 6)) = (uint8_t)__popcll(bal);
         if (lane == 0 && bal != 0) {
             const uint32_t c = (uint32_t)__popcll(bal);
             atomicAdd(&s_sc[(gr >> a.log2Lsc) - firstSc], c);
@@ -406,18 +517,18 @@ __device__ __forceinline__ void gr_span_impl(const GrArgs &a, const GrStep &st, 
 
     for (uint32_t i = tid; i < a.numGO; i += NT) {
         const uint32_t c = s_goh[i];
-        if (c) atomicAdd(&st.goCnt[i], c);
+        if (c) gadd(st.goCnt + i, c);
     }
     {
         // PF blocks the span touches; blocks at either end may be shared with
         // the neighbouring span, whose counts add in by the same atomics
         const uint32_t nSc = ((first + count - 1) >> a.log2Lsc) - firstSc + 1;
         const uint32_t nBc = ((first + count - 1) >> a.log2Lbc) - firstBc + 1;
-        if ((uint32_t)tid < nSc && s_sc[tid]) atomicAdd(&st.scCnt[firstSc + tid], s_sc[tid]);
-        if ((uint32_t)tid < nBc && s_bc[tid]) atomicAdd(&st.bcCnt[firstBc + tid], s_bc[tid]);
+        if ((uint32_t)tid < nSc && s_sc[tid]) gadd(st.scCnt + firstSc + tid, s_sc[tid]);
+        if ((uint32_t)tid < nBc && s_bc[tid]) gadd(st.bcCnt + firstBc + tid, s_bc[tid]);
     }
 
-    gr_fold<NT>(a, st, rank, ranks);
+    gr_fold<NT, RES>(a, st, rank, ranks);
 }
 
 // Connectivity format of a kernel instantiation: kFmtAny decides per launch
@@ -561,11 +672,6 @@ __global__ void __launch_bounds__(256) pfpc_fold_kernel(const float *__restrict_
 }
 
 // ---------------------------------------------------------------- small kernel
-// Device pointers read from the argument block are generic; casting them to
-// the global address space gives global_load/global_store, which do not share
-// the LDS wait counter the way flat accesses do.
-template <typename T>
-using G = __attribute__((address_space(1))) T;
 
 template <typename T>
 __device__ __forceinline__ G<T> *gp(T *p)
@@ -575,6 +681,20 @@ __device__ __forceinline__ G<T> *gp(T *p)
 
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
+// xld / xst for global-space pointers
+template <bool X, typename T>
+__device__ __forceinline__ T xldg(const G<T> *p)
+{
+    if constexpr (X) return __hip_atomic_load(const_cast<G<T> *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else return *p;
+}
+template <bool X, typename T>
+__device__ __forceinline__ void xstg(G<T> *p, T v)
+{
+    if constexpr (X) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else *p = v;
+}
 
 template <typename T>
 __device__ __forceinline__ G<T> *F(const SmallArgs &a, int f)
@@ -621,14 +741,19 @@ __device__ __forceinline__ float lds_sum_seq(const float *x, uint32_t n, float a
 // Within a phase every loop iteration issues all of its global loads before
 // its first store, so each phase costs about one memory round trip, and each
 // population or synapse group starts on its own wave, so that no wave runs
-// the updates of every small population in turn.
+// the updates of every small population in turn.  X: inside a resident run
+// (the arrays other blocks share go through xldg / xstg).
+template <bool X = false>
 __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &st, const float *errDrive,
                                            uint8_t *lds)
 {
     const cbm_sizes &s = a.s;
     const cbm_act_params &p = a.p;
     const SmallLds &L = a.lds;
-    const int tid = threadIdx.x;
+    int tid = threadIdx.x;
+    // resident: as in gr_span_impl, values derived from the thread index are
+    // computed again in each step instead of being held across the step loop
+    if constexpr (X) asm volatile("" : "+v"(tid));
     const int lane = tid & 63, wave = tid >> 6;
     constexpr int nThr = kSmallThreads, nWave = kSmallThreads / 64;
     const uint32_t par = st.par, parPrev = par ^ 1u;
@@ -675,8 +800,8 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
             const uint32_t im = min(i, s.numMF - 1), ip = min(i, nPop - 1);
             const uint32_t w = mfw[min(i, nMFw - 1)];
             const u32x4 bl = ((const G<u32x4> *)a.idxBlob)[min(i, L.idx16 - 1)];
-            const uint32_t r = rngIn[min(i, nRng - 1)];
-            const uint32_t rn = nextIn[min(i, nNext - 1)];
+            const uint32_t r = xldg<X>(rngIn + min(i, nRng - 1));
+            const uint32_t rn = xldg<X>(nextIn + min(i, nNext - 1));
             const uint32_t pz = ip / L.popStride, pk = ip % L.popStride;
             const G<uint32_t> *pw = pk < 3 ? F<uint32_t>(a, CBM_F_PC_POPACT_CTRS) + pz * 3 + pk
                                            : F<uint32_t>(a, CBM_F_HIST_PC_POPACT) + (size_t)pz * bins +
@@ -820,24 +945,24 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
             // ---- loads, at clamped indices (in bounds, no branch)
             const uint32_t iP = min(cP, nPCc - 1), iB = min(cB, nBCc - 1), iS = min(cS, nSCc - 1);
             const uint32_t iI = min(cI, nIOc - 1), iN = min(cN, nNCc - 1);
-            const float pcPF = gp(a.pfpcRing)[(size_t)st.slotT2 * nPCc + iP];
+            const float pcPF = xldg<X>(gp(a.pfpcRing) + (size_t)st.slotT2 * nPCc + iP);
             const uint32_t pcIn = F<uint32_t>(a, CBM_F_INPUT_BC_PC)[iP];
             const float pcGPF = F<float>(a, CBM_F_G_PF_PC)[iP], pcGBC = F<float>(a, CBM_F_G_BC_PC)[iP];
             const float pcV = F<float>(a, CBM_F_VM_PC)[iP], pcTh = F<float>(a, CBM_F_THRESH_PC)[iP];
             const uint32_t pcAb = F<uint32_t>(a, CBM_F_APBUF_PC)[iP];
-            const uint32_t bcPF = gp(a.bcRing)[(size_t)st.slotT2 * s.numBC + (iB & (s.numBC - 1))];
+            const uint32_t bcPF = xldg<X>(gp(a.bcRing) + (size_t)st.slotT2 * s.numBC + (iB & (s.numBC - 1)));
             const uint32_t bcIn = F<uint32_t>(a, CBM_F_INPUT_PC_BC)[iB];
             const float bcGPF = F<float>(a, CBM_F_G_PF_BC)[iB], bcGPC = F<float>(a, CBM_F_G_PC_BC)[iB];
             const float bcV = F<float>(a, CBM_F_VM_BC)[iB], bcTh = F<float>(a, CBM_F_THRESH_BC)[iB];
             const uint32_t bcAb = F<uint32_t>(a, CBM_F_APBUF_BC)[iB];
-            const uint32_t scPF = gp(a.scRing)[(size_t)st.slotT2 * s.numSC + iS];
+            const uint32_t scPF = xldg<X>(gp(a.scRing) + (size_t)st.slotT2 * s.numSC + iS);
             const float scGPF = F<float>(a, CBM_F_G_PF_SC)[iS], scV = F<float>(a, CBM_F_VM_SC)[iS];
             const float scTh = F<float>(a, CBM_F_THRESH_SC)[iS];
             const uint32_t scAb = F<uint32_t>(a, CBM_F_APBUF_SC)[iS];
             const float ioV = F<float>(a, CBM_F_VM_IO)[iI], ioTh = F<float>(a, CBM_F_THRESH_IO)[iI];
             const float ioVC = F<float>(a, CBM_F_VCOUPLE_IO)[iI];
             const uint32_t ioAb = F<uint32_t>(a, CBM_F_APBUF_IO)[iI];
-            const int32_t ioTm = gp(a.timer2)[(size_t)parPrev * nIOc + iI];
+            const int32_t ioTm = xldg<X>(gp(a.timer2) + (size_t)parPrev * nIOc + iI);
             const float ncV = F<float>(a, CBM_F_VM_NC)[iN], ncTh = F<float>(a, CBM_F_THRESH_NC)[iN];
             const float ncR = F<float>(a, CBM_F_SYN_IO_PRELEASE_NC)[iN];
             const uint32_t ncAb = F<uint32_t>(a, CBM_F_APBUF_NC)[iN];
@@ -913,7 +1038,7 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
                 F<uint32_t>(a, CBM_F_APBUF_IO)[i] = (ioAb << 1) | (ap * 0x00000001u);
                 F<float>(a, CBM_F_THRESH_IO)[i] =
                     p.threshMaxIO * ap + (!ap) * (ioTh + p.threshDecIO * (p.threshRestIO - ioTh));
-                gp(a.timer2)[(size_t)par * nIOc + i] = (!ap) * (ioTm + 1) + ap * p.tsLTDStartAPIO;
+                xstg<X>(gp(a.timer2) + (size_t)par * nIOc + i, (int32_t)((!ap) * (ioTm + 1) + ap * p.tsLTDStartAPIO));
             }
             // ---- NC
             if (hNC) {
@@ -945,7 +1070,7 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
             float gMF = 0, gGR = 0, gGO = 0, glu = 0, mG = 0, gInc = 0, gMG = 0, thv = 0, v = 0, vC = 0, sc = 0, inDep = 0;
             uint32_t totalGRIn = 0, inMF = 0, goAb = 0;
             {
-                totalGRIn = gp(a.goRing)[(size_t)st.slotG * s.numGO + i];
+                totalGRIn = xldg<X>(gp(a.goRing) + (size_t)st.slotG * s.numGO + i);
                 inMF = F<uint32_t>(a, CBM_F_INPUT_MF_GO)[i];
                 inDep = F<float>(a, CBM_F_INPUT_GOGABA_SYNDEP_GO)[i];
                 gMF = F<float>(a, CBM_F_G_MF_GO)[i];
@@ -982,7 +1107,7 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
                 F<float>(a, CBM_F_G_MGLURINC_GO)[i] = gInc;
                 F<float>(a, CBM_F_G_MGLUR_GO)[i] = gMG;
                 F<float>(a, CBM_F_VM_GO)[i] = v;
-                gp(a.apGO2)[(size_t)par * s.numGO + i] = ap;
+                xstg<X>(gp(a.apGO2) + (size_t)par * s.numGO + i, ap);
                 s_apGO[i] = ap;
                 s_scale[i] = sc;
                 if (a.anyCoupling) s_vGO[i] = v;
@@ -1148,9 +1273,9 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
         }
     }
     // the ring slot of step t+1 is the slot PC/BC/SC read above (t-2): clear it now
-    for (uint32_t i = tid; i < s.numBC; i += nThr) gp(a.bcRing)[(size_t)st.slotNext * s.numBC + i] = 0;
-    for (uint32_t i = tid; i < s.numSC; i += nThr) gp(a.scRing)[(size_t)st.slotNext * s.numSC + i] = 0;
-    for (uint32_t i = tid; i < s.numGO; i += nThr) gp(a.goRing)[(size_t)st.slotNext * s.numGO + i] = 0;
+    for (uint32_t i = tid; i < s.numBC; i += nThr) xstg<X>(gp(a.bcRing) + (size_t)st.slotNext * s.numBC + i, 0u);
+    for (uint32_t i = tid; i < s.numSC; i += nThr) xstg<X>(gp(a.scRing) + (size_t)st.slotNext * s.numSC + i, 0u);
+    for (uint32_t i = tid; i < s.numGO; i += nThr) xstg<X>(gp(a.goRing) + (size_t)st.slotNext * s.numGO + i, 0u);
     __syncthreads();
 
     // ======================= phase 3
@@ -1201,7 +1326,7 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
             }
         }
         __threadfence_block();
-        for (uint32_t i = lane; i < nZ * kRngZone; i += 64) gp(a.rngState)[(size_t)par * nZ * kRngZone + i] = s_rng[i];
+        for (uint32_t i = lane; i < nZ * kRngZone; i += 64) xstg<X>(gp(a.rngState) + (size_t)par * nZ * kRngZone + i, s_rng[i]);
     }
     // GO->GO synaptic depression scale (innet.cpp:952-954), after the GABA
     // gather above has used the previous scale
@@ -1241,6 +1366,7 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
 // from the state the previous step left, block gen + 1 into this step's
 // slot, or a copy of the previous slot when that already holds it.  The
 // small step takes a prebuilt block only from a slot an earlier launch wrote.
+template <bool X = false>
 __device__ void sfmt_prebuild(const SmallArgs &a, const SmallStep &st, uint32_t *lds)
 {
     const uint32_t nZ = a.s.numZones, lane = threadIdx.x, z = lane >> 2, k = lane & 3u;
@@ -1249,17 +1375,17 @@ __device__ void sfmt_prebuild(const SmallArgs &a, const SmallStep &st, uint32_t 
     const G<uint32_t> *cur = gp(a.rngState) + ((size_t)parPrev * nZ + z) * kRngZone;
     const G<uint32_t> *nin = gp(a.rngNext) + ((size_t)parPrev * nZ + z) * kRngNext;
     G<uint32_t> *nout = gp(a.rngNext) + ((size_t)par * nZ + z) * kRngNext;
-    const uint32_t gen = cur[353], tag = nin[352];
+    const uint32_t gen = xldg<X>(cur + 353), tag = xldg<X>(nin + 352);
     if (tag == gen + 1u) {
-        for (uint32_t w = k; w < 352; w += 4) nout[w] = nin[w];
-        if (k == 0) nout[352] = tag;
+        for (uint32_t w = k; w < 352; w += 4) xstg<X>(nout + w, xldg<X>(nin + w));
+        if (k == 0) xstg<X>(nout + 352, tag);
         return;
     }
     uint32_t *blk = lds + (size_t)z * 352;  // lane k touches only words 4i + k
-    for (uint32_t w = k; w < 352; w += 4) blk[w] = cur[w];
+    for (uint32_t w = k; w < 352; w += 4) blk[w] = xldg<X>(cur + w);
     sfmt_regen4(blk, (int)k);
-    for (uint32_t w = k; w < 352; w += 4) nout[w] = blk[w];
-    if (k == 0) nout[352] = gen + 1u;
+    for (uint32_t w = k; w < 352; w += 4) xstg<X>(nout + w, blk[w]);
+    if (k == 0) xstg<X>(nout + 352, gen + 1u);
 }
 
 // the small populations alone (the two-stream step, and per-kernel timing)
@@ -1317,6 +1443,203 @@ CBM_STEP_SPAN(3, kFmtArrays)
 CBM_STEP_SPAN(5, kFmtArrays)
 #undef CBM_STEP_SPAN
 
+// ---------------------------------------------------------------- resident run
+// Grid barrier between two steps of a resident run.  Every thread first waits
+// until its own stores and atomics are acknowledged, so what it wrote for the
+// other blocks has reached the point of coherence; then thread 0 of each block
+// adds one to the counter of its group (blocks b and b + 8 share one; eight
+// counters 256 bytes apart, so that 256 arrivals do not queue on one address)
+// and polls the eight until they sum to `target` (the counters only grow; the
+// comparison is modulo 2^32).  The wait is bounded: a block that waits for
+// ~0.2 s records the failure in *status and leaves the run, so a launch whose
+// blocks were not all resident ends instead of hanging.
+constexpr uint64_t kBarrierTicks = 20000000ull;  // s_memrealtime runs at 100 MHz
+
+__device__ __forceinline__ bool grid_sync(uint32_t *bar, uint32_t *status, uint32_t target)
+{
+    __shared__ uint32_t s_ok;
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __hip_atomic_fetch_add(bar + (blockIdx.x & (kBarGroups - 1)) * kBarStride, 1u, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+        uint32_t ok = 1;
+        while (true) {
+            uint32_t c[kBarGroups];
+#pragma unroll
+            for (int q = 0; q < kBarGroups; q++)
+                c[q] = __hip_atomic_load(bar + q * kBarStride, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            uint32_t sum = 0;
+#pragma unroll
+            for (int q = 0; q < kBarGroups; q++) sum += c[q];
+            if ((int32_t)(sum - target) >= 0) break;
+            if (__builtin_amdgcn_s_memrealtime() - t0 > kBarrierTicks) {
+                __hip_atomic_store(status, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                ok = 0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        s_ok = ok;
+    }
+    __syncthreads();
+    return s_ok != 0;
+}
+
+// The small populations' part of a resident run (block 0).
+__device__ __forceinline__ void res_small(const SmallArgs &a, const ResRun &r, float *s_err, uint8_t *lds)
+{
+    const uint32_t nZ = a.s.numZones, nb = gridDim.x;
+    for (uint32_t k = 0; k < r.n; k++) {
+        const uint64_t t = r.t0 + k + 1;
+        SmallStep st;
+        st.mf = r.mf + (size_t)k * r.pitch;
+        st.t = t;
+        st.ioOut = nullptr;
+        st.ioPop = 0;
+        uint32_t tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));  // as in small_step
+        if (tid < (uint32_t)kMaxZones) {
+            const uint32_t z = tid;
+            float e = k == 0 ? r.errDrive0[z] : 0.0f;
+            if (r.err && z < nZ) {
+                const float x = r.err[(size_t)k * nZ + z];
+                if (x != 0.0f) e = x * r.maxExtIncVIO;
+            }
+            s_err[z] = e;
+        }
+        st.gIncGRtoGO = r.gIncGRtoGO;
+        st.par = (uint32_t)(t & 1u);
+        st.slotPrev = sreg((uint32_t)((t + kRing - 1) % kRing));
+        st.slotT2 = sreg((uint32_t)((t + kRing - 2) % kRing));
+        st.slotG = sreg((uint32_t)((t + kRing - a.grgoLag) % kRing));
+        st.slotNext = sreg((uint32_t)((t + 1) % kRing));
+        st.popHist = sreg((t % a.p.tsPerPopHistBinPC) == 0);
+        st.resetMFHist = sreg((t % a.p.numTSinMFHist) == 0);
+        __syncthreads();
+        // the argument block through a pointer the compiler cannot follow
+        // across steps (its fields are loaded again in each step instead of
+        // being held, and spilled, across the loop), in the constant address
+        // space: the block does not change during the run, so its fields are
+        // read with scalar loads that no store of the step has to wait for
+        uint64_t pa = reinterpret_cast<uint64_t>(&a);
+        asm volatile("" : "+s"(pa));
+        const SmallArgs *ap = (const SmallArgs *)(const __attribute__((address_space(4))) SmallArgs *)pa;
+        small_step<true>(*ap, st, s_err, lds);
+        if (k + 1 < r.n && !grid_sync(r.bar, r.status, r.barBase + (k + 1) * nb)) return;
+    }
+}
+
+// The generators' next blocks in a resident run (the last block).
+__device__ __forceinline__ void res_prebuild(const SmallArgs &a, const ResRun &r, uint8_t *lds)
+{
+    const uint32_t nb = gridDim.x;
+    for (uint32_t k = 0; k < r.n; k++) {
+        SmallStep st;
+        st.par = (uint32_t)((r.t0 + k + 1) & 1u);
+        sfmt_prebuild<true>(a, st, reinterpret_cast<uint32_t *>(lds));
+        if (k + 1 < r.n && !grid_sync(r.bar, r.status, r.barBase + (k + 1) * nb)) return;
+    }
+}
+
+// The GR cells of span `rank` in a resident run (blocks 1 .. spans).
+template <int CPT>
+__device__ __forceinline__ void res_gr(const SmallArgs &a, const ResRun &r, const uint32_t rank, uint8_t *lds)
+{
+    const GrArgs &g = r.g;
+    const uint32_t nb = gridDim.x, nZ = g.numZones, nC = g.numGR >> 6, tid = threadIdx.x;
+    const uint32_t perHist = a.p.tsPerHistBinGR;
+    const uint32_t c0 = (uint32_t)((uint64_t)rank * nC / r.spans), c1 = (uint32_t)((uint64_t)(rank + 1) * nC / r.spans);
+    const uint32_t first = c0 << 6, count = (c1 - c0) << 6;
+    uint32_t *smem = reinterpret_cast<uint32_t *>(lds);
+    GrRegs<CPT> R;
+    uint4 *recL = reinterpret_cast<uint4 *>(lds + r.recLds);
+    R.rec = recL;
+    if (count) {
+#pragma unroll
+        for (int j = 0; j < CPT; j++) {
+            const uint32_t gr = first + min((uint32_t)j * 1024u + tid, count - 1u);
+            R.v[j] = g.v[gr];
+            R.th[j] = g.thr[gr];
+            R.gk[j] = g.gKCa[gr];
+            R.gE[j] = g.gE[gr];
+            R.gI[j] = g.gI[gr];
+            R.ab[j] = g.apBuf[gr];
+            R.w0[j] = g.w[gr];
+            recL[j * 1024 + tid] = g.conn[gr];
+        }
+    }
+    for (uint32_t k = 0; k < r.n; k++) {
+        const uint64_t t = r.t0 + k + 1;
+        const uint32_t par = (uint32_t)(t & 1u), parPrev = par ^ 1u;
+        const uint32_t slot = (uint32_t)(t % kRing);
+        float *chunk2 = const_cast<float *>(a.chunk2);
+        // (uniform values in scalar registers: see sptr)
+        GrStep gs;
+        gs.mf = (sptr(r.mf + (size_t)k * r.pitch));
+        gs.apGOprev = (sptr(a.apGO2 + (size_t)parPrev * g.numGO));
+        gs.chunk = (sptr(chunk2 + (size_t)par * nZ * nC));
+        gs.foldSrc = (sptr((k == 0 && r.skipFold0) ? nullptr : chunk2 + (size_t)parPrev * nZ * nC));
+        gs.foldDst = (sptr(a.pfpcRing + (size_t)((t + kRing - 1) % kRing) * nZ * g.numPC));
+        gs.bcCnt = (sptr(a.bcRing + (size_t)slot * a.s.numBC));
+        gs.scCnt = (sptr(a.scRing + (size_t)slot * a.s.numSC));
+        gs.goCnt = (sptr(a.goRing + (size_t)slot * g.numGO));
+        gs.plastTimer = (sptr(a.timer2 + (size_t)parPrev * nZ * g.numIO));
+        gs.doPlast = sreg((t % perHist) == 0);
+        const uint64_t upd = t / perHist;
+        gs.histOld = (sptr(g.hist + (size_t)histPlane(upd, 40) * nC));
+        gs.histNew = (sptr(g.hist + (size_t)histPlane(upd, 0) * nC));
+        gs.pfCnt = (sptr((a.rec && r.recOffPF != 0xFFFFFFFFu)
+                               ? a.rec + (size_t)(t % a.recCap) * a.recRow + r.recOffPF
+                               : nullptr));
+        if (count == 0)
+            gr_fold<kSmallThreads, true>(g, gs, rank, r.spans);
+        else
+            gr_span_impl<CPT, kSmallThreads, false, true, true, true>(g, gs, first, count, rank, r.spans, smem, &R);
+        if (k + 1 < r.n && !grid_sync(r.bar, r.status, r.barBase + (k + 1) * nb)) return;
+    }
+    // the cells' state back into the per-GR arrays
+#pragma unroll
+    for (int j = 0; j < CPT; j++) {
+        if ((uint32_t)j * 1024u + (tid & ~63u) >= count) continue;  // wave-uniform
+        const uint32_t gr = first + (uint32_t)j * 1024u + tid;
+        g.v[gr] = R.v[j];
+        g.thr[gr] = R.th[j];
+        g.gKCa[gr] = R.gk[j];
+        g.gE[gr] = R.gE[j];
+        g.gI[gr] = R.gI[j];
+        g.apBuf[gr] = R.ab[j];
+        g.w[gr] = R.w0[j];
+    }
+}
+
+// n steps of one network in one cooperative launch of spans + 2 blocks of
+// 1024 threads, one per CU: block 0 steps the small populations, blocks
+// 1 .. spans the GR cells of their even split (chunks [r*C/spans,
+// (r+1)*C/spans) of the C 64-GR chunks, as step_span_kernel), whose state
+// stays in registers (GrRegs) from the first step to the last and is stored
+// back at the end, and the last block prebuilds the zone generators' next
+// blocks.  Each step does the work of one step_span_kernel launch, and the
+// grid barrier stands in for the launch boundary.  CPT: cells per thread
+// (every span holds at most 1024 * CPT cells); compact connectivity only.
+template <int CPT>
+__global__ void __launch_bounds__(kSmallThreads) step_resident_kernel(const SmallArgs *__restrict__ args, const ResRun r)
+{
+    extern __shared__ __align__(16) uint8_t lds[];
+    __shared__ float s_err[kMaxZones];  // in LDS: the small step indexes it by zone
+    if (blockIdx.x == 0)
+        res_small(*args, r, s_err, lds);
+    else if (blockIdx.x > r.spans)
+        res_prebuild(*args, r, lds);
+    else
+        res_gr<CPT>(*args, r, blockIdx.x - 1, lds);
+}
+
+template __global__ void step_resident_kernel<2>(const SmallArgs *, const ResRun);
+template __global__ void step_resident_kernel<3>(const SmallArgs *, const ResRun);
+template __global__ void step_resident_kernel<5>(const SmallArgs *, const ResRun);
+
 // ---------------------------------------------------------------- streaming ceiling
 // A per-cell pass with the GR step's streams and no arithmetic to speak of:
 // every cell reads nRead 4-byte values and a 16-byte record, and writes the
@@ -1349,14 +1672,6 @@ __global__ void __launch_bounds__(1024) stream_ceiling_kernel(StreamArgs a)
 }
 
 // ---------------------------------------------------------------- batch
-// Block-uniform values derived in vector arithmetic (the 64-bit remainders
-// below, pointers read from a network's record), moved to scalar registers so
-// that they do not occupy vector registers across the step.
-__device__ __forceinline__ uint32_t sreg(uint32_t x) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)x); }
-__device__ __forceinline__ uint64_t sreg64(uint64_t x)
-{
-    return ((uint64_t)sreg((uint32_t)(x >> 32)) << 32) | sreg((uint32_t)x);
-}
 
 // The small populations of network d at run step kk (block-wide).
 __device__ __forceinline__ void batch_small(const BatchNet &d, uint32_t kk, float *s_err, uint8_t *lds)

@@ -232,9 +232,9 @@ int cbm_create(const cbm_sizes *sizes, const cbm_act_params *params, const cbm_o
     h->fusedCpt = (h->o.grCellsPerThread == 1 || sizes->numGR < 2048) ? 1 : 2;
     const bool shapeOk = (h->nt == 256 && (h->cpt == 1 || h->cpt == 2 || h->cpt == 4)) ||
                          (h->nt == 1024 && (h->cpt == 1 || h->cpt == 2));
-    if (!shapeOk || h->o.stepMode > 4) {
+    if (!shapeOk || h->o.stepMode > 5) {
         delete h;
-        return fail(CBM_E_ARG, "GR kernel shape must be 256 threads x 1, 2 or 4 cells or 1024 x 1 or 2; stepMode 0..4");
+        return fail(CBM_E_ARG, "GR kernel shape must be 256 threads x 1, 2 or 4 cells or 1024 x 1 or 2; stepMode 0..5");
     }
     hipError_t e = hipSetDevice(h->o.device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
@@ -279,6 +279,8 @@ int cbm_destroy(cbm_handle *h)
     if (h->rec) (void)hipFree(h->rec);
     if (h->pinnedMF) (void)hipHostFree(h->pinnedMF);
     if (h->pinnedIO) (void)hipHostFree(h->pinnedIO);
+    if (h->resStatusHost) (void)hipHostFree(h->resStatusHost);
+    if (h->resErr) (void)hipFree(h->resErr);
     if (h->evG) (void)hipEventDestroy(h->evG);
     if (h->evS) (void)hipEventDestroy(h->evS);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -310,6 +312,14 @@ int cbmdev::syncAll(cbm_handle *h)
 {
     HIP_TRY(hipStreamSynchronize(h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream2));
+    if (h->resCheck) {  // the status of the last resident run, copied after it
+        h->resCheck = false;
+        if (*h->resStatusHost) {
+            h->resident = false;
+            return fail(CBM_E_HIP, "resident run: a grid barrier wait timed out (the run's workgroups were not all "
+                                   "resident at once); the network state is undefined");
+        }
+    }
     return CBM_OK;
 }
 
@@ -515,6 +525,54 @@ int cbm_write(cbm_handle *h, int field, int zone, const void *src, size_t bytes)
     }
     if (!h->dfield[field]) return fail(CBM_E_ARG, "field is not writable");
     HIP_TRY(hipMemcpy((uint8_t *)h->dfield[field] + off, src, bytes, hipMemcpyHostToDevice));
+    return CBM_OK;
+}
+
+// the resident-run kernel of a span shape
+static const void *residentKernel(int cpt)
+{
+    return cpt == 2 ? (const void *)step_resident_kernel<2>
+                    : cpt == 3 ? (const void *)step_resident_kernel<3> : (const void *)step_resident_kernel<5>;
+}
+
+// Decide at commit whether cbm_run_mf_train runs resident (stepMode 5; the
+// even split and compact connectivity records; a device that launches
+// cooperatively and holds one 1024-thread workgroup of the kernel on every
+// CU), and set it up: the LDS of the GR workgroups (their span's working LDS
+// and the records of their cells), the barrier counters and the status word.
+static int setupResident(cbm_handle *h, int cus)
+{
+    const cbm_sizes &s = h->s;
+    const int sm = (int)h->o.stepMode;
+    h->resident = false;
+    if (sm == 5 && h->spans && h->conn) {
+        int coop = 0;
+        HIP_TRY(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, h->o.device));
+        const uint32_t recOff = roundUp(grTileLds(1024, h->spanCpt, s.numGO, s.numMF, s.numZones, s.numIO), 16);
+        const size_t lds = std::max<size_t>(h->smallLds, recOff + (size_t)h->spanCpt * 1024 * 16);
+        if (coop && lds <= kSmallLdsMax) {
+            const void *k = residentKernel(h->spanCpt);
+            HIP_TRY(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            int per = 0;
+            HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k, kSmallThreads, lds));
+            if (per >= 1 && (int64_t)per * cus >= (int64_t)h->spans + 2) {
+                int rc;
+                if ((rc = h->dalloc((void **)&h->resBar, (size_t)kBarGroups * kBarStride * 4))) return rc;
+                if ((rc = h->dalloc((void **)&h->resStatus, 4))) return rc;
+                if (!h->resStatusHost) {
+                    HIP_TRY(hipHostMalloc((void **)&h->resStatusHost, 4, hipHostMallocDefault));
+                    *h->resStatusHost = 0;
+                }
+                h->resBarBase = 0;
+                h->residentLds = lds;
+                h->residentRecLds = recOff;
+                h->resident = true;
+            }
+        }
+    }
+    if (sm == 5 && !h->resident)
+        return fail(CBM_E_UNSUPPORTED, "stepMode 5: resident runs need the even GR split, compact connectivity "
+                                       "records and one cooperative 1024-thread workgroup per CU");
     return CBM_OK;
 }
 
@@ -769,23 +827,23 @@ int cbm_commit(cbm_handle *h)
         // fused step: every block gets max(small, GR tile) LDS (one 1024-thread
         // block per CU either way, the small step's registers decide that)
         h->mode = h->o.stepMode ? (int)h->o.stepMode : 1;
+        int cus = 0;
+        HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->o.device));
         // GR work of the fused step: split evenly over one block per CU beside
         // the small step's when every block gets 1,024 to 5,120 cells (mode 1;
         // mode 4 whenever it fits), else tiles of 1024 x fusedCpt cells (mode 3
         // always).  The even split leaves no last round of leftover tiles: on
         // the 1M-GR network it takes 16.8 us per step against 18.7 for tiles.
-        if (h->mode == 1 || h->mode == 4) {
-            int cus = 0;
-            HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->o.device));
+        if (h->mode == 1 || h->mode == 4 || h->mode == 5) {
             // one CU each for the small step and the generator block
             const uint32_t spans = cus > 2 ? (uint32_t)cus - 2 : 1u;
             const uint32_t per = (s.numGR / 64 + spans - 1) / spans * 64;  // most cells of a span
             const int cpt = per <= 2048 ? 2 : per <= 3072 ? 3 : per <= 5120 ? 5 : 0;
-            if (cpt && (h->mode == 4 || per >= 1024)) {
+            if (cpt && (h->mode >= 4 || per >= 1024)) {
                 h->spans = spans;
                 h->spanCpt = cpt;
-            } else if (h->mode == 4) {
-                return fail(CBM_E_UNSUPPORTED, "stepMode 4: more than 5,120 GR per block");
+            } else if (h->mode >= 4) {
+                return fail(CBM_E_UNSUPPORTED, "stepMode 4/5: more than 5,120 GR per block");
             }
         }
         h->fusedLds = std::max<size_t>(
@@ -798,6 +856,7 @@ int cbm_commit(cbm_handle *h)
                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)grLds));
         if ((rc = h->dalloc((void **)&h->smallArgs, sizeof(SmallArgs)))) return rc;
         HIP_TRY(hipMemcpy(h->smallArgs, &a, sizeof(SmallArgs), hipMemcpyHostToDevice));
+        if ((rc = setupResident(h, cus))) return rc;
     }
     h->skipFold = true;
     h->committed = true;
@@ -1286,6 +1345,67 @@ int cbmdev::checkTrain(cbm_handle *h, const uint8_t *mf_dev, size_t pitch, int n
     return CBM_OK;
 }
 
+// Steps of one resident launch at most: the barrier targets grow by the grid
+// size per step and are compared modulo 2^32.
+constexpr int kResMaxSteps = 1 << 20;
+
+// cbm_run_mf_train as resident launches (step_resident_kernel): the error
+// drive goes to the device first, each launch derives its steps' values as
+// launchStep does, and the launches' barrier status is copied to pinned host
+// memory for the next synchronisation to read.
+static int runResident(cbm_handle *h, const uint8_t *mf_dev, size_t pitch, int n, const float *err)
+{
+    const uint32_t nZ = h->s.numZones;
+    if (err) {
+        const size_t need = (size_t)n * nZ;
+        if (need > h->resErrCap) {
+            if (h->resErr) {
+                HIP_TRY(hipStreamSynchronize(h->stream));  // an earlier run may still read it
+                (void)hipFree(h->resErr);
+                h->resErr = nullptr;
+                h->resErrCap = 0;
+            }
+            const hipError_t e = hipMalloc((void **)&h->resErr, need * sizeof(float));
+            if (e != hipSuccess) return fail(CBM_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+            h->resErrCap = need;
+        }
+        HIP_TRY(hipMemcpyAsync(h->resErr, err, need * sizeof(float), hipMemcpyHostToDevice, h->stream));
+    }
+    ResRun r;
+    grConstArgs(h, &r.g);
+    r.pitch = pitch;
+    r.maxExtIncVIO = h->p.maxExtIncVIO;
+    r.gIncGRtoGO = h->gIncGRtoGO;
+    r.spans = h->spans;
+    r.recOffPF = (h->rec && h->recOffPF != 0xFFFFFFFFu) ? h->recOffPF : 0xFFFFFFFFu;
+    r.bar = h->resBar;
+    r.status = h->resStatus;
+    r.recLds = h->residentRecLds;
+    const uint32_t grid = h->spans + 2;
+    const void *kern = residentKernel(h->spanCpt);
+    const SmallArgs *sa = h->smallArgs;
+    for (int k0 = 0; k0 < n;) {
+        const int m = std::min(n - k0, kResMaxSteps);
+        r.mf = mf_dev + (size_t)k0 * pitch;
+        r.err = err ? h->resErr + (size_t)k0 * nZ : nullptr;
+        r.t0 = h->curTime;
+        for (int z = 0; z < kMaxZones; z++) r.errDrive0[z] = h->errDrive[z];
+        r.skipFold0 = h->skipFold ? 1u : 0u;
+        r.n = (uint32_t)m;
+        r.barBase = h->resBarBase;
+        void *args[] = {&sa, &r};
+        HIP_TRY(hipLaunchCooperativeKernel(kern, dim3(grid), dim3(kSmallThreads), args, h->residentLds, h->stream));
+        h->resBarBase += (uint32_t)(m - 1) * grid;  // one barrier between consecutive steps
+        h->curTime += (uint64_t)m;
+        for (int z = 0; z < kMaxZones; z++) h->errDrive[z] = 0.0f;  // used by the first step
+        h->skipFold = false;
+        k0 += m;
+    }
+    HIP_TRY(hipMemcpyAsync(h->resStatusHost, h->resStatus, 4, hipMemcpyDeviceToHost, h->stream));
+    h->resCheck = true;
+    return CBM_OK;
+}
+
 extern "C" {
 
 int cbm_run_mf_train(cbm_handle *h, const uint8_t *mf_dev, size_t pitch, int n, const float *err)
@@ -1293,13 +1413,17 @@ int cbm_run_mf_train(cbm_handle *h, const uint8_t *mf_dev, size_t pitch, int n, 
     int rc = checkTrain(h, mf_dev, pitch, n);
     if (rc) return rc;
     HIP_TRY(hipSetDevice(h->o.device));
-    for (int k = 0; k < n; k++) {
-        if (err)
-            for (uint32_t z = 0; z < h->s.numZones; z++) {
-                const float e = err[(size_t)k * h->s.numZones + z];
-                if (e != 0.0f) h->errDrive[z] = e * h->p.maxExtIncVIO;
-            }
-        if ((rc = launchStep(h, mf_dev + (size_t)k * pitch, kNoTiming))) return rc;
+    if (h->resident && n >= 2) {
+        if ((rc = runResident(h, mf_dev, pitch, n, err))) return rc;
+    } else {
+        for (int k = 0; k < n; k++) {
+            if (err)
+                for (uint32_t z = 0; z < h->s.numZones; z++) {
+                    const float e = err[(size_t)k * h->s.numZones + z];
+                    if (e != 0.0f) h->errDrive[z] = e * h->p.maxExtIncVIO;
+                }
+            if ((rc = launchStep(h, mf_dev + (size_t)k * pitch, kNoTiming))) return rc;
+        }
     }
     if ((rc = join(h))) return rc;
     if (n > 0) {
@@ -1454,6 +1578,13 @@ int cbm_step_mode(cbm_handle *h)
     if (!h) return fail(CBM_E_ARG, "null handle");
     if (!h->committed) return fail(CBM_E_STATE, "cbm_commit has not been called");
     return h->mode;
+}
+
+int cbm_resident(cbm_handle *h)
+{
+    if (!h) return fail(CBM_E_ARG, "null handle");
+    if (!h->committed) return fail(CBM_E_STATE, "cbm_commit has not been called");
+    return h->resident ? 1 : 0;
 }
 
 int cbm_gr_launch(cbm_handle *h, uint32_t *blocks, uint32_t *threads, uint32_t *cellsPerThread, uint32_t *evenSplit)

@@ -173,7 +173,9 @@ class Network:
         small-population kernels on two streams, 3 = fused with GR tiles of
         1024 x 2 cells, 4 = fused with the GR cells split evenly over the CUs
         (mode 1 picks the even split when each block gets 1,024 to 5,120
-        cells) (cbm_options.stepMode)."""
+        cells), 5 = as 4, with run_mf_train runs of two or more steps
+        resident: one cooperative launch, GR state in registers, see
+        `resident` (cbm_options.stepMode)."""
         L = _abi.hip_lib()
         o = Options()
         o.grgoLag, o.device, o.grCellsPerThread = grgo_lag, device, cells_per_thread
@@ -299,6 +301,13 @@ class Network:
         m = _abi.hip_lib().cbm_step_mode(self._h)
         check(min(m, 0), "cbm_step_mode")
         return m
+
+    @property
+    def resident(self) -> bool:
+        """run_mf_train runs of two or more steps are resident (cbm_resident)."""
+        m = _abi.hip_lib().cbm_resident(self._h)
+        check(min(m, 0), "cbm_resident")
+        return m == 1
 
     @property
     def gr_launch(self) -> dict:

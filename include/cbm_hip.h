@@ -145,7 +145,13 @@ typedef struct cbm_options {
      * else into tiles of 1024 x 2 cells), 2 = the GR and small-population
      * kernels on two streams, ordered by events, 3 = fused with GR tiles
      * always, 4 = fused with the even split whenever it gives each block at
-     * most 5,120 cells (else an error).  All give the same bits. */
+     * most 5,120 cells (else an error), 5 = as 4, and cbm_run_mf_train runs
+     * of two or more steps are resident (below), or cbm_commit fails.  Resident:
+     * the whole run is one cooperative launch of one workgroup per CU whose GR
+     * workgroups hold their cells' state in registers (and connectivity in
+     * LDS) from the first step to the last, with a grid barrier between steps
+     * (networks with the even split and compact connectivity records).  All
+     * give the same bits. */
     uint32_t stepMode;
     /* Device layout of the GR connectivity: 0 = pick (one 16-byte record per
      * GR when numMF and numGO are at most 2047 and the fan-ins are at most 4
@@ -412,7 +418,11 @@ int cbm_step_io_end(cbm_handle *h, uint8_t *out);
 /* n steps where step k reads MF spikes from mf_dev + k*pitch (a DEVICE
  * pointer to n rows of numMF bytes, e.g. a pre-generated spike train).
  * err_drive_host, if not NULL, gives [n][numZones] relative error drives
- * (0 = none) applied before each step. */
+ * (0 = none) applied before each step.  A run of two or more steps is
+ * resident with cbm_options.stepMode 5; a resident run
+ * whose workgroups were not all resident at once ends after a bounded wait
+ * with CBM_E_HIP at the next call that waits for the handle's work, and the
+ * network's state is then undefined. */
 int cbm_run_mf_train(cbm_handle *h, const uint8_t *mf_dev, size_t pitch, int n,
                      const float *err_drive_host);
 
@@ -450,9 +460,14 @@ int cbm_profile_split(cbm_handle *h, const uint8_t *mf_dev, size_t pitch, int n,
 int cbm_stream_ceiling(int device, uint64_t cells, int read_words, int write_words, int launches,
                        double *us_per_launch, uint64_t *bytes_per_launch);
 
-/* The launch mode the handle uses (1, 3 or 4 = fused, 2 = two streams; see
+/* The launch mode the handle uses (1, 3, 4 or 5 = fused, 2 = two streams; see
  * cbm_options.stepMode), or a negative CBM_E_* code. */
 int cbm_step_mode(cbm_handle *h);
+
+/* 1 when cbm_run_mf_train runs of two or more steps are resident (see
+ * cbm_options.stepMode), 0 when they launch step by step, or a negative
+ * CBM_E_* code.  No reference counterpart. */
+int cbm_resident(cbm_handle *h);
 
 /* How the step's GR work is launched: *blocks GR workgroups of *threads
  * threads with *cellsPerThread cells each at most, *evenSplit = 1 when the
