@@ -21,6 +21,28 @@ constexpr int kRing = 3;                // input-sum ring slots (t, t-1, t-2)
 // nothing after a host write).  Both are double-buffered by step parity.
 constexpr int kRngZone = 354, kRngNext = 353;
 constexpr int kSmallThreads = 1024;     // threads of the small-population kernel
+// The state fields the small-population step reads and writes, which a
+// resident run keeps in the small block's LDS from the first step to the last
+// (when they fit beside its scratch LDS).
+constexpr int kSmallFieldCount = 64;
+constexpr int kSmallFields[kSmallFieldCount] = {
+    CBM_F_APBUF_BC, CBM_F_APBUF_GO, CBM_F_APBUF_IO, CBM_F_APBUF_MF,
+    CBM_F_APBUF_NC, CBM_F_APBUF_PC, CBM_F_APBUF_SC, CBM_F_AP_BC,
+    CBM_F_AP_IO, CBM_F_AP_NC, CBM_F_AP_PC, CBM_F_AP_SC,
+    CBM_F_GLU_GO, CBM_F_GMAX_AMPA_MF_NC, CBM_F_GMAX_NMDA_MF_NC, CBM_F_GO_GABA_OUT_SYNSCALE_GOGO,
+    CBM_F_G_BC_PC, CBM_F_G_GO_GO, CBM_F_G_GR_GO, CBM_F_G_MF_AMPA_NC,
+    CBM_F_G_MF_GO, CBM_F_G_MF_NMDA_NC, CBM_F_G_MGLURINC_GO, CBM_F_G_MGLUR_GO,
+    CBM_F_G_NC_IO, CBM_F_G_PC_BC, CBM_F_G_PC_NC, CBM_F_G_PC_SCALE_NC,
+    CBM_F_G_PF_BC, CBM_F_G_PF_PC, CBM_F_G_PF_SC, CBM_F_G_SC_PC,
+    CBM_F_HIST_MF, CBM_F_HIST_PC_POPACT, CBM_F_INPUT_BC_PC, CBM_F_INPUT_GOGABA_SYNDEP_GO,
+    CBM_F_INPUT_GO_GO, CBM_F_INPUT_MF_GO, CBM_F_INPUT_MF_NC, CBM_F_INPUT_NC_IO,
+    CBM_F_INPUT_PC_BC, CBM_F_INPUT_PC_NC, CBM_F_INPUT_SC_PC, CBM_F_MFNC_NOLTP_NOLTD,
+    CBM_F_MF_SYNW_NC, CBM_F_MGLUR_GO, CBM_F_PC_POPACT_CTRS, CBM_F_SUM_GO_INPUT_GO,
+    CBM_F_SUM_GR_INPUT_GO, CBM_F_SYN_IO_PRELEASE_NC, CBM_F_THRESH_BC, CBM_F_THRESH_GO,
+    CBM_F_THRESH_IO, CBM_F_THRESH_NC, CBM_F_THRESH_PC, CBM_F_THRESH_SC,
+    CBM_F_VCOUPLE_GO, CBM_F_VCOUPLE_IO, CBM_F_VM_BC, CBM_F_VM_GO,
+    CBM_F_VM_IO, CBM_F_VM_NC, CBM_F_VM_PC, CBM_F_VM_SC};
+constexpr uint32_t kNotInLds = 0xFFFFFFFFu;
 constexpr uint32_t kSmallLdsMax = 152 * 1024;  // dynamic LDS cap of the small kernel
 
 // Granule-cell kernel arguments that do not change from step to step.
@@ -89,6 +111,9 @@ struct SmallLds {
     uint32_t synPC, synN, synA, synP, synIO;
     uint32_t apPC, apBC, apSC, apIO, apNC;  // u8 spikes of this step (zones stacked)
     uint32_t vIO, rel;             // f32 [nZ*numIO], [nZ*numNC]
+    uint32_t inPF, inBC, inSC, inGO, inTm;  // this step's inputs from the other half of the step: PF->PC
+                                   // sums [nZ*numPC], PF->BC/SC and GR->GO counts [numBC/numSC/numGO],
+                                   // plasticity timers [nZ*numIO] (staged in phase 0)
     uint32_t rng;                  // u32 [nZ][kRngZone]: generator words, index, generation
     uint32_t rngNext;              // u32 [nZ][kRngNext]: the prebuilt next block and its tag
     uint32_t pop;                  // u32 [nZ][popStride]: 3 counters, bins, 2 flags
@@ -136,6 +161,12 @@ struct SmallArgs {
     uint8_t *rec;
     uint32_t recRow, recCap;
     uint32_t recOff[6];
+    // resident run with the fields in LDS: fLds[f] is field f's byte offset
+    // in the small block's dynamic LDS (kNotInLds: not resident); the copy
+    // list gives each resident field's offset and bytes (all zones)
+    uint32_t fLds[CBM_F_COUNT];
+    uint32_t nLdsFields;
+    uint32_t ldsField[kSmallFieldCount], ldsBytes[kSmallFieldCount];
 };
 
 // Per-step values of the small-population kernel (passed by value; the
@@ -189,11 +220,13 @@ struct BatchNet {
 
 // A resident run: n steps of one network in one cooperative launch (one block
 // per CU), the GR cells' state held in the registers of their blocks from the
-// first step to the last, a grid barrier between steps.  Step t = t0 + k + 1
+// first step to the last, step counters and flags between steps.  Step t = t0 + k + 1
 // of run step k takes MF row k; its per-step values are derived on the
 // device as launchStep derives them on the host.
-constexpr int kBarGroups = 8;      // barrier counters (one per group of blocks)
-constexpr int kBarStride = 64;     // words between counters (256 bytes)
+constexpr int kBarGroups = 8;      // step counters of the GR blocks (one per group of blocks)
+constexpr int kFlagCopies = 16;    // copies of the small block's step count
+constexpr int kBarStride = 1024;   // words between counters and between flag copies (4 KB: different memory
+                                   // channels, so that the pollers of one do not queue behind another's)
 struct ResRun {
     GrArgs g;
     const uint8_t *mf;          // MF row k at mf + k * pitch (device memory)
@@ -206,8 +239,12 @@ struct ResRun {
     uint32_t spans;             // GR blocks (1 .. spans); block spans + 1 prebuilds generator blocks
     uint32_t skipFold0;         // the host wrote the PF->PC sums of step t0
     uint32_t recOffPF;          // offset of the PF chunk counts in a recorder row (UINT32_MAX: none)
-    uint32_t *bar;              // [kBarGroups * kBarStride] barrier counters (monotonic)
+    uint32_t *bar;              // [kBarGroups * kBarStride] step counters of the GR blocks
     uint32_t barBase;           // their sum before this launch
+    uint32_t *flag;             // [kFlagCopies * kBarStride] the small block's step count, copies
+    uint32_t flagBase;          // its value before this launch
+    uint32_t *pb;               // the generator block's step count
+    uint32_t pbBase;            // its value before this launch
     uint32_t *status;           // set to 1 when a barrier wait timed out
     uint32_t recLds;            // byte offset in dynamic LDS of the GR blocks' records [CPT][1024]
 };

@@ -62,8 +62,12 @@ struct cbm_handle {
     bool resident = false;
     size_t residentLds = 0;           // its dynamic LDS (the small step's, or a span's + its records)
     uint32_t residentRecLds = 0;      // byte offset of the GR records in that LDS
-    uint32_t *resBar = nullptr;       // barrier counters [kBarGroups * kBarStride], monotonic
-    uint32_t resBarBase = 0;          // their sum after the last resident launch
+    uint32_t *resBar = nullptr;       // GR step counters [kBarGroups * kBarStride], the small block's
+                                      // step-count copies [kFlagCopies * kBarStride], the generator
+                                      // block's step count
+    uint32_t resBarBase = 0;          // the counters' sum after the last resident launch
+    uint32_t resFlagBase = 0;         // the small block's step count after it
+    uint32_t resPbBase = 0;           // the generator block's step count after it
     uint32_t *resStatus = nullptr;    // device: 1 when a barrier wait timed out
     uint32_t *resStatusHost = nullptr;  // pinned copy, read at the next synchronisation
     bool resCheck = false;            // a resident run's status is still to be read

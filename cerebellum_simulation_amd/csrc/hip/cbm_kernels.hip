@@ -89,8 +89,9 @@ __device__ __forceinline__ T *sptr(T *p)
 // slots and generator blocks.  gfx950 keeps one L2 per XCD and these L2s are
 // not coherent with each other inside a kernel, so such arrays are accessed
 // with relaxed agent-scope atomics: loads and stores (sc1) that go through to
-// the device's point of coherence.  A grid barrier (grid_sync) orders them
-// between steps.  Data a block alone touches during the run keeps plain
+// the device's point of coherence.  Step counters and flags (res_arrive,
+// res_publish, res_wait) order them between steps.  Data a block alone
+// touches during the run keeps plain
 // accesses and stays in its XCD's caches.  X = false: plain accesses.
 template <bool X, typename T>
 __device__ __forceinline__ T xld(const T *p)
@@ -702,6 +703,15 @@ __device__ __forceinline__ G<T> *F(const SmallArgs &a, int f)
     return (G<T> *)(a.f[f]);
 }
 
+// Field f of the small populations: in the small block's LDS during a
+// resident run that keeps them there (LF), else in global memory.
+template <bool LF, typename T>
+__device__ __forceinline__ auto SF(const SmallArgs &a, int f, uint8_t *lds)
+{
+    if constexpr (LF) return reinterpret_cast<T *>(lds + a.fLds[f]);
+    else return F<T>(a, f);
+}
+
 template <typename T>
 __device__ __forceinline__ T *at(uint8_t *base, uint32_t off)
 {
@@ -742,8 +752,9 @@ __device__ __forceinline__ float lds_sum_seq(const float *x, uint32_t n, float a
 // its first store, so each phase costs about one memory round trip, and each
 // population or synapse group starts on its own wave, so that no wave runs
 // the updates of every small population in turn.  X: inside a resident run
-// (the arrays other blocks share go through xldg / xstg).
-template <bool X = false>
+// (the arrays other blocks share go through xldg / xstg); LF: with the state
+// fields in LDS (SF).
+template <bool X = false, bool LF = false>
 __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &st, const float *errDrive,
                                            uint8_t *lds)
 {
@@ -785,15 +796,17 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
     // every GO it targets; the counts are integers, so the order is free
     {
         const uint32_t nMFw = s.numMF / 4, nRng = nZ * kRngZone, nNext = nZ * kRngNext, nPop = nZ * L.popStride;
-        const uint32_t n = max(max(max(nMFw, L.idx16), max(max(nRng, nNext), nPop)), s.numMF);
+        const uint32_t nPCc = nZ * s.numPC, nIOc = nZ * s.numIO;
+        const uint32_t n = max(max(max(max(nMFw, L.idx16), max(max(nRng, nNext), nPop)), s.numMF),
+                               max(max(nPCc, s.numBC), max(max(s.numSC, s.numGO), nIOc)));
         const G<uint32_t> *rngIn = gp(a.rngState) + (size_t)parPrev * nRng;
         const G<uint32_t> *nextIn = gp(a.rngNext) + (size_t)parPrev * nNext;
         uint32_t *s_next = at<uint32_t>(lds, L.rngNext);
         const G<uint32_t> *mfw = (const G<uint32_t> *)st.mf;
         uint32_t *mfl = at<uint32_t>(lds, L.mf);
         u32x4 *bd = at<u32x4>(lds, L.idx);
-        auto histMF = F<uint8_t>(a, CBM_F_HIST_MF);
-        auto apBufMF = F<uint32_t>(a, CBM_F_APBUF_MF);
+        auto histMF = SF<LF, uint8_t>(a, CBM_F_HIST_MF, lds);
+        auto apBufMF = SF<LF, uint32_t>(a, CBM_F_APBUF_MF, lds);
         for (uint32_t i = tid; i < n; i += nThr) {
             // every array is read at a clamped index (in bounds, no branch),
             // so all of the iteration's loads are issued before the first wait
@@ -802,18 +815,31 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
             const u32x4 bl = ((const G<u32x4> *)a.idxBlob)[min(i, L.idx16 - 1)];
             const uint32_t r = xldg<X>(rngIn + min(i, nRng - 1));
             const uint32_t rn = xldg<X>(nextIn + min(i, nNext - 1));
+            // the inputs phase 1 reads from the other half of the step (PC/BC/SC
+            // the PF sums of t-2, GO the GR->GO count of t-grgoLag, IO the
+            // plasticity timer of t-1): nothing writes them before phase 2
+            const float inPF = xldg<X>(gp(a.pfpcRing) + (size_t)st.slotT2 * nPCc + min(i, nPCc - 1));
+            const uint32_t inBC = xldg<X>(gp(a.bcRing) + (size_t)st.slotT2 * s.numBC + min(i, s.numBC - 1));
+            const uint32_t inSC = xldg<X>(gp(a.scRing) + (size_t)st.slotT2 * s.numSC + min(i, s.numSC - 1));
+            const uint32_t inGO = xldg<X>(gp(a.goRing) + (size_t)st.slotG * s.numGO + min(i, s.numGO - 1));
+            const int32_t inTm = xldg<X>(gp(a.timer2) + (size_t)parPrev * nIOc + min(i, nIOc - 1));
             const uint32_t pz = ip / L.popStride, pk = ip % L.popStride;
-            const G<uint32_t> *pw = pk < 3 ? F<uint32_t>(a, CBM_F_PC_POPACT_CTRS) + pz * 3 + pk
-                                           : F<uint32_t>(a, CBM_F_HIST_PC_POPACT) + (size_t)pz * bins +
+            const auto pw = pk < 3 ? SF<LF, uint32_t>(a, CBM_F_PC_POPACT_CTRS, lds) + pz * 3 + pk
+                                           : SF<LF, uint32_t>(a, CBM_F_HIST_PC_POPACT, lds) + (size_t)pz * bins +
                                                  min(pk - 3, bins - 1);
             const uint32_t pv32 = *pw;
-            const uint8_t pv8 = F<uint8_t>(a, CBM_F_MFNC_NOLTP_NOLTD)[pz * 2 + (pk >= 3 + bins ? pk - 3 - bins : 0)];
+            const uint8_t pv8 = SF<LF, uint8_t>(a, CBM_F_MFNC_NOLTP_NOLTD, lds)[pz * 2 + (pk >= 3 + bins ? pk - 3 - bins : 0)];
             const uint8_t m = gp(st.mf)[im];
             const uint8_t h = histMF[im];
             const uint32_t b = apBufMF[im];
             const uint32_t pv = pk < 3 + bins ? pv32 : pv8;
             if (i < nMFw) mfl[i] = w;
             if (i < L.idx16) bd[i] = bl;
+            if (i < nPCc) at<float>(lds, L.inPF)[i] = inPF;
+            if (i < s.numBC) at<uint32_t>(lds, L.inBC)[i] = inBC;
+            if (i < s.numSC) at<uint32_t>(lds, L.inSC)[i] = inSC;
+            if (i < s.numGO) at<uint32_t>(lds, L.inGO)[i] = inGO;
+            if (i < nIOc) at<int32_t>(lds, L.inTm)[i] = inTm;
             if (i < nRng) s_rng[i] = r;
             if (i < nNext) s_next[i] = rn;
             if (i < nPop) s_pop[i] = pv;
@@ -848,19 +874,19 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
     // PC->NC of calcNCActivities (mzone.cpp:358-383), NC->IO of
     // calcIOActivities (mzone.cpp:272-280)
     {
-        auto gSC = F<float>(a, CBM_F_G_SC_PC);
-        auto inSC = F<uint8_t>(a, CBM_F_INPUT_SC_PC);
-        auto inMF = F<uint8_t>(a, CBM_F_INPUT_MF_NC);
-        auto wMF = F<float>(a, CBM_F_MF_SYNW_NC);
-        auto gmN = F<float>(a, CBM_F_GMAX_NMDA_MF_NC);
-        auto gmA = F<float>(a, CBM_F_GMAX_AMPA_MF_NC);
-        auto gN = F<float>(a, CBM_F_G_MF_NMDA_NC);
-        auto gA = F<float>(a, CBM_F_G_MF_AMPA_NC);
-        auto inPC = F<uint8_t>(a, CBM_F_INPUT_PC_NC);
-        auto scPC = F<float>(a, CBM_F_G_PC_SCALE_NC);
-        auto gPC = F<float>(a, CBM_F_G_PC_NC);
-        auto gNC = F<float>(a, CBM_F_G_NC_IO);
-        auto inNC = F<uint8_t>(a, CBM_F_INPUT_NC_IO);
+        auto gSC = SF<LF, float>(a, CBM_F_G_SC_PC, lds);
+        auto inSC = SF<LF, uint8_t>(a, CBM_F_INPUT_SC_PC, lds);
+        auto inMF = SF<LF, uint8_t>(a, CBM_F_INPUT_MF_NC, lds);
+        auto wMF = SF<LF, float>(a, CBM_F_MF_SYNW_NC, lds);
+        auto gmN = SF<LF, float>(a, CBM_F_GMAX_NMDA_MF_NC, lds);
+        auto gmA = SF<LF, float>(a, CBM_F_GMAX_AMPA_MF_NC, lds);
+        auto gN = SF<LF, float>(a, CBM_F_G_MF_NMDA_NC, lds);
+        auto gA = SF<LF, float>(a, CBM_F_G_MF_AMPA_NC, lds);
+        auto inPC = SF<LF, uint8_t>(a, CBM_F_INPUT_PC_NC, lds);
+        auto scPC = SF<LF, float>(a, CBM_F_G_PC_SCALE_NC, lds);
+        auto gPC = SF<LF, float>(a, CBM_F_G_PC_NC, lds);
+        auto gNC = SF<LF, float>(a, CBM_F_G_NC_IO, lds);
+        auto inNC = SF<LF, uint8_t>(a, CBM_F_INPUT_NC_IO, lds);
         const uint32_t nS = nZ * s.numSC, nM = nZ * s.numNC * kMF, nP = nZ * s.numNC * kPC,
                        nI = nZ * s.numIO * kNCIO;
         const uint32_t rounds = (max(max(nS, nM), max(nP, nI)) + nThr - 1) / nThr;
@@ -920,7 +946,7 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
     // ======================= phase 1
     // MZone::updateMFNCOut (mzone.cpp:514-524): the caller's raw MF bytes
     {
-        auto inMF = F<uint8_t>(a, CBM_F_INPUT_MF_NC);
+        auto inMF = SF<LF, uint8_t>(a, CBM_F_INPUT_MF_NC, lds);
         for (uint32_t q = tid; q < nZ * s.numNC * kMF; q += nThr) inMF[q] = s_mf[s_idx[L.iMFNC + q]];
     }
     // cell updates: MZone::calcPCActivities (mzone.cpp:187-226),
@@ -945,27 +971,27 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
             // ---- loads, at clamped indices (in bounds, no branch)
             const uint32_t iP = min(cP, nPCc - 1), iB = min(cB, nBCc - 1), iS = min(cS, nSCc - 1);
             const uint32_t iI = min(cI, nIOc - 1), iN = min(cN, nNCc - 1);
-            const float pcPF = xldg<X>(gp(a.pfpcRing) + (size_t)st.slotT2 * nPCc + iP);
-            const uint32_t pcIn = F<uint32_t>(a, CBM_F_INPUT_BC_PC)[iP];
-            const float pcGPF = F<float>(a, CBM_F_G_PF_PC)[iP], pcGBC = F<float>(a, CBM_F_G_BC_PC)[iP];
-            const float pcV = F<float>(a, CBM_F_VM_PC)[iP], pcTh = F<float>(a, CBM_F_THRESH_PC)[iP];
-            const uint32_t pcAb = F<uint32_t>(a, CBM_F_APBUF_PC)[iP];
-            const uint32_t bcPF = xldg<X>(gp(a.bcRing) + (size_t)st.slotT2 * s.numBC + (iB & (s.numBC - 1)));
-            const uint32_t bcIn = F<uint32_t>(a, CBM_F_INPUT_PC_BC)[iB];
-            const float bcGPF = F<float>(a, CBM_F_G_PF_BC)[iB], bcGPC = F<float>(a, CBM_F_G_PC_BC)[iB];
-            const float bcV = F<float>(a, CBM_F_VM_BC)[iB], bcTh = F<float>(a, CBM_F_THRESH_BC)[iB];
-            const uint32_t bcAb = F<uint32_t>(a, CBM_F_APBUF_BC)[iB];
-            const uint32_t scPF = xldg<X>(gp(a.scRing) + (size_t)st.slotT2 * s.numSC + iS);
-            const float scGPF = F<float>(a, CBM_F_G_PF_SC)[iS], scV = F<float>(a, CBM_F_VM_SC)[iS];
-            const float scTh = F<float>(a, CBM_F_THRESH_SC)[iS];
-            const uint32_t scAb = F<uint32_t>(a, CBM_F_APBUF_SC)[iS];
-            const float ioV = F<float>(a, CBM_F_VM_IO)[iI], ioTh = F<float>(a, CBM_F_THRESH_IO)[iI];
-            const float ioVC = F<float>(a, CBM_F_VCOUPLE_IO)[iI];
-            const uint32_t ioAb = F<uint32_t>(a, CBM_F_APBUF_IO)[iI];
-            const int32_t ioTm = xldg<X>(gp(a.timer2) + (size_t)parPrev * nIOc + iI);
-            const float ncV = F<float>(a, CBM_F_VM_NC)[iN], ncTh = F<float>(a, CBM_F_THRESH_NC)[iN];
-            const float ncR = F<float>(a, CBM_F_SYN_IO_PRELEASE_NC)[iN];
-            const uint32_t ncAb = F<uint32_t>(a, CBM_F_APBUF_NC)[iN];
+            const float pcPF = at<float>(lds, L.inPF)[iP];
+            const uint32_t pcIn = SF<LF, uint32_t>(a, CBM_F_INPUT_BC_PC, lds)[iP];
+            const float pcGPF = SF<LF, float>(a, CBM_F_G_PF_PC, lds)[iP], pcGBC = SF<LF, float>(a, CBM_F_G_BC_PC, lds)[iP];
+            const float pcV = SF<LF, float>(a, CBM_F_VM_PC, lds)[iP], pcTh = SF<LF, float>(a, CBM_F_THRESH_PC, lds)[iP];
+            const uint32_t pcAb = SF<LF, uint32_t>(a, CBM_F_APBUF_PC, lds)[iP];
+            const uint32_t bcPF = at<uint32_t>(lds, L.inBC)[iB & (s.numBC - 1)];
+            const uint32_t bcIn = SF<LF, uint32_t>(a, CBM_F_INPUT_PC_BC, lds)[iB];
+            const float bcGPF = SF<LF, float>(a, CBM_F_G_PF_BC, lds)[iB], bcGPC = SF<LF, float>(a, CBM_F_G_PC_BC, lds)[iB];
+            const float bcV = SF<LF, float>(a, CBM_F_VM_BC, lds)[iB], bcTh = SF<LF, float>(a, CBM_F_THRESH_BC, lds)[iB];
+            const uint32_t bcAb = SF<LF, uint32_t>(a, CBM_F_APBUF_BC, lds)[iB];
+            const uint32_t scPF = at<uint32_t>(lds, L.inSC)[iS];
+            const float scGPF = SF<LF, float>(a, CBM_F_G_PF_SC, lds)[iS], scV = SF<LF, float>(a, CBM_F_VM_SC, lds)[iS];
+            const float scTh = SF<LF, float>(a, CBM_F_THRESH_SC, lds)[iS];
+            const uint32_t scAb = SF<LF, uint32_t>(a, CBM_F_APBUF_SC, lds)[iS];
+            const float ioV = SF<LF, float>(a, CBM_F_VM_IO, lds)[iI], ioTh = SF<LF, float>(a, CBM_F_THRESH_IO, lds)[iI];
+            const float ioVC = SF<LF, float>(a, CBM_F_VCOUPLE_IO, lds)[iI];
+            const uint32_t ioAb = SF<LF, uint32_t>(a, CBM_F_APBUF_IO, lds)[iI];
+            const int32_t ioTm = at<int32_t>(lds, L.inTm)[iI];
+            const float ncV = SF<LF, float>(a, CBM_F_VM_NC, lds)[iN], ncTh = SF<LF, float>(a, CBM_F_THRESH_NC, lds)[iN];
+            const float ncR = SF<LF, float>(a, CBM_F_SYN_IO_PRELEASE_NC, lds)[iN];
+            const uint32_t ncAb = SF<LF, uint32_t>(a, CBM_F_APBUF_NC, lds)[iN];
             // ---- PC
             if (hPC) {
                 const uint32_t i = iP, z = i >> L.l2PC;
@@ -979,13 +1005,13 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
                      (gSCPCSum * (p.eSCtoPC - vv));
                 th = th + (p.threshDecPC * (p.threshRestPC - th));
                 const uint8_t ap = vv > th;
-                F<float>(a, CBM_F_G_PF_PC)[i] = gPF;
-                F<float>(a, CBM_F_G_BC_PC)[i] = gBC;
-                F<float>(a, CBM_F_VM_PC)[i] = vv;
-                F<uint8_t>(a, CBM_F_AP_PC)[i] = ap;
+                SF<LF, float>(a, CBM_F_G_PF_PC, lds)[i] = gPF;
+                SF<LF, float>(a, CBM_F_G_BC_PC, lds)[i] = gBC;
+                SF<LF, float>(a, CBM_F_VM_PC, lds)[i] = vv;
+                SF<LF, uint8_t>(a, CBM_F_AP_PC, lds)[i] = ap;
                 s_apPC[i] = ap;
-                F<uint32_t>(a, CBM_F_APBUF_PC)[i] = (pcAb << 1) | (ap * 0x00000001u);
-                F<float>(a, CBM_F_THRESH_PC)[i] = ap * p.threshMaxPC + (!ap) * th;
+                SF<LF, uint32_t>(a, CBM_F_APBUF_PC, lds)[i] = (pcAb << 1) | (ap * 0x00000001u);
+                SF<LF, float>(a, CBM_F_THRESH_PC, lds)[i] = ap * p.threshMaxPC + (!ap) * th;
                 if (ap) atomicAdd(&s_pop[z * L.popStride + 2], 1u);
             }
             // ---- BC
@@ -999,13 +1025,13 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
                 vv = vv + (p.gLeakBC * (p.eLeakBC - vv)) - (gPF * vv) + (gPC * (p.ePCtoBC - vv));
                 th = th + p.threshDecBC * (p.threshRestBC - th);
                 const uint8_t ap = vv > th;
-                F<float>(a, CBM_F_G_PF_BC)[i] = gPF;
-                F<float>(a, CBM_F_G_PC_BC)[i] = gPC;
-                F<float>(a, CBM_F_VM_BC)[i] = vv;
-                F<uint8_t>(a, CBM_F_AP_BC)[i] = ap;
+                SF<LF, float>(a, CBM_F_G_PF_BC, lds)[i] = gPF;
+                SF<LF, float>(a, CBM_F_G_PC_BC, lds)[i] = gPC;
+                SF<LF, float>(a, CBM_F_VM_BC, lds)[i] = vv;
+                SF<LF, uint8_t>(a, CBM_F_AP_BC, lds)[i] = ap;
                 s_apBC[i] = ap;
-                F<uint32_t>(a, CBM_F_APBUF_BC)[i] = (bcAb << 1) | (ap * 0x00000001u);
-                F<float>(a, CBM_F_THRESH_BC)[i] = ap * p.threshMaxBC + (!ap) * (th);
+                SF<LF, uint32_t>(a, CBM_F_APBUF_BC, lds)[i] = (bcAb << 1) | (ap * 0x00000001u);
+                SF<LF, float>(a, CBM_F_THRESH_BC, lds)[i] = ap * p.threshMaxBC + (!ap) * (th);
             }
             // ---- SC
             if (hSC) {
@@ -1016,12 +1042,12 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
                 vv = vv + (p.gLeakSC * (p.eLeakSC - vv)) - gPF * vv;
                 const uint8_t ap = vv > th;
                 th = th + p.threshDecSC * (p.threshRestSC - th);
-                F<float>(a, CBM_F_G_PF_SC)[i] = gPF;
-                F<float>(a, CBM_F_VM_SC)[i] = vv;
-                F<uint8_t>(a, CBM_F_AP_SC)[i] = ap;
+                SF<LF, float>(a, CBM_F_G_PF_SC, lds)[i] = gPF;
+                SF<LF, float>(a, CBM_F_VM_SC, lds)[i] = vv;
+                SF<LF, uint8_t>(a, CBM_F_AP_SC, lds)[i] = ap;
                 s_apSC[i] = ap;
-                F<uint32_t>(a, CBM_F_APBUF_SC)[i] = (scAb << 1) | (ap * 0x00000001u);
-                F<float>(a, CBM_F_THRESH_SC)[i] = ap * p.threshMaxSC + (!ap) * (th);
+                SF<LF, uint32_t>(a, CBM_F_APBUF_SC, lds)[i] = (scAb << 1) | (ap * 0x00000001u);
+                SF<LF, float>(a, CBM_F_THRESH_SC, lds)[i] = ap * p.threshMaxSC + (!ap) * (th);
             }
             // ---- IO
             if (hIO) {
@@ -1031,12 +1057,12 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
                 gNCSum = gNCSum / 3.1;
                 vv = vv + p.gLeakIO * (p.eLeakIO - vv) + gNCSum * 1.5 * (p.eNCtoIO - vv) + ioVC + errDrive[z];
                 const uint8_t ap = vv > ioTh;
-                F<float>(a, CBM_F_VM_IO)[i] = vv;
+                SF<LF, float>(a, CBM_F_VM_IO, lds)[i] = vv;
                 s_vIO[i] = vv;
-                F<uint8_t>(a, CBM_F_AP_IO)[i] = ap;
+                SF<LF, uint8_t>(a, CBM_F_AP_IO, lds)[i] = ap;
                 s_apIO[i] = ap;
-                F<uint32_t>(a, CBM_F_APBUF_IO)[i] = (ioAb << 1) | (ap * 0x00000001u);
-                F<float>(a, CBM_F_THRESH_IO)[i] =
+                SF<LF, uint32_t>(a, CBM_F_APBUF_IO, lds)[i] = (ioAb << 1) | (ap * 0x00000001u);
+                SF<LF, float>(a, CBM_F_THRESH_IO, lds)[i] =
                     p.threshMaxIO * ap + (!ap) * (ioTh + p.threshDecIO * (p.threshRestIO - ioTh));
                 xstg<X>(gp(a.timer2) + (size_t)par * nIOc + i, (int32_t)((!ap) * (ioTm + 1) + ap * p.tsLTDStartAPIO));
             }
@@ -1055,14 +1081,14 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
                 vv = vv + p.gLeakNC * (p.eLeakNC - vv) - (gMFNMDASum + gMFAMPASum) * vv + gPCNCSum * (p.ePCtoNC - vv);
                 th = th + p.threshDecNC * (p.threshRestNC - th);
                 const uint8_t ap = vv > th;
-                F<float>(a, CBM_F_VM_NC)[i] = vv;
-                F<uint8_t>(a, CBM_F_AP_NC)[i] = ap;
+                SF<LF, float>(a, CBM_F_VM_NC, lds)[i] = vv;
+                SF<LF, uint8_t>(a, CBM_F_AP_NC, lds)[i] = ap;
                 s_apNC[i] = ap;
-                F<uint32_t>(a, CBM_F_APBUF_NC)[i] = (ncAb << 1) | (ap * 0x00000001u);
-                F<float>(a, CBM_F_THRESH_NC)[i] = ap * p.threshMaxNC + (!ap) * th;
+                SF<LF, uint32_t>(a, CBM_F_APBUF_NC, lds)[i] = (ncAb << 1) | (ap * 0x00000001u);
+                SF<LF, float>(a, CBM_F_THRESH_NC, lds)[i] = ap * p.threshMaxNC + (!ap) * th;
                 r = r * expf(-ms / (p.relPDecTSofNCtoIO * expf(-r / p.relPDecTTofNCtoIO) + p.relPDecT0ofNCtoIO));
                 r = r + ap * p.relPIncNCtoIO * expf(-r / p.relPIncTauNCtoIO);
-                F<float>(a, CBM_F_SYN_IO_PRELEASE_NC)[i] = r;
+                SF<LF, float>(a, CBM_F_SYN_IO_PRELEASE_NC, lds)[i] = r;
                 s_rel[i] = r;
             }
         }
@@ -1070,21 +1096,21 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
             float gMF = 0, gGR = 0, gGO = 0, glu = 0, mG = 0, gInc = 0, gMG = 0, thv = 0, v = 0, vC = 0, sc = 0, inDep = 0;
             uint32_t totalGRIn = 0, inMF = 0, goAb = 0;
             {
-                totalGRIn = xldg<X>(gp(a.goRing) + (size_t)st.slotG * s.numGO + i);
-                inMF = F<uint32_t>(a, CBM_F_INPUT_MF_GO)[i];
-                inDep = F<float>(a, CBM_F_INPUT_GOGABA_SYNDEP_GO)[i];
-                gMF = F<float>(a, CBM_F_G_MF_GO)[i];
-                gGR = F<float>(a, CBM_F_G_GR_GO)[i];
-                gGO = F<float>(a, CBM_F_G_GO_GO)[i];
-                glu = F<float>(a, CBM_F_GLU_GO)[i];
-                mG = F<float>(a, CBM_F_MGLUR_GO)[i];
-                gInc = F<float>(a, CBM_F_G_MGLURINC_GO)[i];
-                gMG = F<float>(a, CBM_F_G_MGLUR_GO)[i];
-                thv = F<float>(a, CBM_F_THRESH_GO)[i];
-                v = F<float>(a, CBM_F_VM_GO)[i];
-                vC = F<float>(a, CBM_F_VCOUPLE_GO)[i];
-                sc = F<float>(a, CBM_F_GO_GABA_OUT_SYNSCALE_GOGO)[i];
-                goAb = F<uint32_t>(a, CBM_F_APBUF_GO)[i];
+                totalGRIn = at<uint32_t>(lds, L.inGO)[i];
+                inMF = SF<LF, uint32_t>(a, CBM_F_INPUT_MF_GO, lds)[i];
+                inDep = SF<LF, float>(a, CBM_F_INPUT_GOGABA_SYNDEP_GO, lds)[i];
+                gMF = SF<LF, float>(a, CBM_F_G_MF_GO, lds)[i];
+                gGR = SF<LF, float>(a, CBM_F_G_GR_GO, lds)[i];
+                gGO = SF<LF, float>(a, CBM_F_G_GO_GO, lds)[i];
+                glu = SF<LF, float>(a, CBM_F_GLU_GO, lds)[i];
+                mG = SF<LF, float>(a, CBM_F_MGLUR_GO, lds)[i];
+                gInc = SF<LF, float>(a, CBM_F_G_MGLURINC_GO, lds)[i];
+                gMG = SF<LF, float>(a, CBM_F_G_MGLUR_GO, lds)[i];
+                thv = SF<LF, float>(a, CBM_F_THRESH_GO, lds)[i];
+                v = SF<LF, float>(a, CBM_F_VM_GO, lds)[i];
+                vC = SF<LF, float>(a, CBM_F_VCOUPLE_GO, lds)[i];
+                sc = SF<LF, float>(a, CBM_F_GO_GABA_OUT_SYNSCALE_GOGO, lds)[i];
+                goAb = SF<LF, uint32_t>(a, CBM_F_APBUF_GO, lds)[i];
             }
             {
                 gMF = inMF * p.gIncMFtoGO + gMF * p.gDecMFtoGO;
@@ -1097,22 +1123,22 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
                 thv = thv + (p.threshRestGO - thv) * p.threshDecGO;
                 v = v + (p.gLeakGO * (p.eLeakGO - v)) + (gGO * (p.eGABAGO - v)) - (gMF + gGR) * v + vC;
                 const uint8_t ap = v > thv;
-                F<uint32_t>(a, CBM_F_SUM_GR_INPUT_GO)[i] = totalGRIn;
-                F<float>(a, CBM_F_SUM_GO_INPUT_GO)[i] = inDep;
-                F<float>(a, CBM_F_G_MF_GO)[i] = gMF;
-                F<float>(a, CBM_F_G_GR_GO)[i] = gGR;
-                F<float>(a, CBM_F_G_GO_GO)[i] = gGO;
-                F<float>(a, CBM_F_GLU_GO)[i] = glu;
-                F<float>(a, CBM_F_MGLUR_GO)[i] = mG;
-                F<float>(a, CBM_F_G_MGLURINC_GO)[i] = gInc;
-                F<float>(a, CBM_F_G_MGLUR_GO)[i] = gMG;
-                F<float>(a, CBM_F_VM_GO)[i] = v;
+                SF<LF, uint32_t>(a, CBM_F_SUM_GR_INPUT_GO, lds)[i] = totalGRIn;
+                SF<LF, float>(a, CBM_F_SUM_GO_INPUT_GO, lds)[i] = inDep;
+                SF<LF, float>(a, CBM_F_G_MF_GO, lds)[i] = gMF;
+                SF<LF, float>(a, CBM_F_G_GR_GO, lds)[i] = gGR;
+                SF<LF, float>(a, CBM_F_G_GO_GO, lds)[i] = gGO;
+                SF<LF, float>(a, CBM_F_GLU_GO, lds)[i] = glu;
+                SF<LF, float>(a, CBM_F_MGLUR_GO, lds)[i] = mG;
+                SF<LF, float>(a, CBM_F_G_MGLURINC_GO, lds)[i] = gInc;
+                SF<LF, float>(a, CBM_F_G_MGLUR_GO, lds)[i] = gMG;
+                SF<LF, float>(a, CBM_F_VM_GO, lds)[i] = v;
                 xstg<X>(gp(a.apGO2) + (size_t)par * s.numGO + i, ap);
                 s_apGO[i] = ap;
                 s_scale[i] = sc;
                 if (a.anyCoupling) s_vGO[i] = v;
-                F<uint32_t>(a, CBM_F_APBUF_GO)[i] = (goAb << 1) | (ap * 0x00000001u);
-                F<float>(a, CBM_F_THRESH_GO)[i] = ap * p.threshMaxGO + (!ap) * thv;
+                SF<LF, uint32_t>(a, CBM_F_APBUF_GO, lds)[i] = (goAb << 1) | (ap * 0x00000001u);
+                SF<LF, float>(a, CBM_F_THRESH_GO, lds)[i] = ap * p.threshMaxGO + (!ap) * thv;
             }
         }
     }
@@ -1123,9 +1149,9 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
     // gathers its GABA inputs in increasing source order, which is the order
     // the reference adds them
     {
-        auto inMFGO = F<uint32_t>(a, CBM_F_INPUT_MF_GO);
-        auto inGO = F<uint32_t>(a, CBM_F_INPUT_GO_GO);
-        auto inDep = F<float>(a, CBM_F_INPUT_GOGABA_SYNDEP_GO);
+        auto inMFGO = SF<LF, uint32_t>(a, CBM_F_INPUT_MF_GO, lds);
+        auto inGO = SF<LF, uint32_t>(a, CBM_F_INPUT_GO_GO, lds);
+        auto inDep = SF<LF, float>(a, CBM_F_INPUT_GOGABA_SYNDEP_GO, lds);
         for (uint32_t g = tid; g < s.numGO; g += nThr) {
             uint32_t c = 0;
             float dep = 0;
@@ -1150,7 +1176,7 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
     // the threshold update is in place in cell order, so it runs on one lane.
     // Without coupling inputs (or with a zero ratio) both terms are zero.
     {
-        auto vC = F<float>(a, CBM_F_VCOUPLE_GO);
+        auto vC = SF<LF, float>(a, CBM_F_VCOUPLE_GO, lds);
         if (a.anyCoupling) {
             const uint32_t kc = s.maxnumpGOCoupInGOGO;
             const float ratio = p.coupleRiRjRatioGO;
@@ -1160,7 +1186,7 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
                 vC[i] = acc;
             }
             if (tid == 0) {
-                auto th = F<float>(a, CBM_F_THRESH_GO);
+                auto th = SF<LF, float>(a, CBM_F_THRESH_GO, lds);
                 for (uint32_t i = 0; i < s.numGO; i++) {
                     float tc = 0;
                     for (int j = 0; j < gp(a.coupN)[i]; j++) tc = tc + ratio * (th[gp(a.coupP)[(size_t)i * kc + j]] - th[i]);
@@ -1175,44 +1201,54 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
     // updateSCPCOut (mzone.cpp:467-477), updatePCOut (mzone.cpp:409-447) and
     // the IO coupling of updateIOOut (mzone.cpp:486-489), from LDS
     {
-        auto inBCPC = F<uint32_t>(a, CBM_F_INPUT_BC_PC);
+        auto inBCPC = SF<LF, uint32_t>(a, CBM_F_INPUT_BC_PC, lds);
         for (uint32_t i = (tid - L.oRBCPC) & (nThr - 1); i < nZ * s.numPC; i += nThr) {
             const uint32_t z = i >> L.l2PC, pc = i & (s.numPC - 1);
             const uint16_t *ell = s_idx + L.iBCPC + (size_t)z * L.slotsBCPC * s.numPC + pc;
             uint32_t c = 0;
+            // (every LDS read at a clamped index, then selects: no read
+            // waits for the one before)
             for (uint32_t k0 = 0; k0 < L.slotsBCPC; k0 += 8) {
-                uint32_t b[8];
+                uint32_t b[8], v[8];
 #pragma unroll
-                for (int k = 0; k < 8; k++) b[k] = k0 + k < L.slotsBCPC ? ell[(size_t)(k0 + k) * s.numPC] : kNoCell;
+                for (int k = 0; k < 8; k++) {
+                    const uint32_t e = ell[(size_t)min(k0 + k, L.slotsBCPC - 1) * s.numPC];
+                    b[k] = k0 + k < L.slotsBCPC ? e : kNoCell;
+                }
 #pragma unroll
-                for (int k = 0; k < 8; k++)
-                    if (b[k] != kNoCell) c += s_apBC[z * s.numBC + b[k]] != 0;
+                for (int k = 0; k < 8; k++) v[k] = s_apBC[z * s.numBC + min(b[k], s.numBC - 1)];
+#pragma unroll
+                for (int k = 0; k < 8; k++) c += (b[k] != kNoCell) & (v[k] != 0);
             }
             inBCPC[i] = c;
         }
-        auto inSC = F<uint8_t>(a, CBM_F_INPUT_SC_PC);
+        auto inSC = SF<LF, uint8_t>(a, CBM_F_INPUT_SC_PC, lds);
         for (uint32_t q = (tid - L.oRSCPC) & (nThr - 1); q < nZ * s.numPC * kSC; q += nThr)
             inSC[q] = s_apSC[s_idx[L.iSCPC + q]];
-        auto inPCBC = F<uint32_t>(a, CBM_F_INPUT_PC_BC);
+        auto inPCBC = SF<LF, uint32_t>(a, CBM_F_INPUT_PC_BC, lds);
         for (uint32_t i = (tid - L.oRPCBC) & (nThr - 1); i < nZ * s.numBC; i += nThr) {
             const uint32_t z = i >> L.l2BC, bc = i & (s.numBC - 1);
             const uint16_t *ell = s_idx + L.iPCBC + (size_t)z * L.slotsPCBC * s.numBC + bc;
             uint32_t c = 0;
             for (uint32_t k0 = 0; k0 < L.slotsPCBC; k0 += 8) {
-                uint32_t pc[8];
+                uint32_t pc[8], v[8];
 #pragma unroll
-                for (int k = 0; k < 8; k++) pc[k] = k0 + k < L.slotsPCBC ? ell[(size_t)(k0 + k) * s.numBC] : kNoCell;
+                for (int k = 0; k < 8; k++) {
+                    const uint32_t e = ell[(size_t)min(k0 + k, L.slotsPCBC - 1) * s.numBC];
+                    pc[k] = k0 + k < L.slotsPCBC ? e : kNoCell;
+                }
 #pragma unroll
-                for (int k = 0; k < 8; k++)
-                    if (pc[k] != kNoCell) c += s_apPC[z * s.numPC + pc[k]];
+                for (int k = 0; k < 8; k++) v[k] = s_apPC[z * s.numPC + min(pc[k], s.numPC - 1)];
+#pragma unroll
+                for (int k = 0; k < 8; k++) c += (pc[k] != kNoCell) * v[k];
             }
             inPCBC[i] = c;
         }
-        auto inPCNC = F<uint8_t>(a, CBM_F_INPUT_PC_NC);
+        auto inPCNC = SF<LF, uint8_t>(a, CBM_F_INPUT_PC_NC, lds);
         const uint32_t per = s.numNC * kPC;
         for (uint32_t q = (tid - L.oRPCNC) & (nThr - 1); q < nZ * per; q += nThr)
             inPCNC[q] = s_apPC[(q / per) * s.numPC + s_idx[L.iPCNC + q]];
-        auto vCIO = F<float>(a, CBM_F_VCOUPLE_IO);
+        auto vCIO = SF<LF, float>(a, CBM_F_VCOUPLE_IO, lds);
         for (uint32_t i = (tid - L.oRIO) & (nThr - 1); i < nZ * s.numIO; i += nThr) {
             const uint32_t z = i >> L.l2IO;
             float vc = 0;
@@ -1249,9 +1285,9 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
         }
         s_doLTD[z] = doLTD;
         s_doLTP[z] = doLTP;
-        for (uint32_t k = 0; k < 3; k++) F<uint32_t>(a, CBM_F_PC_POPACT_CTRS)[z * 3 + k] = ctrs[k];
-        for (uint32_t k = 0; k < bins; k++) F<uint32_t>(a, CBM_F_HIST_PC_POPACT)[(size_t)z * bins + k] = hist[k];
-        for (uint32_t k = 0; k < 2; k++) F<uint8_t>(a, CBM_F_MFNC_NOLTP_NOLTD)[z * 2 + k] = (uint8_t)flags[k];
+        for (uint32_t k = 0; k < 3; k++) SF<LF, uint32_t>(a, CBM_F_PC_POPACT_CTRS, lds)[z * 3 + k] = ctrs[k];
+        for (uint32_t k = 0; k < bins; k++) SF<LF, uint32_t>(a, CBM_F_HIST_PC_POPACT, lds)[(size_t)z * bins + k] = hist[k];
+        for (uint32_t k = 0; k < 2; k++) SF<LF, uint8_t>(a, CBM_F_MFNC_NOLTP_NOLTD, lds)[z * 2 + k] = (uint8_t)flags[k];
     }
     // spike recorder (row t mod recCap), and one population's spikes for the
     // host's step-and-read call straight into its pinned buffer
@@ -1286,7 +1322,7 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
     // block prebuilt by an earlier launch when its tag says it follows the
     // current one, and computes it here otherwise.
     if (wave == 0) {
-        auto inNC = F<uint8_t>(a, CBM_F_INPUT_NC_IO);
+        auto inNC = SF<LF, uint8_t>(a, CBM_F_INPUT_NC_IO, lds);
         const uint32_t n = s.numIO * kNCIO;
         const uint32_t *s_next = at<uint32_t>(lds, L.rngNext);
         for (uint32_t z = 0; z < nZ; z++) {
@@ -1331,7 +1367,7 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
     // GO->GO synaptic depression scale (innet.cpp:952-954), after the GABA
     // gather above has used the previous scale
     {
-        auto scale = F<float>(a, CBM_F_GO_GABA_OUT_SYNSCALE_GOGO);
+        auto scale = SF<LF, float>(a, CBM_F_GO_GABA_OUT_SYNSCALE_GOGO, lds);
         const float rec = p.goGABAGOGOSynRec, depF = p.goGABAGOGOSynDepF;
         for (uint32_t i = tid; i < s.numGO; i += nThr) {
             const float sc = s_scale[i];
@@ -1342,7 +1378,7 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
     // MZone::updateMFNCSyn weights (mzone.cpp:586-600)
     if (st.popHist) {
         const uint32_t per = s.numNC * kMF;
-        auto w = F<float>(a, CBM_F_MF_SYNW_NC);
+        auto w = SF<LF, float>(a, CBM_F_MF_SYNW_NC, lds);
         for (uint32_t q = tid; q < nZ * per; q += nThr) {
             const uint32_t z = q / per;
             const int doLTD = s_doLTD[z], doLTP = s_doLTP[z];
@@ -1356,36 +1392,56 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
     }
     // MF history, cleared every numTSinMFHist steps (innet.cpp:982-995)
     {
-        auto histMF = F<uint8_t>(a, CBM_F_HIST_MF);
+        auto histMF = SF<LF, uint8_t>(a, CBM_F_HIST_MF, lds);
         for (uint32_t i = tid; i < s.numMF; i += nThr) histMF[i] = st.resetMFHist ? 0 : s_hmf[i];
     }
 }
 
 // The zone generators' next blocks, built ahead of use by the last block of
-// a fused launch (one wave; lanes 4z..4z+3 build zone z as sfmt_regen4 does):
-// from the state the previous step left, block gen + 1 into this step's
-// slot, or a copy of the previous slot when that already holds it.  The
-// small step takes a prebuilt block only from a slot an earlier launch wrote.
+// a fused launch: from the state the previous step left, block gen + 1 into
+// this step's slot, or a copy of the previous slot when that already holds
+// it.  The whole block moves the words (one load round trip for all of them),
+// and lanes 4z..4z+3 of wave 0 run zone z's recursion in LDS as sfmt_regen4
+// does.  The small step takes a prebuilt block only from a slot an earlier
+// launch wrote.
 template <bool X = false>
 __device__ void sfmt_prebuild(const SmallArgs &a, const SmallStep &st, uint32_t *lds)
 {
-    const uint32_t nZ = a.s.numZones, lane = threadIdx.x, z = lane >> 2, k = lane & 3u;
-    if (threadIdx.x >= 64 || z >= nZ) return;  // whole quads
+    __shared__ uint32_t s_gen[kMaxZones], s_copy[kMaxZones];
+    const uint32_t nZ = a.s.numZones, tid = threadIdx.x;
     const uint32_t par = st.par, parPrev = par ^ 1u;
-    const G<uint32_t> *cur = gp(a.rngState) + ((size_t)parPrev * nZ + z) * kRngZone;
-    const G<uint32_t> *nin = gp(a.rngNext) + ((size_t)parPrev * nZ + z) * kRngNext;
-    G<uint32_t> *nout = gp(a.rngNext) + ((size_t)par * nZ + z) * kRngNext;
-    const uint32_t gen = xldg<X>(cur + 353), tag = xldg<X>(nin + 352);
-    if (tag == gen + 1u) {
-        for (uint32_t w = k; w < 352; w += 4) xstg<X>(nout + w, xldg<X>(nin + w));
-        if (k == 0) xstg<X>(nout + 352, tag);
-        return;
+    const G<uint32_t> *cur = gp(a.rngState) + (size_t)parPrev * nZ * kRngZone;
+    const G<uint32_t> *nin = gp(a.rngNext) + (size_t)parPrev * nZ * kRngNext;
+    G<uint32_t> *nout = gp(a.rngNext) + (size_t)par * nZ * kRngNext;
+    if (tid < nZ) {
+        const uint32_t gen = xldg<X>(cur + (size_t)tid * kRngZone + 353), tag = xldg<X>(nin + (size_t)tid * kRngNext + 352);
+        s_gen[tid] = gen;
+        s_copy[tid] = tag == gen + 1u;
     }
-    uint32_t *blk = lds + (size_t)z * 352;  // lane k touches only words 4i + k
-    for (uint32_t w = k; w < 352; w += 4) blk[w] = xldg<X>(cur + w);
-    sfmt_regen4(blk, (int)k);
-    for (uint32_t w = k; w < 352; w += 4) xstg<X>(nout + w, blk[w]);
-    if (k == 0) xstg<X>(nout + 352, gen + 1u);
+    __syncthreads();
+    // the previous slot's block (copy) or the current state (regenerate), at
+    // most three words per thread for eight zones, all loads issued first
+    constexpr uint32_t kPer = (kMaxZones * 352 + kSmallThreads - 1) / kSmallThreads;
+    const uint32_t n = nZ * 352;
+    uint32_t v[kPer];
+#pragma unroll
+    for (int j = 0; j < (int)kPer; j++) {
+        const uint32_t i = min(tid + (uint32_t)j * kSmallThreads, n - 1u), z = i / 352, w = i % 352;
+        v[j] = s_copy[z] ? xldg<X>(nin + (size_t)z * kRngNext + w) : xldg<X>(cur + (size_t)z * kRngZone + w);
+    }
+#pragma unroll
+    for (int j = 0; j < (int)kPer; j++) {
+        const uint32_t i = tid + (uint32_t)j * kSmallThreads;
+        if (i < n) lds[i] = v[j];
+    }
+    __syncthreads();
+    {
+        const uint32_t z = tid >> 2, k = tid & 3u;
+        if (tid < 64 && z < nZ && !s_copy[z]) sfmt_regen4(lds + (size_t)z * 352, (int)k);  // whole quads
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < n; i += kSmallThreads) xstg<X>(nout + (size_t)(i / 352) * kRngNext + i % 352, lds[i]);
+    if (tid < nZ) xstg<X>(nout + (size_t)tid * kRngNext + 352, s_gen[tid] + 1u);
 }
 
 // the small populations alone (the two-stream step, and per-kernel timing)
@@ -1444,42 +1500,93 @@ CBM_STEP_SPAN(5, kFmtArrays)
 #undef CBM_STEP_SPAN
 
 // ---------------------------------------------------------------- resident run
-// Grid barrier between two steps of a resident run.  Every thread first waits
-// until its own stores and atomics are acknowledged, so what it wrote for the
-// other blocks has reached the point of coherence; then thread 0 of each block
-// adds one to the counter of its group (blocks b and b + 8 share one; eight
-// counters 256 bytes apart, so that 256 arrivals do not queue on one address)
-// and polls the eight until they sum to `target` (the counters only grow; the
-// comparison is modulo 2^32).  The wait is bounded: a block that waits for
-// ~0.2 s records the failure in *status and leaves the run, so a launch whose
-// blocks were not all resident ends instead of hanging.
-constexpr uint64_t kBarrierTicks = 20000000ull;  // s_memrealtime runs at 100 MHz
+// Step-to-step synchronisation of a resident run, point to point instead of a
+// grid barrier.  Step t of
+//  * the small populations needs every GR block to have finished step t-1
+//    (their ring-slot counts) and the generator block step t-1 (the
+//    prebuilt generator block, the fold of the chunk sums of t-2);
+//  * a GR block needs the small populations to have finished step t-1 (GO
+//    spikes, plasticity timers, cleared ring slot) and with it the generator
+//    block (the fold of the chunk sums of t-2, whose buffer step t
+//    overwrites);
+//  * the generator block needs the small populations to have finished step
+//    t-1 (the generator state) and every GR block step t-1 (the chunk sums
+//    it folds).
+// So a GR block that finishes a step adds one to the counter of its group
+// (blocks b and b + 8 share one; eight counters 4 KB apart), the generator
+// block to a counter of its own, and the small block, once it has finished a
+// step and the generator block has too, writes its step count into
+// kFlagCopies flag words 4 KB apart; each other block polls one of them, so
+// that no word has more than ~16 pollers, with a growing pause between polls
+// (up to ~0.4 us) so that their loads do not crowd the memory channels the
+// small block uses meanwhile.
+// A signalling block first waits until its own stores and atomics are
+// acknowledged, so what it wrote for the other blocks has reached the point
+// of coherence.  Counters and flags only grow (compared modulo 2^32).  Every
+// wait is bounded: a block that waits ~0.2 s records the failure in *status
+// and leaves the run, so a launch whose blocks were not all resident ends
+// instead of hanging.
+constexpr uint64_t kWaitTicks = 20000000ull;  // s_memrealtime runs at 100 MHz
 
-__device__ __forceinline__ bool grid_sync(uint32_t *bar, uint32_t *status, uint32_t target)
+// every thread's stores and atomics acknowledged, then the block joins up
+__device__ __forceinline__ void res_drain()
 {
-    __shared__ uint32_t s_ok;
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
+}
+
+// a GR or generator block: one more step done (on counter ctr)
+__device__ __forceinline__ void res_arrive(uint32_t *ctr)
+{
+    res_drain();
+    if (threadIdx.x == 0) __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// the small block: steps done so far
+__device__ __forceinline__ void res_publish(uint32_t *flag, uint32_t v)
+{
+    res_drain();
+    if (threadIdx.x < (uint32_t)kFlagCopies)
+        __hip_atomic_store(flag + threadIdx.x * kBarStride, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// block-wide wait until the sum of the n words w[q * kBarStride] reaches
+// target; BACKOFF: the pause between polls doubles from 64 to 1,024 cycles
+template <int N, bool BACKOFF>
+__device__ __forceinline__ bool res_wait(const uint32_t *w, uint32_t target, uint32_t *status)
+{
+    __shared__ uint32_t s_ok;
     if (threadIdx.x == 0) {
-        __hip_atomic_fetch_add(bar + (blockIdx.x & (kBarGroups - 1)) * kBarStride, 1u, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
         const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
-        uint32_t ok = 1;
+        uint32_t ok = 1, pause = 0;
         while (true) {
-            uint32_t c[kBarGroups];
+            uint32_t c[N];
 #pragma unroll
-            for (int q = 0; q < kBarGroups; q++)
-                c[q] = __hip_atomic_load(bar + q * kBarStride, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            for (int q = 0; q < N; q++)
+                c[q] = __hip_atomic_load(const_cast<uint32_t *>(w) + q * kBarStride, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
             uint32_t sum = 0;
 #pragma unroll
-            for (int q = 0; q < kBarGroups; q++) sum += c[q];
+            for (int q = 0; q < N; q++) sum += c[q];
             if ((int32_t)(sum - target) >= 0) break;
-            if (__builtin_amdgcn_s_memrealtime() - t0 > kBarrierTicks) {
+            if (__builtin_amdgcn_s_memrealtime() - t0 > kWaitTicks) {
                 __hip_atomic_store(status, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 ok = 0;
                 break;
             }
-            __builtin_amdgcn_s_sleep(1);
+            if (!BACKOFF) {
+                __builtin_amdgcn_s_sleep(1);
+            } else {
+                // s_sleep takes an immediate: 1, 2, 4, 8, 16 x 64 cycles
+                switch (pause) {
+                case 0: __builtin_amdgcn_s_sleep(1); break;
+                case 1: __builtin_amdgcn_s_sleep(2); break;
+                case 2: __builtin_amdgcn_s_sleep(4); break;
+                case 3: __builtin_amdgcn_s_sleep(8); break;
+                default: __builtin_amdgcn_s_sleep(16); break;
+                }
+                pause++;
+            }
         }
         s_ok = ok;
     }
@@ -1487,11 +1594,79 @@ __device__ __forceinline__ bool grid_sync(uint32_t *bar, uint32_t *status, uint3
     return s_ok != 0;
 }
 
-// The small populations' part of a resident run (block 0).
+// The PF->PC fold of the chunk sums of one step (as gr_fold), by the 16 waves
+// of the generator block: two (zone, PC) tasks per wave at a time, the loads
+// of both in flight together, each task summed in gr_fold's order.
+__device__ __forceinline__ void res_fold(const float *src, float *dst, uint32_t nTasks, uint32_t numPC, uint32_t nC,
+                                         uint32_t perPC, uint32_t tid)
+{
+    const uint32_t lane = tid & 63u, wave = tid >> 6;
+    constexpr uint32_t kWaves = kSmallThreads / 64;
+    for (uint32_t t0 = wave; t0 < nTasks; t0 += 2 * kWaves) {  // wave-uniform
+        const uint32_t t1 = min(t0 + kWaves, nTasks - 1u);
+        const float *c0 = src + (size_t)(t0 / numPC) * nC + (size_t)(t0 % numPC) * perPC;
+        const float *c1 = src + (size_t)(t1 / numPC) * nC + (size_t)(t1 % numPC) * perPC;
+        float a0 = 0.0f, a1 = 0.0f;
+        for (uint32_t q0 = lane; q0 < perPC; q0 += 8 * 64) {
+            float x0[8], x1[8];
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                const uint32_t q = min(q0 + (uint32_t)j * 64u, perPC - 1u);
+                x0[j] = xld<true>(c0 + q);
+                x1[j] = xld<true>(c1 + q);
+            }
+#pragma unroll
+            for (int j = 0; j < 8; j++)
+                if (q0 + (uint32_t)j * 64u < perPC) {
+                    a0 = a0 + x0[j];
+                    a1 = a1 + x1[j];
+                }
+        }
+        a0 = wave_sum_lane0(a0);
+        a1 = wave_sum_lane0(a1);
+        if (lane == 0) {
+            xst<true>(dst + t0, a0);
+            if (t0 + kWaves < nTasks) xst<true>(dst + t1, a1);
+        }
+    }
+}
+
+// The small populations' state fields between global memory and the small
+// block's LDS (in: into LDS), 16 bytes at a time and the tail byte by byte.
+__device__ __forceinline__ void res_fields(const SmallArgs &a, uint8_t *lds, bool in, uint32_t tid)
+{
+    for (uint32_t e = 0; e < a.nLdsFields; e++) {
+        const int f = (int)a.ldsField[e];
+        uint8_t *g = reinterpret_cast<uint8_t *>(a.f[f]);
+        uint8_t *l = lds + a.fLds[f];
+        const uint32_t bytes = a.ldsBytes[e], n16 = bytes / 16;
+        for (uint32_t i = tid; i < n16; i += kSmallThreads) {
+            if (in) reinterpret_cast<u32x4 *>(l)[i] = *ga(reinterpret_cast<const u32x4 *>(g) + i);
+            else *ga(reinterpret_cast<u32x4 *>(g) + i) = reinterpret_cast<const u32x4 *>(l)[i];
+        }
+        for (uint32_t i = n16 * 16 + tid; i < bytes; i += kSmallThreads) {
+            if (in) l[i] = *ga(g + i);
+            else *ga(g + i) = l[i];
+        }
+    }
+}
+
+// The small populations' part of a resident run (block 0): their state
+// fields held in LDS from the first step to the last.
 __device__ __forceinline__ void res_small(const SmallArgs &a, const ResRun &r, float *s_err, uint8_t *lds)
 {
-    const uint32_t nZ = a.s.numZones, nb = gridDim.x;
+    const uint32_t nZ = a.s.numZones;
+    res_fields(a, lds, true, threadIdx.x);
+    __syncthreads();
+    // the relative error drive of the next step, loaded a step ahead (thread z: zone z)
+    auto errOf = [&](uint32_t k) {
+        return (r.err && threadIdx.x < nZ && k < r.n) ? r.err[(size_t)k * nZ + threadIdx.x] : 0.0f;
+    };
+    float errNext = errOf(0);
     for (uint32_t k = 0; k < r.n; k++) {
+        // every GR block done with step k - 1 (the generator block was, before
+        // step k - 1 was published)
+        if (!res_wait<kBarGroups, false>(r.bar, r.barBase + k * r.spans, r.status)) return;
         const uint64_t t = r.t0 + k + 1;
         SmallStep st;
         st.mf = r.mf + (size_t)k * r.pitch;
@@ -1503,10 +1678,7 @@ __device__ __forceinline__ void res_small(const SmallArgs &a, const ResRun &r, f
         if (tid < (uint32_t)kMaxZones) {
             const uint32_t z = tid;
             float e = k == 0 ? r.errDrive0[z] : 0.0f;
-            if (r.err && z < nZ) {
-                const float x = r.err[(size_t)k * nZ + z];
-                if (x != 0.0f) e = x * r.maxExtIncVIO;
-            }
+            if (z < nZ && errNext != 0.0f) e = errNext * r.maxExtIncVIO;
             s_err[z] = e;
         }
         st.gIncGRtoGO = r.gIncGRtoGO;
@@ -1526,20 +1698,39 @@ __device__ __forceinline__ void res_small(const SmallArgs &a, const ResRun &r, f
         uint64_t pa = reinterpret_cast<uint64_t>(&a);
         asm volatile("" : "+s"(pa));
         const SmallArgs *ap = (const SmallArgs *)(const __attribute__((address_space(4))) SmallArgs *)pa;
-        small_step<true>(*ap, st, s_err, lds);
-        if (k + 1 < r.n && !grid_sync(r.bar, r.status, r.barBase + (k + 1) * nb)) return;
+        small_step<true, true>(*ap, st, s_err, lds);
+        errNext = errOf(k + 1);
+        // the generator block done with step k (its fold read the buffer the
+        // GR blocks' step k + 1 writes)
+        if (!res_wait<1, false>(r.pb, r.pbBase + k + 1, r.status)) return;
+        res_publish(r.flag, r.flagBase + k + 1);
     }
+    __syncthreads();
+    res_fields(a, lds, false, threadIdx.x);
 }
 
-// The generators' next blocks in a resident run (the last block).
+// The generators' next blocks and the PF->PC fold in a resident run (the
+// last block).
 __device__ __forceinline__ void res_prebuild(const SmallArgs &a, const ResRun &r, uint8_t *lds)
 {
-    const uint32_t nb = gridDim.x;
+    const uint32_t nZ = a.s.numZones, nC = r.g.numGR >> 6;
     for (uint32_t k = 0; k < r.n; k++) {
+        // the small populations done with step k - 1 (the generator state),
+        // every GR block done with step k - 1 (the chunk sums)
+        if (!res_wait<1, true>(r.flag + (blockIdx.x % kFlagCopies) * kBarStride, r.flagBase + k, r.status)) return;
+        if (!res_wait<kBarGroups, true>(r.bar, r.barBase + k * r.spans, r.status)) return;
+        const uint64_t t = r.t0 + k + 1;
+        uint32_t tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));  // as in small_step
+        // the chunk sums of step t-1 into their ring slot (read at step t+1)
+        if (!(k == 0 && r.skipFold0))
+            res_fold(a.chunk2 + (size_t)((t - 1) & 1u) * nZ * nC,
+                     a.pfpcRing + (size_t)sreg((uint32_t)((t + kRing - 1) % kRing)) * nZ * a.s.numPC, nZ * a.s.numPC,
+                     a.s.numPC, nC, r.g.perPC, tid);
         SmallStep st;
-        st.par = (uint32_t)((r.t0 + k + 1) & 1u);
+        st.par = (uint32_t)(t & 1u);
         sfmt_prebuild<true>(a, st, reinterpret_cast<uint32_t *>(lds));
-        if (k + 1 < r.n && !grid_sync(r.bar, r.status, r.barBase + (k + 1) * nb)) return;
+        res_arrive(r.pb);
     }
 }
 
@@ -1548,7 +1739,7 @@ template <int CPT>
 __device__ __forceinline__ void res_gr(const SmallArgs &a, const ResRun &r, const uint32_t rank, uint8_t *lds)
 {
     const GrArgs &g = r.g;
-    const uint32_t nb = gridDim.x, nZ = g.numZones, nC = g.numGR >> 6, tid = threadIdx.x;
+    const uint32_t nZ = g.numZones, nC = g.numGR >> 6, tid = threadIdx.x;
     const uint32_t perHist = a.p.tsPerHistBinGR;
     const uint32_t c0 = (uint32_t)((uint64_t)rank * nC / r.spans), c1 = (uint32_t)((uint64_t)(rank + 1) * nC / r.spans);
     const uint32_t first = c0 << 6, count = (c1 - c0) << 6;
@@ -1571,6 +1762,8 @@ __device__ __forceinline__ void res_gr(const SmallArgs &a, const ResRun &r, cons
         }
     }
     for (uint32_t k = 0; k < r.n; k++) {
+        // the small populations done with step k - 1
+        if (!res_wait<1, true>(r.flag + (blockIdx.x % kFlagCopies) * kBarStride, r.flagBase + k, r.status)) return;
         const uint64_t t = r.t0 + k + 1;
         const uint32_t par = (uint32_t)(t & 1u), parPrev = par ^ 1u;
         const uint32_t slot = (uint32_t)(t % kRing);
@@ -1580,8 +1773,8 @@ __device__ __forceinline__ void res_gr(const SmallArgs &a, const ResRun &r, cons
         gs.mf = (sptr(r.mf + (size_t)k * r.pitch));
         gs.apGOprev = (sptr(a.apGO2 + (size_t)parPrev * g.numGO));
         gs.chunk = (sptr(chunk2 + (size_t)par * nZ * nC));
-        gs.foldSrc = (sptr((k == 0 && r.skipFold0) ? nullptr : chunk2 + (size_t)parPrev * nZ * nC));
-        gs.foldDst = (sptr(a.pfpcRing + (size_t)((t + kRing - 1) % kRing) * nZ * g.numPC));
+        gs.foldSrc = nullptr;  // the small block folds (res_small)
+        gs.foldDst = nullptr;
         gs.bcCnt = (sptr(a.bcRing + (size_t)slot * a.s.numBC));
         gs.scCnt = (sptr(a.scRing + (size_t)slot * a.s.numSC));
         gs.goCnt = (sptr(a.goRing + (size_t)slot * g.numGO));
@@ -1593,11 +1786,9 @@ __device__ __forceinline__ void res_gr(const SmallArgs &a, const ResRun &r, cons
         gs.pfCnt = (sptr((a.rec && r.recOffPF != 0xFFFFFFFFu)
                                ? a.rec + (size_t)(t % a.recCap) * a.recRow + r.recOffPF
                                : nullptr));
-        if (count == 0)
-            gr_fold<kSmallThreads, true>(g, gs, rank, r.spans);
-        else
+        if (count)
             gr_span_impl<CPT, kSmallThreads, false, true, true, true>(g, gs, first, count, rank, r.spans, smem, &R);
-        if (k + 1 < r.n && !grid_sync(r.bar, r.status, r.barBase + (k + 1) * nb)) return;
+        res_arrive(r.bar + (blockIdx.x & (kBarGroups - 1)) * kBarStride);
     }
     // the cells' state back into the per-GR arrays
 #pragma unroll
@@ -1620,8 +1811,9 @@ __device__ __forceinline__ void res_gr(const SmallArgs &a, const ResRun &r, cons
 // (r+1)*C/spans) of the C 64-GR chunks, as step_span_kernel), whose state
 // stays in registers (GrRegs) from the first step to the last and is stored
 // back at the end, and the last block prebuilds the zone generators' next
-// blocks.  Each step does the work of one step_span_kernel launch, and the
-// grid barrier stands in for the launch boundary.  CPT: cells per thread
+// blocks.  Each step does the work of one step_span_kernel launch (the small
+// block also folds the PF->PC chunk sums), and the waits on step counters and
+// flags stand in for the launch boundary.  CPT: cells per thread
 // (every span holds at most 1024 * CPT cells); compact connectivity only.
 template <int CPT>
 __global__ void __launch_bounds__(kSmallThreads) step_resident_kernel(const SmallArgs *__restrict__ args, const ResRun r)

@@ -129,6 +129,11 @@ SmallLds smallLayout(const cbm_sizes &s, uint32_t slotsBCPC, uint32_t slotsPCBC,
     L.apNC = take(nZ * s.numNC);
     L.vIO = take(4 * nZ * s.numIO);
     L.rel = take(4 * nZ * s.numNC);
+    L.inPF = take(4 * nZ * s.numPC);
+    L.inBC = take(4 * s.numBC);
+    L.inSC = take(4 * s.numSC);
+    L.inGO = take(4 * s.numGO);
+    L.inTm = take(4 * nZ * s.numIO);
     L.rng = take(4 * nZ * kRngZone);
     L.rngNext = take(4 * nZ * kRngNext);
     L.popStride = 3 + s.numPopHistBinsPC + 2;
@@ -316,7 +321,7 @@ int cbmdev::syncAll(cbm_handle *h)
         h->resCheck = false;
         if (*h->resStatusHost) {
             h->resident = false;
-            return fail(CBM_E_HIP, "resident run: a grid barrier wait timed out (the run's workgroups were not all "
+            return fail(CBM_E_HIP, "resident run: a wait between steps timed out (the run's workgroups were not all "
                                    "resident at once); the network state is undefined");
         }
     }
@@ -535,21 +540,33 @@ static const void *residentKernel(int cpt)
                     : cpt == 3 ? (const void *)step_resident_kernel<3> : (const void *)step_resident_kernel<5>;
 }
 
-// Decide at commit whether cbm_run_mf_train runs resident (stepMode 5; the
-// even split and compact connectivity records; a device that launches
+// Decide at commit whether cbm_run_mf_train runs resident (stepMode 0 or 5;
+// the even split and compact connectivity records; the small populations' state
+// fields fit in LDS beside the small step's scratch; a device that launches
 // cooperatively and holds one 1024-thread workgroup of the kernel on every
 // CU), and set it up: the LDS of the GR workgroups (their span's working LDS
-// and the records of their cells), the barrier counters and the status word.
+// and the records of their cells), the fields' place in the small block's
+// LDS (h->smallHost), the step counters and the status word.
 static int setupResident(cbm_handle *h, int cus)
 {
     const cbm_sizes &s = h->s;
     const int sm = (int)h->o.stepMode;
     h->resident = false;
-    if (sm == 5 && h->spans && h->conn) {
+    if ((sm == 0 || sm == 5) && h->spans && h->conn) {
         int coop = 0;
         HIP_TRY(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, h->o.device));
         const uint32_t recOff = roundUp(grTileLds(1024, h->spanCpt, s.numGO, s.numMF, s.numZones, s.numIO), 16);
-        const size_t lds = std::max<size_t>(h->smallLds, recOff + (size_t)h->spanCpt * 1024 * 16);
+        SmallArgs &sa = h->smallHost;
+        uint32_t off = roundUp((uint32_t)h->smallLds, 16);
+        for (int e = 0; e < kSmallFieldCount; e++) {
+            const int f = kSmallFields[e];
+            const size_t bytes = h->zoneBytes(f) * (size_t)h->zonesOf(f);
+            sa.fLds[f] = off;
+            sa.ldsField[e] = (uint32_t)f;
+            sa.ldsBytes[e] = (uint32_t)bytes;
+            off = roundUp(off + (uint32_t)bytes, 16);
+        }
+        const size_t lds = std::max<size_t>(off, recOff + (size_t)h->spanCpt * 1024 * 16);
         if (coop && lds <= kSmallLdsMax) {
             const void *k = residentKernel(h->spanCpt);
             HIP_TRY(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -557,22 +574,31 @@ static int setupResident(cbm_handle *h, int cus)
             HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k, kSmallThreads, lds));
             if (per >= 1 && (int64_t)per * cus >= (int64_t)h->spans + 2) {
                 int rc;
-                if ((rc = h->dalloc((void **)&h->resBar, (size_t)kBarGroups * kBarStride * 4))) return rc;
+                if ((rc = h->dalloc((void **)&h->resBar, (size_t)(kBarGroups + kFlagCopies + 1) * kBarStride * 4)))
+                    return rc;
                 if ((rc = h->dalloc((void **)&h->resStatus, 4))) return rc;
                 if (!h->resStatusHost) {
                     HIP_TRY(hipHostMalloc((void **)&h->resStatusHost, 4, hipHostMallocDefault));
                     *h->resStatusHost = 0;
                 }
                 h->resBarBase = 0;
+                h->resFlagBase = 0;
+                h->resPbBase = 0;
                 h->residentLds = lds;
                 h->residentRecLds = recOff;
                 h->resident = true;
+                sa.nLdsFields = kSmallFieldCount;
             }
+        }
+        if (!h->resident) {
+            for (int f = 0; f < CBM_F_COUNT; f++) sa.fLds[f] = kNotInLds;
+            sa.nLdsFields = 0;
         }
     }
     if (sm == 5 && !h->resident)
         return fail(CBM_E_UNSUPPORTED, "stepMode 5: resident runs need the even GR split, compact connectivity "
-                                       "records and one cooperative 1024-thread workgroup per CU");
+                                       "records, the small populations' state in LDS and one cooperative "
+                                       "1024-thread workgroup per CU");
     return CBM_OK;
 }
 
@@ -820,6 +846,8 @@ int cbm_commit(cbm_handle *h)
         a.rec = nullptr;
         a.recRow = a.recCap = 0;
         for (int k = 0; k < 6; k++) a.recOff[k] = 0xFFFFFFFFu;
+        for (int f = 0; f < CBM_F_COUNT; f++) a.fLds[f] = kNotInLds;
+        a.nLdsFields = 0;
         h->smallHost = a;
         if (h->smallLds > 64 * 1024)
             HIP_TRY(hipFuncSetAttribute((const void *)small_step_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -854,9 +882,9 @@ int cbm_commit(cbm_handle *h)
         if (h->spans && grLds > 64 * 1024)
             HIP_TRY(hipFuncSetAttribute(grSpanKernel(h->spanCpt, h->conn != nullptr),
                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)grLds));
+        if ((rc = setupResident(h, cus))) return rc;  // (may lay out the fields in LDS: h->smallHost)
         if ((rc = h->dalloc((void **)&h->smallArgs, sizeof(SmallArgs)))) return rc;
-        HIP_TRY(hipMemcpy(h->smallArgs, &a, sizeof(SmallArgs), hipMemcpyHostToDevice));
-        if ((rc = setupResident(h, cus))) return rc;
+        HIP_TRY(hipMemcpy(h->smallArgs, &h->smallHost, sizeof(SmallArgs), hipMemcpyHostToDevice));
     }
     h->skipFold = true;
     h->committed = true;
@@ -1345,7 +1373,7 @@ int cbmdev::checkTrain(cbm_handle *h, const uint8_t *mf_dev, size_t pitch, int n
     return CBM_OK;
 }
 
-// Steps of one resident launch at most: the barrier targets grow by the grid
+// Steps of one resident launch at most: the step counters grow by the grid
 // size per step and are compared modulo 2^32.
 constexpr int kResMaxSteps = 1 << 20;
 
@@ -1379,6 +1407,8 @@ static int runResident(cbm_handle *h, const uint8_t *mf_dev, size_t pitch, int n
     r.spans = h->spans;
     r.recOffPF = (h->rec && h->recOffPF != 0xFFFFFFFFu) ? h->recOffPF : 0xFFFFFFFFu;
     r.bar = h->resBar;
+    r.flag = h->resBar + kBarGroups * kBarStride;
+    r.pb = h->resBar + (kBarGroups + kFlagCopies) * kBarStride;
     r.status = h->resStatus;
     r.recLds = h->residentRecLds;
     const uint32_t grid = h->spans + 2;
@@ -1393,9 +1423,13 @@ static int runResident(cbm_handle *h, const uint8_t *mf_dev, size_t pitch, int n
         r.skipFold0 = h->skipFold ? 1u : 0u;
         r.n = (uint32_t)m;
         r.barBase = h->resBarBase;
+        r.flagBase = h->resFlagBase;
+        r.pbBase = h->resPbBase;
         void *args[] = {&sa, &r};
         HIP_TRY(hipLaunchCooperativeKernel(kern, dim3(grid), dim3(kSmallThreads), args, h->residentLds, h->stream));
-        h->resBarBase += (uint32_t)(m - 1) * grid;  // one barrier between consecutive steps
+        h->resBarBase += (uint32_t)m * h->spans;  // each GR block counts each step
+        h->resFlagBase += (uint32_t)m;
+        h->resPbBase += (uint32_t)m;
         h->curTime += (uint64_t)m;
         for (int z = 0; z < kMaxZones; z++) h->errDrive[z] = 0.0f;  // used by the first step
         h->skipFold = false;

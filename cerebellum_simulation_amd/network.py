@@ -169,7 +169,8 @@ class Network:
 
     def __init__(self, sizes: Sizes, params: ActParams, *, grgo_lag: int = 2, device: int = 0,
                  cells_per_thread: int = 0, block_threads: int = 0, step_mode: int = 0, conn_format: int = 0):
-        """step_mode: 0 = pick, 1 = one fused launch per step, 2 = GR and
+        """step_mode: 0 = pick (1, with resident run_mf_train runs where the
+        network allows them), 1 = one fused launch per step, 2 = GR and
         small-population kernels on two streams, 3 = fused with GR tiles of
         1024 x 2 cells, 4 = fused with the GR cells split evenly over the CUs
         (mode 1 picks the even split when each block gets 1,024 to 5,120

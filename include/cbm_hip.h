@@ -139,19 +139,21 @@ typedef struct cbm_options {
      * two-stream step; 0 = pick (256 x 1).  The fused step uses 1024-thread
      * blocks with 2 cells per thread, or 1 when grCellsPerThread is 1. */
     uint32_t grCellsPerThread, grBlockThreads;
-    /* How a step is launched: 0 = pick (1), 1 = one fused launch per step (block
+    /* How a step is launched: 0 = pick (1, with resident runs below where the
+     * network allows them), 1 = one fused launch per step (block
      * 0 steps the small populations; the GR cells are split evenly over one
      * block per remaining CU when that gives each block 1,024 to 5,120 cells,
      * else into tiles of 1024 x 2 cells), 2 = the GR and small-population
      * kernels on two streams, ordered by events, 3 = fused with GR tiles
      * always, 4 = fused with the even split whenever it gives each block at
      * most 5,120 cells (else an error), 5 = as 4, and cbm_run_mf_train runs
-     * of two or more steps are resident (below), or cbm_commit fails.  Resident:
+     * of two or more steps are resident, or cbm_commit fails.  Resident:
      * the whole run is one cooperative launch of one workgroup per CU whose GR
      * workgroups hold their cells' state in registers (and connectivity in
-     * LDS) from the first step to the last, with a grid barrier between steps
-     * (networks with the even split and compact connectivity records).  All
-     * give the same bits. */
+     * LDS), and whose small-population workgroup holds theirs in LDS, from
+     * the first step to the last, synchronised point to point between steps
+     * (networks with the even split, compact connectivity records and small
+     * populations that fit in LDS).  All give the same bits. */
     uint32_t stepMode;
     /* Device layout of the GR connectivity: 0 = pick (one 16-byte record per
      * GR when numMF and numGO are at most 2047 and the fan-ins are at most 4
@@ -419,7 +421,8 @@ int cbm_step_io_end(cbm_handle *h, uint8_t *out);
  * pointer to n rows of numMF bytes, e.g. a pre-generated spike train).
  * err_drive_host, if not NULL, gives [n][numZones] relative error drives
  * (0 = none) applied before each step.  A run of two or more steps is
- * resident with cbm_options.stepMode 5; a resident run
+ * resident with cbm_options.stepMode 0 or 5 when the network allows it
+ * (cbm_resident); a resident run
  * whose workgroups were not all resident at once ends after a bounded wait
  * with CBM_E_HIP at the next call that waits for the handle's work, and the
  * network's state is then undefined. */
