@@ -22,26 +22,30 @@ constexpr int kRing = 3;                // input-sum ring slots (t, t-1, t-2)
 constexpr int kRngZone = 354, kRngNext = 353;
 constexpr int kSmallThreads = 1024;     // threads of the small-population kernel
 // The state fields the small-population step reads and writes, which a
-// resident run keeps in the small block's LDS from the first step to the last
-// (when they fit beside its scratch LDS).
-constexpr int kSmallFieldCount = 64;
+// resident run keeps in LDS from the first step to the last (when they fit
+// beside the step's scratch LDS): first the input layer's (GO, SC, the MF
+// spike buffer), held by the block that steps them, then the microzones'.
+constexpr int kSmallFieldCount = 64, kSmallInFields = 23;
 constexpr int kSmallFields[kSmallFieldCount] = {
-    CBM_F_APBUF_BC, CBM_F_APBUF_GO, CBM_F_APBUF_IO, CBM_F_APBUF_MF,
-    CBM_F_APBUF_NC, CBM_F_APBUF_PC, CBM_F_APBUF_SC, CBM_F_AP_BC,
-    CBM_F_AP_IO, CBM_F_AP_NC, CBM_F_AP_PC, CBM_F_AP_SC,
-    CBM_F_GLU_GO, CBM_F_GMAX_AMPA_MF_NC, CBM_F_GMAX_NMDA_MF_NC, CBM_F_GO_GABA_OUT_SYNSCALE_GOGO,
-    CBM_F_G_BC_PC, CBM_F_G_GO_GO, CBM_F_G_GR_GO, CBM_F_G_MF_AMPA_NC,
-    CBM_F_G_MF_GO, CBM_F_G_MF_NMDA_NC, CBM_F_G_MGLURINC_GO, CBM_F_G_MGLUR_GO,
-    CBM_F_G_NC_IO, CBM_F_G_PC_BC, CBM_F_G_PC_NC, CBM_F_G_PC_SCALE_NC,
-    CBM_F_G_PF_BC, CBM_F_G_PF_PC, CBM_F_G_PF_SC, CBM_F_G_SC_PC,
-    CBM_F_HIST_MF, CBM_F_HIST_PC_POPACT, CBM_F_INPUT_BC_PC, CBM_F_INPUT_GOGABA_SYNDEP_GO,
-    CBM_F_INPUT_GO_GO, CBM_F_INPUT_MF_GO, CBM_F_INPUT_MF_NC, CBM_F_INPUT_NC_IO,
-    CBM_F_INPUT_PC_BC, CBM_F_INPUT_PC_NC, CBM_F_INPUT_SC_PC, CBM_F_MFNC_NOLTP_NOLTD,
-    CBM_F_MF_SYNW_NC, CBM_F_MGLUR_GO, CBM_F_PC_POPACT_CTRS, CBM_F_SUM_GO_INPUT_GO,
-    CBM_F_SUM_GR_INPUT_GO, CBM_F_SYN_IO_PRELEASE_NC, CBM_F_THRESH_BC, CBM_F_THRESH_GO,
-    CBM_F_THRESH_IO, CBM_F_THRESH_NC, CBM_F_THRESH_PC, CBM_F_THRESH_SC,
-    CBM_F_VCOUPLE_GO, CBM_F_VCOUPLE_IO, CBM_F_VM_BC, CBM_F_VM_GO,
-    CBM_F_VM_IO, CBM_F_VM_NC, CBM_F_VM_PC, CBM_F_VM_SC};
+    // input layer
+    CBM_F_APBUF_GO, CBM_F_APBUF_MF, CBM_F_APBUF_SC, CBM_F_AP_SC,
+    CBM_F_GLU_GO, CBM_F_GO_GABA_OUT_SYNSCALE_GOGO, CBM_F_G_GO_GO, CBM_F_G_GR_GO,
+    CBM_F_G_MF_GO, CBM_F_G_MGLURINC_GO, CBM_F_G_MGLUR_GO, CBM_F_G_PF_SC,
+    CBM_F_INPUT_GOGABA_SYNDEP_GO, CBM_F_INPUT_GO_GO, CBM_F_INPUT_MF_GO, CBM_F_MGLUR_GO,
+    CBM_F_SUM_GO_INPUT_GO, CBM_F_SUM_GR_INPUT_GO, CBM_F_THRESH_GO, CBM_F_THRESH_SC,
+    CBM_F_VCOUPLE_GO, CBM_F_VM_GO, CBM_F_VM_SC,
+    // microzones
+    CBM_F_APBUF_BC, CBM_F_APBUF_IO, CBM_F_APBUF_NC, CBM_F_APBUF_PC,
+    CBM_F_AP_BC, CBM_F_AP_IO, CBM_F_AP_NC, CBM_F_AP_PC,
+    CBM_F_GMAX_AMPA_MF_NC, CBM_F_GMAX_NMDA_MF_NC, CBM_F_G_BC_PC, CBM_F_G_MF_AMPA_NC,
+    CBM_F_G_MF_NMDA_NC, CBM_F_G_NC_IO, CBM_F_G_PC_BC, CBM_F_G_PC_NC,
+    CBM_F_G_PC_SCALE_NC, CBM_F_G_PF_BC, CBM_F_G_PF_PC, CBM_F_G_SC_PC,
+    CBM_F_HIST_MF, CBM_F_HIST_PC_POPACT, CBM_F_INPUT_BC_PC, CBM_F_INPUT_MF_NC,
+    CBM_F_INPUT_NC_IO, CBM_F_INPUT_PC_BC, CBM_F_INPUT_PC_NC, CBM_F_INPUT_SC_PC,
+    CBM_F_MFNC_NOLTP_NOLTD, CBM_F_MF_SYNW_NC, CBM_F_PC_POPACT_CTRS, CBM_F_SYN_IO_PRELEASE_NC,
+    CBM_F_THRESH_BC, CBM_F_THRESH_IO, CBM_F_THRESH_NC, CBM_F_THRESH_PC,
+    CBM_F_VCOUPLE_IO, CBM_F_VM_BC, CBM_F_VM_IO, CBM_F_VM_NC,
+    CBM_F_VM_PC};
 constexpr uint32_t kNotInLds = 0xFFFFFFFFu;
 constexpr uint32_t kSmallLdsMax = 152 * 1024;  // dynamic LDS cap of the small kernel
 
@@ -165,7 +169,7 @@ struct SmallArgs {
     // in the small block's dynamic LDS (kNotInLds: not resident); the copy
     // list gives each resident field's offset and bytes (all zones)
     uint32_t fLds[CBM_F_COUNT];
-    uint32_t nLdsFields;
+    uint32_t nLdsFields, nInFields;  // entries [0, nInFields) the input-layer part's, the rest the microzones'
     uint32_t ldsField[kSmallFieldCount], ldsBytes[kSmallFieldCount];
 };
 
@@ -185,6 +189,13 @@ struct SmallStep {
     // into ioOut, pinned host memory (nullptr: none)
     uint8_t *ioOut;
     uint32_t ioPop;
+    // a resident run that splits the step over two blocks: the input-layer
+    // part writes this step's SC spikes to apSCout, the microzone part routes
+    // the previous step's from apSCin to PC (scFromPrev; 0 in a launch's
+    // first step, whose INPUT_SC_PC the step before routed)
+    uint8_t *apSCout;
+    const uint8_t *apSCin;
+    uint32_t scFromPrev;
 };
 
 // Arguments of the streaming-ceiling kernel: nRead 4-byte arrays (the first
@@ -236,13 +247,17 @@ struct ResRun {
     float errDrive0[kMaxZones]; // drive pending from the host for the first step
     float maxExtIncVIO, gIncGRtoGO;
     uint32_t n;                 // steps
-    uint32_t spans;             // GR blocks (1 .. spans); block spans + 1 prebuilds generator blocks
+    uint32_t spans;             // GR blocks (1 .. spans); block spans + 1 steps the microzones, spans + 2
+                                // folds the PF->PC sums and prebuilds generator blocks
     uint32_t skipFold0;         // the host wrote the PF->PC sums of step t0
     uint32_t recOffPF;          // offset of the PF chunk counts in a recorder row (UINT32_MAX: none)
     uint32_t *bar;              // [kBarGroups * kBarStride] step counters of the GR blocks
     uint32_t barBase;           // their sum before this launch
-    uint32_t *flag;             // [kFlagCopies * kBarStride] the small block's step count, copies
+    uint32_t *flag;             // [kFlagCopies * kBarStride] the microzone part's step count, copies
     uint32_t flagBase;          // its value before this launch
+    uint32_t *flagIn;           // [kFlagCopies * kBarStride] the input-layer part's step count, copies
+    uint32_t flagInBase;        // its value before this launch
+    uint8_t *apSC3;             // [3][numSC] SC spikes by step mod 3 (input-layer part -> microzone part)
     uint32_t *pb;               // the generator block's step count
     uint32_t pbBase;            // its value before this launch
     uint32_t *status;           // set to 1 when a barrier wait timed out

@@ -62,12 +62,16 @@ struct cbm_handle {
     bool resident = false;
     size_t residentLds = 0;           // its dynamic LDS (the small step's, or a span's + its records)
     uint32_t residentRecLds = 0;      // byte offset of the GR records in that LDS
-    uint32_t *resBar = nullptr;       // GR step counters [kBarGroups * kBarStride], the small block's
-                                      // step-count copies [kFlagCopies * kBarStride], the generator
-                                      // block's step count
+    uint32_t *resBar = nullptr;       // GR step counters [kBarGroups * kBarStride], the microzone and
+                                      // input-layer blocks' step-count copies [2][kFlagCopies * kBarStride],
+                                      // the generator block's step count
     uint32_t resBarBase = 0;          // the counters' sum after the last resident launch
     uint32_t resFlagBase = 0;         // the small block's step count after it
     uint32_t resPbBase = 0;           // the generator block's step count after it
+    uint32_t resFlagInBase = 0;       // the input-layer block's step count after it
+    uint32_t resSpans = 0;            // GR blocks of a resident launch
+    int resCpt = 0;                   // their cells per thread
+    uint8_t *resApSC3 = nullptr;      // SC spikes by step mod 3, input-layer block -> microzone block
     uint32_t *resStatus = nullptr;    // device: 1 when a barrier wait timed out
     uint32_t *resStatusHost = nullptr;  // pinned copy, read at the next synchronisation
     bool resCheck = false;            // a resident run's status is still to be read

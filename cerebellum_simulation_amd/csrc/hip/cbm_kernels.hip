@@ -753,8 +753,14 @@ __device__ __forceinline__ float lds_sum_seq(const float *x, uint32_t n, float a
 // population or synapse group starts on its own wave, so that no wave runs
 // the updates of every small population in turn.  X: inside a resident run
 // (the arrays other blocks share go through xldg / xstg); LF: with the state
-// fields in LDS (SF).
-template <bool X = false, bool LF = false>
+// fields in LDS (SF).  PART: the whole step, or one of its two parts on a
+// block of its own (a resident run): the input layer (GO, SC, MF->GO and
+// GO->GO) or the microzones (PC, BC, IO, NC, their synapses, outputs and
+// plasticity).  The microzone part routes the SC spikes to PC at the start of
+// the next step (from the input-layer part's apSCout) instead of at the end
+// of this one.
+constexpr int kPartAll = 0, kPartIn = 1, kPartMz = 2;
+template <bool X = false, bool LF = false, int PART = kPartAll>
 __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &st, const float *errDrive,
                                            uint8_t *lds)
 {
@@ -772,6 +778,7 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
     const uint32_t kSC = s.numpPCfromSCtoPC, kMF = s.numpNCfromMFtoNC, kPC = s.numpNCfromPCtoNC;
     const uint32_t kNCIO = s.numpIOfromNCtoIO, kIOIO = s.numpIOInIOIO;
     const uint32_t bins = s.numPopHistBinsPC;
+    constexpr bool IN = PART != kPartMz, MZ = PART != kPartIn;
 
     uint8_t *s_mf = lds + L.mf;
     uint8_t *s_hmf = lds + L.hmf;
@@ -789,7 +796,8 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
     __shared__ int s_doLTD[kMaxZones], s_doLTP[kMaxZones];
 
     // ======================= phase 0
-    for (uint32_t i = tid; i < s.numGO; i += nThr) s_goMF[i] = 0;
+    if constexpr (IN)
+        for (uint32_t i = tid; i < s.numGO; i += nThr) s_goMF[i] = 0;
     __syncthreads();
     // staging, InNet::updateMFActivties (innet.cpp:800-815) and
     // InNet::updateMFtoGOOut (innet.cpp:912-931): each spiking MF adds one to
@@ -813,16 +821,18 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
             const uint32_t im = min(i, s.numMF - 1), ip = min(i, nPop - 1);
             const uint32_t w = mfw[min(i, nMFw - 1)];
             const u32x4 bl = ((const G<u32x4> *)a.idxBlob)[min(i, L.idx16 - 1)];
-            const uint32_t r = xldg<X>(rngIn + min(i, nRng - 1));
-            const uint32_t rn = xldg<X>(nextIn + min(i, nNext - 1));
+            const uint32_t r = MZ ? xldg<X>(rngIn + min(i, nRng - 1)) : 0u;
+            const uint32_t rn = MZ ? xldg<X>(nextIn + min(i, nNext - 1)) : 0u;
             // the inputs phase 1 reads from the other half of the step (PC/BC/SC
             // the PF sums of t-2, GO the GR->GO count of t-grgoLag, IO the
             // plasticity timer of t-1): nothing writes them before phase 2
-            const float inPF = xldg<X>(gp(a.pfpcRing) + (size_t)st.slotT2 * nPCc + min(i, nPCc - 1));
-            const uint32_t inBC = xldg<X>(gp(a.bcRing) + (size_t)st.slotT2 * s.numBC + min(i, s.numBC - 1));
-            const uint32_t inSC = xldg<X>(gp(a.scRing) + (size_t)st.slotT2 * s.numSC + min(i, s.numSC - 1));
-            const uint32_t inGO = xldg<X>(gp(a.goRing) + (size_t)st.slotG * s.numGO + min(i, s.numGO - 1));
-            const int32_t inTm = xldg<X>(gp(a.timer2) + (size_t)parPrev * nIOc + min(i, nIOc - 1));
+            const float inPF = MZ ? xldg<X>(gp(a.pfpcRing) + (size_t)st.slotT2 * nPCc + min(i, nPCc - 1)) : 0.0f;
+            const uint32_t inBC = MZ ? xldg<X>(gp(a.bcRing) + (size_t)st.slotT2 * s.numBC + min(i, s.numBC - 1)) : 0u;
+            const uint32_t inSC = IN ? xldg<X>(gp(a.scRing) + (size_t)st.slotT2 * s.numSC + min(i, s.numSC - 1)) : 0u;
+            const uint32_t inGO = IN ? xldg<X>(gp(a.goRing) + (size_t)st.slotG * s.numGO + min(i, s.numGO - 1)) : 0u;
+            const int32_t inTm = MZ ? xldg<X>(gp(a.timer2) + (size_t)parPrev * nIOc + min(i, nIOc - 1)) : 0;
+            // (microzone part) the previous step's SC spikes
+            const uint8_t scPrev = PART == kPartMz ? xldg<X>(gp(st.apSCin) + min(i, s.numSC - 1)) : (uint8_t)0;
             const uint32_t pz = ip / L.popStride, pk = ip % L.popStride;
             const auto pw = pk < 3 ? SF<LF, uint32_t>(a, CBM_F_PC_POPACT_CTRS, lds) + pz * 3 + pk
                                            : SF<LF, uint32_t>(a, CBM_F_HIST_PC_POPACT, lds) + (size_t)pz * bins +
@@ -834,20 +844,21 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
             const uint32_t b = apBufMF[im];
             const uint32_t pv = pk < 3 + bins ? pv32 : pv8;
             if (i < nMFw) mfl[i] = w;
-            if (i < L.idx16) bd[i] = bl;
-            if (i < nPCc) at<float>(lds, L.inPF)[i] = inPF;
-            if (i < s.numBC) at<uint32_t>(lds, L.inBC)[i] = inBC;
-            if (i < s.numSC) at<uint32_t>(lds, L.inSC)[i] = inSC;
-            if (i < s.numGO) at<uint32_t>(lds, L.inGO)[i] = inGO;
-            if (i < nIOc) at<int32_t>(lds, L.inTm)[i] = inTm;
-            if (i < nRng) s_rng[i] = r;
-            if (i < nNext) s_next[i] = rn;
-            if (i < nPop) s_pop[i] = pv;
+            if (MZ && i < L.idx16) bd[i] = bl;
+            if (MZ && i < nPCc) at<float>(lds, L.inPF)[i] = inPF;
+            if (MZ && i < s.numBC) at<uint32_t>(lds, L.inBC)[i] = inBC;
+            if (IN && i < s.numSC) at<uint32_t>(lds, L.inSC)[i] = inSC;
+            if (IN && i < s.numGO) at<uint32_t>(lds, L.inGO)[i] = inGO;
+            if (MZ && i < nIOc) at<int32_t>(lds, L.inTm)[i] = inTm;
+            if (PART == kPartMz && i < s.numSC) s_apSC[i] = scPrev;
+            if (MZ && i < nRng) s_rng[i] = r;
+            if (MZ && i < nNext) s_next[i] = rn;
+            if (MZ && i < nPop) s_pop[i] = pv;
             if (i < s.numMF) {
                 const uint8_t ap = m > 0;
-                s_hmf[i] = h || ap;
-                apBufMF[i] = (b << 1) | (ap * 0x00000001u);
-                if (ap) {
+                if constexpr (MZ) s_hmf[i] = h || ap;
+                if constexpr (IN) apBufMF[i] = (b << 1) | (ap * 0x00000001u);
+                if (IN && ap) {
                     const G<u32x4> *row = (const G<u32x4> *)(a.mfgoOut + (size_t)i * a.mfgoPad);
                     const uint32_t nv = a.mfgoPad / 8;
                     for (uint32_t c0 = 0; c0 < nv; c0 += 4) {
@@ -870,10 +881,13 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
             }
         }
     }
+    // (microzone part) the SC->PC routing of the previous step reads the
+    // staged index lists and SC spikes
+    if constexpr (PART == kPartMz) __syncthreads();
     // synapses: SC->PC of calcPCActivities (mzone.cpp:201-207), MF->NC and
     // PC->NC of calcNCActivities (mzone.cpp:358-383), NC->IO of
     // calcIOActivities (mzone.cpp:272-280)
-    {
+    if constexpr (MZ) {
         auto gSC = SF<LF, float>(a, CBM_F_G_SC_PC, lds);
         auto inSC = SF<LF, uint8_t>(a, CBM_F_INPUT_SC_PC, lds);
         auto inMF = SF<LF, uint8_t>(a, CBM_F_INPUT_MF_NC, lds);
@@ -897,7 +911,9 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
             const uint32_t jP = rd * nThr + ((tid - L.oSynPC) & (nThr - 1)), jI = rd * nThr + ((tid - L.oSynIO) & (nThr - 1));
             const uint32_t qs = min(jS, nS - 1), qm = min(jM, nM - 1), qp = min(jP, nP - 1), qi = min(jI, nI - 1);
             float g1 = gSC[qs];
-            const uint8_t i1 = inSC[qs];
+            // (microzone part: MZone::updateSCPCOut (mzone.cpp:467-477) of the
+            // previous step, here)
+            const uint8_t i1 = (PART == kPartMz && st.scFromPrev) ? s_apSC[s_idx[L.iSCPC + qs]] : inSC[qs];
             const uint8_t i2 = inMF[qm];
             const float w = wMF[qm];
             float mN = gmN[qm], mA = gmA[qm], nn = gN[qm], aa = gA[qm];
@@ -910,6 +926,7 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
                 const uint32_t q = jS;
                 g1 = g1 + p.gIncSCtoPC * (1 - g1) * i1;
                 g1 = g1 * p.gDecSCtoPC;
+                if (PART == kPartMz && st.scFromPrev) inSC[q] = i1;
                 gSC[q] = g1;
                 s_syn[L.synPC + q] = g1;
             }
@@ -945,7 +962,7 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
 
     // ======================= phase 1
     // MZone::updateMFNCOut (mzone.cpp:514-524): the caller's raw MF bytes
-    {
+    if constexpr (MZ) {
         auto inMF = SF<LF, uint8_t>(a, CBM_F_INPUT_MF_NC, lds);
         for (uint32_t q = tid; q < nZ * s.numNC * kMF; q += nThr) inMF[q] = s_mf[s_idx[L.iMFNC + q]];
     }
@@ -993,7 +1010,7 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
             const float ncR = SF<LF, float>(a, CBM_F_SYN_IO_PRELEASE_NC, lds)[iN];
             const uint32_t ncAb = SF<LF, uint32_t>(a, CBM_F_APBUF_NC, lds)[iN];
             // ---- PC
-            if (hPC) {
+            if (MZ && hPC) {
                 const uint32_t i = iP, z = i >> L.l2PC;
                 float gPF = pcGPF, gBC = pcGBC, vv = pcV, th = pcTh;
                 const float gSCPCSum = lds_sum_seq(s_syn + L.synPC + i * kSC, kSC, 0.0f);
@@ -1015,7 +1032,7 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
                 if (ap) atomicAdd(&s_pop[z * L.popStride + 2], 1u);
             }
             // ---- BC
-            if (hBC) {
+            if (MZ && hBC) {
                 const uint32_t i = iB;
                 float gPF = bcGPF, gPC = bcGPC, vv = bcV, th = bcTh;
                 gPF = gPF + (bcPF * p.gIncGRtoBC);
@@ -1034,7 +1051,7 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
                 SF<LF, float>(a, CBM_F_THRESH_BC, lds)[i] = ap * p.threshMaxBC + (!ap) * (th);
             }
             // ---- SC
-            if (hSC) {
+            if (IN && hSC) {
                 const uint32_t i = iS;
                 float gPF = scGPF, vv = scV, th = scTh;
                 gPF = gPF + (scPF * p.gIncGRtoSC);
@@ -1050,7 +1067,7 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
                 SF<LF, float>(a, CBM_F_THRESH_SC, lds)[i] = ap * p.threshMaxSC + (!ap) * (th);
             }
             // ---- IO
-            if (hIO) {
+            if (MZ && hIO) {
                 const uint32_t i = iI, z = i >> L.l2IO;
                 float vv = ioV;
                 float gNCSum = lds_sum_seq(s_syn + L.synIO + i * kNCIO, kNCIO, 0.0f);
@@ -1067,7 +1084,7 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
                 xstg<X>(gp(a.timer2) + (size_t)par * nIOc + i, (int32_t)((!ap) * (ioTm + 1) + ap * p.tsLTDStartAPIO));
             }
             // ---- NC
-            if (hNC) {
+            if (MZ && hNC) {
                 const uint32_t i = iN;
                 float vv = ncV, th = ncTh, r = ncR;
                 const float ms = p.msPerTimeStep;
@@ -1092,7 +1109,7 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
                 s_rel[i] = r;
             }
         }
-        for (uint32_t i = tid; i < nGOc; i += nThr) {
+        for (uint32_t i = tid; IN && i < nGOc; i += nThr) {
             float gMF = 0, gGR = 0, gGO = 0, glu = 0, mG = 0, gInc = 0, gMG = 0, thv = 0, v = 0, vC = 0, sc = 0, inDep = 0;
             uint32_t totalGRIn = 0, inMF = 0, goAb = 0;
             {
@@ -1148,7 +1165,7 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
     // MF->GO counts; InNet::updateGOtoGOOut (innet.cpp:937-950): each GO
     // gathers its GABA inputs in increasing source order, which is the order
     // the reference adds them
-    {
+    if constexpr (IN) {
         auto inMFGO = SF<LF, uint32_t>(a, CBM_F_INPUT_MF_GO, lds);
         auto inGO = SF<LF, uint32_t>(a, CBM_F_INPUT_GO_GO, lds);
         auto inDep = SF<LF, float>(a, CBM_F_INPUT_GOGABA_SYNDEP_GO, lds);
@@ -1175,7 +1192,7 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
     // GO electrical coupling (innet.cpp:961-978): vCoupleGO is a Jacobi sweep;
     // the threshold update is in place in cell order, so it runs on one lane.
     // Without coupling inputs (or with a zero ratio) both terms are zero.
-    {
+    if constexpr (IN) {
         auto vC = SF<LF, float>(a, CBM_F_VCOUPLE_GO, lds);
         if (a.anyCoupling) {
             const uint32_t kc = s.maxnumpGOCoupInGOGO;
@@ -1197,10 +1214,13 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
             for (uint32_t i = tid; i < s.numGO; i += nThr) vC[i] = 0.0f;
         }
     }
+    // (input-layer part) this step's SC spikes for the microzone part
+    if constexpr (PART == kPartIn)
+        for (uint32_t i = tid; i < s.numSC; i += nThr) xstg<X>(gp(st.apSCout) + i, s_apSC[i]);
     // microzone outputs: MZone::updateBCPCOut (mzone.cpp:449-465),
     // updateSCPCOut (mzone.cpp:467-477), updatePCOut (mzone.cpp:409-447) and
     // the IO coupling of updateIOOut (mzone.cpp:486-489), from LDS
-    {
+    if constexpr (MZ) {
         auto inBCPC = SF<LF, uint32_t>(a, CBM_F_INPUT_BC_PC, lds);
         for (uint32_t i = (tid - L.oRBCPC) & (nThr - 1); i < nZ * s.numPC; i += nThr) {
             const uint32_t z = i >> L.l2PC, pc = i & (s.numPC - 1);
@@ -1222,9 +1242,11 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
             }
             inBCPC[i] = c;
         }
-        auto inSC = SF<LF, uint8_t>(a, CBM_F_INPUT_SC_PC, lds);
-        for (uint32_t q = (tid - L.oRSCPC) & (nThr - 1); q < nZ * s.numPC * kSC; q += nThr)
-            inSC[q] = s_apSC[s_idx[L.iSCPC + q]];
+        if constexpr (PART == kPartAll) {  // (microzone part: at the next step's start)
+            auto inSC = SF<LF, uint8_t>(a, CBM_F_INPUT_SC_PC, lds);
+            for (uint32_t q = (tid - L.oRSCPC) & (nThr - 1); q < nZ * s.numPC * kSC; q += nThr)
+                inSC[q] = s_apSC[s_idx[L.iSCPC + q]];
+        }
         auto inPCBC = SF<LF, uint32_t>(a, CBM_F_INPUT_PC_BC, lds);
         for (uint32_t i = (tid - L.oRPCBC) & (nThr - 1); i < nZ * s.numBC; i += nThr) {
             const uint32_t z = i >> L.l2BC, bc = i & (s.numBC - 1);
@@ -1258,7 +1280,7 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
         }
     }
     // MZone::updateMFNCSyn population histogram (mzone.cpp:536-580), one lane per zone
-    if (tid < (int)nZ) {
+    if (MZ && tid < (int)nZ) {
         const uint32_t z = tid;
         uint32_t *ctrs = s_pop + z * L.popStride, *hist = ctrs + 3, *flags = ctrs + 3 + bins;
         int doLTD = 0, doLTP = 0;
@@ -1298,6 +1320,8 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
             uint8_t *row = a.rec + (size_t)(st.t % a.recCap) * a.recRow;
             for (int k = 0; k < 6; k++) {
                 if (a.recOff[k] == 0xFFFFFFFFu) continue;
+                // (a part records its own populations: SC and GO the input layer)
+                if (PART != kPartAll && (PART == kPartIn) != (k == CBM_REC_SC || k == CBM_REC_GO)) continue;
                 for (uint32_t i = tid; i < cnt[k]; i += nThr) gp(row)[a.recOff[k] + i] = src[k][i];
             }
         }
@@ -1309,9 +1333,9 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
         }
     }
     // the ring slot of step t+1 is the slot PC/BC/SC read above (t-2): clear it now
-    for (uint32_t i = tid; i < s.numBC; i += nThr) xstg<X>(gp(a.bcRing) + (size_t)st.slotNext * s.numBC + i, 0u);
-    for (uint32_t i = tid; i < s.numSC; i += nThr) xstg<X>(gp(a.scRing) + (size_t)st.slotNext * s.numSC + i, 0u);
-    for (uint32_t i = tid; i < s.numGO; i += nThr) xstg<X>(gp(a.goRing) + (size_t)st.slotNext * s.numGO + i, 0u);
+    for (uint32_t i = tid; MZ && i < s.numBC; i += nThr) xstg<X>(gp(a.bcRing) + (size_t)st.slotNext * s.numBC + i, 0u);
+    for (uint32_t i = tid; IN && i < s.numSC; i += nThr) xstg<X>(gp(a.scRing) + (size_t)st.slotNext * s.numSC + i, 0u);
+    for (uint32_t i = tid; IN && i < s.numGO; i += nThr) xstg<X>(gp(a.goRing) + (size_t)st.slotNext * s.numGO + i, 0u);
     __syncthreads();
 
     // ======================= phase 3
@@ -1321,7 +1345,7 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
     // regeneration are taken at once, one per lane.  A regeneration takes the
     // block prebuilt by an earlier launch when its tag says it follows the
     // current one, and computes it here otherwise.
-    if (wave == 0) {
+    if (MZ && wave == 0) {
         auto inNC = SF<LF, uint8_t>(a, CBM_F_INPUT_NC_IO, lds);
         const uint32_t n = s.numIO * kNCIO;
         const uint32_t *s_next = at<uint32_t>(lds, L.rngNext);
@@ -1366,7 +1390,7 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
     }
     // GO->GO synaptic depression scale (innet.cpp:952-954), after the GABA
     // gather above has used the previous scale
-    {
+    if constexpr (IN) {
         auto scale = SF<LF, float>(a, CBM_F_GO_GABA_OUT_SYNSCALE_GOGO, lds);
         const float rec = p.goGABAGOGOSynRec, depF = p.goGABAGOGOSynDepF;
         for (uint32_t i = tid; i < s.numGO; i += nThr) {
@@ -1376,7 +1400,7 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
         }
     }
     // MZone::updateMFNCSyn weights (mzone.cpp:586-600)
-    if (st.popHist) {
+    if (MZ && st.popHist) {
         const uint32_t per = s.numNC * kMF;
         auto w = SF<LF, float>(a, CBM_F_MF_SYNW_NC, lds);
         for (uint32_t q = tid; q < nZ * per; q += nThr) {
@@ -1390,8 +1414,9 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
             w[q] = wv;
         }
     }
-    // MF history, cleared every numTSinMFHist steps (innet.cpp:982-995)
-    {
+    // MF history, cleared every numTSinMFHist steps (innet.cpp:982-995), kept
+    // by the microzone part (MF->NC plasticity reads it)
+    if constexpr (MZ) {
         auto histMF = SF<LF, uint8_t>(a, CBM_F_HIST_MF, lds);
         for (uint32_t i = tid; i < s.numMF; i += nThr) histMF[i] = st.resetMFHist ? 0 : s_hmf[i];
     }
@@ -1631,11 +1656,13 @@ __device__ __forceinline__ void res_fold(const float *src, float *dst, uint32_t 
     }
 }
 
-// The small populations' state fields between global memory and the small
-// block's LDS (in: into LDS), 16 bytes at a time and the tail byte by byte.
-__device__ __forceinline__ void res_fields(const SmallArgs &a, uint8_t *lds, bool in, uint32_t tid)
+// The small populations' state fields of copy-list entries [e0, e1) between
+// global memory and the block's LDS (in: into LDS), 16 bytes at a time and
+// the tail byte by byte.
+__device__ __forceinline__ void res_fields(const SmallArgs &a, uint8_t *lds, bool in, uint32_t tid, uint32_t e0,
+                                           uint32_t e1)
 {
-    for (uint32_t e = 0; e < a.nLdsFields; e++) {
+    for (uint32_t e = e0; e < e1; e++) {
         const int f = (int)a.ldsField[e];
         uint8_t *g = reinterpret_cast<uint8_t *>(a.f[f]);
         uint8_t *l = lds + a.fLds[f];
@@ -1651,22 +1678,30 @@ __device__ __forceinline__ void res_fields(const SmallArgs &a, uint8_t *lds, boo
     }
 }
 
-// The small populations' part of a resident run (block 0): their state
-// fields held in LDS from the first step to the last.
-__device__ __forceinline__ void res_small(const SmallArgs &a, const ResRun &r, float *s_err, uint8_t *lds)
+// One part of the small populations' step in a resident run (PART: the
+// input layer on block 0, the microzones on block spans + 1), its state
+// fields held in the block's LDS from the first step to the last.
+template <int PART>
+__device__ __forceinline__ void res_part(const SmallArgs &a, const ResRun &r, float *s_err, uint8_t *lds)
 {
-    const uint32_t nZ = a.s.numZones;
-    res_fields(a, lds, true, threadIdx.x);
+    constexpr bool IN = PART == kPartIn;
+    const uint32_t nZ = a.s.numZones, nSC = a.s.numSC;
+    const uint32_t e0 = IN ? 0u : a.nInFields, e1 = IN ? a.nInFields : a.nLdsFields;
+    res_fields(a, lds, true, threadIdx.x, e0, e1);
     __syncthreads();
-    // the relative error drive of the next step, loaded a step ahead (thread z: zone z)
+    // the relative error drive of the next step, loaded a step ahead (thread
+    // z: zone z; the microzone part's IO reads it)
     auto errOf = [&](uint32_t k) {
-        return (r.err && threadIdx.x < nZ && k < r.n) ? r.err[(size_t)k * nZ + threadIdx.x] : 0.0f;
+        return (!IN && r.err && threadIdx.x < nZ && k < r.n) ? r.err[(size_t)k * nZ + threadIdx.x] : 0.0f;
     };
     float errNext = errOf(0);
     for (uint32_t k = 0; k < r.n; k++) {
-        // every GR block done with step k - 1 (the generator block was, before
-        // step k - 1 was published)
+        // every GR block done with step k - 1 (and for the microzone part the
+        // input-layer part, whose SC spikes of step k - 1 it routes; the
+        // generator block was done with step k - 1 before the microzone part
+        // published it)
         if (!res_wait<kBarGroups, false>(r.bar, r.barBase + k * r.spans, r.status)) return;
+        if (!IN && !res_wait<1, false>(r.flagIn, r.flagInBase + k, r.status)) return;
         const uint64_t t = r.t0 + k + 1;
         SmallStep st;
         st.mf = r.mf + (size_t)k * r.pitch;
@@ -1689,6 +1724,11 @@ __device__ __forceinline__ void res_small(const SmallArgs &a, const ResRun &r, f
         st.slotNext = sreg((uint32_t)((t + 1) % kRing));
         st.popHist = sreg((t % a.p.tsPerPopHistBinPC) == 0);
         st.resetMFHist = sreg((t % a.p.numTSinMFHist) == 0);
+        // the SC spikes of step t (written) and t-1 (read), in three slots:
+        // the input-layer part can be a step ahead of the microzone part
+        st.apSCout = r.apSC3 + (size_t)sreg((uint32_t)(t % 3)) * nSC;
+        st.apSCin = r.apSC3 + (size_t)sreg((uint32_t)((t + 2) % 3)) * nSC;
+        st.scFromPrev = k > 0;
         __syncthreads();
         // the argument block through a pointer the compiler cannot follow
         // across steps (its fields are loaded again in each step instead of
@@ -1698,15 +1738,34 @@ __device__ __forceinline__ void res_small(const SmallArgs &a, const ResRun &r, f
         uint64_t pa = reinterpret_cast<uint64_t>(&a);
         asm volatile("" : "+s"(pa));
         const SmallArgs *ap = (const SmallArgs *)(const __attribute__((address_space(4))) SmallArgs *)pa;
-        small_step<true, true>(*ap, st, s_err, lds);
-        errNext = errOf(k + 1);
-        // the generator block done with step k (its fold read the buffer the
-        // GR blocks' step k + 1 writes)
-        if (!res_wait<1, false>(r.pb, r.pbBase + k + 1, r.status)) return;
-        res_publish(r.flag, r.flagBase + k + 1);
+        small_step<true, true, PART>(*ap, st, s_err, lds);
+        if constexpr (IN) {
+            res_publish(r.flagIn, r.flagInBase + k + 1);
+        } else {
+            errNext = errOf(k + 1);
+            // the generator block done with step k (its fold read the buffer
+            // the GR blocks' step k + 1 writes)
+            if (!res_wait<1, false>(r.pb, r.pbBase + k + 1, r.status)) return;
+            res_publish(r.flag, r.flagBase + k + 1);
+        }
+    }
+    if constexpr (!IN) {
+        // the SC->PC routing of the last step (MZone::updateSCPCOut), which
+        // the microzone part does at the start of the next step
+        if (r.n > 0) {
+            if (!res_wait<1, false>(r.flagIn, r.flagInBase + r.n, r.status)) return;
+            const G<uint8_t> *sc = gp(r.apSC3) + (size_t)((r.t0 + r.n) % 3) * nSC;
+            uint8_t *s_apSC = lds + a.lds.apSC;
+            for (uint32_t i = threadIdx.x; i < nSC; i += kSmallThreads) s_apSC[i] = xldg<true>(sc + i);
+            __syncthreads();
+            auto inSC = SF<true, uint8_t>(a, CBM_F_INPUT_SC_PC, lds);
+            const uint16_t *s_idx = at<uint16_t>(lds, a.lds.idx);
+            for (uint32_t q = threadIdx.x; q < nZ * a.s.numPC * a.s.numpPCfromSCtoPC; q += kSmallThreads)
+                inSC[q] = s_apSC[s_idx[a.lds.iSCPC + q]];
+        }
     }
     __syncthreads();
-    res_fields(a, lds, false, threadIdx.x);
+    res_fields(a, lds, false, threadIdx.x, e0, e1);
 }
 
 // The generators' next blocks and the PF->PC fold in a resident run (the
@@ -1715,8 +1774,8 @@ __device__ __forceinline__ void res_prebuild(const SmallArgs &a, const ResRun &r
 {
     const uint32_t nZ = a.s.numZones, nC = r.g.numGR >> 6;
     for (uint32_t k = 0; k < r.n; k++) {
-        // the small populations done with step k - 1 (the generator state),
-        // every GR block done with step k - 1 (the chunk sums)
+        // the microzone part done with step k - 1 (the generator state), every
+        // GR block done with step k - 1 (the chunk sums)
         if (!res_wait<1, true>(r.flag + (blockIdx.x % kFlagCopies) * kBarStride, r.flagBase + k, r.status)) return;
         if (!res_wait<kBarGroups, true>(r.bar, r.barBase + k * r.spans, r.status)) return;
         const uint64_t t = r.t0 + k + 1;
@@ -1762,8 +1821,10 @@ __device__ __forceinline__ void res_gr(const SmallArgs &a, const ResRun &r, cons
         }
     }
     for (uint32_t k = 0; k < r.n; k++) {
-        // the small populations done with step k - 1
-        if (!res_wait<1, true>(r.flag + (blockIdx.x % kFlagCopies) * kBarStride, r.flagBase + k, r.status)) return;
+        // both parts of the small populations done with step k - 1
+        const uint32_t cp = (blockIdx.x % kFlagCopies) * kBarStride;
+        if (!res_wait<1, true>(r.flagIn + cp, r.flagInBase + k, r.status)) return;
+        if (!res_wait<1, true>(r.flag + cp, r.flagBase + k, r.status)) return;
         const uint64_t t = r.t0 + k + 1;
         const uint32_t par = (uint32_t)(t & 1u), parPrev = par ^ 1u;
         const uint32_t slot = (uint32_t)(t % kRing);
@@ -1773,7 +1834,7 @@ __device__ __forceinline__ void res_gr(const SmallArgs &a, const ResRun &r, cons
         gs.mf = (sptr(r.mf + (size_t)k * r.pitch));
         gs.apGOprev = (sptr(a.apGO2 + (size_t)parPrev * g.numGO));
         gs.chunk = (sptr(chunk2 + (size_t)par * nZ * nC));
-        gs.foldSrc = nullptr;  // the small block folds (res_small)
+        gs.foldSrc = nullptr;  // the generator block folds (res_prebuild)
         gs.foldDst = nullptr;
         gs.bcCnt = (sptr(a.bcRing + (size_t)slot * a.s.numBC));
         gs.scCnt = (sptr(a.scRing + (size_t)slot * a.s.numSC));
@@ -1805,23 +1866,27 @@ __device__ __forceinline__ void res_gr(const SmallArgs &a, const ResRun &r, cons
     }
 }
 
-// n steps of one network in one cooperative launch of spans + 2 blocks of
-// 1024 threads, one per CU: block 0 steps the small populations, blocks
-// 1 .. spans the GR cells of their even split (chunks [r*C/spans,
-// (r+1)*C/spans) of the C 64-GR chunks, as step_span_kernel), whose state
-// stays in registers (GrRegs) from the first step to the last and is stored
-// back at the end, and the last block prebuilds the zone generators' next
-// blocks.  Each step does the work of one step_span_kernel launch (the small
-// block also folds the PF->PC chunk sums), and the waits on step counters and
-// flags stand in for the launch boundary.  CPT: cells per thread
-// (every span holds at most 1024 * CPT cells); compact connectivity only.
+// n steps of one network in one cooperative launch of spans + 3 blocks of
+// 1024 threads, one per CU: block 0 steps the input-layer small populations
+// (GO, SC), block spans + 1 the microzones (PC, BC, IO, NC), each holding its
+// populations' state in LDS, blocks 1 .. spans the GR cells of their even
+// split (chunks [r*C/spans, (r+1)*C/spans) of the C 64-GR chunks, as
+// step_span_kernel), whose state stays in registers (GrRegs) from the first
+// step to the last and is stored back at the end, and block spans + 2 folds
+// the PF->PC chunk sums and prebuilds the zone generators' next blocks.
+// Each step does the work of one step_span_kernel launch, and the waits on
+// step counters and flags stand in for the launch boundary.  CPT: cells per
+// thread (every span holds at most 1024 * CPT cells); compact connectivity
+// only.
 template <int CPT>
 __global__ void __launch_bounds__(kSmallThreads) step_resident_kernel(const SmallArgs *__restrict__ args, const ResRun r)
 {
     extern __shared__ __align__(16) uint8_t lds[];
     __shared__ float s_err[kMaxZones];  // in LDS: the small step indexes it by zone
     if (blockIdx.x == 0)
-        res_small(*args, r, s_err, lds);
+        res_part<kPartIn>(*args, r, s_err, lds);
+    else if (blockIdx.x == r.spans + 1)
+        res_part<kPartMz>(*args, r, s_err, lds);
     else if (blockIdx.x > r.spans)
         res_prebuild(*args, r, lds);
     else
@@ -1876,6 +1941,9 @@ __device__ __forceinline__ void batch_small(const BatchNet &d, uint32_t kk, floa
     st.t = t;
     st.ioOut = nullptr;
     st.ioPop = 0;
+    st.apSCout = nullptr;  // (the whole step on one block)
+    st.apSCin = nullptr;
+    st.scFromPrev = 0;
     if (threadIdx.x < (uint32_t)kMaxZones) {
         const uint32_t z = threadIdx.x;
         float e = kk == 0 ? d.errDrive0[z] : 0.0f;

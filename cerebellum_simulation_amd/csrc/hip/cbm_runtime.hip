@@ -552,48 +552,60 @@ static int setupResident(cbm_handle *h, int cus)
     const cbm_sizes &s = h->s;
     const int sm = (int)h->o.stepMode;
     h->resident = false;
-    if ((sm == 0 || sm == 5) && h->spans && h->conn) {
+    // the GR cells split evenly over one block per CU but three (the two
+    // parts of the small populations and the generator block)
+    const uint32_t spans = cus > 4 ? (uint32_t)cus - 3 : 0u;
+    const uint32_t per = spans ? (s.numGR / 64 + spans - 1) / spans * 64 : 0u;
+    const int cpt = per == 0 ? 0 : per <= 2048 ? 2 : per <= 3072 ? 3 : per <= 5120 ? 5 : 0;
+    if ((sm == 0 || sm == 5) && cpt && h->conn) {
         int coop = 0;
         HIP_TRY(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, h->o.device));
-        const uint32_t recOff = roundUp(grTileLds(1024, h->spanCpt, s.numGO, s.numMF, s.numZones, s.numIO), 16);
+        const uint32_t recOff = roundUp(grTileLds(1024, cpt, s.numGO, s.numMF, s.numZones, s.numIO), 16);
+        // each part's fields after the step's scratch LDS in its own block
         SmallArgs &sa = h->smallHost;
-        uint32_t off = roundUp((uint32_t)h->smallLds, 16);
+        const uint32_t base = roundUp((uint32_t)h->smallLds, 16);
+        uint32_t off = base, top = base;
         for (int e = 0; e < kSmallFieldCount; e++) {
+            if (e == kSmallInFields) off = base;
             const int f = kSmallFields[e];
             const size_t bytes = h->zoneBytes(f) * (size_t)h->zonesOf(f);
             sa.fLds[f] = off;
             sa.ldsField[e] = (uint32_t)f;
             sa.ldsBytes[e] = (uint32_t)bytes;
             off = roundUp(off + (uint32_t)bytes, 16);
+            top = std::max(top, off);
         }
-        const size_t lds = std::max<size_t>(off, recOff + (size_t)h->spanCpt * 1024 * 16);
+        const size_t lds = std::max<size_t>(top, recOff + (size_t)cpt * 1024 * 16);
         if (coop && lds <= kSmallLdsMax) {
-            const void *k = residentKernel(h->spanCpt);
+            const void *k = residentKernel(cpt);
             HIP_TRY(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-            int per = 0;
-            HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k, kSmallThreads, lds));
-            if (per >= 1 && (int64_t)per * cus >= (int64_t)h->spans + 2) {
+            int perCU = 0;
+            HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, k, kSmallThreads, lds));
+            if (perCU >= 1 && (int64_t)perCU * cus >= (int64_t)spans + 3) {
                 int rc;
-                if ((rc = h->dalloc((void **)&h->resBar, (size_t)(kBarGroups + kFlagCopies + 1) * kBarStride * 4)))
+                if ((rc = h->dalloc((void **)&h->resBar,
+                                    (size_t)(kBarGroups + 2 * kFlagCopies + 1) * kBarStride * 4)))
                     return rc;
+                if ((rc = h->dalloc((void **)&h->resApSC3, 3 * (size_t)s.numSC))) return rc;
                 if ((rc = h->dalloc((void **)&h->resStatus, 4))) return rc;
                 if (!h->resStatusHost) {
                     HIP_TRY(hipHostMalloc((void **)&h->resStatusHost, 4, hipHostMallocDefault));
                     *h->resStatusHost = 0;
                 }
-                h->resBarBase = 0;
-                h->resFlagBase = 0;
-                h->resPbBase = 0;
+                h->resBarBase = h->resFlagBase = h->resFlagInBase = h->resPbBase = 0;
+                h->resSpans = spans;
+                h->resCpt = cpt;
                 h->residentLds = lds;
                 h->residentRecLds = recOff;
                 h->resident = true;
                 sa.nLdsFields = kSmallFieldCount;
+                sa.nInFields = kSmallInFields;
             }
         }
-        if (!h->resident) {
-            for (int f = 0; f < CBM_F_COUNT; f++) sa.fLds[f] = kNotInLds;
-            sa.nLdsFields = 0;
-        }
+    }
+    if (!h->resident) {
+        for (int f = 0; f < CBM_F_COUNT; f++) h->smallHost.fLds[f] = kNotInLds;
+        h->smallHost.nLdsFields = h->smallHost.nInFields = 0;
     }
     if (sm == 5 && !h->resident)
         return fail(CBM_E_UNSUPPORTED, "stepMode 5: resident runs need the even GR split, compact connectivity "
@@ -1256,6 +1268,9 @@ static int launchStep(cbm_handle *h, const uint8_t *mf, int timing)
     st.resetMFHist = (t % p.numTSinMFHist) == 0;
     st.ioOut = h->ioOut;
     st.ioPop = h->ioPop;
+    st.apSCout = nullptr;  // (the whole step on one block)
+    st.apSCin = nullptr;
+    st.scFromPrev = 0;
     h->skipFold = false;
 
     if (timing == kSplit) {
@@ -1404,15 +1419,17 @@ static int runResident(cbm_handle *h, const uint8_t *mf_dev, size_t pitch, int n
     r.pitch = pitch;
     r.maxExtIncVIO = h->p.maxExtIncVIO;
     r.gIncGRtoGO = h->gIncGRtoGO;
-    r.spans = h->spans;
+    r.spans = h->resSpans;
     r.recOffPF = (h->rec && h->recOffPF != 0xFFFFFFFFu) ? h->recOffPF : 0xFFFFFFFFu;
     r.bar = h->resBar;
     r.flag = h->resBar + kBarGroups * kBarStride;
-    r.pb = h->resBar + (kBarGroups + kFlagCopies) * kBarStride;
+    r.flagIn = h->resBar + (kBarGroups + kFlagCopies) * kBarStride;
+    r.pb = h->resBar + (kBarGroups + 2 * kFlagCopies) * kBarStride;
+    r.apSC3 = h->resApSC3;
     r.status = h->resStatus;
     r.recLds = h->residentRecLds;
-    const uint32_t grid = h->spans + 2;
-    const void *kern = residentKernel(h->spanCpt);
+    const uint32_t grid = h->resSpans + 3;
+    const void *kern = residentKernel(h->resCpt);
     const SmallArgs *sa = h->smallArgs;
     for (int k0 = 0; k0 < n;) {
         const int m = std::min(n - k0, kResMaxSteps);
@@ -1424,11 +1441,13 @@ static int runResident(cbm_handle *h, const uint8_t *mf_dev, size_t pitch, int n
         r.n = (uint32_t)m;
         r.barBase = h->resBarBase;
         r.flagBase = h->resFlagBase;
+        r.flagInBase = h->resFlagInBase;
         r.pbBase = h->resPbBase;
         void *args[] = {&sa, &r};
         HIP_TRY(hipLaunchCooperativeKernel(kern, dim3(grid), dim3(kSmallThreads), args, h->residentLds, h->stream));
-        h->resBarBase += (uint32_t)m * h->spans;  // each GR block counts each step
+        h->resBarBase += (uint32_t)m * h->resSpans;  // each GR block counts each step
         h->resFlagBase += (uint32_t)m;
+        h->resFlagInBase += (uint32_t)m;
         h->resPbBase += (uint32_t)m;
         h->curTime += (uint64_t)m;
         for (int z = 0; z < kMaxZones; z++) h->errDrive[z] = 0.0f;  // used by the first step
