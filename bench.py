@@ -11,16 +11,21 @@ The timed region also carries the outputs a driver reads each ms: the
 device spike recorder stores every step's PC, NC and IO spikes, and the
 recorded rows are copied to the host before the region ends.
 
-roofline: the step's dominant kernel -- the fused step kernel (one launch
-per step: the GR tiles plus one block for the small populations), or the GR
-kernel when the two-stream step is used -- with the GR algorithmic bytes per
-launch (numGR x 104 B at fan-ins (4,3,2), 120 B at (4,3,4); the compulsory
-per-GR bytes with the reference's data types) divided by its mean duration,
-timed with HIP events on the network's own stream around the launches of the
-timed region, against MI355X's 8 TB/s.  traffic is the HBM bytes per launch
-from the FETCH_SIZE / WRITE_SIZE counters (profiles/traffic_<config>.json),
-and traffic_frac the bandwidth those bytes give over the same duration.  The
-GR and small-population kernels launched one after the other are timed too
+roofline: the step's dominant kernel -- the resident kernel when the timed
+run is resident (the whole run in one launch: GR state in registers, small
+populations in LDS), else the fused step kernel (one launch per step: the GR
+tiles plus one block for the small populations), or the GR kernel when the
+two-stream step is used -- with the GR algorithmic bytes per step (numGR x
+104 B at fan-ins (4,3,2), 120 B at (4,3,4); the compulsory per-GR bytes with
+the reference's data types) divided by its mean duration per step, timed
+with HIP events on the network's own stream around the launches of the timed
+region, against MI355X's 8 TB/s.  A resident run does not stream those bytes
+each step, so its frac can pass 1; per_step_launch gives the fused step
+launched once per step on the same roofline.  traffic is the HBM bytes per
+launch (per step inside a resident run) from the FETCH_SIZE / WRITE_SIZE
+counters (profiles/traffic_<config>[_resident].json), and traffic_frac the
+bandwidth those bytes give over the same duration.  The GR and
+small-population kernels launched one after the other are timed too
 (gr_only_us, small_only_us).
 
 cpu_baseline: the CPU oracle (a C restatement of the same step), timed on
@@ -132,7 +137,8 @@ def batch_bench(which: str, steps: int) -> dict:
     t_build = time.perf_counter()
     nets, trains = [], []
     warm, plain_steps = 10, max(20, steps // 4)
-    total = warm + 2 * 8 + steps + plain_steps
+    graph_steps = 20
+    total = warm + 2 * 8 + graph_steps + steps + plain_steps
     for i, (cls, seed, inh) in enumerate(plan):
         st = configs.build_state(cls, seed=seed, gogo_inhibition=inh)
         nets.append(Network.from_state(st))
@@ -140,11 +146,12 @@ def batch_bench(which: str, steps: int) -> dict:
     gr_total = sum(n.sizes.numGR for n in nets)
     print(f"[bench] batch of {len(nets)} networks ({gr_total} GR) built in {time.perf_counter() - t_build:.1f} s",
           file=sys.stderr, flush=True)
-    graph_steps = 20
     b = Batch(nets, graph_steps=graph_steps)
     b.run_mf_train([t[:warm] for t in trains])
     cpt = b.autotune([t[warm:warm + 8] for t in trains], steps=8)
-    k0 = warm + 8
+    # one window of warm-up steps captures the step graph for the chosen shape
+    b.run_mf_train([t[warm + 8:warm + 8 + graph_steps] for t in trains])
+    k0 = warm + 8 + graph_steps
     b.sync()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -389,11 +396,12 @@ def main():
                                    "achieved": round(alg_bytes / (k_us_alone * 1e-6) / 1e9, 1),
                                    "frac": round(alg_bytes / (k_us_alone * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)}
     if resident:
-        roofline["note"] = ("the timed steps run as one cooperative launch whose GR blocks keep their cells' state "
-                            "in registers and LDS and whose small-population block keeps its populations in LDS: "
-                            "the per-GR arrays are not streamed from HBM each step, so the algorithmic-byte rate "
-                            "is not bounded by the HBM peak; the step is bound by the small-population block's "
-                            "chain of dependent memory round trips and the step-to-step signalling")
+        roofline["note"] = ("the timed steps run as one launch whose GR blocks keep their cells' state "
+                            "in registers and LDS and whose two small-population blocks keep their populations in "
+                            "LDS: the per-GR arrays are not streamed from HBM each step, so the algorithmic-byte "
+                            "rate is not bounded by the HBM peak (traffic: the measured fabric bytes of one step "
+                            "inside such a run); the step is bound by the chain of dependent memory round trips of "
+                            "the small-population blocks and the GR blocks and the signalling between them")
     ceil_us, ceil_bytes = stream_ceiling(s.numGR)
     roofline["stream_ceiling"] = {"us": round(ceil_us, 3), "bytes": ceil_bytes,
                                   "GB/s": round(ceil_bytes / (ceil_us * 1e-6) / 1e9, 1),
@@ -401,7 +409,9 @@ def main():
     traffic_file = os.path.join(ROOT, "profiles", f"traffic_{args.config}{'_resident' if resident else ''}.json")
     if os.path.exists(traffic_file):
         with open(traffic_file) as f:
-            tr_bytes = json.load(f).get("hbm_bytes_per_launch")
+            tr = json.load(f)
+        # a resident run: the bytes of one step inside the run
+        tr_bytes = tr.get("hbm_bytes_per_step") if resident else tr.get("hbm_bytes_per_launch")
         roofline["traffic"] = tr_bytes
         if tr_bytes:
             roofline["traffic_achieved"] = round(tr_bytes / (k_us * 1e-6) / 1e9, 1)

@@ -229,8 +229,7 @@ struct BatchNet {
     uint32_t grgoLag, tsPerHistBinGR, tsPerPopHistBinPC, numTSinMFHist;
 };
 
-// A resident run: n steps of one network in one cooperative launch (one block
-// per CU), the GR cells' state held in the registers of their blocks from the
+// A resident run: n steps of one network in one launch (one block per CU), the GR cells' state held in the registers of their blocks from the
 // first step to the last, step counters and flags between steps.  Step t = t0 + k + 1
 // of run step k takes MF row k; its per-step values are derived on the
 // device as launchStep derives them on the host.

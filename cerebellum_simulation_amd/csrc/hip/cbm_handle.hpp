@@ -57,8 +57,9 @@ struct cbm_handle {
     int spanCpt = 0;         // GR cells per thread of those blocks
     size_t fusedLds = 0;     // dynamic LDS of the fused step kernel
     bool fused() const { return mode != 2; }
-    // resident runs (cbm_run_mf_train of two or more steps): one cooperative
-    // launch of spans + 2 workgroups, GR state in registers, grid barriers
+    // resident runs (cbm_run_mf_train of two or more steps): one launch of
+    // spans + 3 workgroups, one per CU, GR state in registers, step counters
+    // and flags between steps
     bool resident = false;
     size_t residentLds = 0;           // its dynamic LDS (the small step's, or a span's + its records)
     uint32_t residentRecLds = 0;      // byte offset of the GR records in that LDS

@@ -1866,8 +1866,9 @@ __device__ __forceinline__ void res_gr(const SmallArgs &a, const ResRun &r, cons
     }
 }
 
-// n steps of one network in one cooperative launch of spans + 3 blocks of
-// 1024 threads, one per CU: block 0 steps the input-layer small populations
+// n steps of one network in one launch of spans + 3 blocks of 1024 threads,
+// one per CU, all on the device at once (the host launches resident runs of a
+// device one after another and asks for LDS that no two of them fit in): block 0 steps the input-layer small populations
 // (GO, SC), block spans + 1 the microzones (PC, BC, IO, NC), each holding its
 // populations' state in LDS, blocks 1 .. spans the GR cells of their even
 // split (chunks [r*C/spans, (r+1)*C/spans) of the C 64-GR chunks, as

@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -542,11 +543,13 @@ static const void *residentKernel(int cpt)
 
 // Decide at commit whether cbm_run_mf_train runs resident (stepMode 0 or 5;
 // the even split and compact connectivity records; the small populations' state
-// fields fit in LDS beside the small step's scratch; a device that launches
-// cooperatively and holds one 1024-thread workgroup of the kernel on every
-// CU), and set it up: the LDS of the GR workgroups (their span's working LDS
-// and the records of their cells), the fields' place in the small block's
-// LDS (h->smallHost), the step counters and the status word.
+// fields fit in LDS beside the small step's scratch; one 1024-thread workgroup
+// of the kernel fits on every CU), and set it up: the LDS of the GR workgroups
+// (their span's working LDS and the records of their cells), the fields' place
+// in the small block's LDS (h->smallHost), the step counters and the status
+// word.  The launch asks for more than half of a CU's LDS, so that no CU holds
+// two of its workgroups and the grid, one workgroup per CU, is on the device
+// at once when nothing else is.
 static int setupResident(cbm_handle *h, int cus)
 {
     const cbm_sizes &s = h->s;
@@ -558,8 +561,8 @@ static int setupResident(cbm_handle *h, int cus)
     const uint32_t per = spans ? (s.numGR / 64 + spans - 1) / spans * 64 : 0u;
     const int cpt = per == 0 ? 0 : per <= 2048 ? 2 : per <= 3072 ? 3 : per <= 5120 ? 5 : 0;
     if ((sm == 0 || sm == 5) && cpt && h->conn) {
-        int coop = 0;
-        HIP_TRY(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, h->o.device));
+        int ldsPerCU = 0;
+        HIP_TRY(hipDeviceGetAttribute(&ldsPerCU, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, h->o.device));
         const uint32_t recOff = roundUp(grTileLds(1024, cpt, s.numGO, s.numMF, s.numZones, s.numIO), 16);
         // each part's fields after the step's scratch LDS in its own block
         SmallArgs &sa = h->smallHost;
@@ -575,13 +578,14 @@ static int setupResident(cbm_handle *h, int cus)
             off = roundUp(off + (uint32_t)bytes, 16);
             top = std::max(top, off);
         }
-        const size_t lds = std::max<size_t>(top, recOff + (size_t)cpt * 1024 * 16);
-        if (coop && lds <= kSmallLdsMax) {
+        const size_t need = std::max<size_t>(top, recOff + (size_t)cpt * 1024 * 16);
+        const size_t lds = std::max<size_t>(need, roundUp((uint32_t)ldsPerCU / 2 + 16, 16));
+        if (ldsPerCU > 0 && lds <= kSmallLdsMax) {
             const void *k = residentKernel(cpt);
             HIP_TRY(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
             int perCU = 0;
             HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, k, kSmallThreads, lds));
-            if (perCU >= 1 && (int64_t)perCU * cus >= (int64_t)spans + 3) {
+            if (perCU == 1 && cus >= (int)spans + 3) {
                 int rc;
                 if ((rc = h->dalloc((void **)&h->resBar,
                                     (size_t)(kBarGroups + 2 * kFlagCopies + 1) * kBarStride * 4)))
@@ -609,8 +613,8 @@ static int setupResident(cbm_handle *h, int cus)
     }
     if (sm == 5 && !h->resident)
         return fail(CBM_E_UNSUPPORTED, "stepMode 5: resident runs need the even GR split, compact connectivity "
-                                       "records, the small populations' state in LDS and one cooperative "
-                                       "1024-thread workgroup per CU");
+                                       "records, the small populations' state in LDS and one 1024-thread "
+                                       "workgroup per CU");
     return CBM_OK;
 }
 
@@ -1388,6 +1392,31 @@ int cbmdev::checkTrain(cbm_handle *h, const uint8_t *mf_dev, size_t pitch, int n
     return CBM_OK;
 }
 
+// Resident launches of the process go to a device one after another: each
+// waits for the previous one, whichever handle made it, so two runs never
+// hold part of the CUs each while their workgroups wait for the rest.  Other
+// kernels may run beside a resident launch: they only delay the workgroups
+// they keep off the CUs, and a wait that lasts past its bound ends the run
+// with an error.  (A cooperative launch would give the same guarantee, but on
+// this runtime every kernel the process launched after one, on any stream,
+// ran slower: the 256-network batch step by 8 %.)
+static constexpr int kResDevices = 64;
+static std::mutex g_resMu;
+static hipEvent_t g_resLast[kResDevices];
+
+static int launchResident(cbm_handle *h, const void *kern, uint32_t grid, void **args)
+{
+    const int dev = h->o.device;
+    if (dev < 0 || dev >= kResDevices) return fail(CBM_E_UNSUPPORTED, "device ordinal above 63");
+    std::lock_guard<std::mutex> lock(g_resMu);
+    hipEvent_t &last = g_resLast[dev];
+    if (!last) HIP_TRY(hipEventCreateWithFlags(&last, hipEventDisableTiming));
+    else HIP_TRY(hipStreamWaitEvent(h->stream, last, 0));
+    HIP_TRY(hipLaunchKernel(kern, dim3(grid), dim3(kSmallThreads), args, h->residentLds, h->stream));
+    HIP_TRY(hipEventRecord(last, h->stream));
+    return CBM_OK;
+}
+
 // Steps of one resident launch at most: the step counters grow by the grid
 // size per step and are compared modulo 2^32.
 constexpr int kResMaxSteps = 1 << 20;
@@ -1444,7 +1473,8 @@ static int runResident(cbm_handle *h, const uint8_t *mf_dev, size_t pitch, int n
         r.flagInBase = h->resFlagInBase;
         r.pbBase = h->resPbBase;
         void *args[] = {&sa, &r};
-        HIP_TRY(hipLaunchCooperativeKernel(kern, dim3(grid), dim3(kSmallThreads), args, h->residentLds, h->stream));
+        const int rc = launchResident(h, kern, grid, args);
+        if (rc) return rc;
         h->resBarBase += (uint32_t)m * h->resSpans;  // each GR block counts each step
         h->resFlagBase += (uint32_t)m;
         h->resFlagInBase += (uint32_t)m;

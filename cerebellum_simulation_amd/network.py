@@ -175,7 +175,7 @@ class Network:
         1024 x 2 cells, 4 = fused with the GR cells split evenly over the CUs
         (mode 1 picks the even split when each block gets 1,024 to 5,120
         cells), 5 = as 4, with run_mf_train runs of two or more steps
-        resident: one cooperative launch, GR state in registers, see
+        resident: one launch per run, GR state in registers, see
         `resident` (cbm_options.stepMode)."""
         L = _abi.hip_lib()
         o = Options()

@@ -148,12 +148,16 @@ typedef struct cbm_options {
      * always, 4 = fused with the even split whenever it gives each block at
      * most 5,120 cells (else an error), 5 = as 4, and cbm_run_mf_train runs
      * of two or more steps are resident, or cbm_commit fails.  Resident:
-     * the whole run is one cooperative launch of one workgroup per CU whose GR
+     * the whole run is one launch of one workgroup per CU whose GR
      * workgroups hold their cells' state in registers (and connectivity in
      * LDS), and whose small-population workgroup holds theirs in LDS, from
      * the first step to the last, synchronised point to point between steps
      * (networks with the even split, compact connectivity records and small
-     * populations that fit in LDS).  All give the same bits. */
+     * populations that fit in LDS).  The resident launches of a process run
+     * on a device one after another; a kernel of another stream or process
+     * that keeps CUs busy for longer than 0.2 s while a resident run waits
+     * for them makes that run fail (CBM_E_HIP at the next cbm_sync or call;
+     * the network's state is then undefined).  All give the same bits. */
     uint32_t stepMode;
     /* Device layout of the GR connectivity: 0 = pick (one 16-byte record per
      * GR when numMF and numGO are at most 2047 and the fan-ins are at most 4

@@ -1,7 +1,8 @@
-"""Resident runs of cbm_run_mf_train: the whole run is one cooperative launch
-of one workgroup per CU whose GR workgroups keep their cells' state in
-registers (connectivity records in LDS) from the first step to the last, with
-a grid barrier between steps.  They must give the bits of the same steps
+"""Resident runs of cbm_run_mf_train: the whole run is one launch of one
+workgroup per CU whose GR workgroups keep their cells' state in registers
+(connectivity records in LDS) and whose two small-population workgroups keep
+theirs in LDS from the first step to the last, with step counters and flags
+between steps.  They must give the bits of the same steps
 launched one by one (the fused even-split step, which test_gpu_parity,
 test_trial and test_batch_c3 hold to the oracle): every state and run-time
 field, and every recorded spike, over runs of several lengths (a run of one
