@@ -115,6 +115,7 @@ struct SmallLds {
     uint32_t synPC, synN, synA, synP, synIO;
     uint32_t apPC, apBC, apSC, apIO, apNC;  // u8 spikes of this step (zones stacked)
     uint32_t vIO, rel;             // f32 [nZ*numIO], [nZ*numNC]
+    uint32_t ncSum;                // f32 [3][nZ*numNC]: the NC input sums (MF NMDA, MF AMPA, PC) of this step
     uint32_t inPF, inBC, inSC, inGO, inTm;  // this step's inputs from the other half of the step: PF->PC
                                    // sums [nZ*numPC], PF->BC/SC and GR->GO counts [numBC/numSC/numGO],
                                    // plasticity timers [nZ*numIO] (staged in phase 0)
@@ -132,6 +133,7 @@ struct SmallLds {
     // land on different waves instead of all on wave 0
     uint32_t oSynSC, oSynMF, oSynPC, oSynIO;      // SC->PC, MF->NC, PC->NC, NC->IO synapses
     uint32_t oPC, oBC, oSC, oIO, oNC;             // cell updates
+    uint32_t oNCS;                                // NC input sums: three waves (MF NMDA, MF AMPA, PC)
     uint32_t oRBCPC, oRSCPC, oRPCBC, oRPCNC, oRIO; // output routing
     uint32_t l2PC, l2BC, l2IO, l2NC;              // log2 of the power-of-two populations
 };
@@ -234,7 +236,7 @@ struct BatchNet {
 // of run step k takes MF row k; its per-step values are derived on the
 // device as launchStep derives them on the host.
 constexpr int kBarGroups = 8;      // step counters of the GR blocks (one per group of blocks)
-constexpr int kFlagCopies = 16;    // copies of the small block's step count
+constexpr int kFlagCopies = 16;    // copies of a small-population or generator block's step count
 constexpr int kBarStride = 1024;   // words between counters and between flag copies (4 KB: different memory
                                    // channels, so that the pollers of one do not queue behind another's)
 struct ResRun {
@@ -257,7 +259,7 @@ struct ResRun {
     uint32_t *flagIn;           // [kFlagCopies * kBarStride] the input-layer part's step count, copies
     uint32_t flagInBase;        // its value before this launch
     uint8_t *apSC3;             // [3][numSC] SC spikes by step mod 3 (input-layer part -> microzone part)
-    uint32_t *pb;               // the generator block's step count
+    uint32_t *pb;               // [kFlagCopies * kBarStride] the generator block's step count, copies
     uint32_t pbBase;            // its value before this launch
     uint32_t *status;           // set to 1 when a barrier wait timed out
     uint32_t recLds;            // byte offset in dynamic LDS of the GR blocks' records [CPT][1024]

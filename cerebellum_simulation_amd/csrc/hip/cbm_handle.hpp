@@ -63,9 +63,9 @@ struct cbm_handle {
     bool resident = false;
     size_t residentLds = 0;           // its dynamic LDS (the small step's, or a span's + its records)
     uint32_t residentRecLds = 0;      // byte offset of the GR records in that LDS
-    uint32_t *resBar = nullptr;       // GR step counters [kBarGroups * kBarStride], the microzone and
-                                      // input-layer blocks' step-count copies [2][kFlagCopies * kBarStride],
-                                      // the generator block's step count
+    uint32_t *resBar = nullptr;       // GR step counters [kBarGroups * kBarStride], then the step-count
+                                      // copies [kFlagCopies * kBarStride] of the microzone, input-layer and
+                                      // generator blocks
     uint32_t resBarBase = 0;          // the counters' sum after the last resident launch
     uint32_t resFlagBase = 0;         // the small block's step count after it
     uint32_t resPbBase = 0;           // the generator block's step count after it

@@ -978,16 +978,28 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
     {
         const uint32_t nPCc = nZ * s.numPC, nBCc = nZ * s.numBC, nSCc = s.numSC, nGOc = s.numGO;
         const uint32_t nIOc = nZ * s.numIO, nNCc = nZ * s.numNC;
-        const uint32_t rounds = (max(max(nPCc, nBCc), max(max(nSCc, nIOc), nNCc)) + nThr - 1) / nThr;
+        // the NC input sums of calcNCActivities (mzone.cpp:357-384), each in
+        // the reference's order: the MF NMDA, MF AMPA and PC sums of every NC
+        // on three waves of their own, beside the other populations' updates
+        // (at 1M GR the MF sums are 128-term chains, the longest of the
+        // phase); the NC update reads them after a barrier
+        float *ncSum = at<float>(lds, L.ncSum);
+        if constexpr (MZ) {
+            const uint32_t w = (tid - L.oNCS) & (nThr - 1), c = w >> 6;
+            if (c < 3) {
+                const uint32_t n = c == 2 ? kPC : kMF, base = c == 0 ? L.synN : c == 1 ? L.synA : L.synP;
+                for (uint32_t i = w & 63u; i < nNCc; i += 64) ncSum[c * nNCc + i] = lds_sum_seq(s_syn + base + i * n, n, 0.0f);
+            }
+        }
+        const uint32_t rounds = (max(max(nPCc, nBCc), max(nSCc, nIOc)) + nThr - 1) / nThr;
         for (uint32_t rd = 0; rd < rounds; rd++) {
             // population-local cell indices: each population starts on its own wave
             const uint32_t cP = rd * nThr + ((tid - L.oPC) & (nThr - 1)), cB = rd * nThr + ((tid - L.oBC) & (nThr - 1));
             const uint32_t cS = rd * nThr + ((tid - L.oSC) & (nThr - 1)), cI = rd * nThr + ((tid - L.oIO) & (nThr - 1));
-            const uint32_t cN = rd * nThr + ((tid - L.oNC) & (nThr - 1));
-            const bool hPC = cP < nPCc, hBC = cB < nBCc, hSC = cS < nSCc, hIO = cI < nIOc, hNC = cN < nNCc;
+            const bool hPC = cP < nPCc, hBC = cB < nBCc, hSC = cS < nSCc, hIO = cI < nIOc;
             // ---- loads, at clamped indices (in bounds, no branch)
             const uint32_t iP = min(cP, nPCc - 1), iB = min(cB, nBCc - 1), iS = min(cS, nSCc - 1);
-            const uint32_t iI = min(cI, nIOc - 1), iN = min(cN, nNCc - 1);
+            const uint32_t iI = min(cI, nIOc - 1);
             const float pcPF = at<float>(lds, L.inPF)[iP];
             const uint32_t pcIn = SF<LF, uint32_t>(a, CBM_F_INPUT_BC_PC, lds)[iP];
             const float pcGPF = SF<LF, float>(a, CBM_F_G_PF_PC, lds)[iP], pcGBC = SF<LF, float>(a, CBM_F_G_BC_PC, lds)[iP];
@@ -1006,9 +1018,6 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
             const float ioVC = SF<LF, float>(a, CBM_F_VCOUPLE_IO, lds)[iI];
             const uint32_t ioAb = SF<LF, uint32_t>(a, CBM_F_APBUF_IO, lds)[iI];
             const int32_t ioTm = at<int32_t>(lds, L.inTm)[iI];
-            const float ncV = SF<LF, float>(a, CBM_F_VM_NC, lds)[iN], ncTh = SF<LF, float>(a, CBM_F_THRESH_NC, lds)[iN];
-            const float ncR = SF<LF, float>(a, CBM_F_SYN_IO_PRELEASE_NC, lds)[iN];
-            const uint32_t ncAb = SF<LF, uint32_t>(a, CBM_F_APBUF_NC, lds)[iN];
             // ---- PC
             if (MZ && hPC) {
                 const uint32_t i = iP, z = i >> L.l2PC;
@@ -1083,31 +1092,6 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
                     p.threshMaxIO * ap + (!ap) * (ioTh + p.threshDecIO * (p.threshRestIO - ioTh));
                 xstg<X>(gp(a.timer2) + (size_t)par * nIOc + i, (int32_t)((!ap) * (ioTm + 1) + ap * p.tsLTDStartAPIO));
             }
-            // ---- NC
-            if (MZ && hNC) {
-                const uint32_t i = iN;
-                float vv = ncV, th = ncTh, r = ncR;
-                const float ms = p.msPerTimeStep;
-                float gMFNMDASum = lds_sum_seq(s_syn + L.synN + i * kMF, kMF, 0.0f);
-                float gMFAMPASum = lds_sum_seq(s_syn + L.synA + i * kMF, kMF, 0.0f);
-                float gPCNCSum = lds_sum_seq(s_syn + L.synP + i * kPC, kPC, 0.0f);
-                gMFNMDASum = gMFNMDASum * ms / ((float)kMF);
-                gMFAMPASum = gMFAMPASum * ms / ((float)kMF);
-                gMFNMDASum = gMFNMDASum * vv / (-80.0f);
-                gPCNCSum = gPCNCSum * ms / ((float)kPC);
-                vv = vv + p.gLeakNC * (p.eLeakNC - vv) - (gMFNMDASum + gMFAMPASum) * vv + gPCNCSum * (p.ePCtoNC - vv);
-                th = th + p.threshDecNC * (p.threshRestNC - th);
-                const uint8_t ap = vv > th;
-                SF<LF, float>(a, CBM_F_VM_NC, lds)[i] = vv;
-                SF<LF, uint8_t>(a, CBM_F_AP_NC, lds)[i] = ap;
-                s_apNC[i] = ap;
-                SF<LF, uint32_t>(a, CBM_F_APBUF_NC, lds)[i] = (ncAb << 1) | (ap * 0x00000001u);
-                SF<LF, float>(a, CBM_F_THRESH_NC, lds)[i] = ap * p.threshMaxNC + (!ap) * th;
-                r = r * expf(-ms / (p.relPDecTSofNCtoIO * expf(-r / p.relPDecTTofNCtoIO) + p.relPDecT0ofNCtoIO));
-                r = r + ap * p.relPIncNCtoIO * expf(-r / p.relPIncTauNCtoIO);
-                SF<LF, float>(a, CBM_F_SYN_IO_PRELEASE_NC, lds)[i] = r;
-                s_rel[i] = r;
-            }
         }
         for (uint32_t i = tid; IN && i < nGOc; i += nThr) {
             float gMF = 0, gGR = 0, gGO = 0, glu = 0, mG = 0, gInc = 0, gMG = 0, thv = 0, v = 0, vC = 0, sc = 0, inDep = 0;
@@ -1156,6 +1140,33 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
                 if (a.anyCoupling) s_vGO[i] = v;
                 SF<LF, uint32_t>(a, CBM_F_APBUF_GO, lds)[i] = (goAb << 1) | (ap * 0x00000001u);
                 SF<LF, float>(a, CBM_F_THRESH_GO, lds)[i] = ap * p.threshMaxGO + (!ap) * thv;
+            }
+        }
+        // ---- NC, once its input sums are in
+        if constexpr (MZ) {
+            __syncthreads();
+            for (uint32_t i = (tid - L.oNC) & (nThr - 1); i < nNCc; i += nThr) {
+                float vv = SF<LF, float>(a, CBM_F_VM_NC, lds)[i], th = SF<LF, float>(a, CBM_F_THRESH_NC, lds)[i];
+                float r = SF<LF, float>(a, CBM_F_SYN_IO_PRELEASE_NC, lds)[i];
+                const uint32_t ncAb = SF<LF, uint32_t>(a, CBM_F_APBUF_NC, lds)[i];
+                const float ms = p.msPerTimeStep;
+                float gMFNMDASum = ncSum[i], gMFAMPASum = ncSum[nNCc + i], gPCNCSum = ncSum[2 * nNCc + i];
+                gMFNMDASum = gMFNMDASum * ms / ((float)kMF);
+                gMFAMPASum = gMFAMPASum * ms / ((float)kMF);
+                gMFNMDASum = gMFNMDASum * vv / (-80.0f);
+                gPCNCSum = gPCNCSum * ms / ((float)kPC);
+                vv = vv + p.gLeakNC * (p.eLeakNC - vv) - (gMFNMDASum + gMFAMPASum) * vv + gPCNCSum * (p.ePCtoNC - vv);
+                th = th + p.threshDecNC * (p.threshRestNC - th);
+                const uint8_t ap = vv > th;
+                SF<LF, float>(a, CBM_F_VM_NC, lds)[i] = vv;
+                SF<LF, uint8_t>(a, CBM_F_AP_NC, lds)[i] = ap;
+                s_apNC[i] = ap;
+                SF<LF, uint32_t>(a, CBM_F_APBUF_NC, lds)[i] = (ncAb << 1) | (ap * 0x00000001u);
+                SF<LF, float>(a, CBM_F_THRESH_NC, lds)[i] = ap * p.threshMaxNC + (!ap) * th;
+                r = r * expf(-ms / (p.relPDecTSofNCtoIO * expf(-r / p.relPDecTTofNCtoIO) + p.relPDecT0ofNCtoIO));
+                r = r + ap * p.relPIncNCtoIO * expf(-r / p.relPIncTauNCtoIO);
+                SF<LF, float>(a, CBM_F_SYN_IO_PRELEASE_NC, lds)[i] = r;
+                s_rel[i] = r;
             }
         }
     }
@@ -1575,26 +1586,34 @@ __device__ __forceinline__ void res_publish(uint32_t *flag, uint32_t v)
         __hip_atomic_store(flag + threadIdx.x * kBarStride, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// block-wide wait until the sum of the n words w[q * kBarStride] reaches
-// target; BACKOFF: the pause between polls doubles from 64 to 1,024 cycles
-template <int N, bool BACKOFF>
-__device__ __forceinline__ bool res_wait(const uint32_t *w, uint32_t target, uint32_t *status)
+// Block-wide wait until the sum of the N0 words w0[q * kBarStride] reaches t0
+// and, when N1 / N2 is 1, the word w1 reaches t1 and w2 reaches t2: thread 0
+// loads every word of a poll at once, so the conditions cost one memory round
+// trip together.  BACKOFF: the pause between polls doubles from 64 to 1,024
+// cycles.  False (and the status word set) when the wait timed out.
+template <int N0, int N1, int N2, bool BACKOFF>
+__device__ __forceinline__ bool res_wait3(const uint32_t *w0, uint32_t t0, const uint32_t *w1, uint32_t t1,
+                                          const uint32_t *w2, uint32_t t2, uint32_t *status)
 {
     __shared__ uint32_t s_ok;
     if (threadIdx.x == 0) {
-        const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+        const uint64_t start = __builtin_amdgcn_s_memrealtime();
         uint32_t ok = 1, pause = 0;
+        auto ld = [](const uint32_t *p) {
+            return __hip_atomic_load(const_cast<uint32_t *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        };
         while (true) {
-            uint32_t c[N];
+            uint32_t c[N0], c1 = 0, c2 = 0;
 #pragma unroll
-            for (int q = 0; q < N; q++)
-                c[q] = __hip_atomic_load(const_cast<uint32_t *>(w) + q * kBarStride, __ATOMIC_RELAXED,
-                                         __HIP_MEMORY_SCOPE_AGENT);
+            for (int q = 0; q < N0; q++) c[q] = ld(w0 + q * kBarStride);
+            if (N1) c1 = ld(w1);
+            if (N2) c2 = ld(w2);
             uint32_t sum = 0;
 #pragma unroll
-            for (int q = 0; q < N; q++) sum += c[q];
-            if ((int32_t)(sum - target) >= 0) break;
-            if (__builtin_amdgcn_s_memrealtime() - t0 > kWaitTicks) {
+            for (int q = 0; q < N0; q++) sum += c[q];
+            if ((int32_t)(sum - t0) >= 0 && (!N1 || (int32_t)(c1 - t1) >= 0) && (!N2 || (int32_t)(c2 - t2) >= 0))
+                break;
+            if (__builtin_amdgcn_s_memrealtime() - start > kWaitTicks) {
                 __hip_atomic_store(status, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 ok = 0;
                 break;
@@ -1617,6 +1636,13 @@ __device__ __forceinline__ bool res_wait(const uint32_t *w, uint32_t target, uin
     }
     __syncthreads();
     return s_ok != 0;
+}
+
+// block-wide wait until the sum of the n words w[q * kBarStride] reaches target
+template <int N, bool BACKOFF>
+__device__ __forceinline__ bool res_wait(const uint32_t *w, uint32_t target, uint32_t *status)
+{
+    return res_wait3<N, 0, 0, BACKOFF>(w, target, nullptr, 0, nullptr, 0, status);
 }
 
 // The PF->PC fold of the chunk sums of one step (as gr_fold), by the 16 waves
@@ -1696,12 +1722,13 @@ __device__ __forceinline__ void res_part(const SmallArgs &a, const ResRun &r, fl
     };
     float errNext = errOf(0);
     for (uint32_t k = 0; k < r.n; k++) {
-        // every GR block done with step k - 1 (and for the microzone part the
-        // input-layer part, whose SC spikes of step k - 1 it routes; the
-        // generator block was done with step k - 1 before the microzone part
-        // published it)
-        if (!res_wait<kBarGroups, false>(r.bar, r.barBase + k * r.spans, r.status)) return;
-        if (!IN && !res_wait<1, false>(r.flagIn, r.flagInBase + k, r.status)) return;
+        // every GR block done with step k - 1, and for the microzone part the
+        // input-layer part (whose SC spikes of step k - 1 it routes) and the
+        // generator block (whose fold of step k - 1 it reads)
+        if (IN ? !res_wait<kBarGroups, false>(r.bar, r.barBase + k * r.spans, r.status)
+               : !res_wait3<kBarGroups, 1, 1, false>(r.bar, r.barBase + k * r.spans, r.flagIn, r.flagInBase + k,
+                                                      r.pb, r.pbBase + k, r.status))
+            return;
         const uint64_t t = r.t0 + k + 1;
         SmallStep st;
         st.mf = r.mf + (size_t)k * r.pitch;
@@ -1743,9 +1770,6 @@ __device__ __forceinline__ void res_part(const SmallArgs &a, const ResRun &r, fl
             res_publish(r.flagIn, r.flagInBase + k + 1);
         } else {
             errNext = errOf(k + 1);
-            // the generator block done with step k (its fold read the buffer
-            // the GR blocks' step k + 1 writes)
-            if (!res_wait<1, false>(r.pb, r.pbBase + k + 1, r.status)) return;
             res_publish(r.flag, r.flagBase + k + 1);
         }
     }
@@ -1776,8 +1800,10 @@ __device__ __forceinline__ void res_prebuild(const SmallArgs &a, const ResRun &r
     for (uint32_t k = 0; k < r.n; k++) {
         // the microzone part done with step k - 1 (the generator state), every
         // GR block done with step k - 1 (the chunk sums)
-        if (!res_wait<1, true>(r.flag + (blockIdx.x % kFlagCopies) * kBarStride, r.flagBase + k, r.status)) return;
-        if (!res_wait<kBarGroups, true>(r.bar, r.barBase + k * r.spans, r.status)) return;
+        if (!res_wait3<kBarGroups, 1, 0, true>(r.bar, r.barBase + k * r.spans,
+                                                r.flag + (blockIdx.x % kFlagCopies) * kBarStride, r.flagBase + k,
+                                                nullptr, 0, r.status))
+            return;
         const uint64_t t = r.t0 + k + 1;
         uint32_t tid = threadIdx.x;
         asm volatile("" : "+v"(tid));  // as in small_step
@@ -1789,7 +1815,7 @@ __device__ __forceinline__ void res_prebuild(const SmallArgs &a, const ResRun &r
         SmallStep st;
         st.par = (uint32_t)(t & 1u);
         sfmt_prebuild<true>(a, st, reinterpret_cast<uint32_t *>(lds));
-        res_arrive(r.pb);
+        res_publish(r.pb, r.pbBase + k + 1);
     }
 }
 
@@ -1823,8 +1849,11 @@ __device__ __forceinline__ void res_gr(const SmallArgs &a, const ResRun &r, cons
     for (uint32_t k = 0; k < r.n; k++) {
         // both parts of the small populations done with step k - 1
         const uint32_t cp = (blockIdx.x % kFlagCopies) * kBarStride;
-        if (!res_wait<1, true>(r.flagIn + cp, r.flagInBase + k, r.status)) return;
-        if (!res_wait<1, true>(r.flag + cp, r.flagBase + k, r.status)) return;
+        // (and the generator block done with step k - 1: its fold read the
+        // chunk sums this step overwrites)
+        if (!res_wait3<1, 1, 1, true>(r.flagIn + cp, r.flagInBase + k, r.flag + cp, r.flagBase + k, r.pb + cp,
+                                      r.pbBase + k, r.status))
+            return;
         const uint64_t t = r.t0 + k + 1;
         const uint32_t par = (uint32_t)(t & 1u), parPrev = par ^ 1u;
         const uint32_t slot = (uint32_t)(t % kRing);

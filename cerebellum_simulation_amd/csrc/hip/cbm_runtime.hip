@@ -130,6 +130,7 @@ SmallLds smallLayout(const cbm_sizes &s, uint32_t slotsBCPC, uint32_t slotsPCBC,
     L.apNC = take(nZ * s.numNC);
     L.vIO = take(4 * nZ * s.numIO);
     L.rel = take(4 * nZ * s.numNC);
+    L.ncSum = take(4 * 3 * nZ * s.numNC);
     L.inPF = take(4 * nZ * s.numPC);
     L.inBC = take(4 * s.numBC);
     L.inSC = take(4 * s.numSC);
@@ -164,7 +165,7 @@ SmallLds smallLayout(const cbm_sizes &s, uint32_t slotsBCPC, uint32_t slotsPCBC,
     place({{&L.oSynIO, nZ * s.numIO * s.numpIOfromNCtoIO}, {&L.oSynPC, nZ * s.numNC * s.numpNCfromPCtoNC},
            {&L.oSynSC, nZ * s.numSC}, {&L.oSynMF, nZ * s.numNC * s.numpNCfromMFtoNC}});
     place({{&L.oIO, nZ * s.numIO}, {&L.oNC, nZ * s.numNC}, {&L.oPC, nZ * s.numPC}, {&L.oBC, nZ * s.numBC},
-           {&L.oSC, s.numSC}});
+           {&L.oSC, s.numSC}, {&L.oNCS, 3 * 64}});
     place({{&L.oRIO, nZ * s.numIO}, {&L.oRBCPC, nZ * s.numPC}, {&L.oRPCBC, nZ * s.numBC},
            {&L.oRPCNC, nZ * s.numNC * s.numpNCfromPCtoNC}, {&L.oRSCPC, nZ * s.numPC * s.numpPCfromSCtoPC}});
     L.l2PC = ilog2(s.numPC);
@@ -587,8 +588,7 @@ static int setupResident(cbm_handle *h, int cus)
             HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, k, kSmallThreads, lds));
             if (perCU == 1 && cus >= (int)spans + 3) {
                 int rc;
-                if ((rc = h->dalloc((void **)&h->resBar,
-                                    (size_t)(kBarGroups + 2 * kFlagCopies + 1) * kBarStride * 4)))
+                if ((rc = h->dalloc((void **)&h->resBar, (size_t)(kBarGroups + 3 * kFlagCopies) * kBarStride * 4)))
                     return rc;
                 if ((rc = h->dalloc((void **)&h->resApSC3, 3 * (size_t)s.numSC))) return rc;
                 if ((rc = h->dalloc((void **)&h->resStatus, 4))) return rc;
