@@ -723,8 +723,23 @@ __device__ __forceinline__ float lds_sum_seq(const float *x, uint32_t n, float a
 {
     if ((n & 3u) == 0 && ((uintptr_t)x & 15u) == 0) {
         const float4 *x4 = reinterpret_cast<const float4 *>(x);
+        uint32_t j = 0;
+        // chunks of 32 terms: the chunk's 8 loads are in flight together and
+        // each add waits only for the load of its own term
+        for (; j + 8 <= n / 4; j += 8) {
+            float4 v[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) v[q] = x4[j + q];
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                acc = acc + v[q].x;
+                acc = acc + v[q].y;
+                acc = acc + v[q].z;
+                acc = acc + v[q].w;
+            }
+        }
 #pragma unroll 4
-        for (uint32_t j = 0; j < n / 4; j++) {
+        for (; j < n / 4; j++) {
             const float4 v = x4[j];
             acc = acc + v.x;
             acc = acc + v.y;
