@@ -154,6 +154,8 @@ def hip_lib() -> C.CDLL:
         L.cbm_sync.argtypes = [P]
         L.cbm_step_io_begin.argtypes = [P, V, I]
         L.cbm_step_io_end.argtypes = [P, V]
+        L.cbm_loop_begin.argtypes = [P, I, I]
+        L.cbm_loop_end.argtypes = [P, C.POINTER(C.c_int)]
         L.cbm_stream_ceiling.argtypes = [I, C.c_uint64, I, I, I, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]
         L.cbm_stream.argtypes = [P]
         L.cbm_stream.restype = P

@@ -116,11 +116,28 @@ int cbm_cartpole_run(cbm_cartpole *c, int n, float *trace, int *done)
         // The loop of SimThread::run with one host round trip per step: the
         // MF spikes go in, the step runs and every zone's NC spikes come back
         // in one cbm_step_io pair, and the MF generator draws the next step's
-        // uniforms while the network steps.
+        // uniforms while the network steps.  On a resident network the pairs
+        // go to one closed-loop launch (cbm_loop_begin), unless steps write
+        // the state to files.
         int k = 0;
         const unsigned nNC = c->sz.numNC, nMF = c->sz.numMF;
         cbm_handle *h = c->core->handle();
         std::vector<uint8_t> nc((size_t)c->sz.numZones * nNC);
+        struct Loop {  // ends the closed loop on every way out
+            cbm_handle *h = nullptr;
+            int end()
+            {
+                cbm_handle *x = h;
+                h = nullptr;
+                return x ? cbm_loop_end(x, nullptr) : CBM_OK;
+            }
+            ~Loop() { (void)end(); }
+        } loop;
+        if (n > 0 && cbm_resident(h) == 1 && !c->env->savesState()) {
+            const int rc = cbm_loop_begin(h, n, CBM_REC_NC);
+            if (rc) return fail(rc, std::string("cbm_loop_begin: ") + cbm_last_error());
+            loop.h = h;
+        }
         for (; k < n && !c->env->terminated(); k++) {
             float *freq = c->env->getState();
             const ct_uint8_t *ap = c->mfs->calcActivity(freq);
@@ -143,6 +160,8 @@ int cbm_cartpole_run(cbm_cartpole *c, int n, float *trace, int *done)
                            (int)(c->env->rightMZ().errorsDelivered() - r0), ncL, ncR, spikes);
             }
         }
+        const int rc = loop.end();
+        if (rc) return fail(rc, std::string("cbm_loop_end: ") + cbm_last_error());
         if (done) *done = k;
         return CBM_OK;
     });

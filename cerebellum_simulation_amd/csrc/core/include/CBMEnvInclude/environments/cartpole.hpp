@@ -61,6 +61,8 @@ public:
     // The same with no core: errors are only counted by the microzones.
     void stepDetached(const ct_uint8_t *apNCLeft, const ct_uint8_t *apNCRight);
     bool terminated();
+    // whether steps write the simulation state to files (simStateDir given)
+    bool savesState() const { return !saveStateDir.empty(); }
 
     float deg2rad(float deg) { return (M_PI * deg) / 180.0; }
     float rad2deg(float rad) { return (180.0 * rad) / M_PI; }

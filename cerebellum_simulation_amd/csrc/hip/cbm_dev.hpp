@@ -263,7 +263,26 @@ struct ResRun {
     uint32_t pbBase;            // its value before this launch
     uint32_t *status;           // set to 1 when a barrier wait timed out
     uint32_t recLds;            // byte offset in dynamic LDS of the GR blocks' records [CPT][1024]
+    uint64_t waitTicks;         // bound of a wait (s_memrealtime ticks)
+    // a closed loop (cbm_loop_begin; hostSeq nullptr: none): the host posts
+    // step k's MF row and error drive in hostIn[k & 1] and then k + 1 in
+    // *hostSeq (kHostStop | k: stop before step k); the generator block copies
+    // them to mf + k * pitch and errDev[k] and publishes k + 1 in mfFlag
+    // (+ kStopBit to stop); the microzone part stores the step's spikes of
+    // population ioPop in hostOut[k & 1] and then k + 1 in *hostOutSeq
+    const uint32_t *hostSeq;
+    const uint8_t *hostIn;      // [2][inPitch]: MF bytes, then numZones floats at inErr
+    uint32_t inPitch, inErr;
+    uint32_t *hostOutSeq;
+    uint32_t *hostOut;          // [2][outWords]: the spikes, 4 to a word
+    uint32_t outWords, ioPop;
+    float *errDev;              // [n][numZones] error drive (device)
+    uint32_t *mfFlag;           // [kFlagCopies * kBarStride] steps whose MF row is in place (0 at launch)
 };
+constexpr uint32_t kHostStop = 1u << 31;  // in hostSeq: stop
+constexpr uint64_t kResWaitTicks = 20000000ull;    // bound of a wait between blocks: 0.2 s of s_memrealtime (100 MHz)
+constexpr uint64_t kLoopWaitTicks = 200000000ull;  // in a closed loop, where the host posts each step: 2 s
+constexpr uint32_t kStopBit = 1u << 30;   // in mfFlag: stop
 
 // Per network of a surrogate sweep: cluster weights [numZones][nCl] and the
 // cluster size (2^log2Cells consecutive GRs).
@@ -280,22 +299,30 @@ struct SweepRun {
     uint64_t pitch;
 };
 
-// Compact GR connectivity: up to 4 MF inputs, 3 GO inputs and 2 GO targets as
-// 11-bit cell indices (kNoCell11 = empty) and the three delays as 6-bit bit
-// indices (kNever6 = never), in two 64-bit halves:
-//   lo: m0 m1 m2 m3 g0 (11 bits each, from bit 0) | PF delay (bits 55..60)
-//   hi: g1 g2 o0 o1 (11 bits each, from bit 0) | delay o0 (44..49) | delay o1 (50..55)
-// Usable when numMF and numGO are at most 2047.
+// Compact GR connectivity: up to 4 MF inputs as 12-bit cell indices (numMF =
+// empty), 3 GO inputs and 2 GO targets as 11-bit cell indices (kNoCell11 =
+// empty) and the three delays as 6-bit bit indices (kNever6 = never), in two
+// 64-bit halves:
+//   lo: m0 m1 m2 m3 (12 bits each, from bit 0) | g0 (48..58)
+//   hi: g1 g2 o0 o1 (11 bits each, from bit 0) | delay o0 (44..49) | delay o1 (50..55) | PF delay (56..61)
+// Usable when numMF is at most 4095 and numGO at most 2047.  An empty input
+// reads a bit of the spike bitmaps that is always zero: bit numMF of the MF
+// bitmap (it holds numMF + 1 bits at least), bit 2047 of the GO bitmap (at
+// least 2,048 bits).
+constexpr uint32_t kNoMF12 = 0xFFFu;  // (the field mask)
 constexpr uint32_t kNoCell11 = 0x7FFu;
 constexpr uint32_t kNever6 = 0x3Fu;
+constexpr uint32_t kMinMFBitmapWords = 64, kMinGOBitmapWords = 64;
 
 __host__ __device__ inline uint4 packGrConn(const uint16_t m[4], const uint16_t g[3], const uint16_t o[2],
-                                            uint8_t dBsp, uint8_t dO0, uint8_t dO1)
+                                            uint8_t dBsp, uint8_t dO0, uint8_t dO1, uint32_t numMF)
 {
+    auto mx = [numMF](uint16_t v) -> uint64_t { return v == kNoCell ? numMF : v; };
     auto ix = [](uint16_t v) -> uint64_t { return v == kNoCell ? kNoCell11 : v; };
     auto dl = [](uint8_t v) -> uint64_t { return v == kNever ? kNever6 : v; };
-    const uint64_t lo = ix(m[0]) | ix(m[1]) << 11 | ix(m[2]) << 22 | ix(m[3]) << 33 | ix(g[0]) << 44 | dl(dBsp) << 55;
-    const uint64_t hi = ix(g[1]) | ix(g[2]) << 11 | ix(o[0]) << 22 | ix(o[1]) << 33 | dl(dO0) << 44 | dl(dO1) << 50;
+    const uint64_t lo = mx(m[0]) | mx(m[1]) << 12 | mx(m[2]) << 24 | mx(m[3]) << 36 | ix(g[0]) << 48;
+    const uint64_t hi = ix(g[1]) | ix(g[2]) << 11 | ix(o[0]) << 22 | ix(o[1]) << 33 | dl(dO0) << 44 | dl(dO1) << 50 |
+                        dl(dBsp) << 56;
     return make_uint4((uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32));
 }
 
@@ -304,7 +331,8 @@ __host__ __device__ constexpr uint32_t grTileLds(uint32_t nt, uint32_t cpt, uint
                                                  uint32_t numZones, uint32_t numIO)
 {
     return numGO * 4 + 2 * (nt * cpt / 64 + 1) * 4 + numZones * numIO * 4 +
-           ((numMF >> 5) + 1 > 64 ? (numMF >> 5) + 1 : 64) * 4 + ((numGO >> 5) + 1 > 64 ? (numGO >> 5) + 1 : 64) * 4;
+           ((numMF >> 5) + 1 > kMinMFBitmapWords ? (numMF >> 5) + 1 : kMinMFBitmapWords) * 4 +
+           ((numGO >> 5) + 1 > kMinGOBitmapWords ? (numGO >> 5) + 1 : kMinGOBitmapWords) * 4;
 }
 
 __host__ __device__ inline uint32_t ilog2(uint32_t x)

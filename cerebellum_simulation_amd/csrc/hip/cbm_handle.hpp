@@ -21,7 +21,7 @@ __global__ void step_span_kernel(const SmallArgs *args, const SmallStep st, cons
 template <int CPT, int FMT>
 __global__ void gr_span_kernel(const GrArgs a, const GrStep st, const uint32_t spans);
 __global__ void small_step_kernel(const SmallArgs *args, const SmallStep st);
-template <int CPT>
+template <int CPT, bool LOOP>  // LOOP: a closed loop (cbm_loop_begin)
 __global__ void step_resident_kernel(const SmallArgs *args, const ResRun r);
 __global__ void stream_ceiling_kernel(StreamArgs a);
 __global__ void pfpc_fold_kernel(const float *chunk, float *out, uint32_t nChunks, uint32_t perPC, uint32_t numPC,
@@ -73,6 +73,19 @@ struct cbm_handle {
     uint32_t resSpans = 0;            // GR blocks of a resident launch
     int resCpt = 0;                   // their cells per thread
     uint8_t *resApSC3 = nullptr;      // SC spikes by step mod 3, input-layer block -> microzone block
+    uint32_t *resMfFlag = nullptr;    // the MF-row step count the blocks poll: zero outside closed loops
+    // a closed loop (cbm_loop_begin .. cbm_loop_end): one resident launch
+    // that takes each step's MF row and error drive from pinned memory and
+    // returns the step's spikes of loopPop there
+    bool looping = false;
+    int loopK = 0, loopMax = 0, loopPop = 0;
+    uint8_t *loopHost = nullptr;      // pinned, coherent: step counts, [2] input rows, [2] output rows
+    size_t loopHostCap = 0;
+    uint32_t loopInPitch = 0, loopInErr = 0, loopOutOff = 0, loopOutWords = 0, loopOutBytes = 0;
+    uint8_t *loopMF = nullptr;        // device: [loopCap][loopPitch] MF rows
+    float *loopErr = nullptr;         // device: [loopCap][numZones] error drive
+    uint32_t *loopFlag = nullptr;     // device: the MF-row step count of the loop's launch
+    size_t loopCap = 0, loopPitch = 0;
     uint32_t *resStatus = nullptr;    // device: 1 when a barrier wait timed out
     uint32_t *resStatusHost = nullptr;  // pinned copy, read at the next synchronisation
     bool resCheck = false;            // a resident run's status is still to be read

@@ -250,7 +250,7 @@ __device__ __forceinline__ void gr_span_impl(const GrArgs &a, const GrStep &st, 
     const uint32_t nStep = a.numZones * a.numIO;
     // bitmap words: one past the cells, and at least 64 so that every 11-bit
     // index of a compact record (0x7FF when absent) reads a zero bit
-    const uint32_t nMFb = max((a.numMF >> 5) + 1, 64u), nGOb = max((a.numGO >> 5) + 1, 64u);
+    const uint32_t nMFb = max((a.numMF >> 5) + 1, kMinMFBitmapWords), nGOb = max((a.numGO >> 5) + 1, kMinGOBitmapWords);
     uint32_t *s_goh = smem;                          // [numGO] GR->GO counts of the span
     uint32_t *s_sc = s_goh + a.numGO;                // [kSeg]
     uint32_t *s_bc = s_sc + kSeg;                    // [kSeg]
@@ -395,20 +395,22 @@ __device__ __forceinline__ void gr_span_impl(const GrArgs &a, const GrStep &st, 
         // (>= 32: never), and up to four GO targets with their delays
         uint32_t in[8], dBsp, tg[4], dg[4];
         if constexpr (COMPACT) {
-            // packGrConn: 11-bit indices (0x7FF absent: a zero bit), 6-bit delays (0x3F never)
+            // packGrConn: 12-bit MF and 11-bit GO indices (numMF and 0x7FF
+            // absent: a zero bit), 6-bit delays (0x3F never)
             const uint64_t lo = (uint64_t)rj.x | (uint64_t)rj.y << 32;
             const uint64_t hi = (uint64_t)rj.z | (uint64_t)rj.w << 32;
+            auto f12 = [](uint64_t w, int sh) { return (uint32_t)(w >> sh) & kNoMF12; };
             auto f11 = [](uint64_t w, int sh) { return (uint32_t)(w >> sh) & kNoCell11; };
             auto f6 = [](uint64_t w, int sh) { return (uint32_t)(w >> sh) & kNever6; };
-            in[0] = f11(lo, 0);
-            in[1] = f11(lo, 11);
-            in[2] = f11(lo, 22);
-            in[3] = f11(lo, 33);
-            in[4] = f11(lo, 44);
+            in[0] = f12(lo, 0);
+            in[1] = f12(lo, 12);
+            in[2] = f12(lo, 24);
+            in[3] = f12(lo, 36);
+            in[4] = f11(lo, 48);
             in[5] = f11(hi, 0);
             in[6] = f11(hi, 11);
             in[7] = a.numGO;
-            dBsp = f6(lo, 55);
+            dBsp = f6(hi, 56);
             tg[0] = f11(hi, 22);
             tg[1] = f11(hi, 33);
             tg[2] = tg[3] = kNoCell11;
@@ -1577,7 +1579,7 @@ CBM_STEP_SPAN(5, kFmtArrays)
 // wait is bounded: a block that waits ~0.2 s records the failure in *status
 // and leaves the run, so a launch whose blocks were not all resident ends
 // instead of hanging.
-constexpr uint64_t kWaitTicks = 20000000ull;  // s_memrealtime runs at 100 MHz
+constexpr uint64_t kWaitTicks = kResWaitTicks;
 
 // every thread's stores and atomics acknowledged, then the block joins up
 __device__ __forceinline__ void res_drain()
@@ -1602,33 +1604,40 @@ __device__ __forceinline__ void res_publish(uint32_t *flag, uint32_t v)
 }
 
 // Block-wide wait until the sum of the N0 words w0[q * kBarStride] reaches t0
-// and, when N1 / N2 is 1, the word w1 reaches t1 and w2 reaches t2: thread 0
-// loads every word of a poll at once, so the conditions cost one memory round
-// trip together.  BACKOFF: the pause between polls doubles from 64 to 1,024
-// cycles.  False (and the status word set) when the wait timed out.
-template <int N0, int N1, int N2, bool BACKOFF>
-__device__ __forceinline__ bool res_wait3(const uint32_t *w0, uint32_t t0, const uint32_t *w1, uint32_t t1,
-                                          const uint32_t *w2, uint32_t t2, uint32_t *status)
+// and, when N1 / N2 / N3 is 1, the word w1 reaches t1, w2 reaches t2 and w3
+// reaches t3: thread 0 loads every word of a poll at once, so the conditions
+// cost one memory round trip together.  BACKOFF: the pause between polls
+// doubles from 64 to 1,024 cycles.  0 (and the status word set) when the wait
+// took longer than `ticks`; 2 when w3 carries kStopBit above t3; else 1.
+template <int N0, int N1, int N2, int N3, bool BACKOFF>
+__device__ __forceinline__ int res_waitn(const uint32_t *w0, uint32_t t0, const uint32_t *w1, uint32_t t1,
+                                         const uint32_t *w2, uint32_t t2, const uint32_t *w3, uint32_t t3,
+                                         uint32_t *status, uint64_t ticks = kWaitTicks)
 {
-    __shared__ uint32_t s_ok;
+    __shared__ int s_ok;
     if (threadIdx.x == 0) {
         const uint64_t start = __builtin_amdgcn_s_memrealtime();
-        uint32_t ok = 1, pause = 0;
+        uint32_t pause = 0;
+        int ok = 1;
         auto ld = [](const uint32_t *p) {
             return __hip_atomic_load(const_cast<uint32_t *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         };
         while (true) {
-            uint32_t c[N0], c1 = 0, c2 = 0;
+            uint32_t c[N0 ? N0 : 1], c1 = 0, c2 = 0, c3 = 0;
 #pragma unroll
             for (int q = 0; q < N0; q++) c[q] = ld(w0 + q * kBarStride);
             if (N1) c1 = ld(w1);
             if (N2) c2 = ld(w2);
+            if (N3) c3 = ld(w3);
             uint32_t sum = 0;
 #pragma unroll
             for (int q = 0; q < N0; q++) sum += c[q];
-            if ((int32_t)(sum - t0) >= 0 && (!N1 || (int32_t)(c1 - t1) >= 0) && (!N2 || (int32_t)(c2 - t2) >= 0))
+            if ((!N0 || (int32_t)(sum - t0) >= 0) && (!N1 || (int32_t)(c1 - t1) >= 0) &&
+                (!N2 || (int32_t)(c2 - t2) >= 0) && (!N3 || (int32_t)(c3 - t3) >= 0)) {
+                if (N3 && c3 - t3 >= kStopBit) ok = 2;
                 break;
-            if (__builtin_amdgcn_s_memrealtime() - start > kWaitTicks) {
+            }
+            if (__builtin_amdgcn_s_memrealtime() - start > ticks) {
                 __hip_atomic_store(status, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 ok = 0;
                 break;
@@ -1650,7 +1659,14 @@ __device__ __forceinline__ bool res_wait3(const uint32_t *w0, uint32_t t0, const
         s_ok = ok;
     }
     __syncthreads();
-    return s_ok != 0;
+    return s_ok;
+}
+
+template <int N0, int N1, int N2, bool BACKOFF>
+__device__ __forceinline__ bool res_wait3(const uint32_t *w0, uint32_t t0, const uint32_t *w1, uint32_t t1,
+                                          const uint32_t *w2, uint32_t t2, uint32_t *status)
+{
+    return res_waitn<N0, N1, N2, 0, BACKOFF>(w0, t0, w1, t1, w2, t2, nullptr, 0, status) != 0;
 }
 
 // block-wide wait until the sum of the n words w[q * kBarStride] reaches target
@@ -1722,7 +1738,7 @@ __device__ __forceinline__ void res_fields(const SmallArgs &a, uint8_t *lds, boo
 // One part of the small populations' step in a resident run (PART: the
 // input layer on block 0, the microzones on block spans + 1), its state
 // fields held in the block's LDS from the first step to the last.
-template <int PART>
+template <int PART, bool LOOP>
 __device__ __forceinline__ void res_part(const SmallArgs &a, const ResRun &r, float *s_err, uint8_t *lds)
 {
     constexpr bool IN = PART == kPartIn;
@@ -1736,14 +1752,26 @@ __device__ __forceinline__ void res_part(const SmallArgs &a, const ResRun &r, fl
         return (!IN && r.err && threadIdx.x < nZ && k < r.n) ? r.err[(size_t)k * nZ + threadIdx.x] : 0.0f;
     };
     float errNext = errOf(0);
+    uint32_t kDone = r.n;
     for (uint32_t k = 0; k < r.n; k++) {
         // every GR block done with step k - 1, and for the microzone part the
         // input-layer part (whose SC spikes of step k - 1 it routes) and the
-        // generator block (whose fold of step k - 1 it reads)
-        if (IN ? !res_wait<kBarGroups, false>(r.bar, r.barBase + k * r.spans, r.status)
-               : !res_wait3<kBarGroups, 1, 1, false>(r.bar, r.barBase + k * r.spans, r.flagIn, r.flagInBase + k,
-                                                      r.pb, r.pbBase + k, r.status))
-            return;
+        // generator block (whose fold of step k - 1 it reads); in a closed
+        // loop, the step's MF row in place (or the stop)
+        const uint32_t *mfw = r.mfFlag + (blockIdx.x % kFlagCopies) * kBarStride;
+        const uint64_t ticks = LOOP ? r.waitTicks : kWaitTicks;
+        const int w = IN ? res_waitn<kBarGroups, 0, 0, LOOP, false>(r.bar, r.barBase + k * r.spans, nullptr, 0,
+                                                                    nullptr, 0, mfw, k + 1, r.status, ticks)
+                         : res_waitn<kBarGroups, 1, 1, LOOP, false>(r.bar, r.barBase + k * r.spans, r.flagIn,
+                                                                    r.flagInBase + k, r.pb, r.pbBase + k, mfw, k + 1,
+                                                                    r.status, ticks);
+        if (!w) return;
+        if constexpr (LOOP) {
+            if (w == 2) {
+                kDone = k;
+                break;
+            }
+        }
         const uint64_t t = r.t0 + k + 1;
         SmallStep st;
         st.mf = r.mf + (size_t)k * r.pitch;
@@ -1756,6 +1784,7 @@ __device__ __forceinline__ void res_part(const SmallArgs &a, const ResRun &r, fl
             const uint32_t z = tid;
             float e = k == 0 ? r.errDrive0[z] : 0.0f;
             if (z < nZ && errNext != 0.0f) e = errNext * r.maxExtIncVIO;
+            if (!IN && LOOP && z < nZ) e = xldg<true>(gp(r.errDev) + (size_t)k * nZ + z);  // (closed loop)
             s_err[z] = e;
         }
         st.gIncGRtoGO = r.gIncGRtoGO;
@@ -1785,15 +1814,31 @@ __device__ __forceinline__ void res_part(const SmallArgs &a, const ResRun &r, fl
             res_publish(r.flagIn, r.flagInBase + k + 1);
         } else {
             errNext = errOf(k + 1);
+            // (closed loop) the step's spikes of the requested population to
+            // the host, four to a word, through to system memory; the drain of
+            // the publish orders them before the step count the host polls
+            if constexpr (LOOP) {
+                const uint32_t pops[4] = {a.lds.apPC, a.lds.apBC, a.lds.apIO, a.lds.apNC};
+                const uint32_t pop = r.ioPop == CBM_REC_PC ? 0u : r.ioPop == CBM_REC_BC ? 1u : r.ioPop == CBM_REC_IO ? 2u : 3u;
+                const uint8_t *src = lds + pops[pop];
+                uint32_t *dst = r.hostOut + (size_t)(k & 1u) * r.outWords;
+                for (uint32_t i = threadIdx.x; i < r.outWords; i += kSmallThreads) {
+                    const uint32_t v = (uint32_t)src[4 * i] | (uint32_t)src[4 * i + 1] << 8 |
+                                       (uint32_t)src[4 * i + 2] << 16 | (uint32_t)src[4 * i + 3] << 24;
+                    __hip_atomic_store(dst + i, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                }
+            }
             res_publish(r.flag, r.flagBase + k + 1);
+            if (LOOP && threadIdx.x == 0)
+                __hip_atomic_store(r.hostOutSeq, k + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
         }
     }
     if constexpr (!IN) {
         // the SC->PC routing of the last step (MZone::updateSCPCOut), which
         // the microzone part does at the start of the next step
-        if (r.n > 0) {
-            if (!res_wait<1, false>(r.flagIn, r.flagInBase + r.n, r.status)) return;
-            const G<uint8_t> *sc = gp(r.apSC3) + (size_t)((r.t0 + r.n) % 3) * nSC;
+        if (kDone > 0) {
+            if (!res_wait<1, false>(r.flagIn, r.flagInBase + kDone, r.status)) return;
+            const G<uint8_t> *sc = gp(r.apSC3) + (size_t)((r.t0 + kDone) % 3) * nSC;
             uint8_t *s_apSC = lds + a.lds.apSC;
             for (uint32_t i = threadIdx.x; i < nSC; i += kSmallThreads) s_apSC[i] = xldg<true>(sc + i);
             __syncthreads();
@@ -1809,10 +1854,49 @@ __device__ __forceinline__ void res_part(const SmallArgs &a, const ResRun &r, fl
 
 // The generators' next blocks and the PF->PC fold in a resident run (the
 // last block).
+template <bool LOOP>
 __device__ __forceinline__ void res_prebuild(const SmallArgs &a, const ResRun &r, uint8_t *lds)
 {
     const uint32_t nZ = a.s.numZones, nC = r.g.numGR >> 6;
     for (uint32_t k = 0; k < r.n; k++) {
+        if constexpr (LOOP) {
+            // a closed loop: the host's step k (or its stop), copied from
+            // pinned memory to the device and published to the other blocks
+            __shared__ uint32_t s_v;
+            if (threadIdx.x == 0) {
+                const uint64_t start = __builtin_amdgcn_s_memrealtime();
+                uint32_t v;
+                while (true) {
+                    v = __hip_atomic_load(const_cast<uint32_t *>(r.hostSeq), __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_SYSTEM);
+                    if ((v & kHostStop) || (int32_t)(v - (k + 1)) >= 0) break;
+                    if (__builtin_amdgcn_s_memrealtime() - start > r.waitTicks) {
+                        __hip_atomic_store(r.status, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        v = 0xFFFFFFFFu;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+                s_v = v;
+            }
+            __syncthreads();
+            const uint32_t v = s_v;
+            if (v == 0xFFFFFFFFu) return;
+            if (v & kHostStop) {
+                res_publish(r.mfFlag, k + 1 + kStopBit);
+                break;
+            }
+            const uint32_t *in = reinterpret_cast<const uint32_t *>(r.hostIn + (size_t)(k & 1u) * r.inPitch);
+            uint32_t *row = reinterpret_cast<uint32_t *>(const_cast<uint8_t *>(r.mf) + (size_t)k * r.pitch);
+            const uint32_t nW = a.s.numMF / 4;
+            for (uint32_t i = threadIdx.x; i < nW + nZ; i += kSmallThreads) {
+                const uint32_t x = __hip_atomic_load(const_cast<uint32_t *>(i < nW ? in + i : in + r.inErr / 4 + (i - nW)),
+                                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                if (i < nW) xstg<true>(gp(row) + i, x);
+                else xstg<true>(gp(reinterpret_cast<uint32_t *>(r.errDev)) + (size_t)k * nZ + (i - nW), x);
+            }
+            res_publish(r.mfFlag, k + 1);
+        }
         // the microzone part done with step k - 1 (the generator state), every
         // GR block done with step k - 1 (the chunk sums)
         if (!res_wait3<kBarGroups, 1, 0, true>(r.bar, r.barBase + k * r.spans,
@@ -1835,7 +1919,7 @@ __device__ __forceinline__ void res_prebuild(const SmallArgs &a, const ResRun &r
 }
 
 // The GR cells of span `rank` in a resident run (blocks 1 .. spans).
-template <int CPT>
+template <int CPT, bool LOOP>
 __device__ __forceinline__ void res_gr(const SmallArgs &a, const ResRun &r, const uint32_t rank, uint8_t *lds)
 {
     const GrArgs &g = r.g;
@@ -1865,10 +1949,14 @@ __device__ __forceinline__ void res_gr(const SmallArgs &a, const ResRun &r, cons
         // both parts of the small populations done with step k - 1
         const uint32_t cp = (blockIdx.x % kFlagCopies) * kBarStride;
         // (and the generator block done with step k - 1: its fold read the
-        // chunk sums this step overwrites)
-        if (!res_wait3<1, 1, 1, true>(r.flagIn + cp, r.flagInBase + k, r.flag + cp, r.flagBase + k, r.pb + cp,
-                                      r.pbBase + k, r.status))
-            return;
+        // chunk sums this step overwrites; in a closed loop, the step's MF
+        // row in place, or the stop)
+        const int w = res_waitn<1, 1, 1, LOOP, true>(r.flagIn + cp, r.flagInBase + k, r.flag + cp, r.flagBase + k,
+                                                     r.pb + cp, r.pbBase + k, r.mfFlag + cp, k + 1, r.status,
+                                                     LOOP ? r.waitTicks : kWaitTicks);
+        if (!w) return;
+        if constexpr (LOOP)
+            if (w == 2) break;
         const uint64_t t = r.t0 + k + 1;
         const uint32_t par = (uint32_t)(t & 1u), parPrev = par ^ 1u;
         const uint32_t slot = (uint32_t)(t % kRing);
@@ -1923,24 +2011,27 @@ __device__ __forceinline__ void res_gr(const SmallArgs &a, const ResRun &r, cons
 // step counters and flags stand in for the launch boundary.  CPT: cells per
 // thread (every span holds at most 1024 * CPT cells); compact connectivity
 // only.
-template <int CPT>
+template <int CPT, bool LOOP>
 __global__ void __launch_bounds__(kSmallThreads) step_resident_kernel(const SmallArgs *__restrict__ args, const ResRun r)
 {
     extern __shared__ __align__(16) uint8_t lds[];
     __shared__ float s_err[kMaxZones];  // in LDS: the small step indexes it by zone
     if (blockIdx.x == 0)
-        res_part<kPartIn>(*args, r, s_err, lds);
+        res_part<kPartIn, LOOP>(*args, r, s_err, lds);
     else if (blockIdx.x == r.spans + 1)
-        res_part<kPartMz>(*args, r, s_err, lds);
+        res_part<kPartMz, LOOP>(*args, r, s_err, lds);
     else if (blockIdx.x > r.spans)
-        res_prebuild(*args, r, lds);
+        res_prebuild<LOOP>(*args, r, lds);
     else
-        res_gr<CPT>(*args, r, blockIdx.x - 1, lds);
+        res_gr<CPT, LOOP>(*args, r, blockIdx.x - 1, lds);
 }
 
-template __global__ void step_resident_kernel<2>(const SmallArgs *, const ResRun);
-template __global__ void step_resident_kernel<3>(const SmallArgs *, const ResRun);
-template __global__ void step_resident_kernel<5>(const SmallArgs *, const ResRun);
+#define CBM_RESIDENT(CPT)                                                                     \
+    template __global__ void step_resident_kernel<CPT, false>(const SmallArgs *, const ResRun); \
+    template __global__ void step_resident_kernel<CPT, true>(const SmallArgs *, const ResRun);
+CBM_RESIDENT(2)
+CBM_RESIDENT(3)
+CBM_RESIDENT(5)
 
 // ---------------------------------------------------------------- streaming ceiling
 // A per-cell pass with the GR step's streams and no arithmetic to speak of:

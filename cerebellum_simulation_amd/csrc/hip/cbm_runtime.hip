@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -280,12 +281,20 @@ int cbm_destroy(cbm_handle *h)
     if (!h) return fail(CBM_E_ARG, "null handle");
     if (h->batch) return fail(CBM_E_STATE, "the network belongs to a batch: destroy the batch first");
     (void)hipSetDevice(h->o.device);
+    if (h->looping) {  // stop the closed loop's launch before its buffers go
+        __atomic_store_n(reinterpret_cast<uint32_t *>(h->loopHost), kHostStop | (uint32_t)h->loopK, __ATOMIC_RELEASE);
+        h->looping = false;
+    }
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     if (h->stream2) (void)hipStreamSynchronize(h->stream2);
     for (void *p : h->allocs) (void)hipFree(p);
     if (h->rec) (void)hipFree(h->rec);
     if (h->pinnedMF) (void)hipHostFree(h->pinnedMF);
     if (h->pinnedIO) (void)hipHostFree(h->pinnedIO);
+    if (h->loopHost) (void)hipHostFree(h->loopHost);
+    if (h->loopMF) (void)hipFree(h->loopMF);
+    if (h->loopErr) (void)hipFree(h->loopErr);
+    if (h->loopFlag) (void)hipFree(h->loopFlag);
     if (h->resStatusHost) (void)hipHostFree(h->resStatusHost);
     if (h->resErr) (void)hipFree(h->resErr);
     if (h->evG) (void)hipEventDestroy(h->evG);
@@ -315,8 +324,14 @@ static uint32_t ringSlot(const cbm_handle *h, int r)
 
 }  // extern "C"
 
+// A closed loop's pinned block: the step count the host posts (offset 0), the
+// step count the microzone block stores after a step's spikes (256), the two
+// input rows (MF bytes then the zones' error drive), then the two output rows.
+constexpr size_t kLoopOutSeqOff = 256, kLoopInOff = 512;
+
 int cbmdev::syncAll(cbm_handle *h)
 {
+    if (h->looping) return fail(CBM_E_STATE, "a closed loop is running on the network: cbm_loop_end first");
     HIP_TRY(hipStreamSynchronize(h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream2));
     if (h->resCheck) {  // the status of the last resident run, copied after it
@@ -536,10 +551,11 @@ int cbm_write(cbm_handle *h, int field, int zone, const void *src, size_t bytes)
 }
 
 // the resident-run kernel of a span shape
-static const void *residentKernel(int cpt)
+static const void *residentKernel(int cpt, bool loop = false)
 {
-    return cpt == 2 ? (const void *)step_resident_kernel<2>
-                    : cpt == 3 ? (const void *)step_resident_kernel<3> : (const void *)step_resident_kernel<5>;
+#define CBM_RK(C) (loop ? (const void *)step_resident_kernel<C, true> : (const void *)step_resident_kernel<C, false>)
+    return cpt == 2 ? CBM_RK(2) : cpt == 3 ? CBM_RK(3) : CBM_RK(5);
+#undef CBM_RK
 }
 
 // Decide at commit whether cbm_run_mf_train runs resident (stepMode 0 or 5;
@@ -582,15 +598,18 @@ static int setupResident(cbm_handle *h, int cus)
         const size_t need = std::max<size_t>(top, recOff + (size_t)cpt * 1024 * 16);
         const size_t lds = std::max<size_t>(need, roundUp((uint32_t)ldsPerCU / 2 + 16, 16));
         if (ldsPerCU > 0 && lds <= kSmallLdsMax) {
-            const void *k = residentKernel(cpt);
+            const void *k = residentKernel(cpt), *kl = residentKernel(cpt, true);
             HIP_TRY(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-            int perCU = 0;
+            HIP_TRY(hipFuncSetAttribute(kl, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            int perCU = 0, perCUl = 0;
             HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, k, kSmallThreads, lds));
-            if (perCU == 1 && cus >= (int)spans + 3) {
+            HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCUl, kl, kSmallThreads, lds));
+            if (perCU == 1 && perCUl == 1 && cus >= (int)spans + 3) {
                 int rc;
                 if ((rc = h->dalloc((void **)&h->resBar, (size_t)(kBarGroups + 3 * kFlagCopies) * kBarStride * 4)))
                     return rc;
                 if ((rc = h->dalloc((void **)&h->resApSC3, 3 * (size_t)s.numSC))) return rc;
+                if ((rc = h->dalloc((void **)&h->resMfFlag, (size_t)kFlagCopies * kBarStride * 4))) return rc;
                 if ((rc = h->dalloc((void **)&h->resStatus, 4))) return rc;
                 if (!h->resStatusHost) {
                     HIP_TRY(hipHostMalloc((void **)&h->resStatusHost, 4, hipHostMallocDefault));
@@ -778,7 +797,7 @@ int cbm_commit(cbm_handle *h)
     }
 
     // ---- device storage
-    const bool compact = h->o.connFormat != 1 && s.numMF <= 2047 && s.numGO <= 2047 && s.maxnumpGRfromMFtoGR <= 4 &&
+    const bool compact = h->o.connFormat != 1 && s.numMF <= 4095 && s.numGO <= 2047 && s.maxnumpGRfromMFtoGR <= 4 &&
                          s.maxnumpGRfromGOtoGR <= 3 && s.maxnumpGRfromGRtoGO <= 2;
     if (compact) {
         std::vector<uint4> conn(nGR);
@@ -788,7 +807,8 @@ int cbm_commit(cbm_handle *h)
             const uint16_t g[3] = {(uint16_t)(goIn[i].x & 0xFFFF), (uint16_t)(goIn[i].x >> 16),
                                    (uint16_t)(goIn[i].y & 0xFFFF)};
             const uint16_t o[2] = {(uint16_t)(goOut[i].x & 0xFFFF), (uint16_t)(goOut[i].x >> 16)};
-            conn[i] = packGrConn(m, g, o, bsp[i], (uint8_t)(goOutDly[i] & 0xFF), (uint8_t)((goOutDly[i] >> 8) & 0xFF));
+            conn[i] = packGrConn(m, g, o, bsp[i], (uint8_t)(goOutDly[i] & 0xFF), (uint8_t)((goOutDly[i] >> 8) & 0xFF),
+                                 s.numMF);
         }
         if ((rc = h->upload(&h->conn, conn))) return rc;
     } else if ((rc = h->upload(&h->mfIn, mfIn)) || (rc = h->upload(&h->goIn, goIn)) ||
@@ -1098,6 +1118,7 @@ int cbm_set_err_drive(cbm_handle *h, int zone, float rel)
 int cbm_set_grpc_steps(cbm_handle *h, int zone, float ltd, float ltp)
 {
     if (!h || zone < 0 || zone >= (int)h->s.numZones) return fail(CBM_E_ARG, "bad zone");
+    if (h->looping) return fail(CBM_E_STATE, "a closed loop is running: the PF->PC steps change between loops");
     h->ltd[zone] = ltd;
     h->ltp[zone] = ltp;
     return CBM_OK;
@@ -1112,6 +1133,7 @@ int cbm_reset_grpc_steps(cbm_handle *h, int zone)
 int cbm_set_ginc_grgo(cbm_handle *h, float inc)
 {
     if (!h) return fail(CBM_E_ARG, "null handle");
+    if (h->looping) return fail(CBM_E_STATE, "a closed loop is running: gIncGRtoGO changes between loops");
     h->gIncGRtoGO = inc;
     return CBM_OK;
 }
@@ -1119,6 +1141,7 @@ int cbm_set_ginc_grgo(cbm_handle *h, float inc)
 int cbm_reset_ginc_grgo(cbm_handle *h)
 {
     if (!h) return fail(CBM_E_ARG, "null handle");
+    if (h->looping) return fail(CBM_E_STATE, "a closed loop is running: gIncGRtoGO changes between loops");
     h->gIncGRtoGO = h->p.gIncGRtoGO;
     return CBM_OK;
 }
@@ -1334,6 +1357,19 @@ int cbm_step_io_begin(cbm_handle *h, const uint8_t *mf, int population)
     if (h->ioPending) return fail(CBM_E_STATE, "cbm_step_io_end has not been called for the last step");
     if (population < CBM_REC_PC || population > CBM_REC_GO)
         return fail(CBM_E_ARG, "cbm_step_io_begin: population must be PC, BC, SC, IO, NC or GO");
+    if (h->looping) {
+        // a closed loop: the step's MF row and error drive into the input row
+        // of its parity, then the step count the generator block polls
+        if (population != h->loopPop) return fail(CBM_E_ARG, "cbm_step_io_begin: not the closed loop's population");
+        if (h->loopK >= h->loopMax) return fail(CBM_E_STATE, "cbm_step_io_begin: the closed loop's steps are used up");
+        uint8_t *in = h->loopHost + kLoopInOff + (size_t)(h->loopK & 1) * h->loopInPitch;
+        std::memcpy(in, mf, h->s.numMF);
+        std::memcpy(in + h->loopInErr, h->errDrive, 4 * (size_t)h->s.numZones);
+        for (int z = 0; z < kMaxZones; z++) h->errDrive[z] = 0.0f;
+        __atomic_store_n(reinterpret_cast<uint32_t *>(h->loopHost), (uint32_t)(h->loopK + 1), __ATOMIC_RELEASE);
+        h->ioPending = true;
+        return CBM_OK;
+    }
     HIP_TRY(hipSetDevice(h->o.device));
     int rc = syncAll(h);  // the pinned stages of the last step are free again
     if (rc) return rc;
@@ -1357,6 +1393,23 @@ int cbm_step_io_end(cbm_handle *h, uint8_t *out)
 {
     if (!h || !out) return fail(CBM_E_ARG, "null argument");
     if (!h->ioPending) return fail(CBM_E_STATE, "cbm_step_io_begin has not been called");
+    if (h->looping) {
+        // a closed loop: wait for the step count the microzone block stores
+        // after the step's spikes (bounded: the kernel bounds its waits too)
+        const volatile uint32_t *seq = reinterpret_cast<const uint32_t *>(h->loopHost + kLoopOutSeqOff);
+        const uint32_t want = (uint32_t)(h->loopK + 1);
+        const auto t0 = std::chrono::steady_clock::now();
+        uint32_t spin = 0;
+        while ((int32_t)(__atomic_load_n(seq, __ATOMIC_ACQUIRE) - want) < 0) {
+            if (++spin % 1024 == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::seconds(5))
+                return fail(CBM_E_HIP, "closed loop: the step's output did not arrive (the loop's launch ended; "
+                                       "cbm_loop_end reports why)");
+        }
+        std::memcpy(out, h->loopHost + h->loopOutOff + (size_t)(h->loopK & 1) * 4 * h->loopOutWords, h->loopOutBytes);
+        h->loopK++;
+        h->ioPending = false;
+        return CBM_OK;
+    }
     HIP_TRY(hipSetDevice(h->o.device));
     h->ioPending = false;
     HIP_TRY(hipStreamSynchronize(h->stream));
@@ -1425,6 +1478,9 @@ constexpr int kResMaxSteps = 1 << 20;
 // drive goes to the device first, each launch derives its steps' values as
 // launchStep does, and the launches' barrier status is copied to pinned host
 // memory for the next synchronisation to read.
+static void runFields(cbm_handle *h, ResRun *rp);
+static void resAdvance(cbm_handle *h, uint32_t m);
+
 static int runResident(cbm_handle *h, const uint8_t *mf_dev, size_t pitch, int n, const float *err)
 {
     const uint32_t nZ = h->s.numZones;
@@ -1444,19 +1500,8 @@ static int runResident(cbm_handle *h, const uint8_t *mf_dev, size_t pitch, int n
         HIP_TRY(hipMemcpyAsync(h->resErr, err, need * sizeof(float), hipMemcpyHostToDevice, h->stream));
     }
     ResRun r;
-    grConstArgs(h, &r.g);
+    runFields(h, &r);
     r.pitch = pitch;
-    r.maxExtIncVIO = h->p.maxExtIncVIO;
-    r.gIncGRtoGO = h->gIncGRtoGO;
-    r.spans = h->resSpans;
-    r.recOffPF = (h->rec && h->recOffPF != 0xFFFFFFFFu) ? h->recOffPF : 0xFFFFFFFFu;
-    r.bar = h->resBar;
-    r.flag = h->resBar + kBarGroups * kBarStride;
-    r.flagIn = h->resBar + (kBarGroups + kFlagCopies) * kBarStride;
-    r.pb = h->resBar + (kBarGroups + 2 * kFlagCopies) * kBarStride;
-    r.apSC3 = h->resApSC3;
-    r.status = h->resStatus;
-    r.recLds = h->residentRecLds;
     const uint32_t grid = h->resSpans + 3;
     const void *kern = residentKernel(h->resCpt);
     const SmallArgs *sa = h->smallArgs;
@@ -1475,13 +1520,7 @@ static int runResident(cbm_handle *h, const uint8_t *mf_dev, size_t pitch, int n
         void *args[] = {&sa, &r};
         const int rc = launchResident(h, kern, grid, args);
         if (rc) return rc;
-        h->resBarBase += (uint32_t)m * h->resSpans;  // each GR block counts each step
-        h->resFlagBase += (uint32_t)m;
-        h->resFlagInBase += (uint32_t)m;
-        h->resPbBase += (uint32_t)m;
-        h->curTime += (uint64_t)m;
-        for (int z = 0; z < kMaxZones; z++) h->errDrive[z] = 0.0f;  // used by the first step
-        h->skipFold = false;
+        resAdvance(h, (uint32_t)m);
         k0 += m;
     }
     HIP_TRY(hipMemcpyAsync(h->resStatusHost, h->resStatus, 4, hipMemcpyDeviceToHost, h->stream));
@@ -1489,7 +1528,155 @@ static int runResident(cbm_handle *h, const uint8_t *mf_dev, size_t pitch, int n
     return CBM_OK;
 }
 
+// The fields of a resident run that do not depend on its steps.
+static void runFields(cbm_handle *h, ResRun *rp)
+{
+    ResRun &r = *rp;
+    grConstArgs(h, &r.g);
+    r.maxExtIncVIO = h->p.maxExtIncVIO;
+    r.gIncGRtoGO = h->gIncGRtoGO;
+    r.spans = h->resSpans;
+    r.recOffPF = (h->rec && h->recOffPF != 0xFFFFFFFFu) ? h->recOffPF : 0xFFFFFFFFu;
+    r.bar = h->resBar;
+    r.flag = h->resBar + kBarGroups * kBarStride;
+    r.flagIn = h->resBar + (kBarGroups + kFlagCopies) * kBarStride;
+    r.pb = h->resBar + (kBarGroups + 2 * kFlagCopies) * kBarStride;
+    r.apSC3 = h->resApSC3;
+    r.waitTicks = kResWaitTicks;
+    r.hostSeq = nullptr;
+    r.hostIn = nullptr;
+    r.inPitch = r.inErr = 0;
+    r.hostOutSeq = nullptr;
+    r.hostOut = nullptr;
+    r.outWords = r.ioPop = 0;
+    r.errDev = nullptr;
+    r.mfFlag = h->resMfFlag;
+    r.status = h->resStatus;
+    r.recLds = h->residentRecLds;
+}
+
+// The host's records after m steps of a resident launch.
+static void resAdvance(cbm_handle *h, uint32_t m)
+{
+    h->resBarBase += m * h->resSpans;  // each GR block counts each step
+    h->resFlagBase += m;
+    h->resFlagInBase += m;
+    h->resPbBase += m;
+    h->curTime += (uint64_t)m;
+    for (int z = 0; z < kMaxZones; z++) h->errDrive[z] = 0.0f;  // used by the first step
+    h->skipFold = false;
+}
+
 extern "C" {
+
+int cbm_loop_begin(cbm_handle *h, int max_steps, int population)
+{
+    if (!h) return fail(CBM_E_ARG, "null handle");
+    if (!h->committed) return fail(CBM_E_STATE, "cbm_commit has not been called");
+    if (h->looping) return fail(CBM_E_STATE, "cbm_loop_begin: a closed loop is running already");
+    if (h->ioPending) return fail(CBM_E_STATE, "cbm_step_io_end has not been called for the last step");
+    if (!h->resident) return fail(CBM_E_UNSUPPORTED, "cbm_loop_begin: the network does not run resident (cbm_resident)");
+    if (population != CBM_REC_PC && population != CBM_REC_BC && population != CBM_REC_IO && population != CBM_REC_NC)
+        return fail(CBM_E_ARG, "cbm_loop_begin: population must be PC, BC, IO or NC");
+    if (max_steps < 1 || max_steps > kResMaxSteps) return fail(CBM_E_ARG, "cbm_loop_begin: max_steps out of range");
+    HIP_TRY(hipSetDevice(h->o.device));
+    int rc = syncAll(h);
+    if (rc) return rc;
+    const cbm_sizes &s = h->s;
+    const uint32_t nZ = s.numZones;
+    h->loopInErr = roundUp(s.numMF, 16);
+    h->loopInPitch = roundUp(h->loopInErr + 4 * nZ, 256);
+    h->loopOutBytes = (uint32_t)recCells(s, population);
+    h->loopOutWords = (h->loopOutBytes + 3) / 4;
+    h->loopOutOff = (uint32_t)(kLoopInOff + 2 * (size_t)h->loopInPitch);
+    const size_t hostBytes = h->loopOutOff + 2 * roundUp(4 * h->loopOutWords, 256);
+    if (hostBytes > h->loopHostCap) {
+        if (h->loopHost) (void)hipHostFree(h->loopHost);
+        h->loopHost = nullptr;
+        h->loopHostCap = 0;
+        HIP_TRY(hipHostMalloc((void **)&h->loopHost, hostBytes, hipHostMallocCoherent));
+        h->loopHostCap = hostBytes;
+    }
+    std::memset(h->loopHost, 0, hostBytes);
+    h->loopPitch = roundUp(s.numMF, 256);
+    if ((size_t)max_steps > h->loopCap) {
+        if (h->loopMF) (void)hipFree(h->loopMF);
+        if (h->loopErr) (void)hipFree(h->loopErr);
+        h->loopMF = nullptr;
+        h->loopErr = nullptr;
+        h->loopCap = 0;
+        hipError_t e = hipMalloc((void **)&h->loopMF, (size_t)max_steps * h->loopPitch);
+        if (e == hipSuccess) e = hipMalloc((void **)&h->loopErr, (size_t)max_steps * nZ * sizeof(float));
+        if (e != hipSuccess) return fail(CBM_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+        h->loopCap = (size_t)max_steps;
+    }
+    if (!h->loopFlag) {
+        const hipError_t e = hipMalloc((void **)&h->loopFlag, (size_t)kFlagCopies * kBarStride * 4);
+        if (e != hipSuccess) return fail(CBM_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    HIP_TRY(hipMemsetAsync(h->loopFlag, 0, (size_t)kFlagCopies * kBarStride * 4, h->stream));
+    ResRun r;
+    runFields(h, &r);
+    r.mf = h->loopMF;
+    r.pitch = h->loopPitch;
+    r.err = nullptr;
+    r.t0 = h->curTime;
+    for (int z = 0; z < kMaxZones; z++) r.errDrive0[z] = 0.0f;  // (each step's drive comes with its MF row)
+    r.skipFold0 = h->skipFold ? 1u : 0u;
+    r.n = (uint32_t)max_steps;
+    r.barBase = h->resBarBase;
+    r.flagBase = h->resFlagBase;
+    r.flagInBase = h->resFlagInBase;
+    r.pbBase = h->resPbBase;
+    r.waitTicks = kLoopWaitTicks;
+    r.hostSeq = reinterpret_cast<const uint32_t *>(h->loopHost);
+    r.hostIn = h->loopHost + kLoopInOff;
+    r.inPitch = h->loopInPitch;
+    r.inErr = h->loopInErr;
+    r.hostOutSeq = reinterpret_cast<uint32_t *>(h->loopHost + kLoopOutSeqOff);
+    r.hostOut = reinterpret_cast<uint32_t *>(h->loopHost + h->loopOutOff);
+    r.outWords = h->loopOutWords;
+    r.ioPop = (uint32_t)population;
+    r.errDev = h->loopErr;
+    r.mfFlag = h->loopFlag;
+    const SmallArgs *sa = h->smallArgs;
+    void *args[] = {&sa, &r};
+    if ((rc = launchResident(h, residentKernel(h->resCpt, true), h->resSpans + 3, args))) return rc;
+    h->looping = true;
+    h->loopK = 0;
+    h->loopMax = max_steps;
+    h->loopPop = population;
+    return CBM_OK;
+}
+
+int cbm_loop_end(cbm_handle *h, int *steps)
+{
+    if (!h) return fail(CBM_E_ARG, "null handle");
+    if (!h->looping) return fail(CBM_E_STATE, "cbm_loop_end: no closed loop is running");
+    HIP_TRY(hipSetDevice(h->o.device));
+    if (h->ioPending) {  // a step posted and not collected: collect it, so that it counts
+        std::vector<uint8_t> out(h->loopOutBytes);
+        const int rc = cbm_step_io_end(h, out.data());
+        if (rc) {
+            h->looping = false;
+            h->ioPending = false;
+            return rc;
+        }
+    }
+    // the stop before the next step; the launch ends after the steps posted
+    const uint32_t m = (uint32_t)h->loopK;
+    __atomic_store_n(reinterpret_cast<uint32_t *>(h->loopHost), kHostStop | m, __ATOMIC_RELEASE);
+    h->looping = false;
+    h->ioPending = false;
+    if (m > 0)  // the MF input field holds the last step's row, as after cbm_step_io
+        HIP_TRY(hipMemcpyAsync(h->dmf, h->loopMF + (size_t)(m - 1) * h->loopPitch, h->s.numMF,
+                               hipMemcpyDeviceToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(h->resStatusHost, h->resStatus, 4, hipMemcpyDeviceToHost, h->stream));
+    h->resCheck = true;
+    resAdvance(h, m);
+    if (steps) *steps = (int)m;
+    return syncAll(h);
+}
 
 int cbm_run_mf_train(cbm_handle *h, const uint8_t *mf_dev, size_t pitch, int n, const float *err)
 {

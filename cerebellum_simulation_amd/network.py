@@ -351,6 +351,19 @@ class Network:
         check(L.cbm_step_io_end(self._h, out.ctypes.data), "cbm_step_io_end")
         return out
 
+    def loop_begin(self, max_steps: int, population: str = "NC") -> None:
+        """Start a closed loop (cbm_loop_begin): the step_io calls that follow
+        go to one resident run of up to max_steps steps, with no launch or
+        stream synchronisation per step.  population: PC, BC, IO or NC."""
+        k = self.STEP_IO_POPULATIONS.index(population)
+        check(_abi.hip_lib().cbm_loop_begin(self._h, max_steps, k), "cbm_loop_begin")
+
+    def loop_end(self) -> int:
+        """Stop the closed loop after the steps posted; their number."""
+        n = C.c_int()
+        check(_abi.hip_lib().cbm_loop_end(self._h, C.byref(n)), "cbm_loop_end")
+        return n.value
+
     def device_bytes(self) -> int:
         b = C.c_size_t()
         check(_abi.hip_lib().cbm_device_bytes(self._h, C.byref(b)), "cbm_device_bytes")

@@ -160,7 +160,7 @@ typedef struct cbm_options {
      * the network's state is then undefined).  All give the same bits. */
     uint32_t stepMode;
     /* Device layout of the GR connectivity: 0 = pick (one 16-byte record per
-     * GR when numMF and numGO are at most 2047 and the fan-ins are at most 4
+     * GR when numMF is at most 4095, numGO at most 2047 and the fan-ins at most 4
      * MF, 3 GO and 2 GO targets; else 29 bytes in five arrays), 1 = the
      * five arrays always.  Both give the same bits. */
     uint32_t connFormat;
@@ -420,6 +420,26 @@ int cbm_step(cbm_handle *h, int n);
  * needs its _end before the next _begin. */
 int cbm_step_io_begin(cbm_handle *h, const uint8_t *mf, int population);
 int cbm_step_io_end(cbm_handle *h, uint8_t *out);
+
+/* A closed loop on a resident network (cbm_resident): cbm_loop_begin
+ * launches one resident run of up to max_steps steps that takes each step's
+ * input from pinned host memory, and the cbm_step_io_begin / _end pairs that
+ * follow (population: the one given here, CBM_REC_PC, _BC, _IO or _NC) post
+ * the step's MF bytes and the error drive set by cbm_set_err_drive, and
+ * collect the step's spikes, with no launch and no stream synchronisation per
+ * step.  The results are those of the same pairs without the loop.
+ * cbm_loop_end stops the run after the steps posted (*steps: their number)
+ * and reports a wait that timed out as CBM_E_HIP.  Between the two only
+ * cbm_step_io_begin / _end and cbm_set_err_drive may be called on the handle
+ * (other calls fail with CBM_E_STATE); the host must post each step within
+ * 2 s of the last one's output, or the run ends with an error.  The loop's
+ * launch holds every CU until cbm_loop_end: kernels of other handles or
+ * streams of the process wait for it.  No reference
+ * counterpart: it serves the closed-loop drivers of SimThread::run
+ * (Matthew_Sim_CBM/src/simthread.cpp:60-165), whose per-ms round trip it
+ * shortens. */
+int cbm_loop_begin(cbm_handle *h, int max_steps, int population);
+int cbm_loop_end(cbm_handle *h, int *steps);
 
 /* n steps where step k reads MF spikes from mf_dev + k*pitch (a DEVICE
  * pointer to n rows of numMF bytes, e.g. a pre-generated spike train).
