@@ -283,7 +283,8 @@ def cartpole_bench(which: str) -> dict:
                         "per zone (the sizes of CBM_Params/conParams.txt)" if which == "full" else
                         f"cart-pole, two microzones, closed loop; {s.numGR:,} GR"),
            "steps": done, "sim_ms_per_s": round(done / el, 1), "us_per_step": round(el / done * 1e6, 2),
-           "network_step_only_us": round(step_only / steps * 1e6, 2)}
+           "network_step_only_us": round(step_only / steps * 1e6, 2),
+           "resident_loop": bool(net.resident)}
     env.close()
     net.close()
     return out

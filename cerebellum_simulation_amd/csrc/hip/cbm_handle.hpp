@@ -21,7 +21,7 @@ __global__ void step_span_kernel(const SmallArgs *args, const SmallStep st, cons
 template <int CPT, int FMT>
 __global__ void gr_span_kernel(const GrArgs a, const GrStep st, const uint32_t spans);
 __global__ void small_step_kernel(const SmallArgs *args, const SmallStep st);
-template <int CPT, bool LOOP>  // LOOP: a closed loop (cbm_loop_begin)
+template <int CPT, bool LOOP, bool BIGNC>  // LOOP: a closed loop (cbm_loop_begin); SPILL: some small-population fields left in global memory
 __global__ void step_resident_kernel(const SmallArgs *args, const ResRun r);
 __global__ void stream_ceiling_kernel(StreamArgs a);
 __global__ void pfpc_fold_kernel(const float *chunk, float *out, uint32_t nChunks, uint32_t perPC, uint32_t numPC,
@@ -61,6 +61,7 @@ struct cbm_handle {
     // spans + 3 workgroups, one per CU, GR state in registers, step counters
     // and flags between steps
     bool resident = false;
+    bool resSpill = false;            // some small-population fields stay in global memory
     size_t residentLds = 0;           // its dynamic LDS (the small step's, or a span's + its records)
     uint32_t residentRecLds = 0;      // byte offset of the GR records in that LDS
     uint32_t *resBar = nullptr;       // GR step counters [kBarGroups * kBarStride], then the step-count

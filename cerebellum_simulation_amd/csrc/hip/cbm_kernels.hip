@@ -706,12 +706,16 @@ __device__ __forceinline__ G<T> *F(const SmallArgs &a, int f)
 }
 
 // Field f of the small populations: in the small block's LDS during a
-// resident run that keeps them there (LF), else in global memory.
-template <bool LF, typename T>
+// resident run that keeps them there (LF 1), else in global memory (LF 0);
+// LF 2: in LDS unless the run left it in global memory (fLds[f] kNotInLds)
+template <int LF, typename T>
 __device__ __forceinline__ auto SF(const SmallArgs &a, int f, uint8_t *lds)
 {
-    if constexpr (LF) return reinterpret_cast<T *>(lds + a.fLds[f]);
-    else return F<T>(a, f);
+    if constexpr (LF == 1) return reinterpret_cast<T *>(lds + a.fLds[f]);
+    else if constexpr (LF == 2) {
+        const uint32_t o = a.fLds[f];
+        return o != kNotInLds ? reinterpret_cast<T *>(lds + o) : reinterpret_cast<T *>(a.f[f]);
+    } else return F<T>(a, f);
 }
 
 template <typename T>
@@ -775,15 +779,17 @@ __device__ __forceinline__ float lds_sum_seq(const float *x, uint32_t n, float a
 // GO->GO) or the microzones (PC, BC, IO, NC, their synapses, outputs and
 // plasticity).  The microzone part routes the SC spikes to PC at the start of
 // the next step (from the input-layer part's apSCout) instead of at the end
-// of this one.
+// of this one.  SPILL: with the fields in LDS, a field the run could not fit
+// there stays in global memory (fLds[f] kNotInLds; checked per field).
 constexpr int kPartAll = 0, kPartIn = 1, kPartMz = 2;
-template <bool X = false, bool LF = false, int PART = kPartAll>
+template <bool X = false, bool LF = false, int PART = kPartAll, bool SPILL = false>
 __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &st, const float *errDrive,
                                            uint8_t *lds)
 {
     const cbm_sizes &s = a.s;
     const cbm_act_params &p = a.p;
     const SmallLds &L = a.lds;
+    constexpr int LFM = LF ? (SPILL ? 2 : 1) : 0;  // where SF finds the fields
     int tid = threadIdx.x;
     // resident: as in gr_span_impl, values derived from the thread index are
     // computed again in each step instead of being held across the step loop
@@ -830,8 +836,8 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
         const G<uint32_t> *mfw = (const G<uint32_t> *)st.mf;
         uint32_t *mfl = at<uint32_t>(lds, L.mf);
         u32x4 *bd = at<u32x4>(lds, L.idx);
-        auto histMF = SF<LF, uint8_t>(a, CBM_F_HIST_MF, lds);
-        auto apBufMF = SF<LF, uint32_t>(a, CBM_F_APBUF_MF, lds);
+        auto histMF = SF<LFM, uint8_t>(a, CBM_F_HIST_MF, lds);
+        auto apBufMF = SF<LFM, uint32_t>(a, CBM_F_APBUF_MF, lds);
         for (uint32_t i = tid; i < n; i += nThr) {
             // every array is read at a clamped index (in bounds, no branch),
             // so all of the iteration's loads are issued before the first wait
@@ -851,11 +857,11 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
             // (microzone part) the previous step's SC spikes
             const uint8_t scPrev = PART == kPartMz ? xldg<X>(gp(st.apSCin) + min(i, s.numSC - 1)) : (uint8_t)0;
             const uint32_t pz = ip / L.popStride, pk = ip % L.popStride;
-            const auto pw = pk < 3 ? SF<LF, uint32_t>(a, CBM_F_PC_POPACT_CTRS, lds) + pz * 3 + pk
-                                           : SF<LF, uint32_t>(a, CBM_F_HIST_PC_POPACT, lds) + (size_t)pz * bins +
+            const auto pw = pk < 3 ? SF<LFM, uint32_t>(a, CBM_F_PC_POPACT_CTRS, lds) + pz * 3 + pk
+                                           : SF<LFM, uint32_t>(a, CBM_F_HIST_PC_POPACT, lds) + (size_t)pz * bins +
                                                  min(pk - 3, bins - 1);
             const uint32_t pv32 = *pw;
-            const uint8_t pv8 = SF<LF, uint8_t>(a, CBM_F_MFNC_NOLTP_NOLTD, lds)[pz * 2 + (pk >= 3 + bins ? pk - 3 - bins : 0)];
+            const uint8_t pv8 = SF<LFM, uint8_t>(a, CBM_F_MFNC_NOLTP_NOLTD, lds)[pz * 2 + (pk >= 3 + bins ? pk - 3 - bins : 0)];
             const uint8_t m = gp(st.mf)[im];
             const uint8_t h = histMF[im];
             const uint32_t b = apBufMF[im];
@@ -905,19 +911,19 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
     // PC->NC of calcNCActivities (mzone.cpp:358-383), NC->IO of
     // calcIOActivities (mzone.cpp:272-280)
     if constexpr (MZ) {
-        auto gSC = SF<LF, float>(a, CBM_F_G_SC_PC, lds);
-        auto inSC = SF<LF, uint8_t>(a, CBM_F_INPUT_SC_PC, lds);
-        auto inMF = SF<LF, uint8_t>(a, CBM_F_INPUT_MF_NC, lds);
-        auto wMF = SF<LF, float>(a, CBM_F_MF_SYNW_NC, lds);
-        auto gmN = SF<LF, float>(a, CBM_F_GMAX_NMDA_MF_NC, lds);
-        auto gmA = SF<LF, float>(a, CBM_F_GMAX_AMPA_MF_NC, lds);
-        auto gN = SF<LF, float>(a, CBM_F_G_MF_NMDA_NC, lds);
-        auto gA = SF<LF, float>(a, CBM_F_G_MF_AMPA_NC, lds);
-        auto inPC = SF<LF, uint8_t>(a, CBM_F_INPUT_PC_NC, lds);
-        auto scPC = SF<LF, float>(a, CBM_F_G_PC_SCALE_NC, lds);
-        auto gPC = SF<LF, float>(a, CBM_F_G_PC_NC, lds);
-        auto gNC = SF<LF, float>(a, CBM_F_G_NC_IO, lds);
-        auto inNC = SF<LF, uint8_t>(a, CBM_F_INPUT_NC_IO, lds);
+        auto gSC = SF<LFM, float>(a, CBM_F_G_SC_PC, lds);
+        auto inSC = SF<LFM, uint8_t>(a, CBM_F_INPUT_SC_PC, lds);
+        auto inMF = SF<LFM, uint8_t>(a, CBM_F_INPUT_MF_NC, lds);
+        auto wMF = SF<LFM, float>(a, CBM_F_MF_SYNW_NC, lds);
+        auto gmN = SF<LFM, float>(a, CBM_F_GMAX_NMDA_MF_NC, lds);
+        auto gmA = SF<LFM, float>(a, CBM_F_GMAX_AMPA_MF_NC, lds);
+        auto gN = SF<LFM, float>(a, CBM_F_G_MF_NMDA_NC, lds);
+        auto gA = SF<LFM, float>(a, CBM_F_G_MF_AMPA_NC, lds);
+        auto inPC = SF<LFM, uint8_t>(a, CBM_F_INPUT_PC_NC, lds);
+        auto scPC = SF<LFM, float>(a, CBM_F_G_PC_SCALE_NC, lds);
+        auto gPC = SF<LFM, float>(a, CBM_F_G_PC_NC, lds);
+        auto gNC = SF<LFM, float>(a, CBM_F_G_NC_IO, lds);
+        auto inNC = SF<LFM, uint8_t>(a, CBM_F_INPUT_NC_IO, lds);
         const uint32_t nS = nZ * s.numSC, nM = nZ * s.numNC * kMF, nP = nZ * s.numNC * kPC,
                        nI = nZ * s.numIO * kNCIO;
         const uint32_t rounds = (max(max(nS, nM), max(nP, nI)) + nThr - 1) / nThr;
@@ -980,7 +986,7 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
     // ======================= phase 1
     // MZone::updateMFNCOut (mzone.cpp:514-524): the caller's raw MF bytes
     if constexpr (MZ) {
-        auto inMF = SF<LF, uint8_t>(a, CBM_F_INPUT_MF_NC, lds);
+        auto inMF = SF<LFM, uint8_t>(a, CBM_F_INPUT_MF_NC, lds);
         for (uint32_t q = tid; q < nZ * s.numNC * kMF; q += nThr) inMF[q] = s_mf[s_idx[L.iMFNC + q]];
     }
     // cell updates: MZone::calcPCActivities (mzone.cpp:187-226),
@@ -1018,22 +1024,22 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
             const uint32_t iP = min(cP, nPCc - 1), iB = min(cB, nBCc - 1), iS = min(cS, nSCc - 1);
             const uint32_t iI = min(cI, nIOc - 1);
             const float pcPF = at<float>(lds, L.inPF)[iP];
-            const uint32_t pcIn = SF<LF, uint32_t>(a, CBM_F_INPUT_BC_PC, lds)[iP];
-            const float pcGPF = SF<LF, float>(a, CBM_F_G_PF_PC, lds)[iP], pcGBC = SF<LF, float>(a, CBM_F_G_BC_PC, lds)[iP];
-            const float pcV = SF<LF, float>(a, CBM_F_VM_PC, lds)[iP], pcTh = SF<LF, float>(a, CBM_F_THRESH_PC, lds)[iP];
-            const uint32_t pcAb = SF<LF, uint32_t>(a, CBM_F_APBUF_PC, lds)[iP];
+            const uint32_t pcIn = SF<LFM, uint32_t>(a, CBM_F_INPUT_BC_PC, lds)[iP];
+            const float pcGPF = SF<LFM, float>(a, CBM_F_G_PF_PC, lds)[iP], pcGBC = SF<LFM, float>(a, CBM_F_G_BC_PC, lds)[iP];
+            const float pcV = SF<LFM, float>(a, CBM_F_VM_PC, lds)[iP], pcTh = SF<LFM, float>(a, CBM_F_THRESH_PC, lds)[iP];
+            const uint32_t pcAb = SF<LFM, uint32_t>(a, CBM_F_APBUF_PC, lds)[iP];
             const uint32_t bcPF = at<uint32_t>(lds, L.inBC)[iB & (s.numBC - 1)];
-            const uint32_t bcIn = SF<LF, uint32_t>(a, CBM_F_INPUT_PC_BC, lds)[iB];
-            const float bcGPF = SF<LF, float>(a, CBM_F_G_PF_BC, lds)[iB], bcGPC = SF<LF, float>(a, CBM_F_G_PC_BC, lds)[iB];
-            const float bcV = SF<LF, float>(a, CBM_F_VM_BC, lds)[iB], bcTh = SF<LF, float>(a, CBM_F_THRESH_BC, lds)[iB];
-            const uint32_t bcAb = SF<LF, uint32_t>(a, CBM_F_APBUF_BC, lds)[iB];
+            const uint32_t bcIn = SF<LFM, uint32_t>(a, CBM_F_INPUT_PC_BC, lds)[iB];
+            const float bcGPF = SF<LFM, float>(a, CBM_F_G_PF_BC, lds)[iB], bcGPC = SF<LFM, float>(a, CBM_F_G_PC_BC, lds)[iB];
+            const float bcV = SF<LFM, float>(a, CBM_F_VM_BC, lds)[iB], bcTh = SF<LFM, float>(a, CBM_F_THRESH_BC, lds)[iB];
+            const uint32_t bcAb = SF<LFM, uint32_t>(a, CBM_F_APBUF_BC, lds)[iB];
             const uint32_t scPF = at<uint32_t>(lds, L.inSC)[iS];
-            const float scGPF = SF<LF, float>(a, CBM_F_G_PF_SC, lds)[iS], scV = SF<LF, float>(a, CBM_F_VM_SC, lds)[iS];
-            const float scTh = SF<LF, float>(a, CBM_F_THRESH_SC, lds)[iS];
-            const uint32_t scAb = SF<LF, uint32_t>(a, CBM_F_APBUF_SC, lds)[iS];
-            const float ioV = SF<LF, float>(a, CBM_F_VM_IO, lds)[iI], ioTh = SF<LF, float>(a, CBM_F_THRESH_IO, lds)[iI];
-            const float ioVC = SF<LF, float>(a, CBM_F_VCOUPLE_IO, lds)[iI];
-            const uint32_t ioAb = SF<LF, uint32_t>(a, CBM_F_APBUF_IO, lds)[iI];
+            const float scGPF = SF<LFM, float>(a, CBM_F_G_PF_SC, lds)[iS], scV = SF<LFM, float>(a, CBM_F_VM_SC, lds)[iS];
+            const float scTh = SF<LFM, float>(a, CBM_F_THRESH_SC, lds)[iS];
+            const uint32_t scAb = SF<LFM, uint32_t>(a, CBM_F_APBUF_SC, lds)[iS];
+            const float ioV = SF<LFM, float>(a, CBM_F_VM_IO, lds)[iI], ioTh = SF<LFM, float>(a, CBM_F_THRESH_IO, lds)[iI];
+            const float ioVC = SF<LFM, float>(a, CBM_F_VCOUPLE_IO, lds)[iI];
+            const uint32_t ioAb = SF<LFM, uint32_t>(a, CBM_F_APBUF_IO, lds)[iI];
             const int32_t ioTm = at<int32_t>(lds, L.inTm)[iI];
             // ---- PC
             if (MZ && hPC) {
@@ -1048,13 +1054,13 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
                      (gSCPCSum * (p.eSCtoPC - vv));
                 th = th + (p.threshDecPC * (p.threshRestPC - th));
                 const uint8_t ap = vv > th;
-                SF<LF, float>(a, CBM_F_G_PF_PC, lds)[i] = gPF;
-                SF<LF, float>(a, CBM_F_G_BC_PC, lds)[i] = gBC;
-                SF<LF, float>(a, CBM_F_VM_PC, lds)[i] = vv;
-                SF<LF, uint8_t>(a, CBM_F_AP_PC, lds)[i] = ap;
+                SF<LFM, float>(a, CBM_F_G_PF_PC, lds)[i] = gPF;
+                SF<LFM, float>(a, CBM_F_G_BC_PC, lds)[i] = gBC;
+                SF<LFM, float>(a, CBM_F_VM_PC, lds)[i] = vv;
+                SF<LFM, uint8_t>(a, CBM_F_AP_PC, lds)[i] = ap;
                 s_apPC[i] = ap;
-                SF<LF, uint32_t>(a, CBM_F_APBUF_PC, lds)[i] = (pcAb << 1) | (ap * 0x00000001u);
-                SF<LF, float>(a, CBM_F_THRESH_PC, lds)[i] = ap * p.threshMaxPC + (!ap) * th;
+                SF<LFM, uint32_t>(a, CBM_F_APBUF_PC, lds)[i] = (pcAb << 1) | (ap * 0x00000001u);
+                SF<LFM, float>(a, CBM_F_THRESH_PC, lds)[i] = ap * p.threshMaxPC + (!ap) * th;
                 if (ap) atomicAdd(&s_pop[z * L.popStride + 2], 1u);
             }
             // ---- BC
@@ -1068,13 +1074,13 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
                 vv = vv + (p.gLeakBC * (p.eLeakBC - vv)) - (gPF * vv) + (gPC * (p.ePCtoBC - vv));
                 th = th + p.threshDecBC * (p.threshRestBC - th);
                 const uint8_t ap = vv > th;
-                SF<LF, float>(a, CBM_F_G_PF_BC, lds)[i] = gPF;
-                SF<LF, float>(a, CBM_F_G_PC_BC, lds)[i] = gPC;
-                SF<LF, float>(a, CBM_F_VM_BC, lds)[i] = vv;
-                SF<LF, uint8_t>(a, CBM_F_AP_BC, lds)[i] = ap;
+                SF<LFM, float>(a, CBM_F_G_PF_BC, lds)[i] = gPF;
+                SF<LFM, float>(a, CBM_F_G_PC_BC, lds)[i] = gPC;
+                SF<LFM, float>(a, CBM_F_VM_BC, lds)[i] = vv;
+                SF<LFM, uint8_t>(a, CBM_F_AP_BC, lds)[i] = ap;
                 s_apBC[i] = ap;
-                SF<LF, uint32_t>(a, CBM_F_APBUF_BC, lds)[i] = (bcAb << 1) | (ap * 0x00000001u);
-                SF<LF, float>(a, CBM_F_THRESH_BC, lds)[i] = ap * p.threshMaxBC + (!ap) * (th);
+                SF<LFM, uint32_t>(a, CBM_F_APBUF_BC, lds)[i] = (bcAb << 1) | (ap * 0x00000001u);
+                SF<LFM, float>(a, CBM_F_THRESH_BC, lds)[i] = ap * p.threshMaxBC + (!ap) * (th);
             }
             // ---- SC
             if (IN && hSC) {
@@ -1085,12 +1091,12 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
                 vv = vv + (p.gLeakSC * (p.eLeakSC - vv)) - gPF * vv;
                 const uint8_t ap = vv > th;
                 th = th + p.threshDecSC * (p.threshRestSC - th);
-                SF<LF, float>(a, CBM_F_G_PF_SC, lds)[i] = gPF;
-                SF<LF, float>(a, CBM_F_VM_SC, lds)[i] = vv;
-                SF<LF, uint8_t>(a, CBM_F_AP_SC, lds)[i] = ap;
+                SF<LFM, float>(a, CBM_F_G_PF_SC, lds)[i] = gPF;
+                SF<LFM, float>(a, CBM_F_VM_SC, lds)[i] = vv;
+                SF<LFM, uint8_t>(a, CBM_F_AP_SC, lds)[i] = ap;
                 s_apSC[i] = ap;
-                SF<LF, uint32_t>(a, CBM_F_APBUF_SC, lds)[i] = (scAb << 1) | (ap * 0x00000001u);
-                SF<LF, float>(a, CBM_F_THRESH_SC, lds)[i] = ap * p.threshMaxSC + (!ap) * (th);
+                SF<LFM, uint32_t>(a, CBM_F_APBUF_SC, lds)[i] = (scAb << 1) | (ap * 0x00000001u);
+                SF<LFM, float>(a, CBM_F_THRESH_SC, lds)[i] = ap * p.threshMaxSC + (!ap) * (th);
             }
             // ---- IO
             if (MZ && hIO) {
@@ -1100,12 +1106,12 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
                 gNCSum = gNCSum / 3.1;
                 vv = vv + p.gLeakIO * (p.eLeakIO - vv) + gNCSum * 1.5 * (p.eNCtoIO - vv) + ioVC + errDrive[z];
                 const uint8_t ap = vv > ioTh;
-                SF<LF, float>(a, CBM_F_VM_IO, lds)[i] = vv;
+                SF<LFM, float>(a, CBM_F_VM_IO, lds)[i] = vv;
                 s_vIO[i] = vv;
-                SF<LF, uint8_t>(a, CBM_F_AP_IO, lds)[i] = ap;
+                SF<LFM, uint8_t>(a, CBM_F_AP_IO, lds)[i] = ap;
                 s_apIO[i] = ap;
-                SF<LF, uint32_t>(a, CBM_F_APBUF_IO, lds)[i] = (ioAb << 1) | (ap * 0x00000001u);
-                SF<LF, float>(a, CBM_F_THRESH_IO, lds)[i] =
+                SF<LFM, uint32_t>(a, CBM_F_APBUF_IO, lds)[i] = (ioAb << 1) | (ap * 0x00000001u);
+                SF<LFM, float>(a, CBM_F_THRESH_IO, lds)[i] =
                     p.threshMaxIO * ap + (!ap) * (ioTh + p.threshDecIO * (p.threshRestIO - ioTh));
                 xstg<X>(gp(a.timer2) + (size_t)par * nIOc + i, (int32_t)((!ap) * (ioTm + 1) + ap * p.tsLTDStartAPIO));
             }
@@ -1115,20 +1121,20 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
             uint32_t totalGRIn = 0, inMF = 0, goAb = 0;
             {
                 totalGRIn = at<uint32_t>(lds, L.inGO)[i];
-                inMF = SF<LF, uint32_t>(a, CBM_F_INPUT_MF_GO, lds)[i];
-                inDep = SF<LF, float>(a, CBM_F_INPUT_GOGABA_SYNDEP_GO, lds)[i];
-                gMF = SF<LF, float>(a, CBM_F_G_MF_GO, lds)[i];
-                gGR = SF<LF, float>(a, CBM_F_G_GR_GO, lds)[i];
-                gGO = SF<LF, float>(a, CBM_F_G_GO_GO, lds)[i];
-                glu = SF<LF, float>(a, CBM_F_GLU_GO, lds)[i];
-                mG = SF<LF, float>(a, CBM_F_MGLUR_GO, lds)[i];
-                gInc = SF<LF, float>(a, CBM_F_G_MGLURINC_GO, lds)[i];
-                gMG = SF<LF, float>(a, CBM_F_G_MGLUR_GO, lds)[i];
-                thv = SF<LF, float>(a, CBM_F_THRESH_GO, lds)[i];
-                v = SF<LF, float>(a, CBM_F_VM_GO, lds)[i];
-                vC = SF<LF, float>(a, CBM_F_VCOUPLE_GO, lds)[i];
-                sc = SF<LF, float>(a, CBM_F_GO_GABA_OUT_SYNSCALE_GOGO, lds)[i];
-                goAb = SF<LF, uint32_t>(a, CBM_F_APBUF_GO, lds)[i];
+                inMF = SF<LFM, uint32_t>(a, CBM_F_INPUT_MF_GO, lds)[i];
+                inDep = SF<LFM, float>(a, CBM_F_INPUT_GOGABA_SYNDEP_GO, lds)[i];
+                gMF = SF<LFM, float>(a, CBM_F_G_MF_GO, lds)[i];
+                gGR = SF<LFM, float>(a, CBM_F_G_GR_GO, lds)[i];
+                gGO = SF<LFM, float>(a, CBM_F_G_GO_GO, lds)[i];
+                glu = SF<LFM, float>(a, CBM_F_GLU_GO, lds)[i];
+                mG = SF<LFM, float>(a, CBM_F_MGLUR_GO, lds)[i];
+                gInc = SF<LFM, float>(a, CBM_F_G_MGLURINC_GO, lds)[i];
+                gMG = SF<LFM, float>(a, CBM_F_G_MGLUR_GO, lds)[i];
+                thv = SF<LFM, float>(a, CBM_F_THRESH_GO, lds)[i];
+                v = SF<LFM, float>(a, CBM_F_VM_GO, lds)[i];
+                vC = SF<LFM, float>(a, CBM_F_VCOUPLE_GO, lds)[i];
+                sc = SF<LFM, float>(a, CBM_F_GO_GABA_OUT_SYNSCALE_GOGO, lds)[i];
+                goAb = SF<LFM, uint32_t>(a, CBM_F_APBUF_GO, lds)[i];
             }
             {
                 gMF = inMF * p.gIncMFtoGO + gMF * p.gDecMFtoGO;
@@ -1141,31 +1147,31 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
                 thv = thv + (p.threshRestGO - thv) * p.threshDecGO;
                 v = v + (p.gLeakGO * (p.eLeakGO - v)) + (gGO * (p.eGABAGO - v)) - (gMF + gGR) * v + vC;
                 const uint8_t ap = v > thv;
-                SF<LF, uint32_t>(a, CBM_F_SUM_GR_INPUT_GO, lds)[i] = totalGRIn;
-                SF<LF, float>(a, CBM_F_SUM_GO_INPUT_GO, lds)[i] = inDep;
-                SF<LF, float>(a, CBM_F_G_MF_GO, lds)[i] = gMF;
-                SF<LF, float>(a, CBM_F_G_GR_GO, lds)[i] = gGR;
-                SF<LF, float>(a, CBM_F_G_GO_GO, lds)[i] = gGO;
-                SF<LF, float>(a, CBM_F_GLU_GO, lds)[i] = glu;
-                SF<LF, float>(a, CBM_F_MGLUR_GO, lds)[i] = mG;
-                SF<LF, float>(a, CBM_F_G_MGLURINC_GO, lds)[i] = gInc;
-                SF<LF, float>(a, CBM_F_G_MGLUR_GO, lds)[i] = gMG;
-                SF<LF, float>(a, CBM_F_VM_GO, lds)[i] = v;
+                SF<LFM, uint32_t>(a, CBM_F_SUM_GR_INPUT_GO, lds)[i] = totalGRIn;
+                SF<LFM, float>(a, CBM_F_SUM_GO_INPUT_GO, lds)[i] = inDep;
+                SF<LFM, float>(a, CBM_F_G_MF_GO, lds)[i] = gMF;
+                SF<LFM, float>(a, CBM_F_G_GR_GO, lds)[i] = gGR;
+                SF<LFM, float>(a, CBM_F_G_GO_GO, lds)[i] = gGO;
+                SF<LFM, float>(a, CBM_F_GLU_GO, lds)[i] = glu;
+                SF<LFM, float>(a, CBM_F_MGLUR_GO, lds)[i] = mG;
+                SF<LFM, float>(a, CBM_F_G_MGLURINC_GO, lds)[i] = gInc;
+                SF<LFM, float>(a, CBM_F_G_MGLUR_GO, lds)[i] = gMG;
+                SF<LFM, float>(a, CBM_F_VM_GO, lds)[i] = v;
                 xstg<X>(gp(a.apGO2) + (size_t)par * s.numGO + i, ap);
                 s_apGO[i] = ap;
                 s_scale[i] = sc;
                 if (a.anyCoupling) s_vGO[i] = v;
-                SF<LF, uint32_t>(a, CBM_F_APBUF_GO, lds)[i] = (goAb << 1) | (ap * 0x00000001u);
-                SF<LF, float>(a, CBM_F_THRESH_GO, lds)[i] = ap * p.threshMaxGO + (!ap) * thv;
+                SF<LFM, uint32_t>(a, CBM_F_APBUF_GO, lds)[i] = (goAb << 1) | (ap * 0x00000001u);
+                SF<LFM, float>(a, CBM_F_THRESH_GO, lds)[i] = ap * p.threshMaxGO + (!ap) * thv;
             }
         }
         // ---- NC, once its input sums are in
         if constexpr (MZ) {
             __syncthreads();
             for (uint32_t i = (tid - L.oNC) & (nThr - 1); i < nNCc; i += nThr) {
-                float vv = SF<LF, float>(a, CBM_F_VM_NC, lds)[i], th = SF<LF, float>(a, CBM_F_THRESH_NC, lds)[i];
-                float r = SF<LF, float>(a, CBM_F_SYN_IO_PRELEASE_NC, lds)[i];
-                const uint32_t ncAb = SF<LF, uint32_t>(a, CBM_F_APBUF_NC, lds)[i];
+                float vv = SF<LFM, float>(a, CBM_F_VM_NC, lds)[i], th = SF<LFM, float>(a, CBM_F_THRESH_NC, lds)[i];
+                float r = SF<LFM, float>(a, CBM_F_SYN_IO_PRELEASE_NC, lds)[i];
+                const uint32_t ncAb = SF<LFM, uint32_t>(a, CBM_F_APBUF_NC, lds)[i];
                 const float ms = p.msPerTimeStep;
                 float gMFNMDASum = ncSum[i], gMFAMPASum = ncSum[nNCc + i], gPCNCSum = ncSum[2 * nNCc + i];
                 gMFNMDASum = gMFNMDASum * ms / ((float)kMF);
@@ -1175,14 +1181,14 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
                 vv = vv + p.gLeakNC * (p.eLeakNC - vv) - (gMFNMDASum + gMFAMPASum) * vv + gPCNCSum * (p.ePCtoNC - vv);
                 th = th + p.threshDecNC * (p.threshRestNC - th);
                 const uint8_t ap = vv > th;
-                SF<LF, float>(a, CBM_F_VM_NC, lds)[i] = vv;
-                SF<LF, uint8_t>(a, CBM_F_AP_NC, lds)[i] = ap;
+                SF<LFM, float>(a, CBM_F_VM_NC, lds)[i] = vv;
+                SF<LFM, uint8_t>(a, CBM_F_AP_NC, lds)[i] = ap;
                 s_apNC[i] = ap;
-                SF<LF, uint32_t>(a, CBM_F_APBUF_NC, lds)[i] = (ncAb << 1) | (ap * 0x00000001u);
-                SF<LF, float>(a, CBM_F_THRESH_NC, lds)[i] = ap * p.threshMaxNC + (!ap) * th;
+                SF<LFM, uint32_t>(a, CBM_F_APBUF_NC, lds)[i] = (ncAb << 1) | (ap * 0x00000001u);
+                SF<LFM, float>(a, CBM_F_THRESH_NC, lds)[i] = ap * p.threshMaxNC + (!ap) * th;
                 r = r * expf(-ms / (p.relPDecTSofNCtoIO * expf(-r / p.relPDecTTofNCtoIO) + p.relPDecT0ofNCtoIO));
                 r = r + ap * p.relPIncNCtoIO * expf(-r / p.relPIncTauNCtoIO);
-                SF<LF, float>(a, CBM_F_SYN_IO_PRELEASE_NC, lds)[i] = r;
+                SF<LFM, float>(a, CBM_F_SYN_IO_PRELEASE_NC, lds)[i] = r;
                 s_rel[i] = r;
             }
         }
@@ -1194,9 +1200,9 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
     // gathers its GABA inputs in increasing source order, which is the order
     // the reference adds them
     if constexpr (IN) {
-        auto inMFGO = SF<LF, uint32_t>(a, CBM_F_INPUT_MF_GO, lds);
-        auto inGO = SF<LF, uint32_t>(a, CBM_F_INPUT_GO_GO, lds);
-        auto inDep = SF<LF, float>(a, CBM_F_INPUT_GOGABA_SYNDEP_GO, lds);
+        auto inMFGO = SF<LFM, uint32_t>(a, CBM_F_INPUT_MF_GO, lds);
+        auto inGO = SF<LFM, uint32_t>(a, CBM_F_INPUT_GO_GO, lds);
+        auto inDep = SF<LFM, float>(a, CBM_F_INPUT_GOGABA_SYNDEP_GO, lds);
         for (uint32_t g = tid; g < s.numGO; g += nThr) {
             uint32_t c = 0;
             float dep = 0;
@@ -1221,7 +1227,7 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
     // the threshold update is in place in cell order, so it runs on one lane.
     // Without coupling inputs (or with a zero ratio) both terms are zero.
     if constexpr (IN) {
-        auto vC = SF<LF, float>(a, CBM_F_VCOUPLE_GO, lds);
+        auto vC = SF<LFM, float>(a, CBM_F_VCOUPLE_GO, lds);
         if (a.anyCoupling) {
             const uint32_t kc = s.maxnumpGOCoupInGOGO;
             const float ratio = p.coupleRiRjRatioGO;
@@ -1231,7 +1237,7 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
                 vC[i] = acc;
             }
             if (tid == 0) {
-                auto th = SF<LF, float>(a, CBM_F_THRESH_GO, lds);
+                auto th = SF<LFM, float>(a, CBM_F_THRESH_GO, lds);
                 for (uint32_t i = 0; i < s.numGO; i++) {
                     float tc = 0;
                     for (int j = 0; j < gp(a.coupN)[i]; j++) tc = tc + ratio * (th[gp(a.coupP)[(size_t)i * kc + j]] - th[i]);
@@ -1249,7 +1255,7 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
     // updateSCPCOut (mzone.cpp:467-477), updatePCOut (mzone.cpp:409-447) and
     // the IO coupling of updateIOOut (mzone.cpp:486-489), from LDS
     if constexpr (MZ) {
-        auto inBCPC = SF<LF, uint32_t>(a, CBM_F_INPUT_BC_PC, lds);
+        auto inBCPC = SF<LFM, uint32_t>(a, CBM_F_INPUT_BC_PC, lds);
         for (uint32_t i = (tid - L.oRBCPC) & (nThr - 1); i < nZ * s.numPC; i += nThr) {
             const uint32_t z = i >> L.l2PC, pc = i & (s.numPC - 1);
             const uint16_t *ell = s_idx + L.iBCPC + (size_t)z * L.slotsBCPC * s.numPC + pc;
@@ -1271,11 +1277,11 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
             inBCPC[i] = c;
         }
         if constexpr (PART == kPartAll) {  // (microzone part: at the next step's start)
-            auto inSC = SF<LF, uint8_t>(a, CBM_F_INPUT_SC_PC, lds);
+            auto inSC = SF<LFM, uint8_t>(a, CBM_F_INPUT_SC_PC, lds);
             for (uint32_t q = (tid - L.oRSCPC) & (nThr - 1); q < nZ * s.numPC * kSC; q += nThr)
                 inSC[q] = s_apSC[s_idx[L.iSCPC + q]];
         }
-        auto inPCBC = SF<LF, uint32_t>(a, CBM_F_INPUT_PC_BC, lds);
+        auto inPCBC = SF<LFM, uint32_t>(a, CBM_F_INPUT_PC_BC, lds);
         for (uint32_t i = (tid - L.oRPCBC) & (nThr - 1); i < nZ * s.numBC; i += nThr) {
             const uint32_t z = i >> L.l2BC, bc = i & (s.numBC - 1);
             const uint16_t *ell = s_idx + L.iPCBC + (size_t)z * L.slotsPCBC * s.numBC + bc;
@@ -1294,11 +1300,11 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
             }
             inPCBC[i] = c;
         }
-        auto inPCNC = SF<LF, uint8_t>(a, CBM_F_INPUT_PC_NC, lds);
+        auto inPCNC = SF<LFM, uint8_t>(a, CBM_F_INPUT_PC_NC, lds);
         const uint32_t per = s.numNC * kPC;
         for (uint32_t q = (tid - L.oRPCNC) & (nThr - 1); q < nZ * per; q += nThr)
             inPCNC[q] = s_apPC[(q / per) * s.numPC + s_idx[L.iPCNC + q]];
-        auto vCIO = SF<LF, float>(a, CBM_F_VCOUPLE_IO, lds);
+        auto vCIO = SF<LFM, float>(a, CBM_F_VCOUPLE_IO, lds);
         for (uint32_t i = (tid - L.oRIO) & (nThr - 1); i < nZ * s.numIO; i += nThr) {
             const uint32_t z = i >> L.l2IO;
             float vc = 0;
@@ -1335,9 +1341,9 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
         }
         s_doLTD[z] = doLTD;
         s_doLTP[z] = doLTP;
-        for (uint32_t k = 0; k < 3; k++) SF<LF, uint32_t>(a, CBM_F_PC_POPACT_CTRS, lds)[z * 3 + k] = ctrs[k];
-        for (uint32_t k = 0; k < bins; k++) SF<LF, uint32_t>(a, CBM_F_HIST_PC_POPACT, lds)[(size_t)z * bins + k] = hist[k];
-        for (uint32_t k = 0; k < 2; k++) SF<LF, uint8_t>(a, CBM_F_MFNC_NOLTP_NOLTD, lds)[z * 2 + k] = (uint8_t)flags[k];
+        for (uint32_t k = 0; k < 3; k++) SF<LFM, uint32_t>(a, CBM_F_PC_POPACT_CTRS, lds)[z * 3 + k] = ctrs[k];
+        for (uint32_t k = 0; k < bins; k++) SF<LFM, uint32_t>(a, CBM_F_HIST_PC_POPACT, lds)[(size_t)z * bins + k] = hist[k];
+        for (uint32_t k = 0; k < 2; k++) SF<LFM, uint8_t>(a, CBM_F_MFNC_NOLTP_NOLTD, lds)[z * 2 + k] = (uint8_t)flags[k];
     }
     // spike recorder (row t mod recCap), and one population's spikes for the
     // host's step-and-read call straight into its pinned buffer
@@ -1374,7 +1380,7 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
     // block prebuilt by an earlier launch when its tag says it follows the
     // current one, and computes it here otherwise.
     if (MZ && wave == 0) {
-        auto inNC = SF<LF, uint8_t>(a, CBM_F_INPUT_NC_IO, lds);
+        auto inNC = SF<LFM, uint8_t>(a, CBM_F_INPUT_NC_IO, lds);
         const uint32_t n = s.numIO * kNCIO;
         const uint32_t *s_next = at<uint32_t>(lds, L.rngNext);
         for (uint32_t z = 0; z < nZ; z++) {
@@ -1419,7 +1425,7 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
     // GO->GO synaptic depression scale (innet.cpp:952-954), after the GABA
     // gather above has used the previous scale
     if constexpr (IN) {
-        auto scale = SF<LF, float>(a, CBM_F_GO_GABA_OUT_SYNSCALE_GOGO, lds);
+        auto scale = SF<LFM, float>(a, CBM_F_GO_GABA_OUT_SYNSCALE_GOGO, lds);
         const float rec = p.goGABAGOGOSynRec, depF = p.goGABAGOGOSynDepF;
         for (uint32_t i = tid; i < s.numGO; i += nThr) {
             const float sc = s_scale[i];
@@ -1430,7 +1436,7 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
     // MZone::updateMFNCSyn weights (mzone.cpp:586-600)
     if (MZ && st.popHist) {
         const uint32_t per = s.numNC * kMF;
-        auto w = SF<LF, float>(a, CBM_F_MF_SYNW_NC, lds);
+        auto w = SF<LFM, float>(a, CBM_F_MF_SYNW_NC, lds);
         for (uint32_t q = tid; q < nZ * per; q += nThr) {
             const uint32_t z = q / per;
             const int doLTD = s_doLTD[z], doLTP = s_doLTP[z];
@@ -1445,7 +1451,7 @@ __device__ __forceinline__ void small_step(const SmallArgs &a, const SmallStep &
     // MF history, cleared every numTSinMFHist steps (innet.cpp:982-995), kept
     // by the microzone part (MF->NC plasticity reads it)
     if constexpr (MZ) {
-        auto histMF = SF<LF, uint8_t>(a, CBM_F_HIST_MF, lds);
+        auto histMF = SF<LFM, uint8_t>(a, CBM_F_HIST_MF, lds);
         for (uint32_t i = tid; i < s.numMF; i += nThr) histMF[i] = st.resetMFHist ? 0 : s_hmf[i];
     }
 }
@@ -1738,7 +1744,7 @@ __device__ __forceinline__ void res_fields(const SmallArgs &a, uint8_t *lds, boo
 // One part of the small populations' step in a resident run (PART: the
 // input layer on block 0, the microzones on block spans + 1), its state
 // fields held in the block's LDS from the first step to the last.
-template <int PART, bool LOOP>
+template <int PART, bool LOOP, bool SPILL>
 __device__ __forceinline__ void res_part(const SmallArgs &a, const ResRun &r, float *s_err, uint8_t *lds)
 {
     constexpr bool IN = PART == kPartIn;
@@ -1809,7 +1815,7 @@ __device__ __forceinline__ void res_part(const SmallArgs &a, const ResRun &r, fl
         uint64_t pa = reinterpret_cast<uint64_t>(&a);
         asm volatile("" : "+s"(pa));
         const SmallArgs *ap = (const SmallArgs *)(const __attribute__((address_space(4))) SmallArgs *)pa;
-        small_step<true, true, PART>(*ap, st, s_err, lds);
+        small_step<true, true, PART, SPILL>(*ap, st, s_err, lds);
         if constexpr (IN) {
             res_publish(r.flagIn, r.flagInBase + k + 1);
         } else {
@@ -1842,7 +1848,7 @@ __device__ __forceinline__ void res_part(const SmallArgs &a, const ResRun &r, fl
             uint8_t *s_apSC = lds + a.lds.apSC;
             for (uint32_t i = threadIdx.x; i < nSC; i += kSmallThreads) s_apSC[i] = xldg<true>(sc + i);
             __syncthreads();
-            auto inSC = SF<true, uint8_t>(a, CBM_F_INPUT_SC_PC, lds);
+            auto inSC = SF<SPILL ? 2 : 1, uint8_t>(a, CBM_F_INPUT_SC_PC, lds);
             const uint16_t *s_idx = at<uint16_t>(lds, a.lds.idx);
             for (uint32_t q = threadIdx.x; q < nZ * a.s.numPC * a.s.numpPCfromSCtoPC; q += kSmallThreads)
                 inSC[q] = s_apSC[s_idx[a.lds.iSCPC + q]];
@@ -2011,27 +2017,28 @@ __device__ __forceinline__ void res_gr(const SmallArgs &a, const ResRun &r, cons
 // step counters and flags stand in for the launch boundary.  CPT: cells per
 // thread (every span holds at most 1024 * CPT cells); compact connectivity
 // only.
-template <int CPT, bool LOOP>
+template <int CPT, bool LOOP, bool SPILL>
 __global__ void __launch_bounds__(kSmallThreads) step_resident_kernel(const SmallArgs *__restrict__ args, const ResRun r)
 {
     extern __shared__ __align__(16) uint8_t lds[];
     __shared__ float s_err[kMaxZones];  // in LDS: the small step indexes it by zone
     if (blockIdx.x == 0)
-        res_part<kPartIn, LOOP>(*args, r, s_err, lds);
+        res_part<kPartIn, LOOP, false>(*args, r, s_err, lds);
     else if (blockIdx.x == r.spans + 1)
-        res_part<kPartMz, LOOP>(*args, r, s_err, lds);
+        res_part<kPartMz, LOOP, SPILL>(*args, r, s_err, lds);
     else if (blockIdx.x > r.spans)
         res_prebuild<LOOP>(*args, r, lds);
     else
         res_gr<CPT, LOOP>(*args, r, blockIdx.x - 1, lds);
 }
 
-#define CBM_RESIDENT(CPT)                                                                     \
-    template __global__ void step_resident_kernel<CPT, false>(const SmallArgs *, const ResRun); \
-    template __global__ void step_resident_kernel<CPT, true>(const SmallArgs *, const ResRun);
-CBM_RESIDENT(2)
-CBM_RESIDENT(3)
-CBM_RESIDENT(5)
+#define CBM_RESIDENT(CPT, SPILL)                                                                     \
+    template __global__ void step_resident_kernel<CPT, false, SPILL>(const SmallArgs *, const ResRun); \
+    template __global__ void step_resident_kernel<CPT, true, SPILL>(const SmallArgs *, const ResRun);
+CBM_RESIDENT(2, false)
+CBM_RESIDENT(3, false)
+CBM_RESIDENT(5, false)
+CBM_RESIDENT(5, true)  // (1M-GR networks of many MFs: the cart-pole network)
 
 // ---------------------------------------------------------------- streaming ceiling
 // A per-cell pass with the GR step's streams and no arithmetic to speak of:

@@ -550,12 +550,30 @@ int cbm_write(cbm_handle *h, int field, int zone, const void *src, size_t bytes)
     return CBM_OK;
 }
 
-// the resident-run kernel of a span shape
-static const void *residentKernel(int cpt, bool loop = false)
+// the resident-run kernel of a span shape (spill: the small populations'
+// fields that do not fit in LDS stay in global memory; built for the 5-chunk
+// span only)
+static const void *residentKernel(int cpt, bool loop, bool spill)
 {
-#define CBM_RK(C) (loop ? (const void *)step_resident_kernel<C, true> : (const void *)step_resident_kernel<C, false>)
+    if (spill)
+        return loop ? (const void *)step_resident_kernel<5, true, true> : (const void *)step_resident_kernel<5, false, true>;
+#define CBM_RK(C) \
+    (loop ? (const void *)step_resident_kernel<C, true, false> : (const void *)step_resident_kernel<C, false, false>)
     return cpt == 2 ? CBM_RK(2) : cpt == 3 ? CBM_RK(3) : CBM_RK(5);
 #undef CBM_RK
+}
+static const void *residentKernel(const cbm_handle *h, bool loop)
+{
+    return residentKernel(h->resCpt, loop, h->resSpill);
+}
+
+// the six MF->NC synapse arrays, the first fields a resident run leaves in
+// global memory when not all fit in LDS (large, and each read once a step in
+// order)
+static bool mfNcField(int f)
+{
+    return f == CBM_F_INPUT_MF_NC || f == CBM_F_MF_SYNW_NC || f == CBM_F_GMAX_NMDA_MF_NC ||
+           f == CBM_F_GMAX_AMPA_MF_NC || f == CBM_F_G_MF_NMDA_NC || f == CBM_F_G_MF_AMPA_NC;
 }
 
 // Decide at commit whether cbm_run_mf_train runs resident (stepMode 0 or 5;
@@ -572,6 +590,7 @@ static int setupResident(cbm_handle *h, int cus)
     const cbm_sizes &s = h->s;
     const int sm = (int)h->o.stepMode;
     h->resident = false;
+    h->resSpill = false;
     // the GR cells split evenly over one block per CU but three (the two
     // parts of the small populations and the generator block)
     const uint32_t spans = cus > 4 ? (uint32_t)cus - 3 : 0u;
@@ -581,24 +600,50 @@ static int setupResident(cbm_handle *h, int cus)
         int ldsPerCU = 0;
         HIP_TRY(hipDeviceGetAttribute(&ldsPerCU, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, h->o.device));
         const uint32_t recOff = roundUp(grTileLds(1024, cpt, s.numGO, s.numMF, s.numZones, s.numIO), 16);
-        // each part's fields after the step's scratch LDS in its own block
+        // each part's fields after the step's scratch LDS in its own block.
+        // When they do not all fit, the 5-chunk span's spilling kernel places
+        // the input layer's first in the microzones' synapse scratch (which
+        // the input-layer part does not use), and leaves the microzones'
+        // MF->NC synapse arrays in global memory, then each microzone field
+        // in turn that does not fit in what is left
         SmallArgs &sa = h->smallHost;
         const uint32_t base = roundUp((uint32_t)h->smallLds, 16);
-        uint32_t off = base, top = base;
-        for (int e = 0; e < kSmallFieldCount; e++) {
-            if (e == kSmallInFields) off = base;
-            const int f = kSmallFields[e];
-            const size_t bytes = h->zoneBytes(f) * (size_t)h->zonesOf(f);
-            sa.fLds[f] = off;
-            sa.ldsField[e] = (uint32_t)f;
-            sa.ldsBytes[e] = (uint32_t)bytes;
-            off = roundUp(off + (uint32_t)bytes, 16);
-            top = std::max(top, off);
+        size_t lds = 0;
+        bool big = false;
+        for (int pass = 0; pass < (cpt == 5 ? 2 : 1); pass++) {
+            big = pass == 1;
+            for (int f = 0; f < CBM_F_COUNT; f++) sa.fLds[f] = kNotInLds;
+            uint32_t off = base, top = base, n = 0, nIn = 0;
+            uint32_t hole = sa.lds.syn;
+            const uint32_t holeEnd = sa.lds.apPC;
+            for (int e = 0; e < kSmallFieldCount; e++) {
+                if (e == kSmallInFields) {
+                    off = base;
+                    nIn = n;
+                }
+                const int f = kSmallFields[e];
+                const bool in = e < kSmallInFields;
+                const uint32_t bytes = (uint32_t)(h->zoneBytes(f) * (size_t)h->zonesOf(f));
+                if (big && !in && (mfNcField(f) || off + bytes > kSmallLdsMax)) continue;
+                if (big && in && hole + bytes <= holeEnd) {
+                    sa.fLds[f] = hole;
+                    hole = roundUp(hole + bytes, 16);
+                } else {
+                    sa.fLds[f] = off;
+                    off = roundUp(off + bytes, 16);
+                    top = std::max(top, off);
+                }
+                sa.ldsField[n] = (uint32_t)f;
+                sa.ldsBytes[n++] = bytes;
+            }
+            sa.nLdsFields = n;
+            sa.nInFields = nIn;
+            const size_t need = std::max<size_t>(top, recOff + (size_t)cpt * 1024 * 16);
+            lds = std::max<size_t>(need, roundUp((uint32_t)ldsPerCU / 2 + 16, 16));
+            if (lds <= kSmallLdsMax) break;
         }
-        const size_t need = std::max<size_t>(top, recOff + (size_t)cpt * 1024 * 16);
-        const size_t lds = std::max<size_t>(need, roundUp((uint32_t)ldsPerCU / 2 + 16, 16));
         if (ldsPerCU > 0 && lds <= kSmallLdsMax) {
-            const void *k = residentKernel(cpt), *kl = residentKernel(cpt, true);
+            const void *k = residentKernel(cpt, false, big), *kl = residentKernel(cpt, true, big);
             HIP_TRY(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
             HIP_TRY(hipFuncSetAttribute(kl, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
             int perCU = 0, perCUl = 0;
@@ -620,9 +665,8 @@ static int setupResident(cbm_handle *h, int cus)
                 h->resCpt = cpt;
                 h->residentLds = lds;
                 h->residentRecLds = recOff;
+                h->resSpill = big;
                 h->resident = true;
-                sa.nLdsFields = kSmallFieldCount;
-                sa.nInFields = kSmallInFields;
             }
         }
     }
@@ -1503,7 +1547,7 @@ static int runResident(cbm_handle *h, const uint8_t *mf_dev, size_t pitch, int n
     runFields(h, &r);
     r.pitch = pitch;
     const uint32_t grid = h->resSpans + 3;
-    const void *kern = residentKernel(h->resCpt);
+    const void *kern = residentKernel(h, false);
     const SmallArgs *sa = h->smallArgs;
     for (int k0 = 0; k0 < n;) {
         const int m = std::min(n - k0, kResMaxSteps);
@@ -1641,7 +1685,7 @@ int cbm_loop_begin(cbm_handle *h, int max_steps, int population)
     r.mfFlag = h->loopFlag;
     const SmallArgs *sa = h->smallArgs;
     void *args[] = {&sa, &r};
-    if ((rc = launchResident(h, residentKernel(h->resCpt, true), h->resSpans + 3, args))) return rc;
+    if ((rc = launchResident(h, residentKernel(h, true), h->resSpans + 3, args))) return rc;
     h->looping = true;
     h->loopK = 0;
     h->loopMax = max_steps;

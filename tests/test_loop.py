@@ -16,9 +16,11 @@ from test_gpu_parity import compare
 
 pytestmark = pytest.mark.gpu
 
-# 4K-GR and 8K-GR two-zone networks (one 64-GR chunk per span or none), and
-# the 1M-GR network of the headline metric
-CASES = [("t4k", 70, "NC"), ("t8k2z", 90, "NC"), ("c1", 45, "PC")]
+# 4K-GR and 8K-GR two-zone networks (one 64-GR chunk per span or none), the
+# 1M-GR network of the headline metric, and the 1M-GR two-zone network of
+# 2,048 MFs of the cart-pole task (whose MF->NC synapse arrays do not fit in
+# the microzone block's LDS beside its other fields and stay in global memory)
+CASES = [("t4k", 70, "NC"), ("t8k2z", 90, "NC"), ("c1", 45, "PC"), ("cp1m", 40, "NC")]
 
 
 @pytest.mark.parametrize("name,steps,pop", CASES, ids=[c[0] for c in CASES])
