@@ -577,8 +577,9 @@ static bool mfNcField(int f)
 }
 
 // Decide at commit whether cbm_run_mf_train runs resident (stepMode 0 or 5;
-// the even split and compact connectivity records; the small populations' state
-// fields fit in LDS beside the small step's scratch; one 1024-thread workgroup
+// the even split and compact connectivity records; the input layer's state
+// fields fit in LDS beside the small step's scratch (the microzones' may stay
+// partly in global memory with the 5-chunk span); one 1024-thread workgroup
 // of the kernel fits on every CU), and set it up: the LDS of the GR workgroups
 // (their span's working LDS and the records of their cells), the fields' place
 // in the small block's LDS (h->smallHost), the step counters and the status
@@ -676,7 +677,7 @@ static int setupResident(cbm_handle *h, int cus)
     }
     if (sm == 5 && !h->resident)
         return fail(CBM_E_UNSUPPORTED, "stepMode 5: resident runs need the even GR split, compact connectivity "
-                                       "records, the small populations' state in LDS and one 1024-thread "
+                                       "records, the input layer's state in LDS and one 1024-thread "
                                        "workgroup per CU");
     return CBM_OK;
 }

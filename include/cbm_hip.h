@@ -150,10 +150,11 @@ typedef struct cbm_options {
      * of two or more steps are resident, or cbm_commit fails.  Resident:
      * the whole run is one launch of one workgroup per CU whose GR
      * workgroups hold their cells' state in registers (and connectivity in
-     * LDS), and whose small-population workgroup holds theirs in LDS, from
+     * LDS), and whose small-population workgroups hold theirs in LDS, from
      * the first step to the last, synchronised point to point between steps
-     * (networks with the even split, compact connectivity records and small
-     * populations that fit in LDS).  The resident launches of a process run
+     * (networks with the even split, compact connectivity records and an
+     * input layer whose state fits in LDS; of the microzones' state, what
+     * does not fit stays in global memory).  The resident launches of a process run
      * on a device one after another; a kernel of another stream or process
      * that keeps CUs busy for longer than 0.2 s while a resident run waits
      * for them makes that run fail (CBM_E_HIP at the next cbm_sync or call;
