@@ -278,13 +278,23 @@ def cartpole_bench(which: str) -> dict:
     net.calc_activity(steps)
     net.sync()
     step_only = time.perf_counter() - t0
+    res_only = None
+    if net.resident:
+        # the same network's step inside one resident run, over a synthetic MF train
+        import torch
+        dtr = torch.from_numpy(configs.mf_train(st.sizes.numMF, steps, seed=2)).to("cuda")
+        net.sync()
+        t0 = time.perf_counter()
+        net.run_mf_train(dtr)
+        net.sync()
+        res_only = round((time.perf_counter() - t0) / steps * 1e6, 2)
     s = st.sizes
     out = {"workload": (f"cart-pole, two microzones, closed loop; {s.numGR:,} GR, {s.numMF:,} MF, {s.numNC} NC "
                         "per zone (the sizes of CBM_Params/conParams.txt)" if which == "full" else
                         f"cart-pole, two microzones, closed loop; {s.numGR:,} GR"),
            "steps": done, "sim_ms_per_s": round(done / el, 1), "us_per_step": round(el / done * 1e6, 2),
            "network_step_only_us": round(step_only / steps * 1e6, 2),
-           "resident_loop": bool(net.resident)}
+           "network_resident_step_us": res_only, "resident_loop": bool(net.resident)}
     env.close()
     net.close()
     return out
